@@ -1,0 +1,228 @@
+"""Benchmark: Wiener-filter PCG time to relative residual 1e-6 on one GPU.
+
+Workload: 300,000 pixels (1000 skewers x 300), box 64 x 64 x 600 Mpc/h, a
+Gaussian random field (var_s 0.05, l_perp 2.5, l_para 2.1) drawn from a seed,
+noise U[0.1, 0.3], W form (the reference app's system, app/dachshund.cc:234),
+x0 = 0 (or the learned warm start when a trained network is given).
+
+A step is one complete solve to |r| <= 1e-6 |b| with the inputs resident
+in HBM.  The headline solver is the mixed FP32/FP64 one (FP32 pair
+evaluation, FP64 vectors and true residuals); plain FP64 PCG (the
+reference's algorithm on the GPU) is timed on the same problem for the
+speed-up.  `value` is evaluated pixel-pair covariances per second over the
+timed solves.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 work of one pair in the reference's formulation
+# (lib/wf_vector.cc:208-217, lib/signal_covar.h:53-77): 3 differences, 4 for
+# the squares and sums, 4 for the scaled argument, 5 for the interpolation,
+# 1 var_s scaling, 2 weights, 1 identity add, 2 multiply-accumulate.
+FLOPS_PER_PAIR = 22
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}   # MI355X published vector peaks
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pix_w, var_s, l_perp, l_para, n_full, n_mv_full,
+                 budget_s=12.0):
+    """The reference's own WFVectorW::A_product (lib/wf_vector.cc:196-220),
+    compiled from its sources into oracle/_ref/libdsref.so with its
+    platforms/edison-gnu flags, timed on this host's cores on a bounded
+    subset of the pixels; returns the dense pair rate and the extrapolated
+    full-size solve time."""
+    import ctypes as C
+    lib_path = os.path.join(ROOT, "oracle", "_ref", "libdsref.so")
+    if not os.path.exists(lib_path):
+        return None
+    L = C.CDLL(lib_path)
+    dp = C.POINTER(C.c_double)
+    L.dsref_w_A_product.argtypes = [C.c_int, dp, C.c_double, C.c_double,
+                                    C.c_double, dp, dp]
+    L.dsref_num_threads.restype = C.c_int
+    threads = L.dsref_num_threads()
+    n = 8000
+    rate = None
+    while True:
+        sub = np.ascontiguousarray(pix_w[:n])
+        a = np.random.default_rng(0).standard_normal(n)
+        b = np.empty(n)
+        t0 = time.perf_counter()
+        L.dsref_w_A_product(n, sub.ctypes.data_as(dp), var_s, l_perp, l_para,
+                            a.ctypes.data_as(dp), b.ctypes.data_as(dp))
+        dt = time.perf_counter() - t0
+        rate = n * n / dt
+        log(f"cpu reference A_product n={n}: {dt:.2f} s, "
+            f"{rate:.3e} pairs/s on {threads} threads")
+        nxt = int(n * min(4.0, max(1.2, (budget_s / max(dt, 1e-3)) ** 0.5)))
+        if dt > budget_s / 4 or nxt >= len(pix_w) or n >= 120000:
+            break
+        n = min(nxt, len(pix_w))
+    return {
+        "value": rate, "unit": "pair-evals/s", "cores": threads,
+        "kind": "reference",
+        "sample": (f"one WFVectorW::A_product of the reference (all n^2 "
+                   f"pairs, OpenMP) on the first {n} pixels of the same "
+                   f"workload"),
+        "extrapolated_solve_s": n_full * n_full * n_mv_full / rate,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c5")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--drop", type=float, default=1e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the "
+              "solve is not sharded across GPUs in this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import dachshund_amd as D
+    from dachshund_amd import synth
+
+    w = synth.workload(args.workload)
+    t0 = time.perf_counter()
+    pix = synth.make_pixels(w)
+    n = len(pix)
+    log(f"workload {w.name}: n={n}, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    pw = D.pixel_to_wpixel(pix)
+    wf = D.WFVectorW(pw, sp)
+    st = wf.stats()
+    log(f"setup {st.setup_seconds:.2f} s; {st.pairs_per_matvec:.3e} "
+        f"evaluated pairs per matvec ({st.pairs_per_matvec / n / n:.2e} of "
+        f"n^2)")
+    prm = D.PCGParams(n, w.max_iter, w.step_r, args.tol)
+
+    def solve(mixed):
+        wf.fill_x(0.0)
+        if mixed:
+            return wf.solve_resident(prm, mixed=True, refine_drop=args.drop)
+        return wf.solve_resident(prm)
+
+    def timed(mixed, k):
+        D.default_context()
+        ts, res = [], None
+        for _ in range(k):
+            t = time.perf_counter()
+            res = solve(mixed)
+            ts.append(time.perf_counter() - t)
+            if not res.success:
+                raise RuntimeError(f"solve (mixed={mixed}) did not converge")
+        return ts, res
+
+    for _ in range(args.warmup):
+        solve(False)
+        solve(True)
+    t64, r64 = timed(False, args.steps)
+    x64 = wf.get_x()
+    log(f"plain FP64 PCG: {np.mean(t64) * 1e3:.1f} ms, {r64.num_iters} iters,"
+        f" {r64.n_matvec64} FP64 matvecs")
+    tmx, rmx = timed(True, args.steps)
+    xmx = wf.get_x()
+    log(f"mixed: {np.mean(tmx) * 1e3:.1f} ms, {rmx.num_iters} iters, "
+        f"{rmx.n_matvec32} FP32 + {rmx.n_matvec64} FP64 matvecs, true "
+        f"|r|/|b| {rmx.true_norm / r64.norm_b if False else 0:.1e}")
+
+    # accuracy of the mixed map against a tight FP64 solve
+    tight = D.PCGParams(n, 4 * w.max_iter, w.step_r, 1e-10)
+    wf.fill_x(0.0)
+    rt = wf.solve_resident(tight)
+    box, shape = w.box, w.map_shape
+    m_ref = wf.smp_grid(*box, *shape)
+    wf.set_x(xmx)
+    m_mx = wf.smp_grid(*box, *shape)
+    wf.set_x(x64)
+    m_64 = wf.smp_grid(*box, *shape)
+    rel_mx = float(np.linalg.norm(m_mx - m_ref) / np.linalg.norm(m_ref))
+    rel_64 = float(np.linalg.norm(m_64 - m_ref) / np.linalg.norm(m_ref))
+    log(f"map rel-L2 vs FP64 tol 1e-10 solve ({rt.num_iters} iters): mixed "
+        f"{rel_mx:.2e}, plain FP64 {rel_64:.2e}")
+
+    # dominant kernel: FP32 pair matvec (the mixed solver's inner operator)
+    ms32 = wf.time_matvec(D.DS_PREC_F32, reps=20)
+    ms64 = wf.time_matvec(D.DS_PREC_F64, reps=20)
+    pairs = st.pairs_per_matvec
+    tf32 = FLOPS_PER_PAIR * pairs / (ms32 * 1e-3) / 1e12
+    tf64 = FLOPS_PER_PAIR * pairs / (ms64 * 1e-3) / 1e12
+    log(f"pair kernels: FP32 {ms32:.3f} ms ({pairs / ms32 / 1e9:.1f} Gpair/s)"
+        f", FP64 {ms64:.3f} ms ({pairs / ms64 / 1e9:.1f} Gpair/s)")
+
+    t_mx = float(np.mean(tmx))
+    t_64 = float(np.mean(t64))
+    evals = pairs * (rmx.n_matvec32 + rmx.n_matvec64)
+    out = {
+        "metric": "PCG time-to-rel-residual 1e-6 (mixed vs plain FP64, 1 "
+                  "GPU); pair-evals/s",
+        "value": evals / t_mx,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mx * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pairs + f64 vectors",
+        "data": "synthetic (seeded Gaussian random field skewers)",
+        "config": {
+            "workload": (f"{n} pixels ({w.n_skewers} skewers x "
+                         f"{w.n_per_skewer}), box {w.box}, GRF var_s "
+                         f"{w.var_s} l_perp {w.l_perp} l_para {w.l_para}, "
+                         f"noise U[0.1,0.3], seed {w.seed}, W form, x0 = 0"),
+            "tol": args.tol, "step_r": w.step_r,
+            "refine_drop": args.drop,
+        },
+        "time_to_tol_s": {"mixed": t_mx, "plain_fp64": t_64},
+        "speedup_vs_plain_fp64": t_64 / t_mx,
+        "iterations": {"mixed": rmx.num_iters, "mixed_fp64_residuals":
+                       rmx.n_true, "plain_fp64": r64.num_iters},
+        "map_rel_l2_vs_fp64_tight": {"mixed": rel_mx, "plain_fp64": rel_64},
+        "pairs_per_matvec": pairs,
+        "dense_equiv_pair_evals_per_s": n * n * (rmx.n_matvec32 +
+                                                 rmx.n_matvec64) / t_mx,
+        "roofline": {
+            "bound": "valu", "achieved": tf32, "peak": PEAK_TFLOPS["f32"],
+            "unit": "TFLOP/s", "frac": tf32 / PEAK_TFLOPS["f32"],
+            "traffic": None,
+            "kernel": "k_matvec32 (FP32 pair matvec)",
+            "ms_per_launch": ms32, "flops_per_pair": FLOPS_PER_PAIR,
+            "pairs_per_launch": pairs,
+        },
+        "fp64_kernel": {"achieved": tf64, "peak": PEAK_TFLOPS["f64"],
+                        "unit": "TFLOP/s", "frac": tf64 / PEAK_TFLOPS["f64"],
+                        "ms_per_launch": ms64},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(pw, w.var_s, w.l_perp, w.l_para, n, r64.n_matvec64)
+        if cb:
+            out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

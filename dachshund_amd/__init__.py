@@ -1,0 +1,21 @@
+"""dachshund_amd: gfx950 (MI355X) Wiener-filter reconstruction.
+
+The product is libdachshund_gpu.so (HIP kernels behind the C ABI of
+include/dachshund_gpu.h) and the C++ drop-in of the reference's library API
+built on it; this package is the Python mirror of the same interface.
+"""
+from .wiener import (DS_PREC_F32, DS_PREC_F64, Context, PCGParams, PCGResult,
+                     SignalCovarParams, WFVectorSN, WFVectorW,
+                     default_context, init_uniform_map_coords, map_covar_diag,
+                     npixel_to_pixel, pixel_to_npixel, pixel_to_wpixel,
+                     plan_tiles, points_from_pixels, read_pixel_data,
+                     sigma_m_noise_dom, smp_product, smp_product_grid,
+                     wpixel_to_pixel)
+
+__all__ = [
+    "DS_PREC_F32", "DS_PREC_F64", "Context", "PCGParams", "PCGResult",
+    "SignalCovarParams", "WFVectorSN", "WFVectorW", "default_context",
+    "init_uniform_map_coords", "map_covar_diag", "npixel_to_pixel", "pixel_to_npixel",
+    "pixel_to_wpixel", "plan_tiles", "points_from_pixels", "read_pixel_data",
+    "sigma_m_noise_dom", "smp_product", "smp_product_grid", "wpixel_to_pixel",
+]

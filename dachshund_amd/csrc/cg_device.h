@@ -1,0 +1,178 @@
+// cg_device.h -- device helpers shared by the pair kernels and the CG vector
+// kernels: deterministic reductions, the last-block-done ticket, and the
+// finalisation of the FP64 CG scalars of lib/pcg.h:27-65.
+//
+// Every reduction sums in a fixed order that depends only on the grid shape,
+// so repeated solves are bit-for-bit reproducible.  A kernel that ends a
+// reduction writes its per-block partials, takes a ticket from
+// ds_cg_scalars::counter, and the block holding the last ticket sums the
+// partials and updates the scalars; it then resets the ticket, so no
+// memset is ever needed between launches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ds_internal.h"
+
+namespace ds {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide sum of NV values per thread; result valid in every thread.
+// `red` holds NV * (blockDim.x / 64) doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k * nw + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[k * nw + i];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// Publish this block's NV partial sums (thread 0 holds them) and return true
+// in every thread of the block that took the last ticket.
+template <int NV>
+__device__ __forceinline__ bool publish_partials(const double (&v)[NV],
+                                                 double* partials,
+                                                 unsigned* counter,
+                                                 bool* is_last_lds) {
+  const int G = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&partials[(size_t)k * G + blockIdx.x], v[k],
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    *is_last_lds = (t == (unsigned)G - 1);
+  }
+  __syncthreads();
+  const bool last = *is_last_lds;
+  if (last) __threadfence();
+  return last;
+}
+
+// In the last block: sum the NV partial vectors in a fixed order.
+template <int NV>
+__device__ __forceinline__ void gather_partials(double (&v)[NV],
+                                                const double* partials,
+                                                double* red) {
+  const int G = gridDim.x;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x)
+      s += load_coherent(&partials[(size_t)k * G + b]);
+    v[k] = s;
+  }
+  block_sum<NV>(v, red);
+}
+
+__device__ __forceinline__ void release_ticket(unsigned* counter) {
+  if (threadIdx.x == 0)
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// lib/pcg.h:35-37: alpha = delta / (d . q).
+__device__ __forceinline__ void finalize_alpha(ds_cg_scalars* sc, double dq) {
+  sc->dq = dq;
+  sc->alpha = sc->delta / dq;
+}
+
+// lib/pcg.h:54-61 and :123-135 (or, with init, :100-110): new delta, beta,
+// |r|, residual history and the stop rule.  rs = r . M^-1 r, rr = r . r.
+// Mixed solver (mixed != 0): the recurrence residual never stops the solve
+// itself; when it meets the tolerance or has dropped by `drop` since the
+// last FP64 true residual, the next true-residual kernel runs (want_true).
+__device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
+                                              double rs, double rr, int init,
+                                              int mixed, double drop,
+                                              int is_true) {
+  const double norm_r = sqrt(rr);
+  if (init) {
+    sc->delta = rs;
+    sc->norm_r = norm_r;
+    sc->iter = 0;
+    sc->n_norms = 1;
+    norms[0] = norm_r;
+    if (mixed) {
+      sc->ref_norm = norm_r;
+      sc->true_norm = norm_r;
+      sc->want_true = 0;
+    }
+    return;
+  }
+  // A true residual of the mixed solver replaces the recurrence residual of
+  // the step that just ended: same iteration, new delta/beta/|r|.
+  const double delta_old = is_true ? sc->saved_delta_old : sc->delta;
+  sc->delta = rs;
+  sc->beta = rs / delta_old;
+  sc->norm_r = norm_r;
+  const int iter = is_true ? sc->iter : sc->iter + 1;
+  sc->iter = iter;
+  norms[iter] = norm_r;
+  sc->n_norms = iter + 1;
+  if (!mixed) {
+    if (norm_r <= sc->tol_norm_b) {
+      sc->done = 1;
+      sc->success = 1;
+      sc->num_iters = iter;
+    } else if (iter + 1 >= sc->max_iter) {
+      // lib/pcg.h:118,151-158: the loop ends with iter == max_iter.
+      sc->done = 1;
+      sc->success = 0;
+      sc->num_iters = sc->max_iter;
+    }
+    return;
+  }
+  if (is_true) {
+    sc->n_true += 1;
+    sc->true_norm = norm_r;
+    sc->ref_norm = norm_r;
+    sc->want_true = 0;
+    if (norm_r <= sc->tol_norm_b) {
+      sc->done = 1;
+      sc->success = 1;
+      sc->num_iters = iter;
+      return;
+    }
+  } else {
+    if (norm_r <= sc->tol_norm_b || norm_r <= drop * sc->ref_norm) {
+      sc->want_true = 1;
+      sc->saved_delta_old = delta_old;
+      return;
+    }
+  }
+  if (iter + 1 >= sc->max_iter) {
+    sc->done = 1;
+    sc->success = 0;
+    sc->num_iters = sc->max_iter;
+  }
+}
+
+}  // namespace ds

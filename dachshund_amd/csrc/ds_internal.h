@@ -1,15 +1,19 @@
 // ds_internal.h -- shared types of the gfx950 Wiener-filter library.
 //
-// Layout in HBM (all per problem, in *sorted* pixel order):
-//   pos64   double4[npad]  {x, y, z, w}   w = 1/n (W form) or N = n^2 (SN form)
-//   rhs64   double[npad]   wd (W form) or d (SN form)
-//   loc32   float4[npad]   pre-scaled coordinates relative to the block centre
-//   blk_c   double4[nblk]  block centre in pre-scaled coordinates
-//   pairs   CSR (pair_ptr[nblk+1], pair_idx[npairs]) of source blocks per
-//           target block that survive the exact covariance cutoff
-// Pixels are sorted by a k-d tree into blocks of DS_BS = 64 pixels (one
-// wavefront of target rows); the tail block is padded with copies of the
-// last pixel whose vector entries are kept at zero.
+// Layout in HBM (per problem, all in *sorted* pixel order, npad entries):
+//   pos64   double4  {x, y, z, w}   w = 1/n (W form) or N = n^2 (SN form)
+//   rhs64   double   wd (W form) or d (SN form); 0 on pad slots
+//   loc32   float4   {X, Y, Z, 0}: coordinates relative to the owning block's
+//                    centre, pre-scaled so that X^2+Y^2+Z^2 is the table
+//                    argument (X = dx * sqrt(c_perp * dx_inv), ...)
+//   blk_c   double4  block centre in the same pre-scaled units
+//   tptr/tidx        CSR list, per target block of DS_BS = 64 rows, of the
+//                    source tiles (DS_TS = 16 consecutive slots) that can
+//                    hold a nonzero covariance under the exact table cutoff
+// Pixels are ordered by a k-d tree into blocks of 64, and each block again
+// into 4 tiles of 16.  Slots n..npad-1 are padding (copies of the last
+// pixel); every kernel writes 0 to padded rows and every vector keeps 0
+// there, so padded columns contribute exactly nothing.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,8 +23,11 @@
 
 #include "../../include/dachshund_gpu.h"
 
-#define DS_BS 64
+#define DS_BS 64          // target rows per block (one wavefront)
+#define DS_TS 16          // source slots per tile
+#define DS_TPB (DS_BS / DS_TS)
 #define DS_MAX_TABLE 1024
+#define DS_MAP_T 4        // map tiles are 4x4x4 cells
 
 namespace ds {
 
@@ -31,57 +38,66 @@ struct CovarDev {
   double c_para;
   double dx_inv;
   int table_n;
-  float s_perp;   // sqrt(c_perp * dx_inv): scaled perpendicular coordinate
-  float s_para;   // sqrt(c_para * dx_inv)
+  double s_perp;   // sqrt(c_perp * dx_inv)
+  double s_para;   // sqrt(c_para * dx_inv)
 };
 
 // Host-side result of spatial blocking.
 struct Blocking {
   int n = 0;
-  int nblk = 0;
+  int nblk = 0;                 // blocks of DS_BS
+  int ntile = 0;                // tiles of DS_TS (= nblk * DS_TPB)
   int npad = 0;
-  std::vector<int> perm;        // sorted slot -> original index (npad, tail = last)
-  std::vector<double> lo, hi;   // nblk * 3, physical AABB of the real pixels
+  std::vector<int> perm;        // sorted slot -> original index (npad)
+  std::vector<double> blo, bhi; // nblk * 3, physical AABB of real pixels
+  std::vector<double> tlo, thi; // ntile * 3 (empty tiles: lo > hi)
+  std::vector<int> tcount;      // real pixels per tile
 };
 
-// Sort points (stride doubles apart, x,y,z first) into k-d blocks of DS_BS.
-// scale[3] is the metric used to choose split axes (isotropic in it).
+// Sort points (stride doubles apart, x,y,z first) into k-d blocks of DS_BS,
+// each split again into DS_TPB tiles.  scale[3] weights the axes when
+// choosing splits (the covariance metric).
 void build_blocking(const double* pts, int stride, int n, const double scale[3],
                     Blocking& out);
 
-// Exact culling: keep (I, J) unless every pair between the two AABBs has
-// table index >= table_n - 1 under the reference's own arithmetic.
-void build_pair_lists(const Blocking& tgt, const Blocking& src,
-                      const CovarDev& cv, std::vector<int>& ptr,
-                      std::vector<int>& idx);
-// Same with targets given directly as AABBs (map tiles).
-void build_pair_lists_boxes(int ntgt, const double* tlo, const double* thi,
-                            const Blocking& src, const CovarDev& cv,
-                            std::vector<int>& ptr, std::vector<int>& idx);
+// Exact culling: for each target box keep the source tiles that may hold a
+// pair with table index < table_n - 1 under the reference's arithmetic.
+void build_tile_lists(int ntgt, const double* tlo, const double* thi,
+                      const Blocking& src, const CovarDev& cv,
+                      std::vector<int>& ptr, std::vector<int>& idx);
 
-// Lower bound of the table index for any pair drawn from two boxes, with the
-// operation order of lib/signal_covar.h:67-77 and lib/wf_vector.cc:62-66.
+// Lower bound of the table argument xx for any pair drawn from two boxes,
+// with the operation order of lib/signal_covar.h:67-77 and
+// lib/wf_vector.cc:62-66.
 double box_gap_xx(const double* alo, const double* ahi, const double* blo,
                   const double* bhi, const CovarDev& cv);
 
+CovarDev make_covar(const ds_covar* c);
+
 }  // namespace ds
 
-// Device-resident scalars of one CG solve.
+// Device-resident scalars of one CG solve (FP64).
 struct ds_cg_scalars {
   double delta;        // r . M^-1 r
   double alpha;
   double beta;
+  double dq;           // d . A d
   double norm_r;       // |r|_2 of the current residual
   double tol_norm_b;
-  double max_r;        // mixed solver: largest |r| since the last true residual
-  int iter;            // current iteration number (reference numbering)
+  double ref_norm;     // mixed solver: |r| at the last true residual
+  double true_norm;    // mixed solver: last FP64 true residual norm
+  double saved_delta_old;  // mixed solver: delta before the step whose
+                           // residual a true residual replaces
+  int iter;            // iterations completed (reference numbering)
   int done;            // 1 once converged or out of iterations
   int success;
   int num_iters;
   int n_norms;
-  int full_residual;   // this iteration recomputes r = b - A x
-  int n_true;          // mixed solver: number of FP64 true residuals
-  int pad;
+  int max_iter;
+  int n_true;          // mixed solver: FP64 true residuals taken
+  int want_true;       // mixed solver: next step recomputes r in FP64
+  unsigned counter;    // last-block-done ticket
+  unsigned pad;
 };
 
 struct ds_ctx {
@@ -89,34 +105,104 @@ struct ds_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int num_cu = 0;
+  double* scratch = nullptr;   // reduction partials
+  size_t scratch_n = 0;
+  unsigned* counter = nullptr; // last-block-done ticket for one-off reductions
 };
 
 struct ds_problem {
   ds_ctx* ctx = nullptr;
   int form = DS_FORM_W;
-  int n = 0, nblk = 0, npad = 0;
+  int n = 0, nblk = 0, ntile = 0, npad = 0;
   ds::CovarDev cv{};
   std::vector<double> table;           // host copy, table_n entries
   ds::Blocking blk;
-  std::vector<int> pair_ptr_h;
-  long long npairs = 0;                // block pairs after culling
+  long long ntile_pairs = 0;           // (block, tile) pairs after culling
+  double norm_b = 0.0;                 // |b| in the reference's order
+  double setup_seconds = 0.0;
+  int waves_per_block = 4;             // pair-kernel launch shape
   // device buffers
   double4* pos64 = nullptr;
   double* rhs64 = nullptr;
+  double* diag = nullptr;              // Jacobi diagonal of A
   float4* loc32 = nullptr;
   double4* blk_c = nullptr;
-  int* pair_ptr = nullptr;
-  int* pair_idx = nullptr;
-  double2* table64 = nullptr;          // {T[i], T[i+1]}, last = {0, 0}
-  float2* table32 = nullptr;           // {var_s T[i], var_s (T[i+1]-T[i])}
+  int* tptr = nullptr;
+  int* tidx = nullptr;
+  float4* toff32 = nullptr;            // per list entry: centre offsets
+  double2* table64 = nullptr;          // {T[k], T[k+1]-T[k]}, last = {0, 0}
+  float2* table32 = nullptr;
+  double* table_raw = nullptr;         // T[k] as given (dense/column paths)
   int* perm = nullptr;                 // sorted -> original
   // CG work vectors (npad each)
   double *x = nullptr, *r = nullptr, *d = nullptr, *s = nullptr, *q = nullptr,
-         *v = nullptr, *tmp = nullptr;
-  float* v32 = nullptr;
+         *tmp = nullptr;
+  double* wvec = nullptr;              // w (W form), 0 on pad slots
+  double *du = nullptr, *xu = nullptr; // w .* d, w .* x (W form)
+  float* du32 = nullptr;               // FP32 copy of du
   double* partials = nullptr;          // 4 * nblk
   ds_cg_scalars* sc = nullptr;
   double* norms = nullptr;             // device residual history
   int norms_cap = 0;
-  ds_cg_scalars* sc_host = nullptr;    // pinned mirror
+  ds_cg_scalars* sc_host = nullptr;    // pinned: [0] final, [1..2] polls
+  double* host_tmp = nullptr;          // pinned npad doubles
 };
+
+namespace ds {
+hipError_t launch_scale(const double* v, const double* w, double* u,
+                        float* u32, int npad, hipStream_t st);
+hipError_t launch_precond(const double* b, const double* diag, double* x,
+                          int npad, hipStream_t st);
+hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
+                      double drop, ds_pcg_result* res);
+hipError_t launch_fill(double* v, double value, int n, hipStream_t st);
+hipError_t launch_column(const ds_problem* p, const double* table_dev,
+                         const double* map_pt, double* col, hipStream_t st);
+hipError_t launch_dot1(const double* a, const double* b, int n, double* out,
+                       hipStream_t st);
+hipError_t dense_solve(ds_problem* p, int exact, const double* table_dev,
+                       int* info_out);
+}  // namespace ds
+
+namespace ds {
+
+// Pair-kernel launchers (pairs64.hip / pairs32.hip).  `in` is the matvec
+// input a in sorted order and `u` its scaled copy (w .* a for the W form, a
+// itself for SN); out = A a, or rhs - A a with `residual`.  The CG epilogues
+// fuse the reductions that follow the matvec in lib/pcg.h:35-37 (d . q and
+// alpha) or :52-61 plus :123-135 (precondition, r . M^-1 r, |r|, stop rule).
+enum EpiKind { EK_NONE = 0, EK_CG_DQ = 1, EK_CG_RES = 2 };
+struct CgEpilogue {
+  int kind = EK_NONE;
+  double* out2 = nullptr;   // EK_CG_RES: M^-1 r
+  int init = 0, mixed = 0, is_true = 0, need_want = 0;
+  double drop = 0.0;
+};
+hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
+                              const double* u, double* out,
+                              const CgEpilogue& e, hipStream_t st);
+hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
+                              const double* in, double* out, int cg,
+                              hipStream_t st);
+
+// Map-side launchers.  Sources are a blocked pixel set (pos64 / loc32 with
+// w or 1/N in .w), targets the uniform grid of lib/util.cc:25-46 cut into
+// 4x4x4 tiles, lists per map tile.
+struct MapGrid {
+  double lx, ly, lz;
+  int nx, ny, nz;
+  int tx, ty, tz;      // tiles per axis
+};
+enum MapMode { MM_SMP = 0, MM_SIGMA = 1 };
+hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
+                        const double* u, const int* tptr, const int* tidx,
+                        const double2* table64, const CovarDev& cv,
+                        double* m, hipStream_t st);
+// Arbitrary target points (Point records), targets sorted by a blocking.
+hipError_t launch_points64(int mode, int n_tgt, const double4* tgt,
+                           const int* tgt_perm, const double4* pos,
+                           const double* u, const int* tptr, const int* tidx,
+                           int n_tblk, const double2* table64,
+                           const CovarDev& cv, double* m, hipStream_t st);
+
+}  // namespace ds

@@ -3,13 +3,16 @@
 // The reference evaluates every one of the n^2 pairs (lib/wf_vector.cc:54-73)
 // but its covariance is exactly zero once the table index reaches
 // table_n - 1 (lib/signal_covar.h:57-62).  Pixels are sorted into compact
-// blocks of 64 with a k-d tree, and a (target, source) block pair is dropped
-// only when the smallest separation between the two bounding boxes already
-// gives an index >= table_n - 1.  Rounding is monotone, so evaluating the
-// reference's own expression on the box gaps bounds every pair's computed
-// index from below: the dropped terms are the reference's exact +0.0.
+// blocks of 64 (one wavefront of target rows) with a k-d tree, each block is
+// split into 4 tiles of 16 source slots, and a (target block, source tile)
+// pair is dropped only when the smallest separation between the two bounding
+// boxes already gives an index >= table_n - 1.  Rounding is monotone, so
+// evaluating the reference's own expression on the box gaps bounds every
+// pair's computed index from below: the dropped terms are the reference's
+// exact +0.0.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <thread>
 
@@ -18,10 +21,6 @@
 namespace ds {
 
 namespace {
-
-struct Node {
-  int b, e;
-};
 
 unsigned n_threads() {
   unsigned t = std::thread::hardware_concurrency();
@@ -43,24 +42,18 @@ void parallel_for(int n, F f) {
   for (auto& x : th) x.join();
 }
 
-}  // namespace
-
-void build_blocking(const double* pts, int stride, int n, const double scale[3],
-                    Blocking& out) {
-  out.n = n;
-  out.nblk = (n + DS_BS - 1) / DS_BS;
-  out.npad = out.nblk * DS_BS;
-  std::vector<int> perm(n);
-  std::iota(perm.begin(), perm.end(), 0);
-  // Every left child holds a multiple of DS_BS points, so each leaf starts
-  // at a multiple of DS_BS and only the last leaf can be short.
+// Recursively split perm[b, e) into leaves of `leaf` points; every left child
+// holds a multiple of `leaf` points so only the last leaf can be short.
+void kd_sort(const double* pts, int stride, const double scale[3], int* perm,
+             int b, int e, int leaf) {
+  struct Node { int b, e; };
   std::vector<Node> stack;
-  if (n > 0) stack.push_back({0, n});
+  if (e > b) stack.push_back({b, e});
   while (!stack.empty()) {
     Node nd = stack.back();
     stack.pop_back();
     const int cnt = nd.e - nd.b;
-    if (cnt <= DS_BS) continue;
+    if (cnt <= leaf) continue;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int k = nd.b; k < nd.e; ++k) {
       const double* p = pts + (size_t)stride * perm[k];
@@ -75,36 +68,76 @@ void build_blocking(const double* pts, int stride, int n, const double scale[3],
       const double ext = (hi[a] - lo[a]) * scale[a];
       if (ext > best) { best = ext; axis = a; }
     }
-    const int nleaf = (cnt + DS_BS - 1) / DS_BS;
-    const int left = DS_BS * (nleaf / 2);
-    auto mid = perm.begin() + nd.b + left;
-    std::nth_element(perm.begin() + nd.b, mid, perm.begin() + nd.e,
+    const int nleaf = (cnt + leaf - 1) / leaf;
+    const int left = leaf * (nleaf / 2);
+    std::nth_element(perm + nd.b, perm + nd.b + left, perm + nd.e,
                      [&](int i, int j) {
                        const double a = pts[(size_t)stride * i + axis];
-                       const double b = pts[(size_t)stride * j + axis];
-                       return a < b || (a == b && i < j);
+                       const double c = pts[(size_t)stride * j + axis];
+                       return a < c || (a == c && i < j);
                      });
     stack.push_back({nd.b + left, nd.e});
     stack.push_back({nd.b, nd.b + left});
   }
+}
+
+void box_of(const double* pts, int stride, const int* perm, int b, int e,
+            double* lo, double* hi) {
+  const double inf = std::numeric_limits<double>::infinity();
+  for (int a = 0; a < 3; ++a) { lo[a] = inf; hi[a] = -inf; }
+  for (int k = b; k < e; ++k) {
+    const double* p = pts + (size_t)stride * perm[k];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], p[a]);
+      hi[a] = std::max(hi[a], p[a]);
+    }
+  }
+}
+
+}  // namespace
+
+CovarDev make_covar(const ds_covar* c) {
+  CovarDev cv;
+  cv.var_s = c->var_s;
+  cv.c_perp = c->c_perp;
+  cv.c_para = c->c_para;
+  cv.dx_inv = c->dx_inv;
+  cv.table_n = c->table_n;
+  cv.s_perp = std::sqrt(c->c_perp * c->dx_inv);
+  cv.s_para = std::sqrt(c->c_para * c->dx_inv);
+  return cv;
+}
+
+void build_blocking(const double* pts, int stride, int n, const double scale[3],
+                    Blocking& out) {
+  out.n = n;
+  out.nblk = (n + DS_BS - 1) / DS_BS;
+  out.ntile = out.nblk * DS_TPB;
+  out.npad = out.nblk * DS_BS;
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  kd_sort(pts, stride, scale, perm.data(), 0, n, DS_BS);
+  // split each block into tiles
+  parallel_for(out.nblk, [&](int B) {
+    const int b = B * DS_BS, e = std::min(n, b + DS_BS);
+    kd_sort(pts, stride, scale, perm.data(), b, e, DS_TS);
+  });
   out.perm.assign(out.npad, n > 0 ? perm[n - 1] : 0);
   std::copy(perm.begin(), perm.end(), out.perm.begin());
-  out.lo.assign(3 * (size_t)out.nblk, 0.0);
-  out.hi.assign(3 * (size_t)out.nblk, 0.0);
-  for (int B = 0; B < out.nblk; ++B) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    const int e = std::min(n, (B + 1) * DS_BS);
-    for (int k = B * DS_BS; k < e; ++k) {
-      const double* p = pts + (size_t)stride * out.perm[k];
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = std::min(lo[a], p[a]);
-        hi[a] = std::max(hi[a], p[a]);
-      }
-    }
-    for (int a = 0; a < 3; ++a) {
-      out.lo[3 * (size_t)B + a] = lo[a];
-      out.hi[3 * (size_t)B + a] = hi[a];
-    }
+  out.blo.assign(3 * (size_t)out.nblk, 0.0);
+  out.bhi.assign(3 * (size_t)out.nblk, 0.0);
+  out.tlo.assign(3 * (size_t)out.ntile, 0.0);
+  out.thi.assign(3 * (size_t)out.ntile, 0.0);
+  out.tcount.assign(out.ntile, 0);
+  for (int B = 0; B < out.nblk; ++B)
+    box_of(pts, stride, out.perm.data(), B * DS_BS,
+           std::min(n, (B + 1) * DS_BS), &out.blo[3 * (size_t)B],
+           &out.bhi[3 * (size_t)B]);
+  for (int T = 0; T < out.ntile; ++T) {
+    const int b = T * DS_TS, e = std::max(b, std::min(n, b + DS_TS));
+    out.tcount[T] = e - b;
+    box_of(pts, stride, out.perm.data(), b, e, &out.tlo[3 * (size_t)T],
+           &out.thi[3 * (size_t)T]);
   }
 }
 
@@ -124,17 +157,24 @@ double box_gap_xx(const double* alo, const double* ahi, const double* blo,
   return (ga_perp + ga_para) * cv.dx_inv;
 }
 
-static void lists_from(int ntgt, const double* tlo, const double* thi,
-                       const Blocking& src, const CovarDev& cv,
-                       std::vector<int>& ptr, std::vector<int>& idx) {
+void build_tile_lists(int ntgt, const double* tlo, const double* thi,
+                      const Blocking& src, const CovarDev& cv,
+                      std::vector<int>& ptr, std::vector<int>& idx) {
+  // The reference drops a pair when (int)xx > table_n - 2, i.e. when
+  // xx >= table_n - 1.  The box gap is a lower bound of every pair's
+  // coordinate differences; a relative margin of 1e-12 covers the few ulps
+  // by which the bound's own rounding may exceed a pair's.
   const double limit = (double)(cv.table_n - 1) * (1.0 + 1e-12);
   std::vector<std::vector<int>> lists(ntgt);
   parallel_for(ntgt, [&](int I) {
     std::vector<int>& L = lists[I];
-    for (int J = 0; J < src.nblk; ++J) {
-      const double xx = box_gap_xx(tlo + 3 * (size_t)I, thi + 3 * (size_t)I,
-                                   &src.lo[3 * (size_t)J],
-                                   &src.hi[3 * (size_t)J], cv);
+    const double* alo = tlo + 3 * (size_t)I;
+    const double* ahi = thi + 3 * (size_t)I;
+    if (!(alo[0] <= ahi[0])) return;  // empty target
+    for (int J = 0; J < src.ntile; ++J) {
+      if (src.tcount[J] == 0) continue;
+      const double xx = box_gap_xx(alo, ahi, &src.tlo[3 * (size_t)J],
+                                   &src.thi[3 * (size_t)J], cv);
       if (xx < limit) L.push_back(J);
     }
   });
@@ -143,18 +183,6 @@ static void lists_from(int ntgt, const double* tlo, const double* thi,
   idx.resize(ptr[ntgt]);
   for (int I = 0; I < ntgt; ++I)
     std::copy(lists[I].begin(), lists[I].end(), idx.begin() + ptr[I]);
-}
-
-void build_pair_lists(const Blocking& tgt, const Blocking& src,
-                      const CovarDev& cv, std::vector<int>& ptr,
-                      std::vector<int>& idx) {
-  lists_from(tgt.nblk, tgt.lo.data(), tgt.hi.data(), src, cv, ptr, idx);
-}
-
-void build_pair_lists_boxes(int ntgt, const double* tlo, const double* thi,
-                            const Blocking& src, const CovarDev& cv,
-                            std::vector<int>& ptr, std::vector<int>& idx) {
-  lists_from(ntgt, tlo, thi, src, cv, ptr, idx);
 }
 
 }  // namespace ds

@@ -1,0 +1,345 @@
+// pairs64.hip -- FP64 pixel-pair kernels: the parity path.
+//
+// COMPILE WITH -ffp-contract=off.  The table argument of every pair follows
+// the reference's operation order and rounding exactly
+// (lib/wf_vector.cc:62-66, lib/signal_covar.h:56-58,67-73):
+//     dx = xi - xj; dy = yi - yj; dz = zi - zj
+//     xx = (c_perp * (dx*dx + dy*dy) + c_para * (dz*dz)) * dx_inv
+// so the cutoff decision ((int)xx > table_n - 2 -> 0) and the table index are
+// the reference's.  The interpolation f[i](i+1-xx) + f[i+1](xx-i) is taken as
+// fma(f[i+1]-f[i], xx-i, f[i]) from an LDS table of {f[i], f[i+1]-f[i]} whose
+// entry table_n-1 is {0, 0}: clamping xx to table_n-1 before the index
+// conversion then yields the reference's exact 0 beyond the cutoff.  The
+// per-pair value differs from the reference's by a few ulps; sums run in a
+// different order.  Explicit fma() calls are fused regardless of the flag.
+//
+// Work decomposition: one workgroup per target block of 64 rows (lane = row),
+// its waves striding over the block's list of 16-slot source tiles.  A tile's
+// coordinates and vector entries are wave-uniform and are read with scalar
+// loads; the only per-lane memory traffic in the loop is the table gather.
+#include "cg_device.h"
+#include "ds_internal.h"
+
+namespace ds {
+
+namespace {
+
+enum Epi { EPI_NONE = EK_NONE, EPI_CG_DQ = EK_CG_DQ, EPI_CG_RES = EK_CG_RES };
+
+// Sum over the listed source tiles of g(x_i, x_j) * u_j, with g the table
+// interpolant (SQUARE: g^2 * u_j, for S W S).
+template <bool SQUARE>
+__device__ __forceinline__ double accumulate_tiles64(
+    const double xi, const double yi, const double zi,
+    const int* __restrict__ tidx, int k, const int ke, const int step,
+    const double4* __restrict__ pos, const double* __restrict__ u,
+    const double2* tab, const double cp, const double cl, const double dxi,
+    const double tlast) {
+  double acc = 0.0;
+  for (; k < ke; k += step) {
+    const int j0 = __builtin_amdgcn_readfirstlane(tidx[k]) * DS_TS;
+#pragma unroll
+    for (int jj = 0; jj < DS_TS; ++jj) {
+      const double4 pj = pos[j0 + jj];
+      const double uj = u[j0 + jj];
+      const double dx = xi - pj.x;
+      const double dy = yi - pj.y;
+      const double dz = zi - pj.z;
+      const double x_perp_2 = dx * dx + dy * dy;
+      const double x_para_2 = dz * dz;
+      double xx = (cp * x_perp_2 + cl * x_para_2) * dxi;
+      xx = fmin(xx, tlast);
+      const int ix = (int)xx;
+      const double t = xx - (double)ix;
+      const double2 f = tab[ix];
+      const double gp = fma(f.y, t, f.x);
+      acc = SQUARE ? fma(gp * gp, uj, acc) : fma(gp, uj, acc);
+    }
+  }
+  return acc;
+}
+
+struct MatvecArgs {
+  const double4* pos;     // sorted pixels {x, y, z, w or N}
+  const double* in;       // matvec input a (own rows, epilogue)
+  const double* u;        // scaled input: w .* a (W form) or a (SN form)
+  const double* rhs;      // wd or d (residual)
+  const double* diag;     // Jacobi diagonal (CG residual epilogue)
+  const int* tptr;
+  const int* tidx;
+  const double2* table;
+  double* out;            // A a, or rhs - A a
+  double* out2;           // CG residual: M^-1 r
+  double* partials;
+  ds_cg_scalars* sc;
+  double* norms;
+  double var_s, c_perp, c_para, dx_inv, tlast, drop;
+  int n, table_n;
+  int residual;           // out = rhs - A a
+  int init, mixed, is_true, need_want;
+};
+
+template <int FORM, int EPI>
+__global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
+  extern __shared__ double2 tab[];
+  __shared__ double red[4 * 64];
+  __shared__ double red2[2 * 4];
+  __shared__ bool is_last;
+  if (EPI != EPI_NONE) {
+    if (a.sc->done) return;
+    if (a.need_want && !a.sc->want_true) return;
+  }
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const int I = blockIdx.x;
+  const int row = I * DS_BS + lane;
+  const double4 pi = a.pos[row];
+  const int kb = a.tptr[I], ke = a.tptr[I + 1];
+  const double acc = accumulate_tiles64<false>(
+      pi.x, pi.y, pi.z, a.tidx, kb + wave, ke, nw, a.pos, a.u, tab, a.c_perp,
+      a.c_para, a.dx_inv, a.tlast);
+  red[wave * 64 + lane] = acc;
+  __syncthreads();
+  double v[2] = {0.0, 0.0};
+  if (wave == 0) {
+    double y = 0.0;
+    for (int w = 0; w < nw; ++w) y += red[w * 64 + lane];
+    const double ai = a.in[row];
+    // FORM W: (w S w + I) a;  FORM SN: (S + N) a  (lib/wf_vector.cc:67,62)
+    double bi = (FORM == DS_FORM_W) ? pi.w * (a.var_s * y) + ai
+                                    : a.var_s * y + pi.w * ai;
+    if (a.residual) bi = a.rhs[row] - bi;
+    if (row >= a.n) bi = 0.0;
+    a.out[row] = bi;
+    if (EPI == EPI_CG_DQ) {
+      v[0] = ai * bi;  // d . q
+    } else if (EPI == EPI_CG_RES) {
+      const double si = (row < a.n) ? bi / a.diag[row] : 0.0;
+      a.out2[row] = si;
+      v[0] = bi * si;  // r . M^-1 r
+      v[1] = bi * bi;  // r . r
+    }
+  }
+  if (EPI == EPI_NONE) return;
+  if (wave == 0) {
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+  }
+  if (EPI == EPI_CG_DQ) {
+    double p1[1] = {v[0]};
+    if (!publish_partials<1>(p1, a.partials, &a.sc->counter, &is_last))
+      return;
+    gather_partials<1>(p1, a.partials, red2);
+    if (threadIdx.x == 0) finalize_alpha(a.sc, p1[0]);
+  } else {
+    double p2[2] = {v[0], v[1]};
+    if (!publish_partials<2>(p2, a.partials, &a.sc->counter, &is_last))
+      return;
+    gather_partials<2>(p2, a.partials, red2);
+    if (threadIdx.x == 0)
+      finalize_step(a.sc, a.norms, p2[0], p2[1], a.init, a.mixed, a.drop,
+                    a.is_true);
+  }
+  release_ticket(&a.sc->counter);
+}
+
+// Map targets: the uniform grid of lib/util.cc:25-46, cut into 4x4x4 tiles;
+// lane -> cell (lane>>4, (lane>>2)&3, lane&3) of the tile.
+struct MapArgs {
+  const double4* pos;
+  const double* u;
+  const int* tptr;
+  const int* tidx;
+  const double2* table;
+  double* m;
+  double lx, ly, lz;
+  int nx, ny, nz, tx, ty, tz;
+  double var_s, c_perp, c_para, dx_inv, tlast;
+  int table_n;
+};
+
+template <bool SQUARE>
+__global__ __launch_bounds__(256) void k_map64(MapArgs a) {
+  extern __shared__ double2 tab[];
+  __shared__ double red[4 * 64];
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const int T = blockIdx.x;
+  const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
+  const int ix = tix * DS_MAP_T + (lane >> 4);
+  const int iy = tiy * DS_MAP_T + ((lane >> 2) & 3);
+  const int iz = tiz * DS_MAP_T + (lane & 3);
+  // lib/util.cc:30-39
+  const double dx = a.lx / a.nx, dy = a.ly / a.ny, dz = a.lz / a.nz;
+  const double xi = dx * (ix + 0.5);
+  const double yi = dy * (iy + 0.5);
+  const double zi = dz * (iz + 0.5);
+  const double acc = accumulate_tiles64<SQUARE>(
+      xi, yi, zi, a.tidx, a.tptr[T] + wave, a.tptr[T + 1], nw, a.pos, a.u,
+      tab, a.c_perp, a.c_para, a.dx_inv, a.tlast);
+  red[wave * 64 + lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  double y = 0.0;
+  for (int w = 0; w < nw; ++w) y += red[w * 64 + lane];
+  if (ix < a.nx && iy < a.ny && iz < a.nz) {
+    const size_t i = ((size_t)ix * a.ny + iy) * a.nz + iz;
+    a.m[i] = SQUARE ? (a.var_s * a.var_s) * y : a.var_s * y;
+  }
+}
+
+// Arbitrary target points, sorted into blocks of 64 by their own blocking.
+struct PointsArgs {
+  const double4* tgt;     // sorted targets (npad_t)
+  const int* tgt_perm;    // sorted slot -> output index
+  const double4* pos;
+  const double* u;
+  const int* tptr;
+  const int* tidx;
+  const double2* table;
+  double* m;
+  int n_tgt;
+  double var_s, c_perp, c_para, dx_inv, tlast;
+  int table_n;
+};
+
+template <bool SQUARE>
+__global__ __launch_bounds__(256) void k_points64(PointsArgs a) {
+  extern __shared__ double2 tab[];
+  __shared__ double red[4 * 64];
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const int I = blockIdx.x;
+  const int slot = I * DS_BS + lane;
+  const double4 ti = a.tgt[slot];
+  const double acc = accumulate_tiles64<SQUARE>(
+      ti.x, ti.y, ti.z, a.tidx, a.tptr[I] + wave, a.tptr[I + 1], nw, a.pos,
+      a.u, tab, a.c_perp, a.c_para, a.dx_inv, a.tlast);
+  red[wave * 64 + lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  double y = 0.0;
+  for (int w = 0; w < nw; ++w) y += red[w * 64 + lane];
+  if (slot < a.n_tgt)
+    a.m[a.tgt_perm[slot]] = SQUARE ? (a.var_s * a.var_s) * y : a.var_s * y;
+}
+
+}  // namespace
+
+// ---- launchers ------------------------------------------------------------
+
+hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
+                              const double* u, double* out,
+                              const CgEpilogue& e, hipStream_t st) {
+  MatvecArgs a{};
+  a.pos = p->pos64;
+  a.in = in;
+  a.u = u;
+  a.rhs = p->rhs64;
+  a.diag = p->diag;
+  a.tptr = p->tptr;
+  a.tidx = p->tidx;
+  a.table = p->table64;
+  a.out = out;
+  a.out2 = e.out2;
+  a.partials = p->partials;
+  a.sc = p->sc;
+  a.norms = p->norms;
+  a.var_s = p->cv.var_s;
+  a.c_perp = p->cv.c_perp;
+  a.c_para = p->cv.c_para;
+  a.dx_inv = p->cv.dx_inv;
+  a.tlast = (double)(p->cv.table_n - 1);
+  a.drop = e.drop;
+  a.n = p->n;
+  a.table_n = p->cv.table_n;
+  a.residual = residual;
+  a.init = e.init;
+  a.mixed = e.mixed;
+  a.is_true = e.is_true;
+  a.need_want = e.need_want;
+  const dim3 grid(p->nblk), block(64 * p->waves_per_block);
+  const size_t shm = sizeof(double2) * p->cv.table_n;
+  const bool w = p->form == DS_FORM_W;
+#define DS_L(F, E) \
+  hipLaunchKernelGGL((k_matvec64<F, E>), grid, block, shm, st, a)
+  if (e.kind == EPI_NONE) {
+    if (w) DS_L(DS_FORM_W, EPI_NONE); else DS_L(DS_FORM_SN, EPI_NONE);
+  } else if (e.kind == EPI_CG_DQ) {
+    if (w) DS_L(DS_FORM_W, EPI_CG_DQ); else DS_L(DS_FORM_SN, EPI_CG_DQ);
+  } else {
+    if (w) DS_L(DS_FORM_W, EPI_CG_RES); else DS_L(DS_FORM_SN, EPI_CG_RES);
+  }
+#undef DS_L
+  return hipGetLastError();
+}
+
+hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
+                        const double* u, const int* tptr, const int* tidx,
+                        const double2* table64, const CovarDev& cv,
+                        double* m, hipStream_t st) {
+  MapArgs a{};
+  a.pos = pos;
+  a.u = u;
+  a.tptr = tptr;
+  a.tidx = tidx;
+  a.table = table64;
+  a.m = m;
+  a.lx = g.lx; a.ly = g.ly; a.lz = g.lz;
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+  a.tx = g.tx; a.ty = g.ty; a.tz = g.tz;
+  a.var_s = cv.var_s;
+  a.c_perp = cv.c_perp;
+  a.c_para = cv.c_para;
+  a.dx_inv = cv.dx_inv;
+  a.tlast = (double)(cv.table_n - 1);
+  a.table_n = cv.table_n;
+  const dim3 grid((unsigned)((size_t)g.tx * g.ty * g.tz)), block(256);
+  const size_t shm = sizeof(double2) * cv.table_n;
+  if (mode == MM_SIGMA)
+    hipLaunchKernelGGL(k_map64<true>, grid, block, shm, st, a);
+  else
+    hipLaunchKernelGGL(k_map64<false>, grid, block, shm, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_points64(int mode, int n_tgt, const double4* tgt,
+                           const int* tgt_perm, const double4* pos,
+                           const double* u, const int* tptr, const int* tidx,
+                           int n_tblk, const double2* table64,
+                           const CovarDev& cv, double* m, hipStream_t st) {
+  PointsArgs a{};
+  a.tgt = tgt;
+  a.tgt_perm = tgt_perm;
+  a.pos = pos;
+  a.u = u;
+  a.tptr = tptr;
+  a.tidx = tidx;
+  a.table = table64;
+  a.m = m;
+  a.n_tgt = n_tgt;
+  a.var_s = cv.var_s;
+  a.c_perp = cv.c_perp;
+  a.c_para = cv.c_para;
+  a.dx_inv = cv.dx_inv;
+  a.tlast = (double)(cv.table_n - 1);
+  a.table_n = cv.table_n;
+  const dim3 grid(n_tblk), block(256);
+  const size_t shm = sizeof(double2) * cv.table_n;
+  if (mode == MM_SIGMA)
+    hipLaunchKernelGGL(k_points64<true>, grid, block, shm, st, a);
+  else
+    hipLaunchKernelGGL(k_points64<false>, grid, block, shm, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace ds

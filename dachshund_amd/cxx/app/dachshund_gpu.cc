@@ -1,0 +1,186 @@
+// dachshund_gpu.cc -- the reconstruction app of app/dachshund.cc on the
+// gfx950 library: same config keys and defaults (app/dachshund.cc:15-50,
+// 96-149), same flow and output files (map.bin, map_covar_nd.bin,
+// map_covar.bin; raw FP64).  Unknown keys and unreadable files exit(1)
+// like the reference.  Blank lines are skipped; in the reference a blank
+// line re-applies the previous key and the last line may be applied twice,
+// neither of which changes any setting.
+//
+// Usage: dachshund_gpu.exe <config> [--mixed]
+//   --mixed  solve with the FP32/FP64 mixed-precision solver (same FP64
+//            stopping rule on the true residual).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dachshund.h"
+
+namespace {
+
+struct Config {
+  double lx = 10.0, ly = 10.0, lz = 10.0;
+  int num_pixels = 100;
+  int map_nx = 10, map_ny = 10, map_nz = 10;
+  double corr_var_s = 1.0, corr_l_perp = 1.0, corr_l_para = 1.0;
+  bool solve_via_pcg = true;
+  int pcg_max_iter = 100;
+  double pcg_tol = 1.0;
+  int pcg_step_r = 50;
+  bool option_map_covar = false;
+  bool option_noise_covar = false;
+  std::string pixel_data_path = "pixel_data.bin";
+  std::string map_path = "map.bin";
+  int num_map_points() const { return map_nx * map_ny * map_nz; }
+};
+
+void parse(const std::string& path, Config& c) {
+  printf("Reading config file %s.\n", path.c_str());
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) {
+    fprintf(stderr, "Could not load file %s.\n", path.c_str());
+    exit(1);
+  }
+  using Set = std::function<void(const char*)>;
+  auto dbl = [](double& d) { return Set([&d](const char* v) { d = atof(v); }); };
+  auto integer = [](int& i) { return Set([&i](const char* v) { i = atoi(v); }); };
+  auto flag = [](bool& b) { return Set([&b](const char* v) { b = atoi(v); }); };
+  auto str = [](std::string& s) { return Set([&s](const char* v) { s = v; }); };
+  const std::map<std::string, Set> keys = {
+      {"lx", dbl(c.lx)},
+      {"ly", dbl(c.ly)},
+      {"lz", dbl(c.lz)},
+      {"num_pixels", integer(c.num_pixels)},
+      {"map_nx", integer(c.map_nx)},
+      {"map_ny", integer(c.map_ny)},
+      {"map_nz", integer(c.map_nz)},
+      {"corr_var_s", dbl(c.corr_var_s)},
+      {"corr_l_para", dbl(c.corr_l_para)},
+      {"corr_l_perp", dbl(c.corr_l_perp)},
+      {"solve_via_pcg", flag(c.solve_via_pcg)},
+      {"pcg_tol", dbl(c.pcg_tol)},
+      {"pcg_max_iter", integer(c.pcg_max_iter)},
+      {"pcg_step_r", integer(c.pcg_step_r)},
+      {"option_noise_covar", flag(c.option_noise_covar)},
+      {"option_map_covar", flag(c.option_map_covar)},
+      {"pixel_data_path", str(c.pixel_data_path)},
+      {"map_path", str(c.map_path)},
+  };
+  char line[1000], key[100], value[100];
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%99s = %99s", key, value) != 2) continue;
+    auto it = keys.find(key);
+    if (it == keys.end()) {
+      fprintf(stderr, "Unknown key '%s' in config.\n", key);
+      exit(1);
+    }
+    it->second(value);
+  }
+  fclose(f);
+}
+
+void print(const Config& c) {
+  printf("\nRunning dachshund (gfx950) with:\n");
+  printf("  pixels/map:\n    lx ly lz = %f %f %f\n", c.lx, c.ly, c.lz);
+  printf("    npix = %i\n", c.num_pixels);
+  printf("    map nx ny nz = %i %i %i, n = %i\n", c.map_nx, c.map_ny, c.map_nz,
+         c.num_map_points());
+  printf("  signal:\n    corr var_s = %f\n", c.corr_var_s);
+  printf("    l_perp = %f    l_para = %f\n", c.corr_l_perp, c.corr_l_para);
+  printf("  WF x solve: %i\n", c.solve_via_pcg);
+  printf("  PCG:\n    max iter = %i    step r = %i\n", c.pcg_max_iter,
+         c.pcg_step_r);
+  printf("    tol = %f\n", c.pcg_tol);
+  printf("  options:\n    map covar = %i\n", c.option_map_covar);
+  printf("  paths:\n    pixel data = %s\n", c.pixel_data_path.c_str());
+  printf("    map = %s\n\n", c.map_path.c_str());
+}
+
+void write_doubles(const std::string& path, const double* v, size_t n) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) {
+    fprintf(stderr, "Could not write %s.\n", path.c_str());
+    exit(1);
+  }
+  fwrite(v, sizeof(double), n, f);
+  fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  bool mixed = false;
+  std::vector<std::string> args;
+  for (int i = 1; i < argc; ++i) {
+    if (std::strcmp(argv[i], "--mixed") == 0) mixed = true;
+    else args.push_back(argv[i]);
+  }
+  if (args.size() != 1) {
+    fputs("Bad number of arguments. Expected 1.\n", stderr);
+    puts("Usage: dachshund_gpu.exe input [--mixed]");
+    puts("    input     path to config file.");
+    exit(1);
+  }
+  Config c;
+  parse(args[0], c);
+  print(c);
+  const int n = c.num_pixels, n_map = c.num_map_points();
+  std::vector<Pixel> pixels(n);
+  std::vector<Point> map_coords(n_map);
+  SignalCovarParams s_params(c.corr_var_s, c.corr_l_perp, c.corr_l_para);
+  PCGParams pcg_params = {n, c.pcg_max_iter, c.pcg_step_r, c.pcg_tol, true};
+
+  puts("Reading pixel data.");
+  read_pixel_data(c.pixel_data_path, n, pixels.data());
+  puts("Preparing for x solve.");
+  init_uniform_map_coords(c.lx, c.ly, c.lz, c.map_nx, c.map_ny, c.map_nz,
+                          map_coords.data());
+  WPixel* wpixels = (WPixel*)pixels.data();
+  pixel_to_wpixel(n, pixels.data(), wpixels);
+  std::vector<double> x(n, 0.0), m(n_map);
+  {
+    WFVectorW wf(n, wpixels, &s_params);
+    puts("Starting solve.");
+    if (c.solve_via_pcg) {
+      if (mixed) wf.solve_pcg_mixed(pcg_params, x.data());
+      else wf.solve_pcg(pcg_params, x.data());
+    } else {
+      wf.solve_cf(x.data(), true, true);
+    }
+    // S_mp x on the grid, sources = the pixels (app/dachshund.cc:253-257)
+    std::vector<Point> pc(n);
+    for (int i = 0; i < n; ++i) pc[i] = Point{wpixels[i].x, wpixels[i].y,
+                                              wpixels[i].z};
+    smp_product(n, pc.data(), n_map, map_coords.data(), &s_params, x.data(),
+                m.data());
+  }
+  printf("Writing map file %s.\n", c.map_path.c_str());
+  write_doubles(c.map_path, m.data(), m.size());
+
+  if (c.option_noise_covar) {
+    puts("Computing covariance in N >> S limit.");
+    NPixel* npixels = (NPixel*)pixels.data();
+    wpixel_to_pixel(n, wpixels, pixels.data());
+    pixel_to_npixel(n, pixels.data(), npixels);
+    sigma_m_noise_dom(n, npixels, n_map, map_coords.data(), &s_params,
+                      m.data());
+    write_doubles("map_covar_nd.bin", m.data(), m.size());
+  }
+  if (c.option_map_covar) {
+    puts("Computing covariance diag.");
+    NPixel* npixels = (NPixel*)pixels.data();
+    if (!c.option_noise_covar) {
+      wpixel_to_pixel(n, wpixels, pixels.data());
+      pixel_to_npixel(n, pixels.data(), npixels);
+    }
+    MapCovarDiag mcd(n, npixels, n_map, map_coords.data(), &s_params);
+    mcd.solve_pcg(pcg_params, m.data());
+    write_doubles("map_covar.bin", m.data(), m.size());
+  }
+  printf("Total time: %.2f s.\n", total_timer.elapsed());
+  return 0;
+}

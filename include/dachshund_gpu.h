@@ -2,17 +2,18 @@
  * dachshund_gpu.h -- C ABI of the gfx950 Wiener-filter library
  * (libdachshund_gpu.so).
  *
- * Plain pointers and sizes only.  Every function returns a ds_status (0 = OK);
- * on error ds_ctx_last_error() describes it and nothing is exit()ed.  A context
- * owns one HIP stream; contexts and problems are not internally locked (use
- * one per host thread, like the reference's single-threaded callers).
+ * Plain pointers and sizes only.  Every int-returning function returns a
+ * ds_status (0 = DS_OK); on error ds_ctx_last_error() describes it and
+ * nothing is exit()ed.  A context owns one HIP stream; contexts and problems
+ * are not internally locked (one per host thread, like the reference's
+ * single-threaded callers).  All host arrays are caller-owned.
  *
  * Pixel records are the reference's 5-double AoS structs:
  *   DS_FORM_W : WPixel {x, y, z, w = 1/n, wd = d/n}  (lib/pixel.h:39-45)
  *   DS_FORM_SN: NPixel {x, y, z, N = n^2, d}          (lib/pixel.h:27-33)
- * Points are {x, y, z} (lib/pixel.h:8-12).  Host vectors are in the caller's
- * pixel order; the library sorts pixels internally and undoes the permutation
- * on the way out.
+ * Points are {x, y, z} records (lib/pixel.h:8-12).  Host vectors are in the
+ * caller's pixel order; the library orders pixels spatially on the device
+ * and undoes the permutation on the way out.
  */
 #ifndef DACHSHUND_GPU_H
 #define DACHSHUND_GPU_H
@@ -33,14 +34,16 @@ typedef enum {
 } ds_status;
 
 enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
-/* Arithmetic of the pair evaluation and matvec.
- * DS_PREC_F64: the reference's FP64 table interpolation, per-pair values
- *              bit-identical to lib/signal_covar.h:53-77.
- * DS_PREC_F32: FP32 evaluation of the same interpolant on pre-scaled
- *              block-relative coordinates, FP32 accumulation. */
+
+/* Arithmetic of the pair evaluation.
+ * DS_PREC_F64: the reference's FP64 table interpolation; the table argument
+ *              and cutoff decision of every pair are bit-identical to
+ *              lib/signal_covar.h:53-77 on the same coordinates.
+ * DS_PREC_F32: the same interpolant evaluated in FP32 on pre-scaled,
+ *              block-relative coordinates, FP32 row sums, FP64 epilogue. */
 enum { DS_PREC_F64 = 0, DS_PREC_F32 = 1 };
 
-/* Covariance parameters exactly as SignalCovarParams holds them
+/* Covariance parameters as SignalCovarParams holds them
  * (lib/signal_covar.h:10-46): var_s, gaussian_const_perp/para (= 1/(2 l^2)),
  * func_table_dx_inv, func_table_n, func_table. */
 typedef struct {
@@ -65,48 +68,66 @@ typedef struct {
 typedef struct {
   int success;
   int num_iters;
-  int n_norms;           /* residual norms written (<= norms_cap) */
-  double norm_b;
-  double final_norm;     /* last |r| recorded */
-  double true_norm;      /* |b - A x| in FP64 at exit (mixed solver) */
+  int n_norms;           /* residual norms recorded */
+  int n_true;            /* mixed solver: FP64 true residuals taken */
   int n_matvec64;        /* FP64 pair-evaluation passes */
   int n_matvec32;        /* FP32 pair-evaluation passes */
+  double norm_b;
+  double final_norm;     /* last |r| recorded */
+  double true_norm;      /* mixed solver: last FP64 true residual |b - A x| */
   double seconds;        /* device time of the solve (HIP events) */
 } ds_pcg_result;
 
 /* Statistics of a problem's blocking and culling. */
 typedef struct {
   int n;
-  int n_blocks;
-  long long block_pairs;     /* (target, source) block pairs kept */
-  long long pairs_per_matvec;/* pair evaluations per matvec = block_pairs*64^2 */
-  double setup_seconds;      /* host time of sort + culling + upload */
+  int n_blocks;              /* target blocks of 64 rows */
+  int n_tiles;               /* source tiles of 16 slots */
+  long long tile_pairs;      /* (block, tile) pairs kept after the cutoff */
+  long long pairs_per_matvec;/* evaluated pairs per matvec = tile_pairs*1024 */
+  double setup_seconds;      /* host time of ordering, culling and upload */
 } ds_problem_stats;
+
+/* ---- host-only planning (no device needed) ---------------------------- */
+/* The spatial ordering and cutoff culling a problem of these points would
+ * use: perm_out[npad] (sorted slot -> point index, padding repeats the last
+ * point), tile_ptr[n_blocks+1] and up to idx_cap source-tile indices per
+ * the CSR list.  *n_blocks, *n_idx receive the sizes (call once with
+ * idx_cap = 0 to size the buffers; perm_out may then be NULL).  Points are
+ * records of `stride` doubles starting {x, y, z}.  No reference
+ * counterpart: it exposes the exact culling that replaces the reference's
+ * all-pairs loops (lib/wf_vector.cc:54-73) for inspection. */
+int ds_plan_tiles(int n, const double* pts, int stride, const ds_covar* covar,
+                  int* perm_out, int* tile_ptr, int* tile_idx, int idx_cap,
+                  int* n_blocks, int* n_idx);
 
 /* ---- context ---------------------------------------------------------- */
 int ds_ctx_create(int device, ds_ctx** out);
 int ds_ctx_destroy(ds_ctx* ctx);
 const char* ds_ctx_last_error(const ds_ctx* ctx);
 int ds_ctx_synchronize(ds_ctx* ctx);
-/* Number of compute units of the context's device. */
 int ds_ctx_num_cu(const ds_ctx* ctx);
 
-/* ---- pixel problem (replaces WFVectorSN / WFVectorW construction,
- *      lib/wf_vector.h:33-96; the pixels and covariance are copied to HBM,
- *      sorted into blocks and the cutoff-culled block-pair list is built) --- */
+/* ---- pixel problem: replaces the construction of WFVectorSN / WFVectorW
+ *      (lib/wf_vector.h:33-96).  Pixels and covariance are copied to HBM,
+ *      ordered into blocks, and the cutoff-culled tile lists are built. --- */
 int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
                       const ds_covar* covar, ds_problem** out);
 int ds_problem_destroy(ds_problem* p);
 int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out);
+/* Pair-kernel launch shape: waves (of 64 lanes) per target block, 1, 2 or 4.
+ * Changes only the order of floating-point sums. */
+int ds_problem_set_waves(ds_problem* p, int waves_per_block);
 
 /* WFVector*::A_product (lib/wf_vector.cc:51-74 SN, :196-220 W): b = A a. */
 int ds_A_product(ds_problem* p, int prec, const double* a, double* b);
 /* WFVector*::A_residual (lib/wf_vector.cc:76-103 SN, :222-251 W):
- * r = b - A x with b = d (SN) or wd (W). */
-int ds_A_residual(ds_problem* p, int prec, const double* x, double* r);
+ * r = b - A x with b = d (SN) or wd (W).  FP64 only. */
+int ds_A_residual(ds_problem* p, const double* x, double* r);
 /* WFVector*::preconditioner_solve (lib/wf_vector.cc:42-49, :185-194). */
 int ds_precond(ds_problem* p, const double* b, double* x);
-/* WFVector*::norm_b (lib/wf_vector.cc:105-112, :253-260). */
+/* WFVector*::norm_b (lib/wf_vector.cc:105-112, :253-260), summed in the
+ * reference's order on the host. */
 int ds_norm_b(ds_problem* p, double* out);
 
 /* WFVector*::solve_pcg (lib/wf_vector.cc:114-121, :263-274) over pcg_solve
@@ -117,51 +138,75 @@ int ds_norm_b(ds_problem* p, double* out);
 int ds_solve_pcg(ds_problem* p, const ds_pcg_params* params, double* x,
                  double* residual_norms, int norms_cap, ds_pcg_result* res);
 
-/* Mixed-precision solve of the same system: FP32 pair evaluation inside
- * PCG, FP64 vectors, dot products and true residuals; whenever the
- * recurrence residual has fallen by `refine_drop` since the last true
- * residual, or meets the tolerance, the FP64 true residual replaces it (an
- * outer refinement step).  Stops only when the FP64 true residual meets
- * |r| <= tol |b|.  No reference counterpart; same x in/out convention. */
+/* Mixed-precision solve of the same system (no reference counterpart):
+ * FP32 pair evaluation inside PCG with FP64 vectors and dot products; when
+ * the recurrence residual has fallen by `refine_drop` since the last FP64
+ * true residual, or meets the tolerance, an FP64 true residual replaces it.
+ * Stops only when the FP64 true residual meets |r| <= tol |b|.  Same x
+ * in/out convention as ds_solve_pcg. */
 int ds_solve_mixed(ds_problem* p, const ds_pcg_params* params,
                    double refine_drop, double* x, double* residual_norms,
                    int norms_cap, ds_pcg_result* res);
 
-/* Device-resident variants: the problem keeps its own solution vector in HBM
- * (in sorted order).  set/get move it from/to host order (x in W space for
- * the W form, like the solve_pcg input); the *_resident solves start from it
- * and leave the solution (x = w .* y for W) in it. */
+/* Device-resident solution vector (sorted order in HBM).  set/get move it
+ * from/to host order; the *_resident solves start from it (W space for the
+ * W form) and leave the solution (x = w .* y for W) in it. */
 int ds_problem_set_x(ds_problem* p, const double* x);
 int ds_problem_get_x(ds_problem* p, double* x);
 int ds_solve_pcg_resident(ds_problem* p, const ds_pcg_params* params,
                           ds_pcg_result* res);
 int ds_solve_mixed_resident(ds_problem* p, const ds_pcg_params* params,
                             double refine_drop, ds_pcg_result* res);
-/* Copy the residual history of the last resident solve. */
+/* Set every entry of the resident x to `value` on the device (no host
+ * transfer).  No reference counterpart (the reference's callers zero their
+ * host x, app/dachshund.cc:237-238). */
+int ds_problem_fill_x(ds_problem* p, double value);
+/* Time `reps` back-to-back launches of the pair matvec kernel (prec
+ * DS_PREC_F64 or DS_PREC_F32) with HIP events on the problem's stream; the
+ * resident x is the input.  *ms_per_launch = average launch duration.  No
+ * reference counterpart: measurement hook. */
+int ds_problem_time_matvec(ds_problem* p, int prec, int reps,
+                           double* ms_per_launch);
+/* Residual history of the last solve on this problem. */
 int ds_problem_get_norms(ds_problem* p, double* norms, int cap, int* count);
 
 /* smp_product (lib/map.cc:20-43) onto the uniform grid of
- * init_uniform_map_coords (lib/util.cc:25-46), using the problem's pixels
- * as sources and its resident solution as x:  m_i = sum_j S(map_i, pix_j) x_j.
- * m is a host array of nx*ny*nz doubles, index (ix*ny + iy)*nz + iz. */
-int ds_problem_smp_grid(ds_problem* p, int prec, double lx, double ly,
-                        double lz, int nx, int ny, int nz, double* m);
+ * init_uniform_map_coords (lib/util.cc:25-46), with the problem's pixels as
+ * sources and its resident solution as x:  m_i = sum_j S(map_i, pix_j) x_j.
+ * m: host array of nx*ny*nz doubles, index (ix*ny + iy)*nz + iz. */
+int ds_problem_smp_grid(ds_problem* p, double lx, double ly, double lz,
+                        int nx, int ny, int nz, double* m);
 
-/* smp_product (lib/map.cc:20-43) with arbitrary pixel and map points
- * (Point arrays), host in/out. */
+/* smp_product (lib/map.cc:20-43) with arbitrary pixel and map points. */
 int ds_smp_product(ds_ctx* ctx, int n_pix, const double* pix_pts, int n_map,
                    const double* map_pts, const ds_covar* covar,
                    const double* x, double* m);
 /* smp_product onto the uniform map grid (lib/util.cc:25-46). */
-int ds_smp_product_grid(ds_ctx* ctx, int prec, int n_pix,
-                        const double* pix_pts, double lx, double ly, double lz,
-                        int nx, int ny, int nz, const ds_covar* covar,
-                        const double* x, double* m);
+int ds_smp_product_grid(ds_ctx* ctx, int n_pix, const double* pix_pts,
+                        double lx, double ly, double lz, int nx, int ny,
+                        int nz, const ds_covar* covar, const double* x,
+                        double* m);
 /* sigma_m_noise_dom (lib/map_covar.cc:11-38): sigma_i = sum_j S_ij^2 / N_j,
  * NPixel input, arbitrary map points. */
 int ds_sigma_m_noise_dom(ds_ctx* ctx, int n_pix, const double* npix,
                          int n_map, const double* map_pts,
                          const ds_covar* covar, double* sigma);
+
+/* solve_cf (lib/wf_vector.cc:123-155 SN, :276-319 W): dense A built on the
+ * device (table interpolant, or the exact exponential when exact != 0, as
+ * WFVector*::A(i, j, exact)), Cholesky-factored on the device (the
+ * reference uses Eigen's LDLT), x = A^-1 b (W form: x = w .* y).  Needs
+ * 8 n^2 bytes of device memory; DS_E_STATE if A is not positive definite. */
+int ds_solve_cf(ds_problem* p, int exact, double* x);
+
+/* MapCovarDiag::solve_pcg (lib/map_covar.cc:73-113): for every map point,
+ * an SN-form PCG solve with the column S(map_i, .) as right-hand side from
+ * x = 0, then sigma_i = S(map_i, .) . x (0 when the solve did not succeed).
+ * NPixel input; num_iters / success (may be NULL) get n_map entries. */
+int ds_map_covar_diag(ds_ctx* ctx, int n_pix, const double* npix, int n_map,
+                      const double* map_pts, const ds_covar* covar,
+                      const ds_pcg_params* params, double* sigma,
+                      int* num_iters, int* success);
 
 #ifdef __cplusplus
 }

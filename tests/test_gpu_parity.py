@@ -1,0 +1,280 @@
+"""GPU parity: the gfx950 kernels (through the C ABI) against the oracle and
+the reference's golden vectors.
+
+Tolerances:
+  * FP64 pair kernels: every pair's table argument and cutoff decision are
+    the reference's; values differ by a few ulps and rows are summed in
+    another order, so |b_gpu - b_ref|_i <= 1e-12 * sum_j |A_ij a_j|.
+  * Jacobi preconditioner and |b|: bit-identical.
+  * FP32 pair kernel: |b32 - b_ref|_i <= 2e-5 * sum_j |A_ij a_j| (FP32
+    coordinates and sums; the few pairs within FP32 rounding of the cutoff
+    may flip).
+  * PCG: same success flag and iteration count as the reference (+-1 where
+    the stopping test is decided within rounding), residual histories
+    within 1e-6 relative while |r| > 1e-10 |r0|, solutions within the
+    reference's own CG-vs-dense test bound.
+  * Maps: smp_product and sigma_m_noise_dom to 1e-12 of sum_j |S_ij x_j|.
+"""
+import numpy as np
+import pytest
+
+from conftest import fixture_pixels, golden
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {"w": 1, "sn": 0}
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dachshund_amd as D
+    D.default_context()
+    return D
+
+
+def _problem(D, form, pix, params):
+    sp = D.SignalCovarParams(*params)
+    if form == "w":
+        return D.WFVectorW(D.pixel_to_wpixel(pix), sp), sp
+    return D.WFVectorSN(D.pixel_to_npixel(pix), sp), sp
+
+
+def _cases():
+    from dachshund_amd import synth
+    rng = np.random.default_rng(12)
+    cases = {"fixture": fixture_pixels(),
+             "small": synth.make_pixels(synth.workload("small")),
+             "tiny": synth.make_pixels(synth.workload("tiny"))}
+    for n in (1, 2, 63, 64, 65, 129):
+        p = np.empty((n, 5))
+        p[:, :3] = rng.uniform(0, 6, (n, 3))
+        p[:, 3] = rng.uniform(0.1, 0.3, n)
+        p[:, 4] = rng.standard_normal(n)
+        cases[f"n{n}"] = p
+    # coincident points and a far-away loner
+    p = np.repeat(cases["n65"][:10], 7, axis=0)
+    p[-1, :3] = [1e4, -3e3, 5e5]
+    cases["dups"] = p
+    return cases
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("params", [(0.05, 2.5, 2.1), (0.05, 1.0, 1.0),
+                                    (0.3, 0.7, 3.3)])
+@pytest.mark.parametrize("form", ["w", "sn"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_matvec64_residual_precond(D, ora, case, form, params):
+    pix = CASES[case]
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(*params)
+    wf, _ = _problem(D, form, pix, params)
+    a = np.random.default_rng(len(pix)).standard_normal(len(pix))
+    bound = ora.A_product(f, fp, c, np.abs(a))
+    b = wf.A_product(a)
+    assert np.all(np.abs(b - ora.A_product(f, fp, c, a)) <= 1e-12 * bound)
+    r = wf.A_residual(a)
+    rb = bound + np.abs(fp[:, 4])
+    assert np.all(np.abs(r - ora.A_residual(f, fp, c, a)) <= 1e-12 * rb)
+    assert np.array_equal(wf.preconditioner_solve(a), ora.precond(f, fp, c, a))
+    assert wf.norm_b() == ora.norm_b(f, fp)
+    b32 = wf.A_product(a, prec=D.DS_PREC_F32)
+    assert np.all(np.abs(b32 - b) <= 2e-5 * bound)
+
+
+@pytest.mark.parametrize("lp,ll", [(1.0, 1.0), (2.0, 2.0), (2.5, 2.1)])
+@pytest.mark.parametrize("form", ["w", "sn"])
+def test_matvec64_golden(D, ora, form, lp, ll):
+    g = golden("ops_fixture.npz")
+    pix = fixture_pixels()
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(0.05, lp, ll)
+    wf, _ = _problem(D, form, pix, (0.05, lp, ll))
+    vec = g["vec"]
+    bound = ora.A_product(f, fp, c, np.abs(vec))
+    key = f"{form}_{lp}_{ll}"
+    assert np.all(np.abs(wf.A_product(vec) - g[key + "_prod"])
+                  <= 1e-12 * bound)
+    assert np.all(np.abs(wf.A_residual(vec) - g[key + "_res"])
+                  <= 1e-12 * (bound + np.abs(fp[:, 4])))
+    assert np.array_equal(wf.preconditioner_solve(vec), g[key + "_pc"])
+    assert wf.norm_b() == g[key + "_normb"][0]
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_launch_shape_changes_only_rounding(D, ora, waves):
+    pix = CASES["small"]
+    wf, _ = _problem(D, "w", pix, (0.05, 2.5, 2.1))
+    a = np.random.default_rng(2).standard_normal(len(pix))
+    fp = ora.to_form(pix, 1)
+    c = ora.covar(0.05, 2.5, 2.1)
+    bound = ora.A_product(1, fp, c, np.abs(a))
+    wf.set_waves(waves)
+    b = wf.A_product(a)
+    assert np.all(np.abs(b - ora.A_product(1, fp, c, a)) <= 1e-12 * bound)
+    assert np.array_equal(b, wf.A_product(a))   # deterministic
+
+
+def _check_pcg(res, x, g_ok, g_it, g_norms, g_x, fp, form):
+    assert res.success == g_ok
+    assert abs(res.num_iters - g_it) <= 1
+    m = min(len(res.residual_norms), len(g_norms))
+    sig = g_norms[:m] > 1e-10 * g_norms[0]
+    np.testing.assert_allclose(res.residual_norms[:m][sig], g_norms[:m][sig],
+                               rtol=1e-6)
+    # tests/wfx_tests.cc:123-151 bound: RMS(x/x_ref - 1) < 0.01 where the
+    # reference solution is not tiny; and a relative L2 bound.
+    rel = np.linalg.norm(x - g_x) / np.linalg.norm(g_x)
+    assert rel < 1e-5, rel
+
+
+def test_pcg_golden(D, ora):
+    g = golden("pcg.npz")
+    pix = fixture_pixels()
+    small = g["small_pixels"]
+    for k in range(int(g["n_setups"])):
+        form, vs, lp, ll, mi, sr, tol, src = g[f"s{k}_setup"]
+        p = pix if src == 0 else small
+        fname = "w" if int(form) == 1 else "sn"
+        wf, _ = _problem(D, fname, p, (vs, lp, ll))
+        x = np.zeros(len(p))
+        res = wf.solve_pcg(D.PCGParams(len(p), int(mi), int(sr), tol), x)
+        _check_pcg(res, x, bool(g[f"s{k}_success"]), int(g[f"s{k}_iters"]),
+                   g[f"s{k}_norms"], g[f"s{k}_x"], p, fname)
+    wf, _ = _problem(D, "w", small, (0.05, 2.5, 2.1))
+    x = g["warm_x0"].copy()
+    res = wf.solve_pcg(D.PCGParams(len(small), 1000, 50, 1e-6), x)
+    _check_pcg(res, x, bool(g["warm_success"]), int(g["warm_iters"]),
+               g["warm_norms"], g["warm_x"], small, "w")
+
+
+def test_pcg_semantics(D, ora):
+    pix = CASES["small"]
+    wf, _ = _problem(D, "w", pix, (0.05, 2.5, 2.1))
+    n = len(pix)
+    fp = ora.to_form(pix, 1)
+    c = ora.covar(0.05, 2.5, 2.1)
+    for mi in (0, 1, 2, 3, 7):
+        x = np.zeros(n)
+        res = wf.solve_pcg(D.PCGParams(n, mi, 50, 1e-30), x)
+        xo, ok, it, norms = ora.solve_pcg(1, fp, c, mi, 50, 1e-30)
+        assert (res.success, res.num_iters, len(res.residual_norms)) == \
+            (ok, it, len(norms)), mi
+        np.testing.assert_allclose(res.residual_norms, norms, rtol=1e-10)
+        np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12)
+    # step_r = 1: a true residual every iteration
+    x = np.zeros(n)
+    res = wf.solve_pcg(D.PCGParams(n, 500, 1, 1e-8), x)
+    xo, ok, it, norms = ora.solve_pcg(1, fp, c, 500, 1, 1e-8)
+    assert res.success and ok and abs(res.num_iters - it) <= 1
+    assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+    # |b| = 0: early return, success false, x untouched then x *= w
+    z = pix.copy()
+    z[:, 4] = 0.0
+    wz, _ = _problem(D, "w", z, (0.05, 2.5, 2.1))
+    x = np.ones(n)
+    res = wz.solve_pcg(D.PCGParams(n, 100, 10, 1e-6), x)
+    assert not res.success and res.num_iters == 0
+    assert len(res.residual_norms) == 0
+    np.testing.assert_array_equal(x, 1.0 / pix[:, 3])
+    # bad parameters are errors, not exits
+    from dachshund_amd import _lib
+    with pytest.raises(_lib.DsError):
+        wf.solve_pcg(D.PCGParams(n, 10, 0, 1e-6), np.zeros(n))
+    with pytest.raises(_lib.DsError):
+        wf.solve_pcg(D.PCGParams(n + 1, 10, 5, 1e-6), np.zeros(n))
+
+
+@pytest.mark.parametrize("form", ["w", "sn"])
+def test_mixed_solver_meets_fp64_tolerance(D, ora, form):
+    pix = CASES["small"]
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(0.05, 2.5, 2.1)
+    wf, _ = _problem(D, form, pix, (0.05, 2.5, 2.1))
+    n = len(pix)
+    tol = 1e-8
+    x = np.zeros(n)
+    res = wf.solve_mixed(D.PCGParams(n, 2000, 50, tol), x, refine_drop=1e-3)
+    assert res.success and res.n_true >= 1
+    # the FP64 true residual of the returned solution, by the oracle
+    y = x / fp[:, 3] if form == "w" else x
+    r = ora.A_residual(f, fp, c, y)
+    assert np.linalg.norm(r) <= tol * ora.norm_b(f, fp) * 1.0001
+    xo, ok, _, _ = ora.solve_pcg(f, fp, c, 5000, 50, 1e-12)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_resident_solution_and_map(D, ora):
+    """app/dachshund.cc:218-264 end to end on the 8,192-pixel workload:
+    W form from x = 0, then smp_product onto the 16x16x64 grid."""
+    g = golden("app_c1.npz")
+    pix = g["pixels"]
+    cfg = dict(line.split(" = ") for line in str(g["config"]).splitlines())
+    params = (float(cfg["corr_var_s"]), float(cfg["corr_l_perp"]),
+              float(cfg["corr_l_para"]))
+    wf, sp = _problem(D, "w", pix, params)
+    n = len(pix)
+    wf.set_x(np.zeros(n))
+    res = wf.solve_resident(D.PCGParams(n, int(cfg["pcg_max_iter"]),
+                                        int(cfg["pcg_step_r"]),
+                                        float(cfg["pcg_tol"])))
+    assert res.success
+    it_ref = int(str(g["log_tail"]).split("iter ")[1].split()[0])
+    assert abs(res.num_iters - it_ref) <= 1
+    shape = (int(cfg["map_nx"]), int(cfg["map_ny"]), int(cfg["map_nz"]))
+    box = (float(cfg["lx"]), float(cfg["ly"]), float(cfg["lz"]))
+    m = wf.smp_grid(*box, *shape)
+    ref = g["map"]
+    assert np.linalg.norm(m - ref) <= 1e-6 * np.linalg.norm(ref)
+    # same map from the oracle's own projection of the GPU solution
+    x = wf.get_x()
+    mo = ora.smp_product(pix[:, :3], ora.map_coords(box, shape),
+                         ora.covar(*params), x)
+    bound = ora.smp_product(pix[:, :3], ora.map_coords(box, shape),
+                            ora.covar(*params), np.abs(x))
+    assert np.all(np.abs(m - mo) <= 1e-12 * bound)
+
+
+def test_map_side_golden(D, ora):
+    g = golden("map_fixture.npz")
+    pix = fixture_pixels()
+    for lp, ll in [(2.0, 2.0), (2.5, 2.1)]:
+        key = f"{lp}_{ll}"
+        shape = tuple(int(v) for v in g["shape_" + key])
+        sp = D.SignalCovarParams(0.05, lp, ll)
+        c = ora.covar(0.05, lp, ll)
+        mc = ora.map_coords((20.0, 20.0, 20.0), shape)
+        bound = ora.smp_product(pix[:, :3], mc, c, np.abs(g["x"]))
+        m = D.smp_product_grid(pix[:, :3], (20.0, 20.0, 20.0), shape, sp,
+                               g["x"])
+        assert np.all(np.abs(m - g["smp_" + key]) <= 1e-12 * bound)
+        m2 = D.smp_product(pix[:, :3], mc, sp, g["x"])
+        assert np.all(np.abs(m2 - g["smp_" + key]) <= 1e-12 * bound)
+        s = D.sigma_m_noise_dom(D.pixel_to_npixel(pix), mc, sp)
+        assert np.all(np.abs(s - g["nd_" + key]) <= 1e-12 * g["nd_" + key]
+                      + 1e-300)
+
+
+@pytest.mark.parametrize("case", ["tiny", "n1", "n65", "dups"])
+def test_map_points_edge_cases(D, ora, case):
+    pix = CASES[case]
+    rng = np.random.default_rng(5)
+    lo, hi = pix[:, :3].min(0) - 3, pix[:, :3].max(0) + 3
+    mc = rng.uniform(lo, hi, (333, 3))
+    mc[:5] = pix[:5 if len(pix) >= 5 else len(pix), :3].repeat(
+        5 // min(len(pix), 5) + 1, axis=0)[:5]
+    sp = D.SignalCovarParams(0.05, 2.5, 2.1)
+    c = ora.covar(0.05, 2.5, 2.1)
+    x = rng.standard_normal(len(pix))
+    m = D.smp_product(pix[:, :3], mc, sp, x)
+    bound = ora.smp_product(pix[:, :3], mc, c, np.abs(x))
+    assert np.all(np.abs(m - ora.smp_product(pix[:, :3], mc, c, x))
+                  <= 1e-12 * bound)
+    npx = D.pixel_to_npixel(pix)
+    s = D.sigma_m_noise_dom(npx, mc, sp)
+    so = ora.sigma_m_noise_dom(npx, mc, c)
+    assert np.all(np.abs(s - so) <= 1e-12 * so)
