@@ -7,12 +7,15 @@ x0 = 0 (or the learned warm start when a trained network is given).
 
 A step is one complete solve to |r| <= 1e-6 |b| with the inputs resident
 in HBM.  The headline solver is the mixed FP32/FP64 one (FP32 pair
-evaluation, FP64 vectors and true residuals); plain FP64 PCG (the
-reference's algorithm on the GPU) is timed on the same problem for the
-speed-up.  `value` is evaluated pixel-pair covariances per second over the
-timed solves.
+evaluation, FP64 vectors and true residuals) started from the learned
+warm start (neighbour search + network inference, inside the timed step);
+plain FP64 PCG from x0 = 0 (the reference's algorithm on the GPU) is timed
+on the same problem for the speed-up.  The benchmark realization (seed
+2000) is held out from the network's training seeds (1000-1199).  `value`
+is evaluated pixel-pair covariances per second over the timed solves.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                  [--warmstart dachshund_amd/warmstart_c5.npz | none]
 """
 from __future__ import annotations
 
@@ -93,6 +96,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--drop", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--warmstart", default=os.path.join(
+        ROOT, "dachshund_amd", "warmstart_c5.npz"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the "
@@ -118,18 +123,30 @@ def main():
         f"n^2)")
     prm = D.PCGParams(n, w.max_iter, w.step_r, args.tol)
 
-    def solve(mixed):
-        wf.fill_x(0.0)
+    ws = None
+    if args.warmstart != "none" and os.path.exists(args.warmstart):
+        from dachshund_amd.warmstart import WarmStart
+        ws = WarmStart()
+        ws.load(args.warmstart)
+        log(f"learned warm start: {args.warmstart}")
+    else:
+        log("no trained warm start: headline solver starts from x0 = 0")
+
+    def solve(mixed, learned=False):
+        if learned:
+            ws.predict_into(wf)
+        else:
+            wf.fill_x(0.0)
         if mixed:
             return wf.solve_resident(prm, mixed=True, refine_drop=args.drop)
         return wf.solve_resident(prm)
 
-    def timed(mixed, k):
+    def timed(mixed, k, learned=False):
         D.default_context()
         ts, res = [], None
         for _ in range(k):
             t = time.perf_counter()
-            res = solve(mixed)
+            res = solve(mixed, learned)
             ts.append(time.perf_counter() - t)
             if not res.success:
                 raise RuntimeError(f"solve (mixed={mixed}) did not converge")
@@ -138,15 +155,22 @@ def main():
     for _ in range(args.warmup):
         solve(False)
         solve(True)
+        if ws:
+            solve(True, True)
     t64, r64 = timed(False, args.steps)
     x64 = wf.get_x()
     log(f"plain FP64 PCG: {np.mean(t64) * 1e3:.1f} ms, {r64.num_iters} iters,"
         f" {r64.n_matvec64} FP64 matvecs")
-    tmx, rmx = timed(True, args.steps)
+    tm0, rm0 = timed(True, args.steps)
+    log(f"mixed, x0 = 0: {np.mean(tm0) * 1e3:.1f} ms, {rm0.num_iters} iters, "
+        f"{rm0.n_matvec32} FP32 + {rm0.n_matvec64} FP64 matvecs")
+    if ws:
+        tmx, rmx = timed(True, args.steps, learned=True)
+        log(f"mixed + learned: {np.mean(tmx) * 1e3:.1f} ms, {rmx.num_iters} "
+            f"iters, {rmx.n_matvec32} FP32 + {rmx.n_matvec64} FP64 matvecs")
+    else:
+        tmx, rmx = tm0, rm0
     xmx = wf.get_x()
-    log(f"mixed: {np.mean(tmx) * 1e3:.1f} ms, {rmx.num_iters} iters, "
-        f"{rmx.n_matvec32} FP32 + {rmx.n_matvec64} FP64 matvecs, true "
-        f"|r|/|b| {rmx.true_norm / r64.norm_b if False else 0:.1e}")
 
     # accuracy of the mixed map against a tight FP64 solve
     tight = D.PCGParams(n, 4 * w.max_iter, w.step_r, 1e-10)
@@ -159,6 +183,7 @@ def main():
     wf.set_x(x64)
     m_64 = wf.smp_grid(*box, *shape)
     rel_mx = float(np.linalg.norm(m_mx - m_ref) / np.linalg.norm(m_ref))
+    rel_hp = float(np.linalg.norm(m_mx - m_64) / np.linalg.norm(m_64))
     rel_64 = float(np.linalg.norm(m_64 - m_ref) / np.linalg.norm(m_ref))
     log(f"map rel-L2 vs FP64 tol 1e-10 solve ({rt.num_iters} iters): mixed "
         f"{rel_mx:.2e}, plain FP64 {rel_64:.2e}")
@@ -176,8 +201,8 @@ def main():
     t_64 = float(np.mean(t64))
     evals = pairs * (rmx.n_matvec32 + rmx.n_matvec64)
     out = {
-        "metric": "PCG time-to-rel-residual 1e-6 (mixed vs plain FP64, 1 "
-                  "GPU); pair-evals/s",
+        "metric": "PCG time-to-rel-residual 1e-6 (mixed+learned vs plain "
+                  "FP64, 1 GPU); pair-evals/s",
         "value": evals / t_mx,
         "unit": "pair-evals/s",
         "n_gpus": 1,
@@ -193,15 +218,22 @@ def main():
             "workload": (f"{n} pixels ({w.n_skewers} skewers x "
                          f"{w.n_per_skewer}), box {w.box}, GRF var_s "
                          f"{w.var_s} l_perp {w.l_perp} l_para {w.l_para}, "
-                         f"noise U[0.1,0.3], seed {w.seed}, W form, x0 = 0"),
+                         f"noise U[0.1,0.3], seed {w.seed} (held out), W "
+                         f"form, headline x0 = "
+                         f"{'learned' if ws else 'zero'}"),
             "tol": args.tol, "step_r": w.step_r,
             "refine_drop": args.drop,
         },
-        "time_to_tol_s": {"mixed": t_mx, "plain_fp64": t_64},
+        "time_to_tol_s": {"headline": t_mx, "mixed_x0_zero":
+                          float(np.mean(tm0)), "plain_fp64": t_64},
         "speedup_vs_plain_fp64": t_64 / t_mx,
-        "iterations": {"mixed": rmx.num_iters, "mixed_fp64_residuals":
-                       rmx.n_true, "plain_fp64": r64.num_iters},
-        "map_rel_l2_vs_fp64_tight": {"mixed": rel_mx, "plain_fp64": rel_64},
+        "iterations": {"headline": rmx.num_iters, "headline_fp64_residuals":
+                       rmx.n_true, "mixed_x0_zero": rm0.num_iters,
+                       "plain_fp64": r64.num_iters},
+        "learned_warm_start": bool(ws),
+        "map_rel_l2_vs_fp64_tight": {"headline": rel_mx,
+                                     "plain_fp64": rel_64},
+        "map_rel_l2_headline_vs_plain_fp64": rel_hp,
         "pairs_per_matvec": pairs,
         "dense_equiv_pair_evals_per_s": n * n * (rmx.n_matvec32 +
                                                  rmx.n_matvec64) / t_mx,

@@ -17,6 +17,7 @@ DS_PREC_F64, DS_PREC_F32 = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 
@@ -60,6 +61,15 @@ SIGNATURES = {
     "ds_problem_destroy": (C.c_int, [_vp]),
     "ds_problem_get_stats": (C.c_int, [_vp, C.POINTER(DsProblemStats)]),
     "ds_problem_set_waves": (C.c_int, [_vp, C.c_int]),
+    "ds_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp,
+                                  C.POINTER(DsCovar), C.POINTER(_vp)]),
+    "ds_batch_num_sub": (C.c_int, [_vp]),
+    "ds_batch_solve": (C.c_int, [_vp, C.POINTER(DsPcgParams), C.c_int,
+                                 C.c_double, C.c_int,
+                                 C.POINTER(DsPcgResult)]),
+    "ds_problem_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ds_problem_autotune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip,
+                                      _dp]),
     "ds_A_product": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "ds_A_residual": (C.c_int, [_vp, _dp, _dp]),
     "ds_precond": (C.c_int, [_vp, _dp, _dp]),
@@ -83,6 +93,21 @@ SIGNATURES = {
     "ds_map_covar_diag": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _dp,
                                     C.POINTER(DsCovar),
                                     C.POINTER(DsPcgParams), _dp, _ip, _ip]),
+    "ds_ws_num_params": (C.c_int, []),
+    "ds_ws_feature_dim": (C.c_int, []),
+    "ds_ws_create": (C.c_int, [_vp, C.c_ulonglong, C.POINTER(_vp)]),
+    "ds_ws_destroy": (C.c_int, [_vp]),
+    "ds_ws_set_params": (C.c_int, [_vp, _fp, C.c_int]),
+    "ds_ws_get_params": (C.c_int, [_vp, _fp, C.c_int]),
+    "ds_ws_set_quantized": (C.c_int, [_vp, C.c_int]),
+    "ds_ws_features": (C.c_int, [_vp, _vp, _fp]),
+    "ds_ws_predict": (C.c_int, [_vp, _vp]),
+    "ds_ws_add_sample": (C.c_int, [_vp, _vp, C.c_ulonglong]),
+    "ds_ws_num_samples": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    "ds_ws_clear_samples": (C.c_int, [_vp]),
+    "ds_ws_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, C.c_float,
+                              C.c_ulonglong, _dp]),
+    "ds_ws_eval_batch": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _fp, _dp]),
     "ds_problem_smp_grid": (C.c_int, [_vp, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_int,
                                       _dp]),

@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <thread>
 #include <vector>
 
 #include "ds_internal.h"
@@ -288,14 +290,20 @@ void problem_free(ds_problem* p) {
   dev_free(p->diag);
   dev_free(p->wvec);
   dev_free(p->loc32);
-  dev_free(p->blk_c);
+  dev_free(p->tgt32);
   dev_free(p->tptr);
   dev_free(p->tidx);
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
   dev_free(p->table_raw);
-  dev_free(p->perm);
+  dev_free(p->blk_sub);
+  dev_free(p->blk_nreal);
+  dev_free(p->blk_order);
+  dev_free(p->sub_blk0);
+  dev_free(p->sub_cv);
+  dev_free(p->tcount);
+  dev_free(p->done_count);
   if (p->xu != p->x) dev_free(p->xu);
   if (p->du != p->d) dev_free(p->du);
   p->xu = p->du = nullptr;
@@ -310,6 +318,8 @@ void problem_free(ds_problem* p) {
   dev_free(p->sc);
   dev_free(p->norms);
   if (p->sc_host) (void)hipHostFree(p->sc_host);
+  if (p->done_host) (void)hipHostFree(p->done_host);
+  p->done_host = nullptr;
   if (p->host_tmp) (void)hipHostFree(p->host_tmp);
   p->sc_host = nullptr;
   p->host_tmp = nullptr;
@@ -318,7 +328,7 @@ void problem_free(ds_problem* p) {
 // Host-order vector -> sorted device vector (pads 0).
 int upload_sorted(ds_problem* p, const double* v, double* dev) {
   for (int k = 0; k < p->npad; ++k)
-    p->host_tmp[k] = k < p->n ? v[p->blk.perm[k]] : 0.0;
+    p->host_tmp[k] = p->perm[k] >= 0 ? v[p->perm[k]] : 0.0;
   HIP_OR_FAIL(p->ctx, hipMemcpyAsync(dev, p->host_tmp,
                                      sizeof(double) * p->npad,
                                      hipMemcpyHostToDevice, p->ctx->stream));
@@ -331,7 +341,8 @@ int download_sorted(ds_problem* p, const double* dev, double* v) {
                                      sizeof(double) * p->npad,
                                      hipMemcpyDeviceToHost, p->ctx->stream));
   HIP_OR_FAIL(p->ctx, hipStreamSynchronize(p->ctx->stream));
-  for (int k = 0; k < p->n; ++k) v[p->blk.perm[k]] = p->host_tmp[k];
+  for (int k = 0; k < p->npad; ++k)
+    if (p->perm[k] >= 0) v[p->perm[k]] = p->host_tmp[k];
   return DS_OK;
 }
 
@@ -416,14 +427,34 @@ int ds_ctx_synchronize(ds_ctx* ctx) {
 
 int ds_ctx_num_cu(const ds_ctx* ctx) { return ctx ? ctx->num_cu : 0; }
 
-int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
-                      const ds_covar* covar, ds_problem** out) {
-  if (!ctx || !out) return DS_E_ARG;
+}  // extern "C"
+
+namespace {
+
+// Build a problem of n_sub independent subvolumes (sub_n[k] pixels each,
+// records consecutive in `pixels`, covariance covars[k]).  Each subvolume is
+// ordered and culled on its own; its blocks are concatenated after the
+// previous subvolume's.
+int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
+                   const double* pixels, const ds_covar* covars,
+                   ds_problem** out) {
+  if (!ctx || !out || n_sub < 1 || !sub_n || !covars) return DS_E_ARG;
   *out = nullptr;
   if (form != DS_FORM_W && form != DS_FORM_SN)
     return fail(ctx, DS_E_ARG, "form must be DS_FORM_W or DS_FORM_SN");
-  if (n <= 0) return fail(ctx, DS_E_ARG, "n must be positive");
-  if (int rc = check_covar(ctx, covar)) return rc;
+  long long ntot = 0;
+  for (int k = 0; k < n_sub; ++k) {
+    if (sub_n[k] <= 0) return fail(ctx, DS_E_ARG, "subvolume %d is empty", k);
+    if (int rc = check_covar(ctx, &covars[k])) return rc;
+    const ds_covar& c0 = covars[0];
+    if (covars[k].table_n != c0.table_n || covars[k].dx_inv != c0.dx_inv ||
+        !std::equal(c0.table, c0.table + c0.table_n, covars[k].table))
+      return fail(ctx, DS_E_ARG, "subvolume %d: every subvolume must use the "
+                  "same covariance table", k);
+    ntot += sub_n[k];
+  }
+  if (ntot > 500000000LL) return fail(ctx, DS_E_ARG, "too many pixels");
+  const int n = (int)ntot;
   if (int rc = check_points(ctx, pixels, 5, n, "pixels")) return rc;
   for (int i = 0; i < n; ++i)
     if (!std::isfinite(pixels[5 * (size_t)i + 3]) ||
@@ -437,76 +468,171 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
   p->ctx = ctx;
   p->form = form;
   p->n = n;
-  p->cv = ds::make_covar(covar);
-  p->table.assign(covar->table, covar->table + covar->table_n);
-  double sc[3];
-  axis_scale(p->cv, sc);
-  ds::build_blocking(pixels, 5, n, sc, p->blk);
-  const Blocking& b = p->blk;
-  p->nblk = b.nblk;
-  p->ntile = b.ntile;
-  p->npad = b.npad;
-  std::vector<int> ptr, idx;
-  ds::build_tile_lists(b.nblk, b.blo.data(), b.bhi.data(), b, p->cv, ptr, idx);
-  p->ntile_pairs = (long long)idx.size();
-  // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
-  double sum_b2 = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double bi = pixels[5 * (size_t)i + 4];
-    sum_b2 += bi * bi;
+  p->n_sub = n_sub;
+  p->cv = ds::make_covar(&covars[0]);
+  p->table.assign(covars[0].table, covars[0].table + covars[0].table_n);
+  p->sub_n.assign(sub_n, sub_n + n_sub);
+  p->sub_off.assign(n_sub + 1, 0);
+  for (int k = 0; k < n_sub; ++k) p->sub_off[k + 1] = p->sub_off[k] + sub_n[k];
+  p->sub_cv_h.resize(n_sub);
+  p->sub_norm_b.assign(n_sub, 0.0);
+  std::vector<Blocking> blks(n_sub);
+  std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
+  auto build = [&](int k) {
+    const ds::CovarDev cv = ds::make_covar(&covars[k]);
+    p->sub_cv_h[k] = cv;
+    double sc[3];
+    axis_scale(cv, sc);
+    const double* pk = pixels + 5 * (size_t)p->sub_off[k];
+    ds::build_blocking(pk, 5, sub_n[k], sc, blks[k]);
+    ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
+                         blks[k], cv, ptrs[k], idxs[k]);
+    // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
+    double sum_b2 = 0.0;
+    for (int i = 0; i < sub_n[k]; ++i) {
+      const double bi = pk[5 * (size_t)i + 4];
+      sum_b2 += bi * bi;
+    }
+    p->sub_norm_b[k] = std::sqrt(sum_b2);
+  };
+  if (n_sub == 1) {
+    build(0);
+  } else {
+    // subvolumes in parallel (each ordering is itself serial)
+    std::vector<std::thread> th;
+    const int nt = std::max(1, std::min(n_sub, 16));
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t]() {
+        for (int k = t; k < n_sub; k += nt) build(k);
+      });
+    for (auto& x : th) x.join();
   }
-  p->norm_b = std::sqrt(sum_b2);
+  p->norm_b = p->sub_norm_b[0];
+  p->sub_blk0_h.assign(n_sub + 1, 0);
+  for (int k = 0; k < n_sub; ++k)
+    p->sub_blk0_h[k + 1] = p->sub_blk0_h[k] + blks[k].nblk;
+  p->nblk = p->sub_blk0_h[n_sub];
+  p->ntile = p->nblk * DS_TPB;
+  p->npad = p->nblk * DS_BS;
+  if (n_sub == 1) p->blk = blks[0];
+  const int nblk = p->nblk, npad = p->npad;
+  // concatenated tile lists
+  std::vector<int> ptr(nblk + 1, 0), idx;
+  for (int k = 0; k < n_sub; ++k) {
+    const int B0 = p->sub_blk0_h[k];
+    for (int I = 0; I < blks[k].nblk; ++I)
+      ptr[B0 + I + 1] = ptr[B0 + I] + (ptrs[k][I + 1] - ptrs[k][I]);
+  }
+  idx.resize(ptr[nblk]);
+  for (int k = 0; k < n_sub; ++k) {
+    const int B0 = p->sub_blk0_h[k];
+    for (int I = 0; I < blks[k].nblk; ++I)
+      for (int e = ptrs[k][I]; e < ptrs[k][I + 1]; ++e)
+        idx[ptr[B0 + I] + (e - ptrs[k][I])] = idxs[k][e] + B0 * DS_TPB;
+  }
+  p->ntile_pairs = (long long)idx.size();
+  p->blk_len_h.resize(nblk);
+  for (int I = 0; I < nblk; ++I) p->blk_len_h[I] = ptr[I + 1] - ptr[I];
 
-  const int npad = p->npad;
-  const double var_s = p->cv.var_s;
+  // per-slot, per-block and per-tile host arrays
   std::vector<double4> hpos(npad);
   std::vector<double> hrhs(npad, 0.0), hdiag(npad, 1.0), hw(npad, 0.0);
-  std::vector<float4> hloc(npad);
-  std::vector<double> cen(3 * (size_t)b.nblk);
-  // FP32 frame of each block: the per-axis median of its real pixels, so an
-  // outlier in a block cannot move the frame away from the others.
-  for (int B = 0; B < b.nblk; ++B) {
-    const int k0 = B * DS_BS, k1 = std::min(n, k0 + DS_BS);
-    std::vector<double> v(k1 - k0);
-    for (int a = 0; a < 3; ++a) {
-      for (int k = k0; k < k1; ++k) v[k - k0] = pixels[5 * (size_t)b.perm[k] + a];
-      std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
-      cen[3 * B + a] = v[v.size() / 2];
+  std::vector<float4> hloc(npad), htgt(npad);
+  std::vector<int> hblk_sub(nblk), hnreal(nblk), htcount(p->ntile);
+  std::vector<double4> hsubcv(n_sub);
+  p->perm.assign(npad, -1);
+  // FP32 frames: every target block and every source tile gets the
+  // per-axis median of its real pixels as origin (an outlier cannot move
+  // it).  Sources are stored relative to their tile's frame, targets
+  // relative to their block's frame, and each list entry carries the
+  // difference of the two frames; for any pair within the cutoff both
+  // points are then near the source frame, which keeps the FP32 expansion
+  // |Xi|^2 + |Xj|^2 - 2 Xi.Xj well conditioned.
+  std::vector<double> cen(3 * (size_t)nblk), tcen(3 * (size_t)p->ntile);
+  double max_src = 0.0;
+  for (int k = 0; k < n_sub; ++k) {
+    const Blocking& b = blks[k];
+    const ds::CovarDev& cv = p->sub_cv_h[k];
+    const int B0 = p->sub_blk0_h[k];
+    const int off = p->sub_off[k];
+    const double* pk = pixels + 5 * (size_t)off;
+    const double var_s = cv.var_s;
+    hsubcv[k] = make_double4(cv.var_s, cv.c_perp, cv.c_para, 0.0);
+    auto median_frame = [&](int k0, int k1, double* c) {
+      std::vector<double> v(k1 - k0);
+      for (int a = 0; a < 3; ++a) {
+        for (int q = k0; q < k1; ++q) v[q - k0] = pk[5 * (size_t)b.perm[q] + a];
+        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+        c[a] = v[v.size() / 2];
+      }
+    };
+    for (int B = 0; B < b.nblk; ++B) {
+      hblk_sub[B0 + B] = k;
+      hnreal[B0 + B] = std::min(DS_BS, b.n - B * DS_BS);
+      median_frame(B * DS_BS, std::min(b.n, (B + 1) * DS_BS),
+                   &cen[3 * (size_t)(B0 + B)]);
+    }
+    for (int T = 0; T < b.ntile; ++T) {
+      const int TG = B0 * DS_TPB + T;
+      htcount[TG] = b.tcount[T];
+      if (b.tcount[T] > 0)
+        median_frame(T * DS_TS, T * DS_TS + b.tcount[T], &tcen[3 * (size_t)TG]);
+      else
+        for (int a = 0; a < 3; ++a)
+          tcen[3 * (size_t)TG + a] = cen[3 * (size_t)(B0 + T / DS_TPB) + a];
+    }
+    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
+    for (int q = 0; q < b.npad; ++q) {
+      const size_t g = (size_t)B0 * DS_BS + q;
+      const int j = b.perm[q];
+      const double* Pj = pk + 5 * (size_t)j;
+      const bool real = q < b.n;
+      if (real) p->perm[g] = off + j;
+      const double wN = real ? Pj[3] : (form == DS_FORM_W ? 0.0 : 1.0);
+      hpos[g] = make_double4(Pj[0], Pj[1], Pj[2], wN);
+      if (real) {
+        hrhs[g] = Pj[4];
+        // lib/wf_vector.cc:190-192 (W) and :44-47 (SN)
+        hdiag[g] = form == DS_FORM_W ? wN * wN * var_s + 1.0 : var_s + wN;
+        hw[g] = form == DS_FORM_W ? wN : 1.0;
+      }
+      const size_t B = (size_t)B0 + q / DS_BS, T = (size_t)B0 * DS_TPB + q / DS_TS;
+      float l[3], t[3];
+      for (int a = 0; a < 3; ++a) {
+        l[a] = (float)((Pj[a] - tcen[3 * T + a]) * s3[a]);
+        t[a] = (float)((Pj[a] - cen[3 * B + a]) * s3[a]);
+        if (real) max_src = std::max(max_src, std::fabs((double)l[a]));
+      }
+      hloc[g] = make_float4(l[0], l[1], l[2],
+                            l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+      htgt[g] = make_float4(t[0], t[1], t[2], 0.0f);
     }
   }
-  const double s3[3] = {p->cv.s_perp, p->cv.s_perp, p->cv.s_para};
-  for (int k = 0; k < npad; ++k) {
-    const int j = b.perm[k];
-    const double* Pj = pixels + 5 * (size_t)j;
-    const bool real = k < n;
-    const double wN = real ? Pj[3] : (form == DS_FORM_W ? 0.0 : 1.0);
-    hpos[k] = make_double4(Pj[0], Pj[1], Pj[2], wN);
-    if (real) {
-      hrhs[k] = Pj[4];
-      // lib/wf_vector.cc:190-192 (W) and :44-47 (SN)
-      hdiag[k] = form == DS_FORM_W ? wN * wN * var_s + 1.0 : var_s + wN;
-      hw[k] = form == DS_FORM_W ? wN : 1.0;
-    }
-    const int B = k / DS_BS;
-    float l[3];
-    for (int a = 0; a < 3; ++a)
-      l[a] = (float)((Pj[a] - cen[3 * B + a]) * s3[a]);
-    hloc[k] = make_float4(l[0], l[1], l[2], 0.0f);
-  }
-  // per list entry: centre of the target block minus centre of the source
-  // tile's block, in pre-scaled units
+  // A tile holding a point far from its median (isolated outliers) would
+  // make the expansion lose its accuracy: use direct differences then.
+  p->f32_direct = max_src > 16.0;
+  // per list entry: target block frame minus source tile frame (pre-scaled)
   std::vector<float4> hoff(idx.size());
-  for (int I = 0; I < b.nblk; ++I)
+  for (int I = 0; I < nblk; ++I) {
+    const ds::CovarDev& cv = p->sub_cv_h[hblk_sub[I]];
+    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
     for (int e = ptr[I]; e < ptr[I + 1]; ++e) {
-      const int B = idx[e] / DS_TPB;
+      const size_t T = (size_t)idx[e];
       float o[3];
       for (int a = 0; a < 3; ++a)
-        o[a] = (float)((cen[3 * I + a] - cen[3 * B + a]) * s3[a]);
+        o[a] = (float)((cen[3 * (size_t)I + a] - tcen[3 * T + a]) * s3[a]);
       hoff[e] = make_float4(o[0], o[1], o[2], 0.0f);
     }
+  }
+  // launch order: longest tile list first (for the autotuner)
+  std::vector<int> order(nblk);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return p->blk_len_h[a] > p->blk_len_h[b];
+  });
   std::vector<double2> t64;
   std::vector<float2> t32;
-  make_tables(covar, t64, t32);
+  make_tables(&covars[0], t64, t32);
 
   hipStream_t st = ctx->stream;
   HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
@@ -514,12 +640,20 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
   HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->wvec, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->loc32, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tgt32, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tptr, ptr.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tidx, idx.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table64, t64.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table32, t32.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table_raw, p->table.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_order, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->done_count, 1));
   double** vecs[] = {&p->x, &p->r, &p->d, &p->s, &p->q, &p->tmp};
   for (double** v : vecs) HIP_OR_FAIL(ctx, dev_alloc(v, npad));
   if (form == DS_FORM_W) {
@@ -530,14 +664,16 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
     p->du = p->d;
   }
   HIP_OR_FAIL(ctx, dev_alloc(&p->du32, npad));
-  const size_t nparts = 2 * (size_t)std::max(p->nblk, npad / 256 + 1);
-  HIP_OR_FAIL(ctx, dev_alloc(&p->partials, nparts));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sc, 1));
-  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->sc_host, 3 * sizeof(ds_cg_scalars),
+  HIP_OR_FAIL(ctx, dev_alloc(&p->partials, 2 * (size_t)nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sc, n_sub));
+  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->sc_host,
+                                 sizeof(ds_cg_scalars) * n_sub,
+                                 hipHostMallocDefault));
+  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->done_host, 2 * sizeof(int),
                                  hipHostMallocDefault));
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->host_tmp, sizeof(double) * npad,
                                  hipHostMallocDefault));
-  std::memset(p->sc_host, 0, 3 * sizeof(ds_cg_scalars));
+  std::memset(p->sc_host, 0, sizeof(ds_cg_scalars) * n_sub);
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
@@ -547,6 +683,7 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
   HIP_OR_FAIL(ctx, up(p->diag, hdiag.data(), sizeof(double) * npad));
   HIP_OR_FAIL(ctx, up(p->wvec, hw.data(), sizeof(double) * npad));
   HIP_OR_FAIL(ctx, up(p->loc32, hloc.data(), sizeof(float4) * npad));
+  HIP_OR_FAIL(ctx, up(p->tgt32, htgt.data(), sizeof(float4) * npad));
   HIP_OR_FAIL(ctx, up(p->tptr, ptr.data(), sizeof(int) * ptr.size()));
   HIP_OR_FAIL(ctx, up(p->tidx, idx.data(), sizeof(int) * idx.size()));
   HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
@@ -554,13 +691,39 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
   HIP_OR_FAIL(ctx, up(p->table_raw, p->table.data(),
                       sizeof(double) * p->table.size()));
+  HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->blk_order, order.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
+                      sizeof(int) * (n_sub + 1)));
+  HIP_OR_FAIL(ctx, up(p->sub_cv, hsubcv.data(), sizeof(double4) * n_sub));
+  HIP_OR_FAIL(ctx, up(p->tcount, htcount.data(), sizeof(int) * p->ntile));
   std::vector<double> zero(npad, 0.0);
   HIP_OR_FAIL(ctx, up(p->x, zero.data(), sizeof(double) * npad));
-  HIP_OR_FAIL(ctx, up(p->sc, p->sc_host, sizeof(ds_cg_scalars)));
+  HIP_OR_FAIL(ctx, up(p->sc, p->sc_host, sizeof(ds_cg_scalars) * n_sub));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
   p->setup_seconds = now_s() - t0;
   *out = P.release();
   return DS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
+                      const ds_covar* covar, ds_problem** out) {
+  if (!ctx || !out) return DS_E_ARG;
+  if (n <= 0) return fail(ctx, DS_E_ARG, "n must be positive");
+  if (int rc = check_covar(ctx, covar)) return rc;
+  return create_problem(ctx, form, 1, &n, pixels, covar, out);
+}
+
+int ds_batch_create(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
+                    const double* pixels, const ds_covar* covars,
+                    ds_problem** out) {
+  if (!ctx || !out) return DS_E_ARG;
+  return create_problem(ctx, form, n_sub, sub_n, pixels, covars, out);
 }
 
 int ds_problem_destroy(ds_problem* p) {
@@ -651,7 +814,9 @@ int ds_solve_pcg_resident(ds_problem* p, const ds_pcg_params* prm,
                           ds_pcg_result* res) {
   if (!p || !res) return DS_E_ARG;
   if (int rc = check_pcg_params(p, prm)) return rc;
-  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 0, 0.0, res));
+  if (p->n_sub != 1)
+    return fail(p->ctx, DS_E_STATE, "batch problem: use ds_batch_solve");
+  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 0, 0.0, res, 0));
   return DS_OK;
 }
 
@@ -661,7 +826,9 @@ int ds_solve_mixed_resident(ds_problem* p, const ds_pcg_params* prm,
   if (int rc = check_pcg_params(p, prm)) return rc;
   if (!(drop > 0.0 && drop < 1.0))
     return fail(p->ctx, DS_E_ARG, "refine_drop must lie in (0, 1)");
-  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 1, drop, res));
+  if (p->n_sub != 1)
+    return fail(p->ctx, DS_E_STATE, "batch problem: use ds_batch_solve");
+  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 1, drop, res, 1));
   return DS_OK;
 }
 
@@ -704,6 +871,8 @@ int ds_problem_smp_grid(ds_problem* p, double lx, double ly, double lz,
   if (!p || !m) return DS_E_ARG;
   ds::MapGrid g;
   if (int rc = check_grid(p->ctx, lx, ly, lz, nx, ny, nz, g)) return rc;
+  if (p->n_sub != 1)
+    return fail(p->ctx, DS_E_STATE, "map projection needs a single volume");
   return map_grid_run(p->ctx, ds::MM_SMP, g, p->blk, p->pos64, p->x,
                       p->table64, p->cv, m);
 }
@@ -777,6 +946,8 @@ int ds_sigma_m_noise_dom(ds_ctx* ctx, int n_pix, const double* npix,
 
 int ds_solve_cf(ds_problem* p, int exact, double* x) {
   if (!p || !x) return DS_E_ARG;
+  if (p->n_sub != 1)
+    return fail(p->ctx, DS_E_STATE, "dense solve needs a single volume");
   if ((double)p->n * p->n * 8.0 > 1.5e11)
     return fail(p->ctx, DS_E_ARG, "dense solve of %d pixels needs %.0f GB",
                 p->n, (double)p->n * p->n * 8e-9);
@@ -829,10 +1000,11 @@ int ds_map_covar_diag(ds_ctx* ctx, int n_pix, const double* npix, int n_map,
       sum_b2 += bj * bj;
     }
     p->norm_b = std::sqrt(sum_b2);
+    p->sub_norm_b[0] = p->norm_b;
     p->rhs64 = col.p;
     HIP_OR_FAIL(ctx, ds::launch_fill(p->x, 0.0, p->npad, st));
     ds_pcg_result res;
-    const hipError_t e = ds::run_solver(p, prm, 0, 0.0, &res);
+    const hipError_t e = ds::run_solver(p, prm, 0, 0.0, &res, 0);
     p->rhs64 = rhs_saved;
     HIP_OR_FAIL(ctx, e);
     double s = 0.0;
@@ -853,10 +1025,60 @@ int ds_map_covar_diag(ds_ctx* ctx, int n_pix, const double* npix, int n_map,
 int ds_problem_fill_x(ds_problem* p, double value) {
   if (!p) return DS_E_ARG;
   hipStream_t st = p->ctx->stream;
-  HIP_OR_FAIL(p->ctx, ds::launch_fill(p->x, value, p->n, st));
-  if (p->npad > p->n)
-    HIP_OR_FAIL(p->ctx, ds::launch_fill(p->x + p->n, 0.0, p->npad - p->n, st));
-  HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+  if (value == 0.0) {
+    HIP_OR_FAIL(p->ctx, ds::launch_fill(p->x, 0.0, p->npad, st));
+    HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+    return DS_OK;
+  }
+  // padding slots must stay 0
+  std::vector<double> v(p->n, value);
+  return upload_sorted(p, v.data(), p->x);
+}
+
+int ds_batch_num_sub(const ds_problem* p) { return p ? p->n_sub : 0; }
+
+int ds_batch_solve(ds_problem* p, const ds_pcg_params* prm, int mixed,
+                   double drop, int use_graph, ds_pcg_result* res) {
+  if (!p || !res) return DS_E_ARG;
+  if (int rc = check_pcg_params(p, prm)) return rc;
+  if (mixed && !(drop > 0.0 && drop < 1.0))
+    return fail(p->ctx, DS_E_ARG, "refine_drop must lie in (0, 1)");
+  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, mixed ? 1 : 0, drop, res,
+                                     use_graph ? 1 : 0));
+  return DS_OK;
+}
+
+int ds_problem_set_launch(ds_problem* p, int waves, int order_by_cost) {
+  if (!p) return DS_E_ARG;
+  if (waves != 1 && waves != 2 && waves != 4)
+    return fail(p->ctx, DS_E_ARG, "waves_per_block must be 1, 2 or 4");
+  p->waves_per_block = waves;
+  p->order_by_cost = order_by_cost ? 1 : 0;
+  return DS_OK;
+}
+
+int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
+                        int* order_out, double* ms_out) {
+  if (!p || reps < 1) return DS_E_ARG;
+  double best = 1e300;
+  int bw = 4, bo = 0;
+  for (int order = 0; order < 2; ++order)
+    for (int waves : {1, 2, 4}) {
+      p->waves_per_block = waves;
+      p->order_by_cost = order;
+      double ms = 0.0;
+      if (int rc = ds_problem_time_matvec(p, prec, reps, &ms)) return rc;
+      if (ms < best) {
+        best = ms;
+        bw = waves;
+        bo = order;
+      }
+    }
+  p->waves_per_block = bw;
+  p->order_by_cost = bo;
+  if (waves_out) *waves_out = bw;
+  if (order_out) *order_out = bo;
+  if (ms_out) *ms_out = best;
   return DS_OK;
 }
 

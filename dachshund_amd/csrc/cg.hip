@@ -25,11 +25,16 @@ namespace {
 
 constexpr int kVecThreads = 256;
 
+// The CG vector kernels run one 64-thread block per target block, so every
+// block belongs to one subvolume and its reduction group is that
+// subvolume's blocks.
 struct UpdArgs {
   const double* d;
   const double* q;
   const double* diag;
   const double* wvec;     // w (W form) or nullptr
+  const int* blk_sub;
+  const int* sub_blk0;
   double* x;
   double* xu;             // w .* x (W form); may alias x
   double* r;
@@ -37,57 +42,62 @@ struct UpdArgs {
   double* partials;
   ds_cg_scalars* sc;
   double* norms;
+  int* done_count;
   double drop;
-  int npad;
+  int nblk, norms_cap;
   int only_x, mixed;
 };
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
-__global__ __launch_bounds__(kVecThreads) void k_cg_update(UpdArgs a) {
-  __shared__ double red[2 * (kVecThreads / 64)];
+__global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
+  __shared__ double red[2];
   __shared__ bool is_last;
-  if (a.sc->done) return;
-  const double alpha = a.sc->alpha;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = blockIdx.x;
+  const int sv = a.blk_sub[I];
+  ds_cg_scalars* sc = a.sc + sv;
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  const int i = I * DS_BS + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (i < a.npad) {
-    const double xi = a.x[i] + alpha * a.d[i];
-    a.x[i] = xi;
-    if (a.wvec) a.xu[i] = a.wvec[i] * xi;
-    if (!a.only_x) {
-      const double ri = a.r[i] - alpha * a.q[i];
-      a.r[i] = ri;
-      const double si = ri / a.diag[i];
-      a.s[i] = si;
-      v[0] = ri * si;
-      v[1] = ri * ri;
-    }
-  }
+  const double xi = a.x[i] + alpha * a.d[i];
+  a.x[i] = xi;
+  if (a.wvec) a.xu[i] = a.wvec[i] * xi;
   if (a.only_x) return;
+  const double ri = a.r[i] - alpha * a.q[i];
+  a.r[i] = ri;
+  const double si = ri / a.diag[i];
+  a.s[i] = si;
+  v[0] = ri * si;
+  v[1] = ri * ri;
   block_sum<2>(v, red);
-  if (!publish_partials<2>(v, a.partials, &a.sc->counter, &is_last)) return;
-  gather_partials<2>(v, a.partials, red);
+  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
+  if (!publish_partials<2>(v, a.partials + b0, a.nblk, I - b0, nb,
+                           &sc->counter, &is_last))
+    return;
+  gather_partials<2>(v, a.partials + b0, a.nblk, nb, red);
   if (threadIdx.x == 0)
-    finalize_step(a.sc, a.norms, v[0], v[1], 0, a.mixed, a.drop, 0);
-  release_ticket(&a.sc->counter);
+    finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0], v[1], 0,
+                  a.mixed, a.drop, 0, a.done_count);
+  release_ticket(&sc->counter);
 }
 
 struct DirArgs {
   const double* s;
   const double* wvec;
+  const int* blk_sub;
   double* d;
   double* du;
   float* du32;
   ds_cg_scalars* sc;
-  int npad;
 };
 
 // lib/pcg.h:61-64: d = s + beta d (and its scaled copies).
-__global__ __launch_bounds__(kVecThreads) void k_cg_dir(DirArgs a) {
-  if (a.sc->done) return;
-  const double beta = a.sc->beta;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.npad) return;
+__global__ __launch_bounds__(64) void k_cg_dir(DirArgs a) {
+  const int I = blockIdx.x;
+  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
+  if (sc->done) return;
+  const double beta = sc->beta;
+  const int i = I * DS_BS + threadIdx.x;
   const double di = a.s[i] + beta * a.d[i];
   a.d[i] = di;
   const double ui = a.wvec ? a.wvec[i] * di : di;
@@ -151,6 +161,8 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.q = p->q;
   a.diag = p->diag;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
+  a.blk_sub = p->blk_sub;
+  a.sub_blk0 = p->sub_blk0;
   a.x = p->x;
   a.xu = p->xu;
   a.r = p->r;
@@ -158,12 +170,13 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.partials = p->partials;
   a.sc = p->sc;
   a.norms = p->norms;
+  a.done_count = p->done_count;
   a.drop = drop;
-  a.npad = p->npad;
+  a.nblk = p->nblk;
+  a.norms_cap = p->norms_cap;
   a.only_x = only_x;
   a.mixed = mixed;
-  hipLaunchKernelGGL(k_cg_update, vec_grid(p->npad), dim3(kVecThreads), 0, st,
-                     a);
+  hipLaunchKernelGGL(k_cg_update, dim3(p->nblk), dim3(DS_BS), 0, st, a);
   return hipGetLastError();
 }
 
@@ -171,12 +184,12 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   DirArgs a{};
   a.s = p->s;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
+  a.blk_sub = p->blk_sub;
   a.d = p->d;
   a.du = p->du;
   a.du32 = mixed ? p->du32 : nullptr;
   a.sc = p->sc;
-  a.npad = p->npad;
-  hipLaunchKernelGGL(k_cg_dir, vec_grid(p->npad), dim3(kVecThreads), 0, st, a);
+  hipLaunchKernelGGL(k_cg_dir, dim3(p->nblk), dim3(DS_BS), 0, st, a);
   return hipGetLastError();
 }
 
@@ -214,36 +227,45 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
 
 }  // namespace
 
-// Solve with the problem's resident x (sorted order, W space on entry) as the
-// initial guess; on return x holds the solution (x = w .* y for the W form,
-// lib/wf_vector.cc:268-271).
+// Solve every subvolume with the problem's resident x (sorted order, W
+// space on entry) as the initial guess; on return x holds the solutions
+// (x = w .* y for the W form, lib/wf_vector.cc:268-271).  res receives one
+// entry per subvolume.  With use_graph the iterations of each chunk are
+// captured once into a hipGraph and replayed (mixed solver only: its
+// iterations all launch the same kernels with the same arguments).
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
-                      double drop, ds_pcg_result* res) {
+                      double drop, ds_pcg_result* res, int use_graph) {
   hipStream_t st = p->ctx->stream;
   const int max_iter = prm->max_iter;
+  const int ns = p->n_sub;
   const int cap = std::max(max_iter, 1) + 1;
   if (cap > p->norms_cap) {
     if (p->norms) DS_TRY(hipFree(p->norms));
     p->norms = nullptr;
-    DS_TRY(hipMalloc(&p->norms, sizeof(double) * (size_t)cap));
+    DS_TRY(hipMalloc(&p->norms, sizeof(double) * (size_t)cap * ns));
     p->norms_cap = cap;
   }
-  std::memset(res, 0, sizeof(*res));
-  res->norm_b = p->norm_b;
   hipEvent_t e0, e1;
   DS_TRY(hipEventCreate(&e0));
   DS_TRY(hipEventCreate(&e1));
   DS_TRY(hipEventRecord(e0, st));
-  ds_cg_scalars& h = *p->sc_host;
-  std::memset(&h, 0, sizeof(h));
-  h.tol_norm_b = prm->tol * p->norm_b;
-  h.max_iter = max_iter;
+  ds_cg_scalars* h = p->sc_host;
+  std::memset(h, 0, sizeof(ds_cg_scalars) * ns);
+  int live = 0;
+  for (int k = 0; k < ns; ++k) {
+    h[k].tol_norm_b = prm->tol * p->sub_norm_b[k];
+    h[k].max_iter = max_iter;
+    // lib/pcg.h:82-86: |b| <= 0 returns at once, x untouched (then x *= w)
+    if (p->sub_norm_b[k] <= 0.0) h[k].done = 1;
+    else ++live;
+  }
   const double* w = p->form == DS_FORM_W ? p->wvec : nullptr;
-  if (p->norm_b <= 0.0) {
-    // lib/pcg.h:82-86: early return, x untouched (then x *= w).
-    h.done = 1;
-  } else {
-    DS_TRY(hipMemcpyAsync(p->sc, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  p->done_host[0] = ns - live;
+  DS_TRY(hipMemcpyAsync(p->done_count, &p->done_host[0], sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  DS_TRY(hipMemcpyAsync(p->sc, h, sizeof(ds_cg_scalars) * ns,
+                        hipMemcpyHostToDevice, st));
+  if (live > 0) {
     // lib/pcg.h:97-110: r = b - A x; d = M^-1 r; delta = r . d; |r|.
     DS_TRY(launch_scale(p->x, w, w ? p->xu : nullptr, nullptr, p->npad, st));
     CgEpilogue init;
@@ -254,37 +276,63 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init, st));
     DS_TRY(launch_scale(p->d, w, w ? p->du : nullptr,
                         mixed ? p->du32 : nullptr, p->npad, st));
-    // Enqueue iterations in chunks; after each chunk a snapshot of the
-    // scalars is copied to pinned memory (two slots, alternating) and the
-    // host inspects the previous chunk's snapshot while this one runs.
+    // Enqueue iterations in chunks; after each chunk the number of finished
+    // subvolumes is copied to pinned memory (two slots, alternating) and the
+    // host inspects the previous chunk's count while this one runs.
     const int chunk = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const bool graphs = use_graph && mixed;
     hipEvent_t ev[2];
     DS_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     DS_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int iter = 1, k = 0;
-    while (iter < max_iter) {
+    hipError_t err = hipSuccess;
+    while (iter < max_iter && err == hipSuccess) {
       const int end = std::min(max_iter, iter + chunk);
-      for (; iter < end; ++iter)
-        DS_TRY(enqueue_iteration(p, iter, prm->step_r, mixed, drop, st));
-      DS_TRY(hipMemcpyAsync(&p->sc_host[1 + (k & 1)], p->sc, sizeof(h),
-                            hipMemcpyDeviceToHost, st));
-      DS_TRY(hipEventRecord(ev[k & 1], st));
-      if (k > 0) {
-        DS_TRY(hipEventSynchronize(ev[(k - 1) & 1]));
-        if (p->sc_host[1 + ((k - 1) & 1)].done) break;
+      if (graphs && end - iter == chunk) {
+        if (!gexec) {
+          err = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+          for (int it = 0; it < chunk && err == hipSuccess; ++it)
+            err = enqueue_iteration(p, 1, prm->step_r, mixed, drop, st);
+          hipError_t e2 = hipStreamEndCapture(st, &graph);
+          if (err == hipSuccess) err = e2;
+          if (err == hipSuccess)
+            err = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        }
+        if (err == hipSuccess) err = hipGraphLaunch(gexec, st);
+        iter = end;
+      } else {
+        for (; iter < end && err == hipSuccess; ++iter)
+          err = enqueue_iteration(p, iter, prm->step_r, mixed, drop, st);
+      }
+      if (err != hipSuccess) break;
+      err = hipMemcpyAsync(&p->done_host[k & 1], p->done_count, sizeof(int),
+                           hipMemcpyDeviceToHost, st);
+      if (err == hipSuccess) err = hipEventRecord(ev[k & 1], st);
+      if (err == hipSuccess && k > 0) {
+        err = hipEventSynchronize(ev[(k - 1) & 1]);
+        if (p->done_host[(k - 1) & 1] >= ns) break;
       }
       ++k;
     }
-    DS_TRY(hipMemcpyAsync(&h, p->sc, sizeof(h), hipMemcpyDeviceToHost, st));
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    DS_TRY(err);
+    DS_TRY(hipMemcpyAsync(h, p->sc, sizeof(ds_cg_scalars) * ns,
+                          hipMemcpyDeviceToHost, st));
     DS_TRY(hipStreamSynchronize(st));
-    DS_TRY(hipEventDestroy(ev[0]));
-    DS_TRY(hipEventDestroy(ev[1]));
     if (max_iter <= 1) {
       // lib/pcg.h:118,152-158: no step taken, iter == 1.
-      h.num_iters = 1;
-      h.success = (max_iter != 1);
-      h.n_norms = 1;
-      h.done = 1;
+      for (int s = 0; s < ns; ++s)
+        if (p->sub_norm_b[s] > 0.0) {
+          h[s].num_iters = 1;
+          h[s].success = (max_iter != 1);
+          h[s].n_norms = 1;
+          h[s].done = 1;
+        }
     }
   }
   // lib/wf_vector.cc:268-271: x = w .* y.
@@ -295,22 +343,26 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
   DS_TRY(hipEventElapsedTime(&ms, e0, e1));
   DS_TRY(hipEventDestroy(e0));
   DS_TRY(hipEventDestroy(e1));
-  res->success = h.success;
-  res->num_iters = h.num_iters;
-  res->n_norms = h.n_norms;
-  res->final_norm = h.norm_r;
-  res->true_norm = mixed ? h.true_norm : h.norm_r;
-  res->n_true = h.n_true;
-  res->seconds = ms * 1e-3;
-  // pair-evaluation passes: init residual + one per iteration (+ refreshes)
-  if (p->norm_b > 0.0) {
-    const int steps = std::max(0, std::min(h.num_iters, max_iter - 1));
-    if (mixed) {
-      res->n_matvec32 = steps;
-      res->n_matvec64 = 1 + h.n_true;
-    } else {
-      res->n_matvec32 = 0;
-      res->n_matvec64 = 1 + steps + steps / std::max(prm->step_r, 1);
+  for (int s = 0; s < ns; ++s) {
+    ds_pcg_result& r = res[s];
+    std::memset(&r, 0, sizeof(r));
+    r.norm_b = p->sub_norm_b[s];
+    r.success = h[s].success;
+    r.num_iters = h[s].num_iters;
+    r.n_norms = h[s].n_norms;
+    r.final_norm = h[s].norm_r;
+    r.true_norm = mixed ? h[s].true_norm : h[s].norm_r;
+    r.n_true = h[s].n_true;
+    r.seconds = ms * 1e-3;
+    // pair-evaluation passes: initial residual + one per step (+ refreshes)
+    if (r.norm_b > 0.0) {
+      const int steps = std::max(0, std::min(h[s].num_iters, max_iter - 1));
+      if (mixed) {
+        r.n_matvec32 = steps;
+        r.n_matvec64 = 1 + h[s].n_true;
+      } else {
+        r.n_matvec64 = 1 + steps + steps / std::max(prm->step_r, 1);
+      }
     }
   }
   return hipSuccess;

@@ -53,22 +53,24 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   __syncthreads();
 }
 
-// Publish this block's NV partial sums (thread 0 holds them) and return true
-// in every thread of the block that took the last ticket.
+// Publish this block's NV partial sums (thread 0 holds them) into slot
+// `slot` of a reduction group of `count` blocks (component k at
+// partials[k * stride + slot]) and return true in every thread of the block
+// that took the group's last ticket.
 template <int NV>
 __device__ __forceinline__ bool publish_partials(const double (&v)[NV],
                                                  double* partials,
-                                                 unsigned* counter,
+                                                 size_t stride, int slot,
+                                                 int count, unsigned* counter,
                                                  bool* is_last_lds) {
-  const int G = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&partials[(size_t)k * G + blockIdx.x], v[k],
+      __hip_atomic_store(&partials[(size_t)k * stride + slot], v[k],
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
     const unsigned t = atomicAdd(counter, 1u);
-    *is_last_lds = (t == (unsigned)G - 1);
+    *is_last_lds = (t == (unsigned)count - 1);
   }
   __syncthreads();
   const bool last = *is_last_lds;
@@ -76,17 +78,29 @@ __device__ __forceinline__ bool publish_partials(const double (&v)[NV],
   return last;
 }
 
-// In the last block: sum the NV partial vectors in a fixed order.
+// In the last block: sum the group's NV partial vectors in a fixed order.
+// The acquire fence of publish_partials has invalidated this CU's caches,
+// so plain loads see every block's partial; they are issued eight at a time
+// so their memory latencies overlap.
 template <int NV>
 __device__ __forceinline__ void gather_partials(double (&v)[NV],
                                                 const double* partials,
+                                                size_t stride, int count,
                                                 double* red) {
-  const int G = gridDim.x;
+  const int step = blockDim.x;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
+    const double* pk = partials + (size_t)k * stride;
     double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += blockDim.x)
-      s += load_coherent(&partials[(size_t)k * G + b]);
+    int b = threadIdx.x;
+    for (; b + 7 * step < count; b += 8 * step) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = pk[b + u * step];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += t[u];
+    }
+    for (; b < count; b += step) s += pk[b];
     v[k] = s;
   }
   block_sum<NV>(v, red);
@@ -109,10 +123,18 @@ __device__ __forceinline__ void finalize_alpha(ds_cg_scalars* sc, double dq) {
 // Mixed solver (mixed != 0): the recurrence residual never stops the solve
 // itself; when it meets the tolerance or has dropped by `drop` since the
 // last FP64 true residual, the next true-residual kernel runs (want_true).
+__device__ __forceinline__ void mark_done(ds_cg_scalars* sc, int success,
+                                          int num_iters, int* done_count) {
+  sc->done = 1;
+  sc->success = success;
+  sc->num_iters = num_iters;
+  atomicAdd(done_count, 1);
+}
+
 __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
                                               double rs, double rr, int init,
                                               int mixed, double drop,
-                                              int is_true) {
+                                              int is_true, int* done_count) {
   const double norm_r = sqrt(rr);
   if (init) {
     sc->delta = rs;
@@ -139,14 +161,10 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
   sc->n_norms = iter + 1;
   if (!mixed) {
     if (norm_r <= sc->tol_norm_b) {
-      sc->done = 1;
-      sc->success = 1;
-      sc->num_iters = iter;
+      mark_done(sc, 1, iter, done_count);
     } else if (iter + 1 >= sc->max_iter) {
       // lib/pcg.h:118,151-158: the loop ends with iter == max_iter.
-      sc->done = 1;
-      sc->success = 0;
-      sc->num_iters = sc->max_iter;
+      mark_done(sc, 0, sc->max_iter, done_count);
     }
     return;
   }
@@ -156,9 +174,7 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
     sc->ref_norm = norm_r;
     sc->want_true = 0;
     if (norm_r <= sc->tol_norm_b) {
-      sc->done = 1;
-      sc->success = 1;
-      sc->num_iters = iter;
+      mark_done(sc, 1, iter, done_count);
       return;
     }
   } else {
@@ -168,11 +184,7 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
       return;
     }
   }
-  if (iter + 1 >= sc->max_iter) {
-    sc->done = 1;
-    sc->success = 0;
-    sc->num_iters = sc->max_iter;
-  }
+  if (iter + 1 >= sc->max_iter) mark_done(sc, 0, sc->max_iter, done_count);
 }
 
 }  // namespace ds

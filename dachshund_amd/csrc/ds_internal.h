@@ -3,17 +3,22 @@
 // Layout in HBM (per problem, all in *sorted* pixel order, npad entries):
 //   pos64   double4  {x, y, z, w}   w = 1/n (W form) or N = n^2 (SN form)
 //   rhs64   double   wd (W form) or d (SN form); 0 on pad slots
-//   loc32   float4   {X, Y, Z, 0}: coordinates relative to the owning block's
-//                    centre, pre-scaled so that X^2+Y^2+Z^2 is the table
-//                    argument (X = dx * sqrt(c_perp * dx_inv), ...)
-//   blk_c   double4  block centre in the same pre-scaled units
+//   loc32   float4   source coordinates {X, Y, Z, X^2+Y^2+Z^2} relative to
+//                    the frame (per-axis median) of the pixel's tile,
+//                    pre-scaled so that squared distances are table
+//                    arguments (X = x sqrt(c_perp dx_inv), Z = z sqrt(...))
+//   tgt32   float4   target coordinates relative to the pixel's block frame
+//   toff32  float4   per list entry: block frame - tile frame
 //   tptr/tidx        CSR list, per target block of DS_BS = 64 rows, of the
 //                    source tiles (DS_TS = 16 consecutive slots) that can
 //                    hold a nonzero covariance under the exact table cutoff
 // Pixels are ordered by a k-d tree into blocks of 64, and each block again
-// into 4 tiles of 16.  Slots n..npad-1 are padding (copies of the last
-// pixel); every kernel writes 0 to padded rows and every vector keeps 0
-// there, so padded columns contribute exactly nothing.
+// into 4 tiles of 16.  A problem may hold several independent subvolumes
+// (a batch): each occupies a contiguous range of blocks, ordered and culled
+// on its own, with its own covariance constants and CG scalars; a single
+// volume is the one-subvolume case.  The last block of each subvolume is
+// padded (copies of its last pixel); every kernel writes 0 to padded rows
+// and every vector keeps 0 there, so padded columns contribute nothing.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -114,26 +119,43 @@ struct ds_problem {
   ds_ctx* ctx = nullptr;
   int form = DS_FORM_W;
   int n = 0, nblk = 0, ntile = 0, npad = 0;
-  ds::CovarDev cv{};
+  ds::CovarDev cv{};                   // subvolume 0 (table shared by all)
   std::vector<double> table;           // host copy, table_n entries
-  ds::Blocking blk;
+  ds::Blocking blk;                    // one-subvolume problems: its blocking
+  std::vector<int> perm;               // slot -> caller index, -1 for padding
   long long ntile_pairs = 0;           // (block, tile) pairs after culling
-  double norm_b = 0.0;                 // |b| in the reference's order
+  double norm_b = 0.0;                 // |b| of subvolume 0
+  // subvolumes
+  int n_sub = 1;
+  std::vector<int> sub_n;              // pixels per subvolume
+  std::vector<int> sub_off;            // first caller index per subvolume
+  std::vector<int> sub_blk0_h;         // first block per subvolume (n_sub+1)
+  std::vector<ds::CovarDev> sub_cv_h;
+  std::vector<double> sub_norm_b;      // |b| per subvolume, reference order
+  std::vector<int> blk_len_h;          // list length per block
+  int* blk_sub = nullptr;              // subvolume of each block
+  int* blk_nreal = nullptr;            // real rows of each block
+  int* sub_blk0 = nullptr;             // first block per subvolume
+  double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}
+  int* tcount = nullptr;               // real slots per tile
+  int* done_count = nullptr;           // subvolumes finished
+  int* blk_order = nullptr;            // launch order of target blocks
   double setup_seconds = 0.0;
   int waves_per_block = 4;             // pair-kernel launch shape
+  int order_by_cost = 0;               // launch blocks longest list first
   // device buffers
   double4* pos64 = nullptr;
   double* rhs64 = nullptr;
   double* diag = nullptr;              // Jacobi diagonal of A
-  float4* loc32 = nullptr;
-  double4* blk_c = nullptr;
+  float4* loc32 = nullptr;             // sources: tile frame, .w = |X|^2
+  float4* tgt32 = nullptr;             // targets: block frame
+  bool f32_direct = false;             // FP32 kernel uses direct differences
   int* tptr = nullptr;
   int* tidx = nullptr;
   float4* toff32 = nullptr;            // per list entry: centre offsets
   double2* table64 = nullptr;          // {T[k], T[k+1]-T[k]}, last = {0, 0}
   float2* table32 = nullptr;
   double* table_raw = nullptr;         // T[k] as given (dense/column paths)
-  int* perm = nullptr;                 // sorted -> original
   // CG work vectors (npad each)
   double *x = nullptr, *r = nullptr, *d = nullptr, *s = nullptr, *q = nullptr,
          *tmp = nullptr;
@@ -141,10 +163,11 @@ struct ds_problem {
   double *du = nullptr, *xu = nullptr; // w .* d, w .* x (W form)
   float* du32 = nullptr;               // FP32 copy of du
   double* partials = nullptr;          // 4 * nblk
-  ds_cg_scalars* sc = nullptr;
-  double* norms = nullptr;             // device residual history
+  ds_cg_scalars* sc = nullptr;         // n_sub entries
+  double* norms = nullptr;             // residual histories, n_sub x cap
   int norms_cap = 0;
-  ds_cg_scalars* sc_host = nullptr;    // pinned: [0] final, [1..2] polls
+  ds_cg_scalars* sc_host = nullptr;    // pinned, n_sub entries (final)
+  int* done_host = nullptr;            // pinned poll slots (2)
   double* host_tmp = nullptr;          // pinned npad doubles
 };
 
@@ -154,7 +177,7 @@ hipError_t launch_scale(const double* v, const double* w, double* u,
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st);
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
-                      double drop, ds_pcg_result* res);
+                      double drop, ds_pcg_result* res, int use_graph);
 hipError_t launch_fill(double* v, double value, int n, hipStream_t st);
 hipError_t launch_column(const ds_problem* p, const double* table_dev,
                          const double* map_pt, double* col, hipStream_t st);

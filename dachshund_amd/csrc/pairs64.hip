@@ -68,13 +68,19 @@ struct MatvecArgs {
   const int* tptr;
   const int* tidx;
   const double2* table;
+  const int* blk_sub;     // subvolume of each block
+  const int* blk_nreal;   // real rows of each block
+  const int* sub_blk0;    // first block of each subvolume
+  const double4* sub_cv;  // {var_s, c_perp, c_para, 0} per subvolume
+  const int* blk_order;   // launch slot -> block (may be null)
   double* out;            // A a, or rhs - A a
   double* out2;           // CG residual: M^-1 r
   double* partials;
-  ds_cg_scalars* sc;
-  double* norms;
-  double var_s, c_perp, c_para, dx_inv, tlast, drop;
-  int n, table_n;
+  ds_cg_scalars* sc;      // per subvolume
+  double* norms;          // per subvolume, norms_cap apart
+  int* done_count;
+  double dx_inv, tlast, drop;
+  int nblk, table_n, norms_cap;
   int residual;           // out = rhs - A a
   int init, mixed, is_true, need_want;
 };
@@ -85,22 +91,27 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   __shared__ double red[4 * 64];
   __shared__ double red2[2 * 4];
   __shared__ bool is_last;
+  const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
+  const int sv = a.blk_sub[I];
+  ds_cg_scalars* sc = a.sc + sv;
   if (EPI != EPI_NONE) {
-    if (a.sc->done) return;
-    if (a.need_want && !a.sc->want_true) return;
+    if (sc->done) return;
+    if (a.need_want && !sc->want_true) return;
   }
   for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
   __syncthreads();
+  const double4 cvs = a.sub_cv[sv];
+  const double var_s = cvs.x;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
-  const int I = blockIdx.x;
   const int row = I * DS_BS + lane;
+  const bool real = lane < a.blk_nreal[I];
   const double4 pi = a.pos[row];
   const int kb = a.tptr[I], ke = a.tptr[I + 1];
   const double acc = accumulate_tiles64<false>(
-      pi.x, pi.y, pi.z, a.tidx, kb + wave, ke, nw, a.pos, a.u, tab, a.c_perp,
-      a.c_para, a.dx_inv, a.tlast);
+      pi.x, pi.y, pi.z, a.tidx, kb + wave, ke, nw, a.pos, a.u, tab, cvs.y,
+      cvs.z, a.dx_inv, a.tlast);
   red[wave * 64 + lane] = acc;
   __syncthreads();
   double v[2] = {0.0, 0.0};
@@ -109,15 +120,15 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
     for (int w = 0; w < nw; ++w) y += red[w * 64 + lane];
     const double ai = a.in[row];
     // FORM W: (w S w + I) a;  FORM SN: (S + N) a  (lib/wf_vector.cc:67,62)
-    double bi = (FORM == DS_FORM_W) ? pi.w * (a.var_s * y) + ai
-                                    : a.var_s * y + pi.w * ai;
+    double bi = (FORM == DS_FORM_W) ? pi.w * (var_s * y) + ai
+                                    : var_s * y + pi.w * ai;
     if (a.residual) bi = a.rhs[row] - bi;
-    if (row >= a.n) bi = 0.0;
+    if (!real) bi = 0.0;
     a.out[row] = bi;
     if (EPI == EPI_CG_DQ) {
       v[0] = ai * bi;  // d . q
     } else if (EPI == EPI_CG_RES) {
-      const double si = (row < a.n) ? bi / a.diag[row] : 0.0;
+      const double si = real ? bi / a.diag[row] : 0.0;
       a.out2[row] = si;
       v[0] = bi * si;  // r . M^-1 r
       v[1] = bi * bi;  // r . r
@@ -128,22 +139,25 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
     v[0] = wave_sum(v[0]);
     v[1] = wave_sum(v[1]);
   }
+  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
   if (EPI == EPI_CG_DQ) {
     double p1[1] = {v[0]};
-    if (!publish_partials<1>(p1, a.partials, &a.sc->counter, &is_last))
+    if (!publish_partials<1>(p1, a.partials + b0, a.nblk, I - b0, nb,
+                             &sc->counter, &is_last))
       return;
-    gather_partials<1>(p1, a.partials, red2);
-    if (threadIdx.x == 0) finalize_alpha(a.sc, p1[0]);
+    gather_partials<1>(p1, a.partials + b0, a.nblk, nb, red2);
+    if (threadIdx.x == 0) finalize_alpha(sc, p1[0]);
   } else {
     double p2[2] = {v[0], v[1]};
-    if (!publish_partials<2>(p2, a.partials, &a.sc->counter, &is_last))
+    if (!publish_partials<2>(p2, a.partials + b0, a.nblk, I - b0, nb,
+                             &sc->counter, &is_last))
       return;
-    gather_partials<2>(p2, a.partials, red2);
+    gather_partials<2>(p2, a.partials + b0, a.nblk, nb, red2);
     if (threadIdx.x == 0)
-      finalize_step(a.sc, a.norms, p2[0], p2[1], a.init, a.mixed, a.drop,
-                    a.is_true);
+      finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, p2[0], p2[1],
+                    a.init, a.mixed, a.drop, a.is_true, a.done_count);
   }
-  release_ticket(&a.sc->counter);
+  release_ticket(&sc->counter);
 }
 
 // Map targets: the uniform grid of lib/util.cc:25-46, cut into 4x4x4 tiles;
@@ -249,19 +263,23 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.tptr = p->tptr;
   a.tidx = p->tidx;
   a.table = p->table64;
+  a.blk_sub = p->blk_sub;
+  a.blk_nreal = p->blk_nreal;
+  a.sub_blk0 = p->sub_blk0;
+  a.sub_cv = p->sub_cv;
+  a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
   a.out = out;
   a.out2 = e.out2;
   a.partials = p->partials;
   a.sc = p->sc;
   a.norms = p->norms;
-  a.var_s = p->cv.var_s;
-  a.c_perp = p->cv.c_perp;
-  a.c_para = p->cv.c_para;
+  a.done_count = p->done_count;
   a.dx_inv = p->cv.dx_inv;
   a.tlast = (double)(p->cv.table_n - 1);
   a.drop = e.drop;
-  a.n = p->n;
+  a.nblk = p->nblk;
   a.table_n = p->cv.table_n;
+  a.norms_cap = p->norms_cap;
   a.residual = residual;
   a.init = e.init;
   a.mixed = e.mixed;

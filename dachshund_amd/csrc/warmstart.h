@@ -1,0 +1,56 @@
+// warmstart.h -- sizes and device state of the learned initial guess.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define WS_K 32                       // nearest neighbours per pixel
+#define WS_FEAT (5 * WS_K + 2)        // 162 inputs
+#define WS_HIDDEN 64
+// flat parameter vector (FP32); weights are stored [in][out]
+#define WS_OFF_W1 0
+#define WS_OFF_B1 (WS_OFF_W1 + WS_FEAT * WS_HIDDEN)
+#define WS_OFF_W2 (WS_OFF_B1 + WS_HIDDEN)
+#define WS_OFF_B2 (WS_OFF_W2 + WS_HIDDEN * WS_HIDDEN)
+#define WS_OFF_W3 (WS_OFF_B2 + WS_HIDDEN)
+#define WS_OFF_B3 (WS_OFF_W3 + WS_HIDDEN * WS_HIDDEN)
+#define WS_OFF_W4 (WS_OFF_B3 + WS_HIDDEN)
+#define WS_OFF_B4 (WS_OFF_W4 + WS_HIDDEN)
+#define WS_NPARAM (WS_OFF_B4 + 1)     // 18817
+#define WS_CHUNK 1024                 // samples per weight-gradient partial
+#define WS_MAX_BATCH 65536
+
+struct ds_problem;
+
+namespace ds {
+
+struct WsTrainer {
+  float* P = nullptr;       // parameters
+  float* m = nullptr;       // Adam moments
+  float* v = nullptr;
+  float* grad = nullptr;
+  float* part = nullptr;    // per-chunk gradient partials
+  unsigned short* act = nullptr;   // minibatch activations (BF16)
+  float* delta = nullptr;   // minibatch deltas
+  float* dy = nullptr;
+  float* loss_part = nullptr;
+  int* iota = nullptr;      // 0..WS_MAX_BATCH-1
+  long long step = 0;
+};
+
+hipError_t ws_features(const ds_problem* p, unsigned short* feat,
+                       hipStream_t st);
+hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
+                      int npad, int quant, const ds_problem* p, double* x_out,
+                      float* t_out, hipStream_t st);
+hipError_t ws_targets(const ds_problem* p, float* t, hipStream_t st);
+hipError_t ws_batch(WsTrainer& T, const unsigned short* feat,
+                    const float* target, const int* idx, int B, int quant,
+                    float* loss_part, hipStream_t st);
+hipError_t ws_adam(WsTrainer& T, float lr, hipStream_t st);
+
+}  // namespace ds
+
+namespace ds {
+hipError_t ws_gather(const unsigned short* fsrc, const float* tsrc,
+                     const int* perm, int n, unsigned short* fdst,
+                     float* tdst, hipStream_t st);
+}  // namespace ds

@@ -1,0 +1,543 @@
+// warmstart.hip -- the learned initial guess of the Wiener-filter solve.
+//
+// A point network predicts each pixel's solution value x_i = [(S+N)^-1 d]_i
+// (the quantity the W-form solve returns, lib/wf_vector.cc:268-271) from its
+// neighbourhood; the prediction enters PCG as the initial guess
+// y0_i = x0_i / w_i, the reference's x in/out argument (lib/pcg.h:98).  The
+// solve's stopping rule is unchanged, so the prediction only changes how
+// many iterations are needed.
+//
+// Features of pixel i (FEAT = 5 K + 2, K = 32 nearest neighbours under the
+// covariance metric, nearest first; absent neighbours are zeros):
+//   per neighbour j: (x_j - x_i)/l_perp, (y_j - y_i)/l_perp, (z_j - z_i)/l_para,
+//                    n_j / sqrt(var_s), d_j / sqrt(var_s)
+//   own:             n_i / sqrt(var_s), d_i / sqrt(var_s)
+// stored as BF16.  Network: FEAT -> 64 -> 64 -> 64 -> 1, ReLU, FP32 weights
+// (layout [in][out]), activations rounded to BF16 (optional), output
+// t = x_i sqrt(var_s).
+//
+// Kernels: k_knn_features (neighbour search over the problem's culled tile
+// lists), k_fwd (inference), k_fwd_bwd (training forward + per-pixel
+// back-propagated deltas), k_wgrad (deterministic chunked weight-gradient
+// sums), k_reduce_chunks, k_adam, k_gather_pred.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cg_device.h"
+#include "ds_internal.h"
+#include "warmstart.h"
+
+namespace ds {
+
+namespace {
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) {
+  return __uint_as_float(((unsigned)h) << 16);
+}
+
+// ---- neighbour search and features --------------------------------------
+
+struct KnnArgs {
+  const double4* pos;     // {x, y, z, w or N}
+  const double* rhs;      // wd (W) or d (SN)
+  const float4* loc;
+  const float4* tgt;
+  const int* tptr;
+  const int* tidx;
+  const float4* toff;
+  unsigned short* feat;   // npad x WS_FEAT
+  int n, form;
+  float inv_lp, inv_ll, inv_sqrt_vs, cutoff;
+};
+
+template <int K>
+__global__ __launch_bounds__(64) void k_knn_features(KnnArgs a) {
+  const int lane = threadIdx.x;
+  const int I = blockIdx.x;
+  const int row = I * DS_BS + lane;
+  const float4 li = a.tgt[row];
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bd[k] = __builtin_inff();
+    bi[k] = -1;
+  }
+  float worst = __builtin_inff();
+  const int ke = a.tptr[I + 1];
+  for (int e = a.tptr[I]; e < ke; ++e) {
+    const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
+    const float4 o = a.toff[e];
+    const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
+    for (int jj = 0; jj < DS_TS; ++jj) {
+      const int j = j0 + jj;
+      if (j >= a.n) break;   // tile slots past n are padding (uniform)
+      const float4 pj = a.loc[j];
+      const float dX = xi - pj.x, dY = yi - pj.y, dZ = zi - pj.z;
+      const float d2 = dX * dX + dY * dY + dZ * dZ;
+      // candidates: inside the table cutoff (d2 is the table argument)
+      if (j != row && d2 < worst && d2 < a.cutoff) {
+        // replace the current worst entry, then find the new worst
+        float nw = 0.0f;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const bool hit = !done && bd[k] == worst;
+          bd[k] = hit ? d2 : bd[k];
+          bi[k] = hit ? j : bi[k];
+          done = done || hit;
+          nw = fmaxf(nw, bd[k]);
+        }
+        worst = nw;
+      }
+    }
+  }
+  // sort ascending by (distance, index): odd-even transposition network
+#pragma unroll
+  for (int pass = 0; pass < K; ++pass) {
+#pragma unroll
+    for (int k = pass & 1; k + 1 < K; k += 2) {
+      const bool sw = bd[k + 1] < bd[k] || (bd[k + 1] == bd[k] &&
+                                            (unsigned)bi[k + 1] <
+                                                (unsigned)bi[k]);
+      const float td = sw ? bd[k + 1] : bd[k];
+      const float tu = sw ? bd[k] : bd[k + 1];
+      const int ti = sw ? bi[k + 1] : bi[k];
+      const int tj = sw ? bi[k] : bi[k + 1];
+      bd[k] = td;
+      bd[k + 1] = tu;
+      bi[k] = ti;
+      bi[k + 1] = tj;
+    }
+  }
+  unsigned short* f = a.feat + (size_t)row * WS_FEAT;
+  if (row >= a.n) {
+    for (int c = 0; c < WS_FEAT; ++c) f[c] = 0;
+    return;
+  }
+  const double4 pi = a.pos[row];
+  auto noise = [&](double wN) {
+    return a.form == DS_FORM_W ? 1.0 / wN : sqrt(wN);
+  };
+  auto datum = [&](int j, double wN) {
+    return a.form == DS_FORM_W ? a.rhs[j] / wN : a.rhs[j];
+  };
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    const int j = bi[k];
+    if (j >= 0) {
+      const double4 pj = a.pos[j];
+      v[0] = (float)(pj.x - pi.x) * a.inv_lp;
+      v[1] = (float)(pj.y - pi.y) * a.inv_lp;
+      v[2] = (float)(pj.z - pi.z) * a.inv_ll;
+      v[3] = (float)noise(pj.w) * a.inv_sqrt_vs;
+      v[4] = (float)datum(j, pj.w) * a.inv_sqrt_vs;
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[5 * k + c] = f2bf(v[c]);
+  }
+  f[5 * K] = f2bf((float)noise(pi.w) * a.inv_sqrt_vs);
+  f[5 * K + 1] = f2bf((float)datum(row, pi.w) * a.inv_sqrt_vs);
+}
+
+// ---- network ------------------------------------------------------------
+
+constexpr int H = WS_HIDDEN;
+
+struct NetPtrs {
+  const float* W1;  // [FEAT][H]
+  const float* b1;
+  const float* W2;  // [H][H]
+  const float* b2;
+  const float* W3;
+  const float* b3;
+  const float* W4;  // [H]
+  const float* b4;
+};
+
+__device__ __forceinline__ NetPtrs net_ptrs(const float* P) {
+  NetPtrs n;
+  n.W1 = P + WS_OFF_W1;
+  n.b1 = P + WS_OFF_B1;
+  n.W2 = P + WS_OFF_W2;
+  n.b2 = P + WS_OFF_B2;
+  n.W3 = P + WS_OFF_W3;
+  n.b3 = P + WS_OFF_B3;
+  n.W4 = P + WS_OFF_W4;
+  n.b4 = P + WS_OFF_B4;
+  return n;
+}
+
+template <bool Q>
+__device__ __forceinline__ float act(float z) {
+  const float r = fmaxf(z, 0.0f);
+  return Q ? bf2f(f2bf(r)) : r;
+}
+
+// h_out = act(b + h_in W) for a 64 -> 64 layer (weights [in][out]).
+template <bool Q>
+__device__ __forceinline__ void dense64(const float* __restrict__ W,
+                                        const float* __restrict__ b,
+                                        const float (&hin)[H], float (&hout)[H]) {
+#pragma unroll
+  for (int o = 0; o < H; ++o) hout[o] = b[o];
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const float x = hin[i];
+#pragma unroll
+    for (int o = 0; o < H; ++o) hout[o] = fmaf(x, W[i * H + o], hout[o]);
+  }
+#pragma unroll
+  for (int o = 0; o < H; ++o) hout[o] = act<Q>(hout[o]);
+}
+
+// Forward pass of one pixel; fills h1..h3 and returns the output.
+template <bool Q>
+__device__ __forceinline__ float forward(const NetPtrs& n,
+                                         const unsigned short* __restrict__ f,
+                                         float (&h1)[H], float (&h2)[H],
+                                         float (&h3)[H]) {
+#pragma unroll
+  for (int o = 0; o < H; ++o) h1[o] = n.b1[o];
+  for (int c = 0; c < WS_FEAT; ++c) {
+    const float x = bf2f(f[c]);
+#pragma unroll
+    for (int o = 0; o < H; ++o) h1[o] = fmaf(x, n.W1[c * H + o], h1[o]);
+  }
+#pragma unroll
+  for (int o = 0; o < H; ++o) h1[o] = act<Q>(h1[o]);
+  dense64<Q>(n.W2, n.b2, h1, h2);
+  dense64<Q>(n.W3, n.b3, h2, h3);
+  float y = n.b4[0];
+#pragma unroll
+  for (int i = 0; i < H; ++i) y = fmaf(h3[i], n.W4[i], y);
+  return y;
+}
+
+struct FwdArgs {
+  const unsigned short* feat;
+  const float* P;
+  const double4* pos;     // prediction -> W-space initial guess
+  double* x_out;          // y0 = x0 / w (W) or x0 (SN); may be null
+  float* t_out;           // raw outputs; may be null
+  int n, npad, form, quant;
+  double inv_sqrt_vs;     // x0 = t / sqrt(var_s)
+};
+
+template <bool Q>
+__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npad) return;
+  if (i >= a.n) {
+    if (a.x_out) a.x_out[i] = 0.0;
+    if (a.t_out) a.t_out[i] = 0.0f;
+    return;
+  }
+  const NetPtrs n = net_ptrs(a.P);
+  float h1[H], h2[H], h3[H];
+  const float t = forward<Q>(n, a.feat + (size_t)i * WS_FEAT, h1, h2, h3);
+  if (a.t_out) a.t_out[i] = t;
+  if (a.x_out) {
+    const double x0 = (double)t * a.inv_sqrt_vs;
+    const double w = a.pos[i].w;
+    a.x_out[i] = a.form == DS_FORM_W ? x0 / w : x0;
+  }
+}
+
+// Training: forward + back-propagation of the squared error of sample
+// idx[b] (b < B) into per-sample activations (BF16) and deltas (FP32).
+struct TrainArgs {
+  const unsigned short* feat;   // store of all samples
+  const float* target;
+  const int* idx;               // minibatch -> sample
+  const float* P;
+  unsigned short* act;          // B x 3H: h1, h2, h3
+  float* delta;                 // B x 3H: d1, d2, d3
+  float* dy;                    // B
+  float* loss_part;             // gridDim.x
+  int B;
+  float scale;                  // d loss / d y = scale * (y - t)
+};
+
+template <bool Q>
+__global__ __launch_bounds__(256) void k_fwd_bwd(TrainArgs a) {
+  __shared__ double red[4];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {0.0};
+  if (b < a.B) {
+    const int s = a.idx[b];
+    const NetPtrs n = net_ptrs(a.P);
+    float h1[H], h2[H], h3[H];
+    const float y = forward<Q>(n, a.feat + (size_t)s * WS_FEAT, h1, h2, h3);
+    const float e = y - a.target[s];
+    v[0] = (double)e * e;
+    const float g = a.scale * e;
+    a.dy[b] = g;
+    unsigned short* A = a.act + (size_t)b * 3 * H;
+    float* D = a.delta + (size_t)b * 3 * H;
+    float d3[H], d2[H];
+#pragma unroll
+    for (int o = 0; o < H; ++o) {
+      d3[o] = h3[o] > 0.0f ? g * n.W4[o] : 0.0f;
+      D[2 * H + o] = d3[o];
+      A[2 * H + o] = f2bf(h3[o]);
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      float s3 = 0.0f;
+#pragma unroll
+      for (int o = 0; o < H; ++o) s3 = fmaf(n.W3[i * H + o], d3[o], s3);
+      d2[i] = h2[i] > 0.0f ? s3 : 0.0f;
+    }
+#pragma unroll
+    for (int o = 0; o < H; ++o) {
+      D[H + o] = d2[o];
+      A[H + o] = f2bf(h2[o]);
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      float s2 = 0.0f;
+#pragma unroll
+      for (int o = 0; o < H; ++o) s2 = fmaf(n.W2[i * H + o], d2[o], s2);
+      D[i] = h1[i] > 0.0f ? s2 : 0.0f;
+      A[i] = f2bf(h1[i]);
+    }
+  }
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) a.loss_part[blockIdx.x] = (float)v[0];
+}
+
+// Weight gradients G[r][o] = sum_b In[b][r] * D[b][o] over a chunk of the
+// minibatch, one 16 x 64 tile of G per block (r = input row, o = output).
+// layer 0: In = features (BF16, FEAT cols), D = delta1
+// layer 1: In = h1 (act cols 0..63),         D = delta2
+// layer 2: In = h2,                          D = delta3
+// layer 3: In = h3, D = dy (one column)
+// A final row r == nin holds the bias gradient (In = 1).
+struct WgradArgs {
+  const unsigned short* feat;
+  const int* idx;
+  const unsigned short* act;
+  const float* delta;
+  const float* dy;
+  float* part;          // [nchunk][WS_NPARAM]
+  int B, chunk;
+};
+
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a, int layer) {
+  const int nin = layer == 0 ? WS_FEAT : H;
+  const int nout = layer == 3 ? 1 : H;
+  const int rtiles = (nin + 1 + 15) / 16;
+  const int rt = blockIdx.x % rtiles;
+  const int ch = blockIdx.x / rtiles;
+  const int r = rt * 16 + (threadIdx.x >> 4);
+  const int o4 = (threadIdx.x & 15) * 4;
+  const int b0 = ch * a.chunk, b1 = min(a.B, b0 + a.chunk);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  if (r <= nin && o4 < nout) {
+    for (int b = b0; b < b1; ++b) {
+      float x;
+      if (r == nin) x = 1.0f;
+      else if (layer == 0) x = bf2f(a.feat[(size_t)a.idx[b] * WS_FEAT + r]);
+      else x = bf2f(a.act[(size_t)b * 3 * H + (layer - 1) * H + r]);
+      if (layer == 3) {
+        acc[0] = fmaf(x, a.dy[b], acc[0]);
+      } else {
+        const float4 d = *reinterpret_cast<const float4*>(
+            a.delta + (size_t)b * 3 * H + layer * H + o4);
+        acc[0] = fmaf(x, d.x, acc[0]);
+        acc[1] = fmaf(x, d.y, acc[1]);
+        acc[2] = fmaf(x, d.z, acc[2]);
+        acc[3] = fmaf(x, d.w, acc[3]);
+      }
+    }
+  }
+  if (r > nin || o4 >= nout) return;
+  float* G = a.part + (size_t)ch * WS_NPARAM;
+  const int offW = layer == 0 ? WS_OFF_W1 : layer == 1 ? WS_OFF_W2
+                   : layer == 2 ? WS_OFF_W3 : WS_OFF_W4;
+  const int offB = layer == 0 ? WS_OFF_B1 : layer == 1 ? WS_OFF_B2
+                   : layer == 2 ? WS_OFF_B3 : WS_OFF_B4;
+  float* dst = r == nin ? G + offB : G + offW + (size_t)r * nout;
+  const int m = nout == 1 ? 1 : 4;
+  for (int k = 0; k < m; ++k) dst[o4 + k] = acc[k];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const float* part,
+                                                       int nchunk,
+                                                       float* grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= WS_NPARAM) return;
+  float s = 0.0f;
+  for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * WS_NPARAM + p];
+  grad[p] = s;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* P, float* m, float* v,
+                                              const float* g, float lr,
+                                              float b1, float b2, float eps,
+                                              float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= WS_NPARAM) return;
+  const float gi = g[p];
+  const float mi = b1 * m[p] + (1.0f - b1) * gi;
+  const float vi = b2 * v[p] + (1.0f - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  P[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// training targets: t = x sqrt(var_s) from the solved x (sorted order)
+__global__ __launch_bounds__(256) void k_targets(const double* x, int n,
+                                                 double sqrt_vs, float* t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = (float)(x[i] * sqrt_vs);
+}
+
+}  // namespace
+
+// ---- host side ----------------------------------------------------------
+
+#define WS_TRY(expr)                  \
+  do {                                \
+    hipError_t e_ = (expr);           \
+    if (e_ != hipSuccess) return e_;  \
+  } while (0)
+
+hipError_t ws_features(const ds_problem* p, unsigned short* feat,
+                       hipStream_t st) {
+  KnnArgs a{};
+  a.pos = p->pos64;
+  a.rhs = p->rhs64;
+  a.loc = p->loc32;
+  a.tgt = p->tgt32;
+  a.tptr = p->tptr;
+  a.tidx = p->tidx;
+  a.toff = p->toff32;
+  a.feat = feat;
+  a.n = p->n;
+  a.form = p->form;
+  a.inv_lp = (float)std::sqrt(2.0 * p->cv.c_perp);   // 1 / l_perp
+  a.inv_ll = (float)std::sqrt(2.0 * p->cv.c_para);
+  a.inv_sqrt_vs = (float)(1.0 / std::sqrt(p->cv.var_s));
+  a.cutoff = (float)(p->cv.table_n - 1);
+  hipLaunchKernelGGL(k_knn_features<WS_K>, dim3(p->nblk), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
+                      int npad, int quant, const ds_problem* p, double* x_out,
+                      float* t_out, hipStream_t st) {
+  FwdArgs a{};
+  a.feat = feat;
+  a.P = P;
+  a.pos = p ? p->pos64 : nullptr;
+  a.x_out = x_out;
+  a.t_out = t_out;
+  a.n = n;
+  a.npad = npad;
+  a.form = p ? p->form : DS_FORM_SN;
+  a.quant = quant;
+  a.inv_sqrt_vs = p ? 1.0 / std::sqrt(p->cv.var_s) : 1.0;
+  const dim3 g((npad + 255) / 256), b(256);
+  if (quant) hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, a);
+  else hipLaunchKernelGGL(k_fwd<false>, g, b, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t ws_targets(const ds_problem* p, float* t, hipStream_t st) {
+  hipLaunchKernelGGL(k_targets, dim3((p->n + 255) / 256), dim3(256), 0, st,
+                     p->x, p->n, std::sqrt(p->cv.var_s), t);
+  return hipGetLastError();
+}
+
+// One minibatch: forward/backward, gradient into `grad` (and, with lr > 0,
+// an Adam step).  loss_part receives ceil(B/256) partial sums.
+hipError_t ws_batch(WsTrainer& T, const unsigned short* feat,
+                    const float* target, const int* idx, int B, int quant,
+                    float* loss_part, hipStream_t st) {
+  TrainArgs a{};
+  a.feat = feat;
+  a.target = target;
+  a.idx = idx;
+  a.P = T.P;
+  a.act = T.act;
+  a.delta = T.delta;
+  a.dy = T.dy;
+  a.loss_part = loss_part;
+  a.B = B;
+  a.scale = 2.0f / (float)B;
+  const dim3 g((B + 255) / 256), blk(256);
+  if (quant) hipLaunchKernelGGL(k_fwd_bwd<true>, g, blk, 0, st, a);
+  else hipLaunchKernelGGL(k_fwd_bwd<false>, g, blk, 0, st, a);
+  WS_TRY(hipGetLastError());
+  WgradArgs w{};
+  w.feat = feat;
+  w.idx = idx;
+  w.act = T.act;
+  w.delta = T.delta;
+  w.dy = T.dy;
+  w.part = T.part;
+  w.B = B;
+  w.chunk = WS_CHUNK;
+  const int nchunk = (B + WS_CHUNK - 1) / WS_CHUNK;
+  for (int layer = 0; layer < 4; ++layer) {
+    const int nin = layer == 0 ? WS_FEAT : WS_HIDDEN;
+    const int rtiles = (nin + 1 + 15) / 16;
+    hipLaunchKernelGGL(k_wgrad, dim3(rtiles * nchunk), dim3(256), 0, st, w,
+                       layer);
+    WS_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((WS_NPARAM + 255) / 256),
+                     dim3(256), 0, st, T.part, nchunk, T.grad);
+  return hipGetLastError();
+}
+
+hipError_t ws_adam(WsTrainer& T, float lr, hipStream_t st) {
+  T.step += 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f - std::pow(b1, (float)T.step);
+  const float c2 = 1.0f - std::pow(b2, (float)T.step);
+  hipLaunchKernelGGL(k_adam, dim3((WS_NPARAM + 255) / 256), dim3(256), 0, st,
+                     T.P, T.m, T.v, T.grad, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace ds
+
+namespace ds {
+namespace {
+__global__ __launch_bounds__(256) void k_gather_rows(
+    const unsigned short* src, const int* perm, int n, unsigned short* dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)n * WS_FEAT;
+  if (t >= total) return;
+  const size_t r = t / WS_FEAT, c = t % WS_FEAT;
+  dst[t] = src[(size_t)perm[r] * WS_FEAT + c];
+}
+__global__ __launch_bounds__(256) void k_gather_f(const float* src,
+                                                  const int* perm, int n,
+                                                  float* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+}  // namespace
+
+hipError_t ws_gather(const unsigned short* fsrc, const float* tsrc,
+                     const int* perm, int n, unsigned short* fdst,
+                     float* tdst, hipStream_t st) {
+  const size_t total = (size_t)n * WS_FEAT;
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((total + 255) / 256)),
+                     dim3(256), 0, st, fsrc, perm, n, fdst);
+  hipLaunchKernelGGL(k_gather_f, dim3((n + 255) / 256), dim3(256), 0, st,
+                     tsrc, perm, n, tdst);
+  return hipGetLastError();
+}
+}  // namespace ds

@@ -69,9 +69,10 @@ def workload(name: str) -> Workload:
     if name == "c3":   # 100,000 px, 32 noise realisations
         return Workload("c3", (40.0, 40.0, 500.0), 400, 250,
                         map_shape=(40, 40, 500), tol=1e-6, seed=3)
-    if name == "c5":   # 300,000 px, learned warm start
+    if name == "c5":   # 300,000 px, learned warm start; seeds 1000-1199
+        # train the network, 2000-2019 are held out (2000 is the benchmark)
         return Workload("c5", (64.0, 64.0, 600.0), 1000, 300,
-                        map_shape=(64, 64, 600), tol=1e-6, seed=1000)
+                        map_shape=(64, 64, 600), tol=1e-6, seed=2000)
     if name == "tiny":
         return Workload("tiny", (8.0, 8.0, 16.0), 12, 24, grid_cell=0.25,
                         map_shape=(4, 4, 8), tol=1e-8, seed=7)

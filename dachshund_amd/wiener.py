@@ -299,6 +299,10 @@ class _WFVector:
         check(lib().ds_problem_get_x(self.h, _dp(x)), self.ctx.h)
         return x
 
+    def set_launch(self, waves, order_by_cost=False):
+        check(lib().ds_problem_set_launch(self.h, waves, int(order_by_cost)),
+              self.ctx.h)
+
     def solve_resident(self, params: PCGParams, mixed=False,
                        refine_drop=1e-3):
         res = DsPcgResult()
@@ -336,6 +340,56 @@ class _WFVector:
         check(lib().ds_problem_smp_grid(self.h, lx, ly, lz, nx, ny, nz,
                                         _dp(m)), self.ctx.h, "smp_grid")
         return m
+
+
+class BatchWF(_WFVector):
+    """Many independent subvolumes solved together (ds_batch_*): every pair
+    and CG kernel processes all of them in one launch; each has its own
+    covariance, CG scalars and stopping test.  `pixels` is a list of (n_k,
+    5) record arrays in the given form, `s_params` a list of
+    SignalCovarParams (all with the same table)."""
+
+    def __init__(self, pixels, s_params, form="w", ctx=None):
+        self.ctx = ctx or default_context()
+        self.FORM = DS_FORM_W if form == "w" else DS_FORM_SN
+        parts = [_f64(p) for p in pixels]
+        self.sub_n = np.array([len(p) for p in parts], dtype=np.int32)
+        self.pixels = np.ascontiguousarray(np.concatenate(parts))
+        self.n = len(self.pixels)
+        self.n_sub = len(parts)
+        self.s_params = list(s_params)
+        covs = (DsCovar * self.n_sub)(*[sp.c_struct() for sp in s_params])
+        h = C.c_void_p()
+        check(lib().ds_batch_create(self.ctx.h, self.FORM, self.n_sub,
+                                    self.sub_n.ctypes.data_as(
+                                        C.POINTER(C.c_int)),
+                                    _dp(self.pixels), covs, C.byref(h)),
+              self.ctx.h, "batch_create")
+        self.h = h
+        self.offsets = np.concatenate([[0], np.cumsum(self.sub_n)])
+
+    def split(self, v):
+        return [v[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+    def solve(self, params: PCGParams, mixed=True, refine_drop=1e-3,
+              use_graph=True):
+        """Solve all subvolumes from the resident x; one PCGResult each."""
+        res = (DsPcgResult * self.n_sub)()
+        prm = params.c_struct()
+        check(lib().ds_batch_solve(self.h, C.byref(prm), int(mixed),
+                                   refine_drop, int(use_graph), res),
+              self.ctx.h, "batch_solve")
+        return [self._result(r, None) for r in res]
+
+    def set_launch(self, waves, order_by_cost):
+        check(lib().ds_problem_set_launch(self.h, waves, int(order_by_cost)),
+              self.ctx.h)
+
+    def autotune(self, prec=DS_PREC_F32, reps=5):
+        w, o, ms = C.c_int(), C.c_int(), C.c_double()
+        check(lib().ds_problem_autotune(self.h, prec, reps, C.byref(w),
+                                        C.byref(o), C.byref(ms)), self.ctx.h)
+        return w.value, bool(o.value), ms.value
 
 
 class WFVectorSN(_WFVector):

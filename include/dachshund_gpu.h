@@ -119,6 +119,33 @@ int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out);
  * Changes only the order of floating-point sums. */
 int ds_problem_set_waves(ds_problem* p, int waves_per_block);
 
+/* ---- batches of independent subvolumes (no reference counterpart) ------
+ * n_sub subvolumes with sub_n[k] pixels each (records consecutive in
+ * `pixels`) and covariance covars[k] (all with the same table), held in one
+ * problem: each is ordered and culled on its own, every pair kernel and CG
+ * kernel processes all of them in one launch, and each has its own CG
+ * scalars and stopping test.  The operator entry points (A_product, ...,
+ * set_x/get_x) act on the concatenated vector; a one-subvolume batch is an
+ * ordinary problem. */
+int ds_batch_create(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
+                    const double* pixels, const ds_covar* covars,
+                    ds_problem** out);
+int ds_batch_num_sub(const ds_problem* p);
+/* Solve every subvolume from the resident x (W space for the W form); res
+ * gets one entry per subvolume.  mixed: the FP32/FP64 solver of
+ * ds_solve_mixed with refine_drop.  use_graph: capture the iterations into
+ * a hipGraph and replay it (mixed solver). */
+int ds_batch_solve(ds_problem* p, const ds_pcg_params* params, int mixed,
+                   double refine_drop, int use_graph, ds_pcg_result* res);
+/* Pair-kernel launch shape: waves per block (1, 2, 4) and whether blocks
+ * are launched longest tile list first.  Changes only the order of sums. */
+int ds_problem_set_launch(ds_problem* p, int waves_per_block,
+                          int order_by_cost);
+/* Time every launch shape on this problem's pair kernel (prec) and keep
+ * the fastest. */
+int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
+                        int* order_out, double* ms_out);
+
 /* WFVector*::A_product (lib/wf_vector.cc:51-74 SN, :196-220 W): b = A a. */
 int ds_A_product(ds_problem* p, int prec, const double* a, double* b);
 /* WFVector*::A_residual (lib/wf_vector.cc:76-103 SN, :222-251 W):
@@ -207,6 +234,46 @@ int ds_map_covar_diag(ds_ctx* ctx, int n_pix, const double* npix, int n_map,
                       const double* map_pts, const ds_covar* covar,
                       const ds_pcg_params* params, double* sigma,
                       int* num_iters, int* success);
+
+/* ---- learned warm start (no reference counterpart) --------------------
+ * A point network predicts each pixel's solution value x_i from its 32
+ * nearest neighbours (offsets / l, noise / sqrt(var_s), data / sqrt(var_s))
+ * and its own noise and datum; the prediction becomes the initial guess of
+ * the solve (W form: y0 = x0 / w), the x in/out argument the reference's
+ * solve already takes (lib/pcg.h:98, app/dachshund.cc:237-244).  The
+ * stopping rule is unchanged.  Network: 162 -> 64 -> 64 -> 64 -> 1, ReLU,
+ * FP32 weights, BF16 activations; trained on the device by its own forward,
+ * backward and Adam kernels on solutions the solver produced. */
+typedef struct ds_warmstart ds_warmstart;
+int ds_ws_num_params(void);
+int ds_ws_feature_dim(void);
+int ds_ws_create(ds_ctx* ctx, unsigned long long seed, ds_warmstart** out);
+int ds_ws_destroy(ds_warmstart* ws);
+int ds_ws_set_params(ds_warmstart* ws, const float* params, int n);
+int ds_ws_get_params(ds_warmstart* ws, float* params, int n);
+/* BF16 rounding of hidden activations (default on). */
+int ds_ws_set_quantized(ds_warmstart* ws, int on);
+/* Features of every pixel of p, host order, n x ds_ws_feature_dim(). */
+int ds_ws_features(ds_warmstart* ws, ds_problem* p, float* feat_out);
+/* Predict the initial guess into p's resident x (W space for the W form). */
+int ds_ws_predict(ds_warmstart* ws, ds_problem* p);
+/* Add p as a training sample: its features and, as target, its resident x,
+ * which must hold a solution (x = w .* y after a W-form solve).  Samples
+ * stay in device memory, rows shuffled with `seed`. */
+int ds_ws_add_sample(ds_warmstart* ws, ds_problem* p, unsigned long long seed);
+int ds_ws_num_samples(ds_warmstart* ws, long long* count);
+int ds_ws_clear_samples(ds_warmstart* ws);
+/* Adam on the mean squared error over all samples; lr * lr_decay^epoch;
+ * loss_per_epoch (may be NULL) gets the mean loss of each epoch. */
+int ds_ws_train(ds_warmstart* ws, int epochs, int batch, float lr,
+                float lr_decay, unsigned long long seed,
+                double* loss_per_epoch);
+/* Forward and backward on n given feature rows (host, rounded to BF16 on
+ * upload) and targets: predictions, the gradient of the mean squared error
+ * (ds_ws_num_params() floats) and the loss.  For gradient checks. */
+int ds_ws_eval_batch(ds_warmstart* ws, int n, const float* feat,
+                     const float* target, float* pred, float* grad,
+                     double* loss);
 
 #ifdef __cplusplus
 }

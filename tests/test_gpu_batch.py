@@ -1,0 +1,84 @@
+"""Batched solves of independent subvolumes (one launch per kernel for all
+of them, hipGraph-captured iterations, autotuned launch shape) against the
+same subvolumes solved one by one.
+
+Each subvolume's arithmetic and reduction order in a batch are those of the
+single-volume solve with the same launch shape, so results must be
+bit-identical; graph replay must not change them; the autotuner may change
+only the order of sums (solutions then agree to the solve tolerance).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _subvolumes(count=6, seed=4):
+    from dachshund_amd import synth
+    subs = synth.batch_workloads(seed=seed, count=count, n_min=500,
+                                 n_max=6000)
+    return subs, [synth.make_pixels(w) for w in subs]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_bit_identical_to_single(mixed):
+    import dachshund_amd as D
+    subs, pix = _subvolumes()
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    wpix = [D.pixel_to_wpixel(p) for p in pix]
+    batch = D.BatchWF(wpix, sps)
+    batch.set_launch(4, False)
+    n = batch.n
+    prm = D.PCGParams(n, 1000, 50, 1e-6)
+    batch.fill_x(0.0)
+    res = batch.solve(prm, mixed=mixed, use_graph=False)
+    xb = batch.split(batch.get_x())
+    for k, (w, p, sp) in enumerate(zip(subs, wpix, sps)):
+        wf = D.WFVectorW(p, sp)
+        wf.set_launch(4, False)
+        x = np.zeros(len(p))
+        r = (wf.solve_mixed(D.PCGParams(len(p), 1000, 50, 1e-6), x)
+             if mixed else wf.solve_pcg(D.PCGParams(len(p), 1000, 50, 1e-6),
+                                        x))
+        assert res[k].success and r.success
+        assert res[k].num_iters == r.num_iters, k
+        assert np.array_equal(xb[k], x), k
+    # graph replay of the same iterations
+    if mixed:
+        batch.fill_x(0.0)
+        res_g = batch.solve(prm, mixed=True, use_graph=True)
+        assert [r.num_iters for r in res_g] == [r.num_iters for r in res]
+        for a, b in zip(batch.split(batch.get_x()), xb):
+            assert np.array_equal(a, b)
+
+
+def test_batch_meets_fp64_tolerance_per_subvolume(ora):
+    import dachshund_amd as D
+    subs, pix = _subvolumes(count=5, seed=9)
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    batch = D.BatchWF([D.pixel_to_wpixel(p) for p in pix], sps)
+    waves, order, ms = batch.autotune()
+    assert waves in (1, 2, 4) and ms > 0
+    batch.fill_x(0.0)
+    res = batch.solve(D.PCGParams(batch.n, 1000, 50, 1e-6), mixed=True)
+    for k, (w, p, x) in enumerate(zip(subs, pix, batch.split(batch.get_x()))):
+        assert res[k].success
+        fp = ora.to_form(p, ora.FORM_W)
+        c = ora.covar(w.var_s, w.l_perp, w.l_para)
+        r = ora.A_residual(ora.FORM_W, fp, c, x / fp[:, 3])
+        assert np.linalg.norm(r) <= 1.0001e-6 * ora.norm_b(ora.FORM_W, fp), k
+
+
+def test_batch_operator_is_block_diagonal(ora):
+    import dachshund_amd as D
+    subs, pix = _subvolumes(count=3, seed=2)
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    batch = D.BatchWF([D.pixel_to_wpixel(p) for p in pix], sps)
+    a = np.random.default_rng(0).standard_normal(batch.n)
+    b = batch.split(batch.A_product(a))
+    for k, (w, p, ak) in enumerate(zip(subs, pix, batch.split(a))):
+        fp = ora.to_form(p, ora.FORM_W)
+        c = ora.covar(w.var_s, w.l_perp, w.l_para)
+        bound = ora.A_product(ora.FORM_W, fp, c, np.abs(ak))
+        assert np.all(np.abs(b[k] - ora.A_product(ora.FORM_W, fp, c, ak))
+                      <= 1e-12 * bound), k
