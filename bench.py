@@ -16,6 +16,8 @@ is evaluated pixel-pair covariances per second over the timed solves.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                   [--warmstart dachshund_amd/warmstart_c5.npz | none]
+  python bench.py --workload c3    # 32-RHS explicit BF16 block (MFMA)
+  python bench.py --workload c4    # 512 subvolumes, batched
 """
 from __future__ import annotations
 
@@ -87,6 +89,170 @@ def cpu_baseline(pix_w, var_s, l_perp, l_para, n_full, n_mv_full,
     }
 
 
+def bench_block(args):
+    """100,000 pixels, 32 noise realizations of one field: multi-RHS mixed
+    solve whose inner operator is the explicit BF16 S on MFMA (one GEMM
+    per iteration for all right-hand sides), FP64 true residuals; against
+    the matrix-free FP32 inner operator and plain FP64 PCG on the same
+    batch.  Roofline of the GEMM: HBM bytes per launch (S + U + partials)
+    over its time, and MFMA FLOP/s."""
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    w = synth.workload("c3")
+    R = 32
+    t0 = time.perf_counter()
+    pix, ds = synth.make_pixels(w, n_real=R)
+    sets = []
+    for k in range(R):
+        p = pix.copy()
+        p[:, 4] = ds[k]
+        sets.append(D.pixel_to_wpixel(p))
+    n = len(pix)
+    log(f"c3: {R} realizations of {n} pixels in {time.perf_counter() - t0:.1f}"
+        f" s")
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    batch = D.BatchWF(sets, [sp] * R)
+    t = time.perf_counter()
+    batch.build_dense_bf16()
+    t_build = time.perf_counter() - t
+    log(f"explicit BF16 S ({2 * n * n / 1e9:.1f} GB) built in {t_build:.2f} s")
+    prm = D.PCGParams(batch.n, 2000, w.step_r, args.tol)
+
+    def run(mode, k):
+        ts, res = [], None
+        for _ in range(k):
+            batch.fill_x(0.0)
+            t = time.perf_counter()
+            if mode == "dense":
+                batch.set_dense(True)
+                res = batch.solve(prm, mixed=True, use_graph=True)
+            elif mode == "mixed":
+                batch.set_dense(False)
+                res = batch.solve(prm, mixed=True, use_graph=True)
+            else:
+                res = batch.solve(prm, mixed=False, use_graph=False)
+            ts.append(time.perf_counter() - t)
+            assert all(r.success for r in res), mode
+        return float(np.mean(ts)), res
+
+    out = {}
+    for mode in ("dense", "mixed", "fp64"):
+        run(mode, args.warmup)
+        tm, res = run(mode, args.steps)
+        out[mode] = {"s": tm, "iters_max": max(r.num_iters for r in res),
+                     "fp64_true_residuals": max(r.n_true for r in res)}
+        log(f"{mode}: {tm * 1e3:.1f} ms for {R} RHS, iterations <= "
+            f"{out[mode]['iters_max']}")
+        if mode == "dense":
+            x_dense = batch.get_x()
+    # accuracy against tight FP64 solves of every right-hand side
+    batch.fill_x(0.0)
+    batch.solve(D.PCGParams(batch.n, 4000, w.step_r, 1e-10), mixed=False,
+                use_graph=False)
+    x_tight = batch.get_x()
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            for a, b in zip(batch.split(x_dense), batch.split(x_tight))]
+    ms = batch.time_dense(reps=10)
+    # algorithmic bytes of one block product: S once (BF16), the R
+    # directions (BF16) and the R results (FP32)
+    bytes_launch = 2.0 * n * n + 2.0 * R * n + 4.0 * R * n
+    flops = 2.0 * n * n * R
+    gbs = bytes_launch / (ms * 1e-3) / 1e9
+    tfs = flops / (ms * 1e-3) / 1e12
+    log(f"GEMM {ms:.3f} ms: {gbs:.0f} GB/s, {tfs:.0f} TFLOP/s")
+    print(json.dumps({
+        "metric": "multi-RHS PCG time-to-rel-residual 1e-6 (32 RHS, explicit "
+                  "BF16 S on MFMA, FP64 refinement)",
+        "value": R * n * n * (out["dense"]["iters_max"]) / out["dense"]["s"],
+        "unit": "dense-equivalent pair-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": out["dense"]["s"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 S, fp32 accumulate, fp64 vectors",
+        "data": "synthetic (seeded GRF, 32 noise realizations)",
+        "config": {"workload": f"{n} pixels (400 skewers x 250), box "
+                               f"{w.box}, 32 RHS, seed {w.seed}",
+                   "tol": args.tol},
+        "time_to_tol_s": {k: v["s"] for k, v in out.items()},
+        "iterations_max": {k: v["iters_max"] for k, v in out.items()},
+        "s_build_explicit_S": t_build,
+        "x_rel_l2_vs_fp64_tight_max": max(errs),
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_gemm_bf16",
+                     "ms_per_launch": ms, "bytes_per_launch": bytes_launch,
+                     "mfma_tflops": tfs, "mfma_peak_tflops": 2500.0},
+    }), flush=True)
+
+
+def bench_batch(args):
+    """512 subvolumes with log-uniform pixel counts in [2000, 40000] and
+    per-subvolume correlation lengths: one batched mixed solve (every
+    kernel launched once for all subvolumes, hipGraph-captured iterations,
+    autotuned launch shape) against solving the subvolumes one by one."""
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    subs = synth.batch_workloads(seed=4, count=args.count)
+    t0 = time.perf_counter()
+    pix = []
+    for k, w in enumerate(subs):
+        pix.append(D.pixel_to_wpixel(synth.make_pixels(w)))
+        if k % 64 == 0:
+            log(f"subvolume {k}: n={len(pix[-1])}, "
+                f"{time.perf_counter() - t0:.1f} s")
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    batch = D.BatchWF(pix, sps)
+    st = batch.stats()
+    log(f"batch: {batch.n} pixels, setup {st.setup_seconds:.2f} s, "
+        f"{st.pairs_per_matvec:.3e} pairs per batched matvec")
+    waves, order, ms_tuned = batch.autotune()
+    log(f"autotuner: waves {waves}, longest-first {order}, {ms_tuned:.3f} ms")
+    prm = D.PCGParams(batch.n, 2000, 50, args.tol)
+    for _ in range(args.warmup):
+        batch.fill_x(0.0)
+        batch.solve(prm, mixed=True, use_graph=True)
+    ts = []
+    for _ in range(args.steps):
+        batch.fill_x(0.0)
+        t = time.perf_counter()
+        res = batch.solve(prm, mixed=True, use_graph=True)
+        ts.append(time.perf_counter() - t)
+        assert all(r.success for r in res)
+    t_batch = float(np.mean(ts))
+    log(f"batched: {t_batch * 1e3:.1f} ms, iterations <= "
+        f"{max(r.num_iters for r in res)}")
+    # the same subvolumes one after another
+    t_seq = 0.0
+    for p, sp in zip(pix, sps):
+        wf = D.WFVectorW(p, sp)
+        wf.fill_x(0.0)
+        t = time.perf_counter()
+        r = wf.solve_resident(D.PCGParams(wf.n, 2000, 50, args.tol),
+                              mixed=True)
+        t_seq += time.perf_counter() - t
+        wf.close()
+    log(f"one by one: {t_seq * 1e3:.1f} ms")
+    pairs = st.pairs_per_matvec * max(r.num_iters for r in res)
+    print(json.dumps({
+        "metric": "batched PCG time-to-rel-residual 1e-6 (512 subvolumes, "
+                  "one launch per kernel, hipGraph)",
+        "value": pairs / t_batch, "unit": "evaluated pair-evals/s (upper "
+                                          "bound: all subvolumes to the "
+                                          "longest)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pairs + f64 vectors",
+        "data": "synthetic (seeded GRF subvolumes)",
+        "config": {"workload": f"{len(subs)} subvolumes, {batch.n} pixels",
+                   "tol": args.tol},
+        "time_to_tol_s": {"batched": t_batch, "one_by_one": t_seq},
+        "speedup_vs_one_by_one": t_seq / t_batch,
+        "autotuned": {"waves_per_block": waves, "longest_first": order,
+                      "matvec_ms": ms_tuned},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,12 +264,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--warmstart", default=os.path.join(
         ROOT, "dachshund_amd", "warmstart_c5.npz"))
+    ap.add_argument("--count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the "
               "solve is not sharded across GPUs in this build.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return bench_block(args)
+    if args.workload == "c4":
+        return bench_batch(args)
 
     import dachshund_amd as D
     from dachshund_amd import synth

@@ -68,6 +68,10 @@ SIGNATURES = {
                                  C.c_double, C.c_int,
                                  C.POINTER(DsPcgResult)]),
     "ds_problem_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ds_problem_build_dense_bf16": (C.c_int, [_vp]),
+    "ds_problem_set_dense": (C.c_int, [_vp, C.c_int]),
+    "ds_problem_dense_product": (C.c_int, [_vp, _dp, _dp]),
+    "ds_problem_time_dense": (C.c_int, [_vp, C.c_int, _dp]),
     "ds_problem_autotune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip,
                                       _dp]),
     "ds_A_product": (C.c_int, [_vp, C.c_int, _dp, _dp]),

@@ -304,6 +304,12 @@ void problem_free(ds_problem* p) {
   dev_free(p->sub_cv);
   dev_free(p->tcount);
   dev_free(p->done_count);
+  dev_free(p->sub_grp0);
+  dev_free(p->gpart);
+  dev_free(p->gcount);
+  dev_free(p->dense_s);
+  dev_free(p->dense_y);
+  dev_free(p->du16);
   if (p->xu != p->x) dev_free(p->xu);
   if (p->du != p->d) dev_free(p->du);
   p->xu = p->du = nullptr;
@@ -508,6 +514,23 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     for (auto& x : th) x.join();
   }
   p->norm_b = p->sub_norm_b[0];
+  // do all subvolumes share one geometry (coordinates and weights)?
+  p->shared_geometry = true;
+  for (int k = 1; k < n_sub && p->shared_geometry; ++k) {
+    if (sub_n[k] != sub_n[0] || covars[k].var_s != covars[0].var_s ||
+        covars[k].c_perp != covars[0].c_perp ||
+        covars[k].c_para != covars[0].c_para) {
+      p->shared_geometry = false;
+      break;
+    }
+    const double* a = pixels + 5 * (size_t)p->sub_off[k];
+    for (int i = 0; i < sub_n[0] && p->shared_geometry; ++i)
+      for (int c = 0; c < 4; ++c)
+        if (a[5 * (size_t)i + c] != pixels[5 * (size_t)i + c]) {
+          p->shared_geometry = false;
+          break;
+        }
+  }
   p->sub_blk0_h.assign(n_sub + 1, 0);
   for (int k = 0; k < n_sub; ++k)
     p->sub_blk0_h[k + 1] = p->sub_blk0_h[k] + blks[k].nblk;
@@ -635,6 +658,13 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   make_tables(&covars[0], t64, t32);
 
   hipStream_t st = ctx->stream;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
+                 : hipSuccess;
+  };
+  auto up_int = [&](int* dst, const std::vector<int>& v) {
+    return up(dst, v.data(), sizeof(int) * v.size());
+  };
   HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->rhs64, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
@@ -654,6 +684,17 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
   HIP_OR_FAIL(ctx, dev_alloc(&p->done_count, 1));
+  // reduction groups of 64 blocks per subvolume
+  std::vector<int> grp0(n_sub + 1, 0);
+  for (int k = 0; k < n_sub; ++k)
+    grp0[k + 1] = grp0[k] + (blks[k].nblk + 63) / 64;
+  p->ngrp = grp0[n_sub];
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_grp0, n_sub + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gpart, 2 * (size_t)p->ngrp));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gcount, p->ngrp));
+  HIP_OR_FAIL(ctx, hipMemsetAsync(p->gcount, 0, sizeof(unsigned) * p->ngrp,
+                                  st));
+  HIP_OR_FAIL(ctx, up_int(p->sub_grp0, grp0));
   double** vecs[] = {&p->x, &p->r, &p->d, &p->s, &p->q, &p->tmp};
   for (double** v : vecs) HIP_OR_FAIL(ctx, dev_alloc(v, npad));
   if (form == DS_FORM_W) {
@@ -674,10 +715,6 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->host_tmp, sizeof(double) * npad,
                                  hipHostMallocDefault));
   std::memset(p->sc_host, 0, sizeof(ds_cg_scalars) * n_sub);
-  auto up = [&](void* dst, const void* src, size_t bytes) {
-    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
-                 : hipSuccess;
-  };
   HIP_OR_FAIL(ctx, up(p->pos64, hpos.data(), sizeof(double4) * npad));
   HIP_OR_FAIL(ctx, up(p->rhs64, hrhs.data(), sizeof(double) * npad));
   HIP_OR_FAIL(ctx, up(p->diag, hdiag.data(), sizeof(double) * npad));
@@ -1113,6 +1150,99 @@ int ds_problem_time_matvec(ds_problem* p, int prec, int reps,
   HIP_OR_FAIL(p->ctx, e);
   *ms_per_launch = (double)ms / reps;
   return DS_OK;
+}
+
+int ds_problem_build_dense_bf16(ds_problem* p) {
+  if (!p) return DS_E_ARG;
+  if (!p->shared_geometry || p->n_sub > DS_DENSE_R)
+    return fail(p->ctx, DS_E_STATE, "the explicit block path needs at most "
+                "%d subvolumes sharing one geometry", DS_DENSE_R);
+  const int ld = p->npad / p->n_sub;
+  const int mpad = (ld + 127) / 128 * 128;
+  const double bytes = 2.0 * (double)mpad * ld;
+  if (bytes > 2.4e11)
+    return fail(p->ctx, DS_E_NOMEM, "explicit S would need %.1f GB",
+                bytes * 1e-9);
+  hipStream_t st = p->ctx->stream;
+  dev_free(p->dense_s);
+  dev_free(p->dense_y);
+  dev_free(p->du16);
+  p->dense_ld = ld;
+  p->dense_mpad = mpad;
+  HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_s, (size_t)mpad * ld));
+  HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_y,
+                                (size_t)p->dense_split * DS_DENSE_R * mpad));
+  HIP_OR_FAIL(p->ctx, dev_alloc(&p->du16, (size_t)DS_DENSE_R * ld));
+  // unused right-hand-side columns stay zero
+  HIP_OR_FAIL(p->ctx, hipMemsetAsync(p->du16, 0, sizeof(unsigned short) *
+                                     (size_t)DS_DENSE_R * ld, st));
+  HIP_OR_FAIL(p->ctx, ds::dense_build(p, st));
+  HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+  p->use_dense = true;
+  return DS_OK;
+}
+
+int ds_problem_set_dense(ds_problem* p, int on) {
+  if (!p) return DS_E_ARG;
+  if (on && !p->dense_s)
+    return fail(p->ctx, DS_E_STATE, "no explicit block built");
+  p->use_dense = on != 0;
+  return DS_OK;
+}
+
+int ds_problem_time_dense(ds_problem* p, int reps, double* ms_per_launch) {
+  if (!p || !ms_per_launch || reps < 1) return DS_E_ARG;
+  if (!p->dense_s) return fail(p->ctx, DS_E_STATE, "no explicit block built");
+  hipStream_t st = p->ctx->stream;
+  // the timing input: bf16 copy of the resident x (any vector will do);
+  // done_count must read "not finished" for the kernel to run
+  HIP_OR_FAIL(p->ctx, ds::launch_scale(p->x, nullptr, nullptr, p->du32,
+                                       p->npad, st));
+  HIP_OR_FAIL(p->ctx, ds::launch_to_bf16(p->du32, p->du16, p->npad, st));
+  const int zero = 0;
+  HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->done_count, &zero, sizeof(int),
+                                     hipMemcpyHostToDevice, st));
+  hipEvent_t e0, e1;
+  HIP_OR_FAIL(p->ctx, hipEventCreate(&e0));
+  HIP_OR_FAIL(p->ctx, hipEventCreate(&e1));
+  hipError_t e = ds::dense_gemm(p, st);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = ds::dense_gemm(p, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_OR_FAIL(p->ctx, e);
+  *ms_per_launch = (double)ms / reps;
+  return DS_OK;
+}
+
+int ds_problem_dense_product(ds_problem* p, const double* a, double* b) {
+  // b = A a through the explicit BF16 block (every subvolume at once);
+  // the pair of kernels the mixed solver uses, without the CG epilogue's
+  // scalar updates.
+  if (!p || !a || !b) return DS_E_ARG;
+  if (!p->dense_s) return fail(p->ctx, DS_E_STATE, "no explicit block built");
+  hipStream_t st = p->ctx->stream;
+  if (int rc = upload_sorted(p, a, p->d)) return rc;
+  const double* w = p->form == DS_FORM_W ? p->wvec : nullptr;
+  HIP_OR_FAIL(p->ctx, ds::launch_scale(p->d, w, nullptr, p->du32, p->npad,
+                                       st));
+  HIP_OR_FAIL(p->ctx, ds::launch_to_bf16(p->du32, p->du16, p->npad, st));
+  // scalars: not done, zero tickets (the epilogue also computes alpha)
+  std::vector<ds_cg_scalars> h(p->n_sub);
+  std::memset(h.data(), 0, sizeof(ds_cg_scalars) * p->n_sub);
+  const int zero = 0;
+  HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->sc, h.data(),
+                                     sizeof(ds_cg_scalars) * p->n_sub,
+                                     hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->done_count, &zero, sizeof(int),
+                                     hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(p->ctx, ds::dense_gemm(p, st));
+  HIP_OR_FAIL(p->ctx, ds::dense_epilogue(p, st));
+  return download_sorted(p, p->q, b);
 }
 
 }  // extern "C"

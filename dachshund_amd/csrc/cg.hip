@@ -34,17 +34,16 @@ struct UpdArgs {
   const double* diag;
   const double* wvec;     // w (W form) or nullptr
   const int* blk_sub;
-  const int* sub_blk0;
   double* x;
   double* xu;             // w .* x (W form); may alias x
   double* r;
   double* s;
-  double* partials;
+  RedCtx red;
   ds_cg_scalars* sc;
   double* norms;
   int* done_count;
   double drop;
-  int nblk, norms_cap;
+  int norms_cap;
   int only_x, mixed;
 };
 
@@ -70,15 +69,11 @@ __global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
   v[0] = ri * si;
   v[1] = ri * ri;
   block_sum<2>(v, red);
-  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
-  if (!publish_partials<2>(v, a.partials + b0, a.nblk, I - b0, nb,
-                           &sc->counter, &is_last))
+  if (!reduce_subvolume<2>(v, a.red, sv, I, &sc->counter, red, &is_last))
     return;
-  gather_partials<2>(v, a.partials + b0, a.nblk, nb, red);
   if (threadIdx.x == 0)
     finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0], v[1], 0,
                   a.mixed, a.drop, 0, a.done_count);
-  release_ticket(&sc->counter);
 }
 
 struct DirArgs {
@@ -88,8 +83,15 @@ struct DirArgs {
   double* d;
   double* du;
   float* du32;
+  unsigned short* du16;   // BF16 copy for the explicit block path
   ds_cg_scalars* sc;
 };
+
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+  return (unsigned short)(u >> 16);
+}
 
 // lib/pcg.h:61-64: d = s + beta d (and its scaled copies).
 __global__ __launch_bounds__(64) void k_cg_dir(DirArgs a) {
@@ -103,6 +105,14 @@ __global__ __launch_bounds__(64) void k_cg_dir(DirArgs a) {
   const double ui = a.wvec ? a.wvec[i] * di : di;
   if (a.wvec) a.du[i] = ui;
   if (a.du32) a.du32[i] = (float)ui;
+  if (a.du16) a.du16[i] = to_bf16((float)ui);
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_to_bf16(const float* src,
+                                                         unsigned short* dst,
+                                                         int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = to_bf16(src[i]);
 }
 
 // u = w .* v (w may be null: u = v), optional FP32 copy.
@@ -139,6 +149,13 @@ hipError_t launch_scale(const double* v, const double* w, double* u,
   return hipGetLastError();
 }
 
+hipError_t launch_to_bf16(const float* src, unsigned short* dst, int n,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16, vec_grid(n), dim3(kVecThreads), 0, st, src,
+                     dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st) {
   hipLaunchKernelGGL(k_precond, vec_grid(npad), dim3(kVecThreads), 0, st, b,
@@ -162,17 +179,15 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.diag = p->diag;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
   a.blk_sub = p->blk_sub;
-  a.sub_blk0 = p->sub_blk0;
   a.x = p->x;
   a.xu = p->xu;
   a.r = p->r;
   a.s = p->s;
-  a.partials = p->partials;
+  a.red = red_ctx(p);
   a.sc = p->sc;
   a.norms = p->norms;
   a.done_count = p->done_count;
   a.drop = drop;
-  a.nblk = p->nblk;
   a.norms_cap = p->norms_cap;
   a.only_x = only_x;
   a.mixed = mixed;
@@ -188,6 +203,7 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   a.d = p->d;
   a.du = p->du;
   a.du32 = mixed ? p->du32 : nullptr;
+  a.du16 = mixed && p->use_dense ? p->du16 : nullptr;
   a.sc = p->sc;
   hipLaunchKernelGGL(k_cg_dir, dim3(p->nblk), dim3(DS_BS), 0, st, a);
   return hipGetLastError();
@@ -199,7 +215,13 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
   CgEpilogue dq;
   dq.kind = EK_CG_DQ;
   if (mixed) {
-    DS_TRY(launch_matvec32_ex(p, p->du32, p->d, p->q, 1, st));
+    if (p->use_dense) {
+      // explicit BF16 block: q = w .* (S (w .* d)) + d for every RHS at once
+      DS_TRY(dense_gemm(p, st));
+      DS_TRY(dense_epilogue(p, st));
+    } else {
+      DS_TRY(launch_matvec32_ex(p, p->du32, p->d, p->q, 1, st));
+    }
     DS_TRY(launch_update(p, 0, 1, drop, st));
     CgEpilogue tr;
     tr.kind = EK_CG_RES;
@@ -276,6 +298,8 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init, st));
     DS_TRY(launch_scale(p->d, w, w ? p->du : nullptr,
                         mixed ? p->du32 : nullptr, p->npad, st));
+    if (mixed && p->use_dense)
+      DS_TRY(launch_to_bf16(p->du32, p->du16, p->npad, st));
     // Enqueue iterations in chunks; after each chunk the number of finished
     // subvolumes is copied to pinned memory (two slots, alternating) and the
     // host inspects the previous chunk's count while this one runs.

@@ -112,6 +112,60 @@ __device__ __forceinline__ void release_ticket(unsigned* counter) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Where a kernel's per-block partial sums go.  Blocks of a subvolume are
+// grouped by 64; level 1 reduces each group into a group partial (<= 64
+// tickets per counter), level 2 reduces a subvolume's group partials.
+// Slots, groups and summation order depend only on the layout, so results
+// are reproducible.
+struct RedCtx {
+  double* partials;       // NV x nblk
+  double* gpart;          // NV x ngrp
+  unsigned* gcount;       // ngrp tickets
+  const int* sub_blk0;    // first block per subvolume
+  const int* sub_grp0;    // first group per subvolume
+  int nblk, ngrp;
+};
+
+// Two-level deterministic reduction of the block sums v (thread 0 holds
+// them) over subvolume sv's blocks; true (in every thread of the block)
+// for the one block that ends up holding the subvolume's total in v.
+// `red` holds NV * blockDim.x / 64 doubles of LDS.
+template <int NV>
+__device__ __forceinline__ bool reduce_subvolume(double (&v)[NV],
+                                                 const RedCtx& R, int sv,
+                                                 int I, unsigned* sub_counter,
+                                                 double* red, bool* flag) {
+  const int b0 = R.sub_blk0[sv], nb = R.sub_blk0[sv + 1] - b0;
+  const int slot = I - b0, g = slot >> 6;
+  const int gsize = min(64, nb - (g << 6));
+  const int ng = (nb + 63) >> 6;
+  const int G0 = R.sub_grp0[sv];
+  double* pg = R.partials + b0 + (g << 6);
+  if (!publish_partials<NV>(v, pg, R.nblk, slot & 63, gsize, &R.gcount[G0 + g],
+                            flag))
+    return false;
+  gather_partials<NV>(v, pg, R.nblk, gsize, red);
+  release_ticket(&R.gcount[G0 + g]);
+  if (!publish_partials<NV>(v, R.gpart + G0, R.ngrp, g, ng, sub_counter,
+                            flag))
+    return false;
+  gather_partials<NV>(v, R.gpart + G0, R.ngrp, ng, red);
+  release_ticket(sub_counter);
+  return true;
+}
+
+inline RedCtx red_ctx(const ds_problem* p) {
+  RedCtx r;
+  r.partials = p->partials;
+  r.gpart = p->gpart;
+  r.gcount = p->gcount;
+  r.sub_blk0 = p->sub_blk0;
+  r.sub_grp0 = p->sub_grp0;
+  r.nblk = p->nblk;
+  r.ngrp = p->ngrp;
+  return r;
+}
+
 // lib/pcg.h:35-37: alpha = delta / (d . q).
 __device__ __forceinline__ void finalize_alpha(ds_cg_scalars* sc, double dq) {
   sc->dq = dq;

@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_dense_A(const double4* pos, int n,
                                                  int form, int exact,
                                                  ElemArgs e, double* A,
                                                  size_t ld) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  const int i = blockIdx.x;   // rows on x: grid y is limited to 65536
+  const int j = blockIdx.y * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const double4 pi = pos[i], pj = pos[j];
   const double S = pair_covar(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, e, exact);
@@ -140,7 +140,7 @@ hipError_t dense_solve(ds_problem* p, int exact, const double* table_dev,
   rocblas_handle h = nullptr;
   rocblas_status rs = rocblas_create_handle(&h);
   if (rs == rocblas_status_success) rs = rocblas_set_stream(h, st);
-  hipLaunchKernelGGL(k_dense_A, dim3((n + 255) / 256, n), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_dense_A, dim3(n, (n + 255) / 256), dim3(256), 0, st,
                      p->pos64, n, p->form, exact, elem_args(p, table_dev), A,
                      ld);
   e = hipGetLastError();

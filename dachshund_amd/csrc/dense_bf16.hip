@@ -1,0 +1,208 @@
+// dense_bf16.hip -- explicit BF16 covariance block times many right-hand
+// sides on the matrix cores (the multi-RHS inner operator).
+//
+// A batch whose subvolumes share one geometry (several data vectors over the
+// same pixels, e.g. noise realizations) has one S for all of them.  S is
+// built once, explicitly, in BF16 (k_build_s: the FP64 parity interpolant
+// of lib/signal_covar.h:53-77 rounded to nearest even), and the inner matvec
+// of the mixed solver becomes Y = S U for the n x R block U of scaled
+// directions (w .* d per right-hand side, BF16), on MFMA with FP32
+// accumulation (k_gemm_bf16), followed by the per-right-hand-side epilogue
+// q = w .* Y + d (W form) or Y + N .* d (SN form) with the d . q reduction
+// and alpha (k_dense_epi).  FP64 true residuals (the matrix-free parity
+// kernel) keep the stopping rule exact.
+//
+// k_gemm_bf16: 256 threads = 4 waves; a wave owns rows [row0, row0+32) x all
+// R = 32 columns of a K range (split K: blockIdx.y).  Per K-chunk of 64 a
+// lane of half h = lane/32 reads the contiguous run k0 + 32h .. k0 + 32h +
+// 31 of its row of S and of its column of U (the MFMA's k-slots of a half
+// are fed from that run; A and B use the same assignment, so every product
+// S[row][k] U[k][col] of the chunk is formed exactly once).  Each row's 128
+// bytes per chunk are read by two lanes.  Output tile layout of
+// v_mfma_f32_32x32x8bf16_1k: register j of lane l holds row
+// 8*(j/4) + 4*(l/32) + j%4, column l%32.
+// COMPILE WITH -ffp-contract=off (the S build is FP64 parity arithmetic).
+#include "cg_device.h"
+#include "ds_internal.h"
+
+namespace ds {
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kR = DS_DENSE_R;     // right-hand sides per GEMM
+constexpr int kMT = 128;           // rows per workgroup
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// S[i][j] for i < mpad, j < ld (row-major BF16); zero outside [0, n)^2.
+__global__ __launch_bounds__(256) void k_build_s(const double4* pos, int n,
+                                                 int ld, double var_s,
+                                                 double c_perp, double c_para,
+                                                 double dx_inv,
+                                                 const double* table,
+                                                 int table_n,
+                                                 unsigned short* S) {
+  const int i = blockIdx.x;   // rows on x: grid y is limited to 65536
+  const int j0 = (blockIdx.y * blockDim.x + threadIdx.x) * 8;
+  if (j0 >= ld) return;
+  unsigned short v[8];
+  const double4 pi = i < n ? pos[i] : make_double4(0, 0, 0, 0);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int j = j0 + t;
+    double s = 0.0;
+    if (i < n && j < n) {
+      const double4 pj = pos[j];
+      const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const double x_perp_2 = (dx * dx + dy * dy);
+      const double x_para_2 = dz * dz;
+      const double xx = (c_perp * x_perp_2 + c_para * x_para_2) * dx_inv;
+      if (xx < (double)(table_n - 1)) {
+        const int ix = (int)xx;
+        s = var_s * (table[ix] * ((double)(ix + 1) - xx) +
+                     table[ix + 1] * (xx - (double)ix));
+      }
+    }
+    v[t] = bf16_rne((float)s);
+  }
+  uint4 w;
+  w.x = v[0] | ((unsigned)v[1] << 16);
+  w.y = v[2] | ((unsigned)v[3] << 16);
+  w.z = v[4] | ((unsigned)v[5] << 16);
+  w.w = v[6] | ((unsigned)v[7] << 16);
+  *reinterpret_cast<uint4*>(S + (size_t)i * ld + j0) = w;
+}
+
+// Yp[split][col][row] = sum over the split's K range of S[row][k] U[col][k]
+__global__ __launch_bounds__(256) void k_gemm_bf16(
+    const unsigned short* __restrict__ S, const unsigned short* __restrict__ U,
+    float* __restrict__ Yp, int ld, int mpad, int klen,
+    const int* __restrict__ done_count, int n_sub) {
+  if (*done_count >= n_sub) return;   // every right-hand side converged
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 31, h = lane >> 5;
+  const int row0 = blockIdx.x * kMT + wave * 32;
+  const int split = blockIdx.y;
+  const int kbeg = split * klen;
+  const int kend = min(ld, kbeg + klen);
+  const unsigned short* srow = S + (size_t)(row0 + r) * ld + 32 * h;
+  const unsigned short* ucol = U + (size_t)r * ld + 32 * h;
+  v16f acc = {};
+  for (int k0 = kbeg; k0 < kend; k0 += 64) {
+    uint4 a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = reinterpret_cast<const uint4*>(srow + k0)[q];
+      b[q] = reinterpret_cast<const uint4*>(ucol + k0)[q];
+    }
+    const unsigned* au = reinterpret_cast<const unsigned*>(a);
+    const unsigned* bu = reinterpret_cast<const unsigned*>(b);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      v4s av, bv;
+      av[0] = (short)(au[2 * s] & 0xFFFFu);
+      av[1] = (short)(au[2 * s] >> 16);
+      av[2] = (short)(au[2 * s + 1] & 0xFFFFu);
+      av[3] = (short)(au[2 * s + 1] >> 16);
+      bv[0] = (short)(bu[2 * s] & 0xFFFFu);
+      bv[1] = (short)(bu[2 * s] >> 16);
+      bv[2] = (short)(bu[2 * s + 1] & 0xFFFFu);
+      bv[3] = (short)(bu[2 * s + 1] >> 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+  }
+  float* y = Yp + ((size_t)split * kR + r) * mpad + row0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) y[8 * (j >> 2) + 4 * h + (j & 3)] = acc[j];
+}
+
+// One 64-thread block per target block of the batch: right-hand side
+// sv = blk_sub[I], row i = (I - sub_blk0[sv]) * 64 + lane of the shared
+// geometry.  q = w .* Y + d (W) or Y + N .* d (SN); d . q -> alpha.
+struct EpiArgs {
+  const float* Yp;
+  const double4* pos;
+  const double* d;
+  const int* blk_sub;
+  const int* blk_nreal;
+  const int* sub_blk0;
+  double* q;
+  RedCtx red;
+  ds_cg_scalars* sc;
+  int mpad, nsplit, form;
+};
+
+__global__ __launch_bounds__(64) void k_dense_epi(EpiArgs a) {
+  __shared__ double red[1];
+  __shared__ bool is_last;
+  const int I = blockIdx.x;
+  const int sv = a.blk_sub[I];
+  ds_cg_scalars* sc = a.sc + sv;
+  if (sc->done) return;
+  const int b0 = a.sub_blk0[sv];
+  const int lane = threadIdx.x;
+  const int i = (I - b0) * DS_BS + lane;       // row of the shared geometry
+  const size_t g = (size_t)I * DS_BS + lane;   // slot in the batch vector
+  double y = 0.0;
+  for (int s = 0; s < a.nsplit; ++s)
+    y += (double)a.Yp[((size_t)s * kR + sv) * a.mpad + i];
+  const double di = a.d[g];
+  const double wN = a.pos[g].w;
+  double qi = a.form == DS_FORM_W ? wN * y + di : y + wN * di;
+  if (lane >= a.blk_nreal[I]) qi = 0.0;
+  a.q[g] = qi;
+  double v[1] = {di * qi};
+  block_sum<1>(v, red);
+  if (!reduce_subvolume<1>(v, a.red, sv, I, &sc->counter, red, &is_last))
+    return;
+  if (threadIdx.x == 0) finalize_alpha(sc, v[0]);
+}
+
+}  // namespace
+
+hipError_t dense_build(ds_problem* p, hipStream_t st) {
+  const int n1 = p->sub_n[0];
+  const int ld = p->dense_ld, mpad = p->dense_mpad;
+  const dim3 grid(mpad, (ld / 8 + 255) / 256);
+  hipLaunchKernelGGL(k_build_s, grid, dim3(256), 0, st, p->pos64, n1, ld,
+                     p->cv.var_s, p->cv.c_perp, p->cv.c_para, p->cv.dx_inv,
+                     p->table_raw, p->cv.table_n, p->dense_s);
+  return hipGetLastError();
+}
+
+hipError_t dense_gemm(ds_problem* p, hipStream_t st) {
+  const int ld = p->dense_ld;
+  const int klen = ((ld + p->dense_split - 1) / p->dense_split + 63) / 64 * 64;
+  const dim3 grid(p->dense_mpad / kMT, p->dense_split);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, p->dense_s,
+                     p->du16, p->dense_y, ld, p->dense_mpad, klen,
+                     p->done_count, p->n_sub);
+  return hipGetLastError();
+}
+
+hipError_t dense_epilogue(ds_problem* p, hipStream_t st) {
+  EpiArgs a{};
+  a.Yp = p->dense_y;
+  a.pos = p->pos64;
+  a.d = p->d;
+  a.blk_sub = p->blk_sub;
+  a.blk_nreal = p->blk_nreal;
+  a.sub_blk0 = p->sub_blk0;
+  a.q = p->q;
+  a.red = red_ctx(p);
+  a.sc = p->sc;
+  a.mpad = p->dense_mpad;
+  a.nsplit = p->dense_split;
+  a.form = p->form;
+  hipLaunchKernelGGL(k_dense_epi, dim3(p->nblk), dim3(DS_BS), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace ds

@@ -33,6 +33,7 @@
 #define DS_TPB (DS_BS / DS_TS)
 #define DS_MAX_TABLE 1024
 #define DS_MAP_T 4        // map tiles are 4x4x4 cells
+#define DS_DENSE_R 32     // right-hand sides of the explicit BF16 block path
 
 namespace ds {
 
@@ -136,10 +137,21 @@ struct ds_problem {
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
   int* sub_blk0 = nullptr;             // first block per subvolume
+  int* sub_grp0 = nullptr;             // first reduction group per subvolume
+  double* gpart = nullptr;             // group partials, 2 x ngrp
+  unsigned* gcount = nullptr;          // group tickets
+  int ngrp = 0;
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}
   int* tcount = nullptr;               // real slots per tile
   int* done_count = nullptr;           // subvolumes finished
   int* blk_order = nullptr;            // launch order of target blocks
+  bool shared_geometry = false;        // all subvolumes: same pixels/weights
+  // explicit BF16 block path (shared geometry, <= DS_DENSE_R subvolumes)
+  bool use_dense = false;
+  unsigned short* dense_s = nullptr;   // S, dense_mpad x dense_ld, BF16
+  float* dense_y = nullptr;            // split-K partials of S U
+  unsigned short* du16 = nullptr;      // BF16 scaled directions, R x ld
+  int dense_ld = 0, dense_mpad = 0, dense_split = 4;
   double setup_seconds = 0.0;
   int waves_per_block = 4;             // pair-kernel launch shape
   int order_by_cost = 0;               // launch blocks longest list first
@@ -179,6 +191,11 @@ hipError_t launch_precond(const double* b, const double* diag, double* x,
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
                       double drop, ds_pcg_result* res, int use_graph);
 hipError_t launch_fill(double* v, double value, int n, hipStream_t st);
+hipError_t launch_to_bf16(const float* src, unsigned short* dst, int n,
+                          hipStream_t st);
+hipError_t dense_build(ds_problem* p, hipStream_t st);
+hipError_t dense_gemm(ds_problem* p, hipStream_t st);
+hipError_t dense_epilogue(ds_problem* p, hipStream_t st);
 hipError_t launch_column(const ds_problem* p, const double* table_dev,
                          const double* map_pt, double* col, hipStream_t st);
 hipError_t launch_dot1(const double* a, const double* b, int n, double* out,

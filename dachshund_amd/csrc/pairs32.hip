@@ -44,7 +44,7 @@ struct Matvec32Args {
   const double4* sub_cv;
   const int* blk_order;
   double* out;
-  double* partials;
+  RedCtx red;
   ds_cg_scalars* sc;
   float tlast;
   int nblk, table_n;
@@ -119,13 +119,9 @@ __global__ __launch_bounds__(256) void k_matvec32(
     v[0] = wave_sum(ai * bi);
   }
   if (!CG) return;
-  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
-  if (!publish_partials<1>(v, a.partials + b0, a.nblk, I - b0, nb,
-                           &sc->counter, &is_last))
+  if (!reduce_subvolume<1>(v, a.red, sv, I, &sc->counter, red2, &is_last))
     return;
-  gather_partials<1>(v, a.partials + b0, a.nblk, nb, red2);
   if (threadIdx.x == 0) finalize_alpha(sc, v[0]);
-  release_ticket(&sc->counter);
 }
 
 }  // namespace
@@ -149,7 +145,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.sub_blk0 = p->sub_blk0;
   a.sub_cv = p->sub_cv;
   a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
-  a.partials = p->partials;
+  a.red = red_ctx(p);
   a.sc = p->sc;
   a.tlast = (float)(p->cv.table_n - 1);
   a.nblk = p->nblk;

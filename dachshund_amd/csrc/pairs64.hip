@@ -75,7 +75,7 @@ struct MatvecArgs {
   const int* blk_order;   // launch slot -> block (may be null)
   double* out;            // A a, or rhs - A a
   double* out2;           // CG residual: M^-1 r
-  double* partials;
+  RedCtx red;
   ds_cg_scalars* sc;      // per subvolume
   double* norms;          // per subvolume, norms_cap apart
   int* done_count;
@@ -139,25 +139,19 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
     v[0] = wave_sum(v[0]);
     v[1] = wave_sum(v[1]);
   }
-  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
   if (EPI == EPI_CG_DQ) {
     double p1[1] = {v[0]};
-    if (!publish_partials<1>(p1, a.partials + b0, a.nblk, I - b0, nb,
-                             &sc->counter, &is_last))
+    if (!reduce_subvolume<1>(p1, a.red, sv, I, &sc->counter, red2, &is_last))
       return;
-    gather_partials<1>(p1, a.partials + b0, a.nblk, nb, red2);
     if (threadIdx.x == 0) finalize_alpha(sc, p1[0]);
   } else {
     double p2[2] = {v[0], v[1]};
-    if (!publish_partials<2>(p2, a.partials + b0, a.nblk, I - b0, nb,
-                             &sc->counter, &is_last))
+    if (!reduce_subvolume<2>(p2, a.red, sv, I, &sc->counter, red2, &is_last))
       return;
-    gather_partials<2>(p2, a.partials + b0, a.nblk, nb, red2);
     if (threadIdx.x == 0)
       finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, p2[0], p2[1],
                     a.init, a.mixed, a.drop, a.is_true, a.done_count);
   }
-  release_ticket(&sc->counter);
 }
 
 // Map targets: the uniform grid of lib/util.cc:25-46, cut into 4x4x4 tiles;
@@ -270,7 +264,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
   a.out = out;
   a.out2 = e.out2;
-  a.partials = p->partials;
+  a.red = red_ctx(p);
   a.sc = p->sc;
   a.norms = p->norms;
   a.done_count = p->done_count;

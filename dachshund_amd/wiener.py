@@ -385,6 +385,28 @@ class BatchWF(_WFVector):
         check(lib().ds_problem_set_launch(self.h, waves, int(order_by_cost)),
               self.ctx.h)
 
+    def build_dense_bf16(self):
+        """Explicit BF16 S for subvolumes sharing one geometry; the mixed
+        solver then uses one MFMA GEMM for all right-hand sides."""
+        check(lib().ds_problem_build_dense_bf16(self.h), self.ctx.h,
+              "build_dense_bf16")
+
+    def set_dense(self, on):
+        check(lib().ds_problem_set_dense(self.h, int(bool(on))), self.ctx.h)
+
+    def dense_product(self, a):
+        a = self._vec(a)
+        b = np.empty(self.n)
+        check(lib().ds_problem_dense_product(self.h, _dp(a), _dp(b)),
+              self.ctx.h, "dense_product")
+        return b
+
+    def time_dense(self, reps=10):
+        ms = C.c_double()
+        check(lib().ds_problem_time_dense(self.h, reps, C.byref(ms)),
+              self.ctx.h)
+        return ms.value
+
     def autotune(self, prec=DS_PREC_F32, reps=5):
         w, o, ms = C.c_int(), C.c_int(), C.c_double()
         check(lib().ds_problem_autotune(self.h, prec, reps, C.byref(w),

@@ -146,6 +146,21 @@ int ds_problem_set_launch(ds_problem* p, int waves_per_block,
 int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
                         int* order_out, double* ms_out);
 
+/* Explicit BF16 covariance block for a batch whose subvolumes (at most 32)
+ * share one geometry, e.g. several noise realizations of the same pixels
+ * (no reference counterpart; the dense matrix the reference builds only in
+ * solve_cf, lib/wf_vector.cc:276-290).  S is built once on the device in
+ * BF16 (2 * n^2 bytes) and the mixed solver's inner operator becomes one
+ * MFMA GEMM S [w .* d_1 ... w .* d_R] with FP32 accumulation per iteration;
+ * FP64 matrix-free true residuals keep the stopping rule.  set_dense
+ * switches between it and the matrix-free FP32 operator. */
+int ds_problem_build_dense_bf16(ds_problem* p);
+int ds_problem_set_dense(ds_problem* p, int on);
+/* b = A a through the explicit block (all subvolumes). */
+int ds_problem_dense_product(ds_problem* p, const double* a, double* b);
+/* Average duration of the block GEMM (HIP events, problem's stream). */
+int ds_problem_time_dense(ds_problem* p, int reps, double* ms_per_launch);
+
 /* WFVector*::A_product (lib/wf_vector.cc:51-74 SN, :196-220 W): b = A a. */
 int ds_A_product(ds_problem* p, int prec, const double* a, double* b);
 /* WFVector*::A_residual (lib/wf_vector.cc:76-103 SN, :222-251 W):

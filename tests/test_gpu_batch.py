@@ -82,3 +82,60 @@ def test_batch_operator_is_block_diagonal(ora):
         bound = ora.A_product(ora.FORM_W, fp, c, np.abs(ak))
         assert np.all(np.abs(b[k] - ora.A_product(ora.FORM_W, fp, c, ak))
                       <= 1e-12 * bound), k
+
+
+def _realizations(R, seed=21):
+    from dachshund_amd import synth
+    w = synth.workload("small")
+    pix, ds = synth.make_pixels(w, seed=seed, n_real=R)
+    sets = []
+    for k in range(R):
+        p = pix.copy()
+        p[:, 4] = ds[k]
+        sets.append(p)
+    return w, sets
+
+
+def test_dense_bf16_block_product(ora):
+    """S built in BF16 (relative rounding 2^-9) times BF16 directions, FP32
+    accumulation: each row within 8e-3 of sum_j |S_ij u_j| of the FP64
+    product; the identity part is exact."""
+    import dachshund_amd as D
+    w, sets = _realizations(5)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    batch = D.BatchWF([D.pixel_to_wpixel(p) for p in sets], [sp] * len(sets))
+    batch.build_dense_bf16()
+    a = np.random.default_rng(1).standard_normal(batch.n)
+    b = batch.split(batch.dense_product(a))
+    c = ora.covar(w.var_s, w.l_perp, w.l_para)
+    errs = []
+    for k, (p, ak) in enumerate(zip(sets, batch.split(a))):
+        fp = ora.to_form(p, ora.FORM_W)
+        ref = ora.A_product(ora.FORM_W, fp, c, ak)
+        bound = ora.A_product(ora.FORM_W, fp, c, np.abs(ak)) - np.abs(ak)
+        assert np.all(np.abs(b[k] - ref) <= 8e-3 * bound + 1e-12), k
+        errs.append(np.linalg.norm(b[k] - ref) / np.linalg.norm(ref))
+    assert max(errs) < 3e-3, errs
+
+
+def test_dense_bf16_block_solve(ora):
+    """Multi-RHS mixed solve with the explicit BF16 block as inner operator:
+    every right-hand side meets the FP64 tolerance (oracle residual) and
+    matches its own FP64 solve."""
+    import dachshund_amd as D
+    w, sets = _realizations(6, seed=22)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    batch = D.BatchWF([D.pixel_to_wpixel(p) for p in sets], [sp] * len(sets))
+    batch.build_dense_bf16()
+    batch.fill_x(0.0)
+    tol = 1e-6
+    res = batch.solve(D.PCGParams(batch.n, 2000, 50, tol), mixed=True,
+                      use_graph=True)
+    c = ora.covar(w.var_s, w.l_perp, w.l_para)
+    for k, (p, x) in enumerate(zip(sets, batch.split(batch.get_x()))):
+        assert res[k].success and res[k].n_true >= 1, k
+        fp = ora.to_form(p, ora.FORM_W)
+        r = ora.A_residual(ora.FORM_W, fp, c, x / fp[:, 3])
+        assert np.linalg.norm(r) <= 1.0001 * tol * ora.norm_b(ora.FORM_W, fp)
+        xo, ok, _, _ = ora.solve_pcg(ora.FORM_W, fp, c, 5000, 50, 1e-12)
+        assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo), k
