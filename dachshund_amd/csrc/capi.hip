@@ -304,9 +304,6 @@ void problem_free(ds_problem* p) {
   dev_free(p->sub_cv);
   dev_free(p->tcount);
   dev_free(p->done_count);
-  dev_free(p->sub_grp0);
-  dev_free(p->gpart);
-  dev_free(p->gcount);
   dev_free(p->dense_s);
   dev_free(p->dense_y);
   dev_free(p->du16);
@@ -662,9 +659,6 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  auto up_int = [&](int* dst, const std::vector<int>& v) {
-    return up(dst, v.data(), sizeof(int) * v.size());
-  };
   HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->rhs64, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
@@ -684,17 +678,6 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
   HIP_OR_FAIL(ctx, dev_alloc(&p->done_count, 1));
-  // reduction groups of 64 blocks per subvolume
-  std::vector<int> grp0(n_sub + 1, 0);
-  for (int k = 0; k < n_sub; ++k)
-    grp0[k + 1] = grp0[k] + (blks[k].nblk + 63) / 64;
-  p->ngrp = grp0[n_sub];
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_grp0, n_sub + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gpart, 2 * (size_t)p->ngrp));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gcount, p->ngrp));
-  HIP_OR_FAIL(ctx, hipMemsetAsync(p->gcount, 0, sizeof(unsigned) * p->ngrp,
-                                  st));
-  HIP_OR_FAIL(ctx, up_int(p->sub_grp0, grp0));
   double** vecs[] = {&p->x, &p->r, &p->d, &p->s, &p->q, &p->tmp};
   for (double** v : vecs) HIP_OR_FAIL(ctx, dev_alloc(v, npad));
   if (form == DS_FORM_W) {

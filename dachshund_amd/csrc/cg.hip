@@ -38,26 +38,19 @@ struct UpdArgs {
   double* xu;             // w .* x (W form); may alias x
   double* r;
   double* s;
-  RedCtx red;
-  ds_cg_scalars* sc;
-  double* norms;
-  int* done_count;
-  double drop;
-  int norms_cap;
-  int only_x, mixed;
+  double* partials;
+  const ds_cg_scalars* sc;
+  int nblk;
+  int only_x;
 };
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
 __global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
-  __shared__ double red[2];
-  __shared__ bool is_last;
   const int I = blockIdx.x;
-  const int sv = a.blk_sub[I];
-  ds_cg_scalars* sc = a.sc + sv;
+  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
   if (sc->done) return;
   const double alpha = sc->alpha;
   const int i = I * DS_BS + threadIdx.x;
-  double v[2] = {0.0, 0.0};
   const double xi = a.x[i] + alpha * a.d[i];
   a.x[i] = xi;
   if (a.wvec) a.xu[i] = a.wvec[i] * xi;
@@ -66,14 +59,41 @@ __global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
   a.r[i] = ri;
   const double si = ri / a.diag[i];
   a.s[i] = si;
-  v[0] = ri * si;
-  v[1] = ri * ri;
-  block_sum<2>(v, red);
-  if (!reduce_subvolume<2>(v, a.red, sv, I, &sc->counter, red, &is_last))
-    return;
-  if (threadIdx.x == 0)
-    finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0], v[1], 0,
-                  a.mixed, a.drop, 0, a.done_count);
+  double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
+  store_partials<2>(v, a.partials, a.nblk, I);
+}
+
+struct FinArgs {
+  const double* partials;
+  const int* sub_blk0;
+  ds_cg_scalars* sc;
+  double* norms;
+  int* done_count;
+  double drop;
+  int nblk, norms_cap;
+  int init, mixed, is_true, need_want;
+};
+
+// One block per subvolume: sum its blocks' partials in block order and
+// finalize its scalars (alpha for NV = 1, the step for NV = 2).  Skips the
+// subvolumes whose producing kernel skipped them.
+template <int NV>
+__global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
+  __shared__ double red[NV * 4];
+  const int sv = blockIdx.x;
+  ds_cg_scalars* sc = a.sc + sv;
+  if (sc->done) return;
+  if (a.need_want && !sc->want_true) return;
+  const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
+  double v[NV];
+  gather_partials<NV>(v, a.partials + b0, a.nblk, nb, red);
+  if (threadIdx.x != 0) return;
+  if (NV == 1)
+    finalize_alpha(sc, v[0]);
+  else
+    finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0],
+                  v[NV - 1], a.init, a.mixed, a.drop, a.is_true,
+                  a.done_count);
 }
 
 struct DirArgs {
@@ -163,6 +183,28 @@ hipError_t launch_precond(const double* b, const double* diag, double* x,
   return hipGetLastError();
 }
 
+hipError_t launch_finalize(ds_problem* p, const CgEpilogue& e,
+                           hipStream_t st) {
+  FinArgs a{};
+  a.partials = p->partials;
+  a.sub_blk0 = p->sub_blk0;
+  a.sc = p->sc;
+  a.norms = p->norms;
+  a.done_count = p->done_count;
+  a.drop = e.drop;
+  a.nblk = p->nblk;
+  a.norms_cap = p->norms_cap;
+  a.init = e.init;
+  a.mixed = e.mixed;
+  a.is_true = e.is_true;
+  a.need_want = e.need_want;
+  if (e.kind == EK_CG_DQ)
+    hipLaunchKernelGGL(k_finalize<1>, dim3(p->n_sub), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_finalize<2>, dim3(p->n_sub), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 namespace {
 
 #define DS_TRY(expr)                     \
@@ -183,16 +225,18 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.xu = p->xu;
   a.r = p->r;
   a.s = p->s;
-  a.red = red_ctx(p);
+  a.partials = p->partials;
   a.sc = p->sc;
-  a.norms = p->norms;
-  a.done_count = p->done_count;
-  a.drop = drop;
-  a.norms_cap = p->norms_cap;
+  a.nblk = p->nblk;
   a.only_x = only_x;
-  a.mixed = mixed;
   hipLaunchKernelGGL(k_cg_update, dim3(p->nblk), dim3(DS_BS), 0, st, a);
-  return hipGetLastError();
+  DS_TRY(hipGetLastError());
+  if (only_x) return hipSuccess;
+  CgEpilogue e;
+  e.kind = EK_CG_RES;
+  e.mixed = mixed;
+  e.drop = drop;
+  return launch_finalize(p, e, st);
 }
 
 hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
@@ -277,6 +321,7 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
   for (int k = 0; k < ns; ++k) {
     h[k].tol_norm_b = prm->tol * p->sub_norm_b[k];
     h[k].max_iter = max_iter;
+    h[k].refresh_every = prm->step_r;
     // lib/pcg.h:82-86: |b| <= 0 returns at once, x untouched (then x *= w)
     if (p->sub_norm_b[k] <= 0.0) h[k].done = 1;
     else ++live;

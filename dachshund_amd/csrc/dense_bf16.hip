@@ -134,17 +134,15 @@ struct EpiArgs {
   const int* blk_nreal;
   const int* sub_blk0;
   double* q;
-  RedCtx red;
-  ds_cg_scalars* sc;
-  int mpad, nsplit, form;
+  double* partials;
+  const ds_cg_scalars* sc;
+  int mpad, nsplit, form, nblk;
 };
 
 __global__ __launch_bounds__(64) void k_dense_epi(EpiArgs a) {
-  __shared__ double red[1];
-  __shared__ bool is_last;
   const int I = blockIdx.x;
   const int sv = a.blk_sub[I];
-  ds_cg_scalars* sc = a.sc + sv;
+  const ds_cg_scalars* sc = a.sc + sv;
   if (sc->done) return;
   const int b0 = a.sub_blk0[sv];
   const int lane = threadIdx.x;
@@ -158,11 +156,8 @@ __global__ __launch_bounds__(64) void k_dense_epi(EpiArgs a) {
   double qi = a.form == DS_FORM_W ? wN * y + di : y + wN * di;
   if (lane >= a.blk_nreal[I]) qi = 0.0;
   a.q[g] = qi;
-  double v[1] = {di * qi};
-  block_sum<1>(v, red);
-  if (!reduce_subvolume<1>(v, a.red, sv, I, &sc->counter, red, &is_last))
-    return;
-  if (threadIdx.x == 0) finalize_alpha(sc, v[0]);
+  double v[1] = {wave_sum(di * qi)};
+  store_partials<1>(v, a.partials, a.nblk, I);
 }
 
 }  // namespace
@@ -196,13 +191,18 @@ hipError_t dense_epilogue(ds_problem* p, hipStream_t st) {
   a.blk_nreal = p->blk_nreal;
   a.sub_blk0 = p->sub_blk0;
   a.q = p->q;
-  a.red = red_ctx(p);
+  a.partials = p->partials;
   a.sc = p->sc;
   a.mpad = p->dense_mpad;
   a.nsplit = p->dense_split;
   a.form = p->form;
+  a.nblk = p->nblk;
   hipLaunchKernelGGL(k_dense_epi, dim3(p->nblk), dim3(DS_BS), 0, st, a);
-  return hipGetLastError();
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  CgEpilogue dq;
+  dq.kind = EK_CG_DQ;
+  return launch_finalize(p, dq, st);
 }
 
 }  // namespace ds

@@ -102,8 +102,9 @@ struct ds_cg_scalars {
   int max_iter;
   int n_true;          // mixed solver: FP64 true residuals taken
   int want_true;       // mixed solver: next step recomputes r in FP64
-  unsigned counter;    // last-block-done ticket
-  unsigned pad;
+  int last_true;       // mixed solver: iteration of the last true residual
+  int refresh_every;   // mixed solver: true residual at least this often
+  int pad;
 };
 
 struct ds_ctx {
@@ -113,7 +114,6 @@ struct ds_ctx {
   int num_cu = 0;
   double* scratch = nullptr;   // reduction partials
   size_t scratch_n = 0;
-  unsigned* counter = nullptr; // last-block-done ticket for one-off reductions
 };
 
 struct ds_problem {
@@ -137,10 +137,6 @@ struct ds_problem {
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
   int* sub_blk0 = nullptr;             // first block per subvolume
-  int* sub_grp0 = nullptr;             // first reduction group per subvolume
-  double* gpart = nullptr;             // group partials, 2 x ngrp
-  unsigned* gcount = nullptr;          // group tickets
-  int ngrp = 0;
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}
   int* tcount = nullptr;               // real slots per tile
   int* done_count = nullptr;           // subvolumes finished
@@ -224,6 +220,11 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
 hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                               const double* in, double* out, int cg,
                               hipStream_t st);
+// Sum the per-block partials of the kernel just launched and finalize the
+// CG scalars of every subvolume: EK_CG_DQ -> alpha; EK_CG_RES -> delta,
+// beta, |r|, stop rule (init / mixed / is_true / need_want as in e).
+hipError_t launch_finalize(ds_problem* p, const CgEpilogue& e,
+                           hipStream_t st);
 
 // Map-side launchers.  Sources are a blocked pixel set (pos64 / loc32 with
 // w or 1/N in .w), targets the uniform grid of lib/util.cc:25-46 cut into

@@ -44,8 +44,8 @@ struct Matvec32Args {
   const double4* sub_cv;
   const int* blk_order;
   double* out;
-  RedCtx red;
-  ds_cg_scalars* sc;
+  double* partials;
+  const ds_cg_scalars* sc;
   float tlast;
   int nblk, table_n;
 };
@@ -60,11 +60,9 @@ __global__ __launch_bounds__(256) void k_matvec32(
     Matvec32Args a) {
   extern __shared__ float2 tab32[];
   __shared__ double red[4 * 64];
-  __shared__ double red2[4];
-  __shared__ bool is_last;
   const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
   const int sv = a.blk_sub[I];
-  ds_cg_scalars* sc = a.sc + sv;
+  const ds_cg_scalars* sc = a.sc + sv;
   if (CG && sc->done) return;
   for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
     tab32[k] = a.table[k];
@@ -118,10 +116,7 @@ __global__ __launch_bounds__(256) void k_matvec32(
     a.out[row] = bi;
     v[0] = wave_sum(ai * bi);
   }
-  if (!CG) return;
-  if (!reduce_subvolume<1>(v, a.red, sv, I, &sc->counter, red2, &is_last))
-    return;
-  if (threadIdx.x == 0) finalize_alpha(sc, v[0]);
+  if (CG) store_partials<1>(v, a.partials, a.nblk, I);
 }
 
 }  // namespace
@@ -145,7 +140,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.sub_blk0 = p->sub_blk0;
   a.sub_cv = p->sub_cv;
   a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
-  a.red = red_ctx(p);
+  a.partials = p->partials;
   a.sc = p->sc;
   a.tlast = (float)(p->cv.table_n - 1);
   a.nblk = p->nblk;
@@ -165,7 +160,11 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
     else { if (d) DS_L32(DS_FORM_SN, false, true); else DS_L32(DS_FORM_SN, false, false); }
   }
 #undef DS_L32
-  return hipGetLastError();
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess || !cg) return err;
+  CgEpilogue dq;
+  dq.kind = EK_CG_DQ;
+  return launch_finalize(p, dq, st);
 }
 
 }  // namespace ds

@@ -75,25 +75,21 @@ struct MatvecArgs {
   const int* blk_order;   // launch slot -> block (may be null)
   double* out;            // A a, or rhs - A a
   double* out2;           // CG residual: M^-1 r
-  RedCtx red;
-  ds_cg_scalars* sc;      // per subvolume
-  double* norms;          // per subvolume, norms_cap apart
-  int* done_count;
-  double dx_inv, tlast, drop;
-  int nblk, table_n, norms_cap;
+  double* partials;       // per-block sums for the finalize kernel
+  const ds_cg_scalars* sc;  // per subvolume
+  double dx_inv, tlast;
+  int nblk, table_n;
   int residual;           // out = rhs - A a
-  int init, mixed, is_true, need_want;
+  int need_want;          // skip unless the subvolume wants a true residual
 };
 
 template <int FORM, int EPI>
 __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   extern __shared__ double2 tab[];
   __shared__ double red[4 * 64];
-  __shared__ double red2[2 * 4];
-  __shared__ bool is_last;
   const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
   const int sv = a.blk_sub[I];
-  ds_cg_scalars* sc = a.sc + sv;
+  const ds_cg_scalars* sc = a.sc + sv;
   if (EPI != EPI_NONE) {
     if (sc->done) return;
     if (a.need_want && !sc->want_true) return;
@@ -134,23 +130,13 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
       v[1] = bi * bi;  // r . r
     }
   }
-  if (EPI == EPI_NONE) return;
-  if (wave == 0) {
-    v[0] = wave_sum(v[0]);
-    v[1] = wave_sum(v[1]);
-  }
+  if (EPI == EPI_NONE || wave != 0) return;
   if (EPI == EPI_CG_DQ) {
-    double p1[1] = {v[0]};
-    if (!reduce_subvolume<1>(p1, a.red, sv, I, &sc->counter, red2, &is_last))
-      return;
-    if (threadIdx.x == 0) finalize_alpha(sc, p1[0]);
+    double p1[1] = {wave_sum(v[0])};
+    store_partials<1>(p1, a.partials, a.nblk, I);
   } else {
-    double p2[2] = {v[0], v[1]};
-    if (!reduce_subvolume<2>(p2, a.red, sv, I, &sc->counter, red2, &is_last))
-      return;
-    if (threadIdx.x == 0)
-      finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, p2[0], p2[1],
-                    a.init, a.mixed, a.drop, a.is_true, a.done_count);
+    double p2[2] = {wave_sum(v[0]), wave_sum(v[1])};
+    store_partials<2>(p2, a.partials, a.nblk, I);
   }
 }
 
@@ -264,20 +250,13 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
   a.out = out;
   a.out2 = e.out2;
-  a.red = red_ctx(p);
+  a.partials = p->partials;
   a.sc = p->sc;
-  a.norms = p->norms;
-  a.done_count = p->done_count;
   a.dx_inv = p->cv.dx_inv;
   a.tlast = (double)(p->cv.table_n - 1);
-  a.drop = e.drop;
   a.nblk = p->nblk;
   a.table_n = p->cv.table_n;
-  a.norms_cap = p->norms_cap;
   a.residual = residual;
-  a.init = e.init;
-  a.mixed = e.mixed;
-  a.is_true = e.is_true;
   a.need_want = e.need_want;
   const dim3 grid(p->nblk), block(64 * p->waves_per_block);
   const size_t shm = sizeof(double2) * p->cv.table_n;
@@ -292,7 +271,9 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
     if (w) DS_L(DS_FORM_W, EPI_CG_RES); else DS_L(DS_FORM_SN, EPI_CG_RES);
   }
 #undef DS_L
-  return hipGetLastError();
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess || e.kind == EPI_NONE) return err;
+  return launch_finalize(p, e, st);
 }
 
 hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,

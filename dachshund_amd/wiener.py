@@ -366,14 +366,19 @@ class BatchWF(_WFVector):
                                     _dp(self.pixels), covs, C.byref(h)),
               self.ctx.h, "batch_create")
         self.h = h
+        self._dense = False
         self.offsets = np.concatenate([[0], np.cumsum(self.sub_n)])
 
     def split(self, v):
         return [v[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
-    def solve(self, params: PCGParams, mixed=True, refine_drop=1e-3,
+    def solve(self, params: PCGParams, mixed=True, refine_drop=None,
               use_graph=True):
-        """Solve all subvolumes from the resident x; one PCGResult each."""
+        """Solve all subvolumes from the resident x; one PCGResult each.
+        refine_drop defaults to 0.1 with the explicit BF16 block as inner
+        operator (relative error ~2^-9) and 1e-3 with the FP32 one."""
+        if refine_drop is None:
+            refine_drop = 0.1 if self._dense else 1e-3
         res = (DsPcgResult * self.n_sub)()
         prm = params.c_struct()
         check(lib().ds_batch_solve(self.h, C.byref(prm), int(mixed),
@@ -390,9 +395,11 @@ class BatchWF(_WFVector):
         solver then uses one MFMA GEMM for all right-hand sides."""
         check(lib().ds_problem_build_dense_bf16(self.h), self.ctx.h,
               "build_dense_bf16")
+        self._dense = True
 
     def set_dense(self, on):
         check(lib().ds_problem_set_dense(self.h, int(bool(on))), self.ctx.h)
+        self._dense = bool(on)
 
     def dense_product(self, a):
         a = self._vec(a)

@@ -153,7 +153,9 @@ int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
  * BF16 (2 * n^2 bytes) and the mixed solver's inner operator becomes one
  * MFMA GEMM S [w .* d_1 ... w .* d_R] with FP32 accumulation per iteration;
  * FP64 matrix-free true residuals keep the stopping rule.  set_dense
- * switches between it and the matrix-free FP32 operator. */
+ * switches between it and the matrix-free FP32 operator.  The block's
+ * relative error is about 2^-9 (BF16 S and directions), so its solves want
+ * a refine_drop of about 0.1 (1e-3 suits the FP32 operator). */
 int ds_problem_build_dense_bf16(ds_problem* p);
 int ds_problem_set_dense(ds_problem* p, int on);
 /* b = A a through the explicit block (all subvolumes). */
@@ -183,8 +185,9 @@ int ds_solve_pcg(ds_problem* p, const ds_pcg_params* params, double* x,
 /* Mixed-precision solve of the same system (no reference counterpart):
  * FP32 pair evaluation inside PCG with FP64 vectors and dot products; when
  * the recurrence residual has fallen by `refine_drop` since the last FP64
- * true residual, or meets the tolerance, an FP64 true residual replaces it.
- * Stops only when the FP64 true residual meets |r| <= tol |b|.  Same x
+ * true residual, or meets the tolerance, or step_r iterations have passed
+ * since that one, an FP64 true residual replaces it.  Stops only when the
+ * FP64 true residual meets |r| <= tol |b|.  Same x
  * in/out convention as ds_solve_pcg. */
 int ds_solve_mixed(ds_problem* p, const ds_pcg_params* params,
                    double refine_drop, double* x, double* residual_norms,
