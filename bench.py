@@ -335,7 +335,7 @@ def main():
     tm0, rm0 = timed(True, args.steps)
     log(f"mixed, x0 = 0: {np.mean(tm0) * 1e3:.1f} ms, {rm0.num_iters} iters, "
         f"{rm0.n_matvec32} FP32 + {rm0.n_matvec64} FP64 matvecs")
-    t_pred = None
+    t_pred = t_knn = None
     if ws:
         tmx, rmx = timed(True, args.steps, learned=True)
         log(f"mixed + learned: {np.mean(tmx) * 1e3:.1f} ms, {rmx.num_iters} "
@@ -352,6 +352,15 @@ def main():
         t_pred = float(np.mean(tp))
         log(f"learned prediction alone (features + network): "
             f"{t_pred * 1e3:.2f} ms")
+        # one-time neighbour search of a fresh problem (geometry setup,
+        # cached like the tile lists; not inside the timed solves)
+        wf2 = D.WFVectorW(pw, sp)
+        t = time.perf_counter()
+        ws.predict_into(wf2)
+        t_knn = time.perf_counter() - t - t_pred
+        wf2.close()
+        log(f"one-time neighbour search (geometry setup): "
+            f"{t_knn * 1e3:.2f} ms")
 
     # accuracy of the mixed map against a tight FP64 solve
     tight = D.PCGParams(n, 4 * w.max_iter, w.step_r, 1e-10)
@@ -413,6 +422,8 @@ def main():
                        "plain_fp64": r64.num_iters},
         "learned_warm_start": bool(ws),
         "learned_predict_s": t_pred,
+        "knn_setup_s": t_knn,
+        "setup_s": st.setup_seconds,
         "map_rel_l2_vs_fp64_tight": {"headline": rel_mx,
                                      "plain_fp64": rel_64},
         "map_rel_l2_headline_vs_plain_fp64": rel_hp,

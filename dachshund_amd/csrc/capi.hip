@@ -285,6 +285,7 @@ int upload_table64(ds_ctx* ctx, const ds_covar* c, DevBuf<double2>& t) {
 }
 
 void problem_free(ds_problem* p) {
+  dev_free(p->knn_nbr);
   dev_free(p->pos64);
   dev_free(p->rhs64);
   dev_free(p->diag);

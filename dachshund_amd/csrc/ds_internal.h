@@ -136,6 +136,8 @@ struct ds_problem {
   std::vector<int> blk_len_h;          // list length per block
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
+  int* knn_nbr = nullptr;              // learned warm start: K nearest
+                                       // rows per row, built on first use
   int* sub_blk0 = nullptr;             // first block per subvolume
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}
   int* tcount = nullptr;               // real slots per tile

@@ -36,7 +36,7 @@ struct WsTrainer {
   long long step = 0;
 };
 
-hipError_t ws_features(const ds_problem* p, unsigned short* feat,
+hipError_t ws_features(ds_problem* p, unsigned short* feat,
                        hipStream_t st);
 hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
                       int npad, int quant, const ds_problem* p, double* x_out,

@@ -16,8 +16,8 @@
 // (layout [in][out]), activations rounded to BF16 (optional), output
 // t = x_i sqrt(var_s).
 //
-// Kernels: k_knn_features (neighbour search over the problem's culled tile
-// lists), k_fwd (inference), k_fwd_bwd (training forward + per-pixel
+// Kernels: k_knn_scan / k_knn_select (neighbour search over the problem's
+// culled tile lists, once per problem), k_ws_features, k_fwd (inference), k_fwd_bwd (training forward + per-pixel
 // back-propagated deltas), k_wgrad (deterministic chunked weight-gradient
 // sums), k_reduce_chunks, k_adam, k_gather_pred.
 #include <cmath>
@@ -46,22 +46,162 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
 struct KnnArgs {
   const double4* pos;     // {x, y, z, w or N}
   const double* rhs;      // wd (W) or d (SN)
-  const float4* loc;
   const float4* tgt;
   const int* tptr;
   const int* tidx;
-  const float4* toff;
+  float2* cand;           // npad x KNN_CAP kept candidates {d2, j}
+  int* cnt;               // kept candidates per row
+  int* nbr;               // npad x K neighbour rows, -1 = none
   unsigned short* feat;   // npad x WS_FEAT
   int n, form;
   float inv_lp, inv_ll, inv_sqrt_vs, cutoff;
 };
 
-template <int K>
-__global__ __launch_bounds__(64) void k_knn_features(KnnArgs a) {
+constexpr int KNN_BINS = 64;    // histogram bins per refinement level
+constexpr int KNN_LEVELS = 3;   // refinement levels
+constexpr int KNN_CAP = 64;     // candidates kept per row for the selection
+
+// Visit every candidate neighbour of target `row` (block I): slots of the
+// block's listed source tiles below n, other than row itself, whose table
+// argument d2 (FP32, direct differences in the tile frame) is inside the
+// cutoff.  Every pass evaluates d2 with the same explicit operations.  The
+// tile data is wave-uniform and read-only here (restrict: scalar loads);
+// tile slots exist up to npad, those past n are padding.
+template <class F>
+__device__ __forceinline__ void knn_scan(const KnnArgs& a,
+                                         const float4* __restrict__ loc,
+                                         const float4* __restrict__ toff,
+                                         int I, int row, const float4& li,
+                                         F&& visit) {
+  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]);
+  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  for (int e = kb; e < ke; ++e) {
+    const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
+    const float4 o = toff[e];
+    const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
+#pragma unroll
+    for (int jj = 0; jj < DS_TS; ++jj) {
+      const int j = j0 + jj;
+      const float4 pj = loc[j];
+      const float dX = xi - pj.x, dY = yi - pj.y, dZ = zi - pj.z;
+      const float d2 = fmaf(dZ, dZ, fmaf(dY, dY, dX * dX));
+      if (j < a.n && j != row && d2 < a.cutoff) visit(d2, j);
+    }
+  }
+}
+
+__device__ __forceinline__ int knn_bin(float d2, float lo, float scale) {
+  const int b = (int)((d2 - lo) * scale);
+  return min(max(b, 0), KNN_BINS - 1);
+}
+
+// (d, j) < (e, k) in (distance, index) order.
+__device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
+  return d < e || (d == e && (unsigned)j < (unsigned)k);
+}
+
+// Neighbour search, part 1: keep, per target row, a small candidate set
+// that surely holds its K nearest.  A lane never inserts into a top-K list
+// while scanning (the whole wave would run the insertion whenever any lane
+// accepts a candidate).  Instead histogram passes find a threshold: level 0
+// bins d2 over [0, cutoff); the bin where the cumulative count reaches K is
+// split into KNN_BINS sub-bins at the next level while the candidates below
+// the threshold would exceed KNN_CAP.  The accepted set {bin path <=
+// crossing path} is closed downward in d2 and holds at least K candidates
+// (or all of them), ties at the threshold included; an append pass stores
+// it.  More than KNN_CAP survive only when more than about KNN_CAP - K
+// candidates share one FP32 distance after three refinements; then the
+// first KNN_CAP in scan order are kept.
+__global__ __launch_bounds__(64) void k_knn_scan(
+    const float4* __restrict__ loc, const float4* __restrict__ toff,
+    KnnArgs a, int K) {
+  __shared__ unsigned hist[KNN_BINS * 64];   // [bin][lane], 16 KB
   const int lane = threadIdx.x;
   const int I = blockIdx.x;
   const int row = I * DS_BS + lane;
   const float4 li = a.tgt[row];
+
+  float lo[KNN_LEVELS], sc[KNN_LEVELS];
+  int cb[KNN_LEVELS];
+#pragma unroll
+  for (int L = 0; L < KNN_LEVELS; ++L) {
+    lo[L] = 0.0f;
+    sc[L] = 0.0f;
+    cb[L] = 0;
+  }
+  sc[0] = (float)KNN_BINS / a.cutoff;
+  int nlev = 0;          // levels whose crossing bin is fixed
+  bool settled = false;
+  int below = 0;         // candidates strictly below the current domain
+
+  // -1: below the domain of level p (accepted), 0: inside it, 1: above.
+  auto classify = [&](float d2, int p) {
+    int c = 0;
+#pragma unroll
+    for (int L = 0; L < KNN_LEVELS; ++L) {
+      if (L < p && c == 0) {
+        const int b = knn_bin(d2, lo[L], sc[L]);
+        c = b < cb[L] ? -1 : (b > cb[L] ? 1 : 0);
+      }
+    }
+    return c;
+  };
+
+  for (int p = 0; p < KNN_LEVELS; ++p) {
+    if (!__any(!settled)) break;
+    for (int b = 0; b < KNN_BINS; ++b) hist[b * 64 + lane] = 0u;
+    knn_scan(a, loc, toff, I, row, li, [&](float d2, int j) {
+      if (!settled && classify(d2, p) == 0)   // lane-private column: the
+        atomicAdd(&hist[knn_bin(d2, lo[p], sc[p]) * 64 + lane], 1u);
+    });                                       // atomic returns nothing
+    if (settled) continue;
+    int cum = below, cross = -1, before = below;
+    for (int b = 0; b < KNN_BINS; ++b) {
+      const int h = (int)hist[b * 64 + lane];
+      if (cross < 0 && cum + h >= K) {
+        cross = b;
+        before = cum;
+      }
+      cum += h;
+    }
+    if (cross < 0) {   // fewer than K candidates in all: take the domain
+      nlev = p;
+      settled = true;
+      continue;
+    }
+    cb[p] = cross;
+    nlev = p + 1;
+    const int take = before + (int)hist[cross * 64 + lane];
+    if (take <= KNN_CAP || p + 1 == KNN_LEVELS) {
+      settled = true;
+      continue;
+    }
+    below = before;
+    const float w = 1.0f / sc[p];
+    lo[p + 1] = lo[p] + (float)cross * w;
+    sc[p + 1] = (float)KNN_BINS / w;
+  }
+
+  // append: below the crossing path, or inside the final crossing bin (or
+  // the whole final domain when it holds fewer than K)
+  float2* out = a.cand + (size_t)row * KNN_CAP;
+  int cnt = 0;
+  knn_scan(a, loc, toff, I, row, li, [&](float d2, int j) {
+    if (classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
+      out[cnt] = make_float2(d2, __int_as_float(j));
+      ++cnt;
+    }
+  });
+  a.cnt[row] = row < a.n ? cnt : 0;
+}
+
+// Part 2: the K nearest kept candidates of each row in (d2, index) order
+// (replace-worst selection, then an odd-even transposition sort).
+template <int K>
+__global__ __launch_bounds__(64) void k_knn_select(KnnArgs a) {
+  const int row = blockIdx.x * DS_BS + threadIdx.x;
+  const int cnt = a.cnt[row];
+  const float2* c = a.cand + (size_t)row * KNN_CAP;
   float bd[K];
   int bi[K];
 #pragma unroll
@@ -69,43 +209,34 @@ __global__ __launch_bounds__(64) void k_knn_features(KnnArgs a) {
     bd[k] = __builtin_inff();
     bi[k] = -1;
   }
-  float worst = __builtin_inff();
-  const int ke = a.tptr[I + 1];
-  for (int e = a.tptr[I]; e < ke; ++e) {
-    const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
-    const float4 o = a.toff[e];
-    const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
-    for (int jj = 0; jj < DS_TS; ++jj) {
-      const int j = j0 + jj;
-      if (j >= a.n) break;   // tile slots past n are padding (uniform)
-      const float4 pj = a.loc[j];
-      const float dX = xi - pj.x, dY = yi - pj.y, dZ = zi - pj.z;
-      const float d2 = dX * dX + dY * dY + dZ * dZ;
-      // candidates: inside the table cutoff (d2 is the table argument)
-      if (j != row && d2 < worst && d2 < a.cutoff) {
-        // replace the current worst entry, then find the new worst
-        float nw = 0.0f;
-        bool done = false;
+  float wd = __builtin_inff();
+  int wj = -1;
+  for (int s = 0; s < cnt; ++s) {
+    const float2 e = c[s];
+    const float d2 = e.x;
+    const int j = __float_as_int(e.y);
+    if (!knn_less(d2, j, wd, wj)) continue;
+    bool done = false;
+    float nd = -1.0f;
+    int nj = -1;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const bool hit = !done && bd[k] == worst;
-          bd[k] = hit ? d2 : bd[k];
-          bi[k] = hit ? j : bi[k];
-          done = done || hit;
-          nw = fmaxf(nw, bd[k]);
-        }
-        worst = nw;
-      }
+    for (int k = 0; k < K; ++k) {
+      const bool hit = !done && bd[k] == wd && bi[k] == wj;
+      bd[k] = hit ? d2 : bd[k];
+      bi[k] = hit ? j : bi[k];
+      done = done || hit;
+      const bool gt = knn_less(nd, nj, bd[k], bi[k]);
+      nd = gt ? bd[k] : nd;
+      nj = gt ? bi[k] : nj;
     }
+    wd = nd;
+    wj = nj;
   }
-  // sort ascending by (distance, index): odd-even transposition network
 #pragma unroll
   for (int pass = 0; pass < K; ++pass) {
 #pragma unroll
     for (int k = pass & 1; k + 1 < K; k += 2) {
-      const bool sw = bd[k + 1] < bd[k] || (bd[k + 1] == bd[k] &&
-                                            (unsigned)bi[k + 1] <
-                                                (unsigned)bi[k]);
+      const bool sw = knn_less(bd[k + 1], bi[k + 1], bd[k], bi[k]);
       const float td = sw ? bd[k + 1] : bd[k];
       const float tu = sw ? bd[k] : bd[k + 1];
       const int ti = sw ? bi[k + 1] : bi[k];
@@ -116,35 +247,55 @@ __global__ __launch_bounds__(64) void k_knn_features(KnnArgs a) {
       bi[k + 1] = tj;
     }
   }
-  unsigned short* f = a.feat + (size_t)row * WS_FEAT;
+  int* o = a.nbr + (size_t)row * K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) o[k] = bi[k];
+}
+
+// Features of each row from its neighbour list and the current data,
+// staged in LDS so that the block's 64 contiguous feature rows are written
+// out in coalesced dwords.
+template <int K>
+__global__ __launch_bounds__(64) void k_ws_features(KnnArgs a) {
+  __shared__ unsigned short stage[DS_BS * WS_FEAT];   // 20.25 KB
+  const int lane = threadIdx.x;
+  const int I = blockIdx.x;
+  const int row = I * DS_BS + lane;
+  unsigned short* f = stage + lane * WS_FEAT;
   if (row >= a.n) {
     for (int c = 0; c < WS_FEAT; ++c) f[c] = 0;
-    return;
-  }
-  const double4 pi = a.pos[row];
-  auto noise = [&](double wN) {
-    return a.form == DS_FORM_W ? 1.0 / wN : sqrt(wN);
-  };
-  auto datum = [&](int j, double wN) {
-    return a.form == DS_FORM_W ? a.rhs[j] / wN : a.rhs[j];
-  };
+  } else {
+    const double4 pi = a.pos[row];
+    auto noise = [&](double wN) {
+      return a.form == DS_FORM_W ? 1.0 / wN : sqrt(wN);
+    };
+    auto datum = [&](int j, double wN) {
+      return a.form == DS_FORM_W ? a.rhs[j] / wN : a.rhs[j];
+    };
+    const int* nb = a.nbr + (size_t)row * K;
+#pragma unroll 4
+    for (int k = 0; k < K; ++k) {
+      float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      const int j = nb[k];
+      if (j >= 0) {
+        const double4 pj = a.pos[j];
+        v[0] = (float)(pj.x - pi.x) * a.inv_lp;
+        v[1] = (float)(pj.y - pi.y) * a.inv_lp;
+        v[2] = (float)(pj.z - pi.z) * a.inv_ll;
+        v[3] = (float)noise(pj.w) * a.inv_sqrt_vs;
+        v[4] = (float)datum(j, pj.w) * a.inv_sqrt_vs;
+      }
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    const int j = bi[k];
-    if (j >= 0) {
-      const double4 pj = a.pos[j];
-      v[0] = (float)(pj.x - pi.x) * a.inv_lp;
-      v[1] = (float)(pj.y - pi.y) * a.inv_lp;
-      v[2] = (float)(pj.z - pi.z) * a.inv_ll;
-      v[3] = (float)noise(pj.w) * a.inv_sqrt_vs;
-      v[4] = (float)datum(j, pj.w) * a.inv_sqrt_vs;
+      for (int c = 0; c < 5; ++c) f[5 * k + c] = f2bf(v[c]);
     }
-#pragma unroll
-    for (int c = 0; c < 5; ++c) f[5 * k + c] = f2bf(v[c]);
+    f[5 * K] = f2bf((float)noise(pi.w) * a.inv_sqrt_vs);
+    f[5 * K + 1] = f2bf((float)datum(row, pi.w) * a.inv_sqrt_vs);
   }
-  f[5 * K] = f2bf((float)noise(pi.w) * a.inv_sqrt_vs);
-  f[5 * K + 1] = f2bf((float)datum(row, pi.w) * a.inv_sqrt_vs);
+  __syncthreads();
+  const unsigned* src = reinterpret_cast<const unsigned*>(stage);
+  unsigned* dst = reinterpret_cast<unsigned*>(a.feat + (size_t)I * DS_BS *
+                                                           WS_FEAT);
+  for (int k = lane; k < DS_BS * WS_FEAT / 2; k += 64) dst[k] = src[k];
 }
 
 // ---- network ------------------------------------------------------------
@@ -231,18 +382,31 @@ struct FwdArgs {
   double inv_sqrt_vs;     // x0 = t / sqrt(var_s)
 };
 
+// One wave per 64 pixels: the block's feature rows (contiguous) are copied
+// to LDS in coalesced dwords, then each lane runs its pixel through the
+// network with the weights fetched by scalar loads (P is read-only).
 template <bool Q>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_fwd(const float* __restrict__ P,
+                                            FwdArgs a) {
+  __shared__ unsigned short stage[64 * WS_FEAT];
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * 64;
+  const int rows = min(64, a.npad - r0);
+  const unsigned* src =
+      reinterpret_cast<const unsigned*>(a.feat + (size_t)r0 * WS_FEAT);
+  unsigned* dst = reinterpret_cast<unsigned*>(stage);
+  for (int k = lane; k < rows * (WS_FEAT / 2); k += 64) dst[k] = src[k];
+  __syncthreads();
+  const int i = r0 + lane;
   if (i >= a.npad) return;
   if (i >= a.n) {
     if (a.x_out) a.x_out[i] = 0.0;
     if (a.t_out) a.t_out[i] = 0.0f;
     return;
   }
-  const NetPtrs n = net_ptrs(a.P);
+  const NetPtrs n = net_ptrs(P);
   float h1[H], h2[H], h3[H];
-  const float t = forward<Q>(n, a.feat + (size_t)i * WS_FEAT, h1, h2, h3);
+  const float t = forward<Q>(n, stage + lane * WS_FEAT, h1, h2, h3);
   if (a.t_out) a.t_out[i] = t;
   if (a.x_out) {
     const double x0 = (double)t * a.inv_sqrt_vs;
@@ -411,16 +575,14 @@ __global__ __launch_bounds__(256) void k_targets(const double* x, int n,
     if (e_ != hipSuccess) return e_;  \
   } while (0)
 
-hipError_t ws_features(const ds_problem* p, unsigned short* feat,
+hipError_t ws_features(ds_problem* p, unsigned short* feat,
                        hipStream_t st) {
   KnnArgs a{};
   a.pos = p->pos64;
   a.rhs = p->rhs64;
-  a.loc = p->loc32;
   a.tgt = p->tgt32;
   a.tptr = p->tptr;
   a.tidx = p->tidx;
-  a.toff = p->toff32;
   a.feat = feat;
   a.n = p->n;
   a.form = p->form;
@@ -428,7 +590,38 @@ hipError_t ws_features(const ds_problem* p, unsigned short* feat,
   a.inv_ll = (float)std::sqrt(2.0 * p->cv.c_para);
   a.inv_sqrt_vs = (float)(1.0 / std::sqrt(p->cv.var_s));
   a.cutoff = (float)(p->cv.table_n - 1);
-  hipLaunchKernelGGL(k_knn_features<WS_K>, dim3(p->nblk), dim3(64), 0, st, a);
+  if (!p->knn_nbr) {
+    // The neighbour lists depend on the geometry only: built once per
+    // problem, like its tile lists.
+    int* nbr = nullptr;
+    hipError_t e = hipMalloc(&nbr, sizeof(int) * (size_t)p->npad * WS_K);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&a.cand, sizeof(float2) * (size_t)p->npad * KNN_CAP);
+    if (e == hipSuccess) e = hipMalloc(&a.cnt, sizeof(int) * p->npad);
+    a.nbr = nbr;
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_knn_scan, dim3(p->nblk), dim3(64), 0, st,
+                         p->loc32, p->toff32, a, WS_K);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_knn_select<WS_K>, dim3(p->nblk), dim3(64), 0, st,
+                         a);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (a.cand) (void)hipFree(a.cand);
+    if (a.cnt) (void)hipFree(a.cnt);
+    a.cand = nullptr;
+    a.cnt = nullptr;
+    if (e != hipSuccess) {
+      (void)hipFree(nbr);
+      return e;
+    }
+    p->knn_nbr = nbr;
+  }
+  a.nbr = p->knn_nbr;
+  hipLaunchKernelGGL(k_ws_features<WS_K>, dim3(p->nblk), dim3(64), 0, st, a);
   return hipGetLastError();
 }
 
@@ -446,9 +639,9 @@ hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
   a.form = p ? p->form : DS_FORM_SN;
   a.quant = quant;
   a.inv_sqrt_vs = p ? 1.0 / std::sqrt(p->cv.var_s) : 1.0;
-  const dim3 g((npad + 255) / 256), b(256);
-  if (quant) hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, a);
-  else hipLaunchKernelGGL(k_fwd<false>, g, b, 0, st, a);
+  const dim3 g((npad + 63) / 64), b(64);
+  if (quant) hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, P, a);
+  else hipLaunchKernelGGL(k_fwd<false>, g, b, 0, st, P, a);
   return hipGetLastError();
 }
 
