@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libdachshund_gpu.so")
 DS_OK, DS_E_ARG, DS_E_HIP, DS_E_NOMEM, DS_E_STATE = range(5)
 DS_FORM_SN, DS_FORM_W = 0, 1
 DS_PREC_F64, DS_PREC_F32 = 0, 1
+DS_F32_VALU, DS_F32_MFMA = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -68,6 +69,7 @@ SIGNATURES = {
                                  C.c_double, C.c_int,
                                  C.POINTER(DsPcgResult)]),
     "ds_problem_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ds_problem_set_f32_kernel": (C.c_int, [_vp, C.c_int]),
     "ds_problem_build_dense_bf16": (C.c_int, [_vp]),
     "ds_problem_set_dense": (C.c_int, [_vp, C.c_int]),
     "ds_problem_dense_product": (C.c_int, [_vp, _dp, _dp]),

@@ -1078,25 +1078,38 @@ int ds_problem_set_launch(ds_problem* p, int waves, int order_by_cost) {
   return DS_OK;
 }
 
+int ds_problem_set_f32_kernel(ds_problem* p, int kind) {
+  if (!p) return DS_E_ARG;
+  if (kind != DS_F32_VALU && kind != DS_F32_MFMA)
+    return fail(p->ctx, DS_E_ARG, "unknown FP32 kernel kind %d", kind);
+  p->f32_kind = kind;
+  return DS_OK;
+}
+
 int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
                         int* order_out, double* ms_out) {
   if (!p || reps < 1) return DS_E_ARG;
   double best = 1e300;
-  int bw = 4, bo = 0;
-  for (int order = 0; order < 2; ++order)
-    for (int waves : {1, 2, 4}) {
-      p->waves_per_block = waves;
-      p->order_by_cost = order;
-      double ms = 0.0;
-      if (int rc = ds_problem_time_matvec(p, prec, reps, &ms)) return rc;
-      if (ms < best) {
-        best = ms;
-        bw = waves;
-        bo = order;
+  int bw = 4, bo = 0, bk = p->f32_kind;
+  const int nkind = prec == DS_PREC_F32 ? 2 : 1;
+  for (int kind = 0; kind < nkind; ++kind)
+    for (int order = 0; order < 2; ++order)
+      for (int waves : {1, 2, 4}) {
+        p->waves_per_block = waves;
+        p->order_by_cost = order;
+        if (nkind == 2) p->f32_kind = kind;
+        double ms = 0.0;
+        if (int rc = ds_problem_time_matvec(p, prec, reps, &ms)) return rc;
+        if (ms < best) {
+          best = ms;
+          bw = waves;
+          bo = order;
+          bk = p->f32_kind;
+        }
       }
-    }
   p->waves_per_block = bw;
   p->order_by_cost = bo;
+  p->f32_kind = bk;
   if (waves_out) *waves_out = bw;
   if (order_out) *order_out = bo;
   if (ms_out) *ms_out = best;

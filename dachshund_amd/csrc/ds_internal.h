@@ -152,7 +152,7 @@ struct ds_problem {
   int dense_ld = 0, dense_mpad = 0, dense_split = 4;
   double setup_seconds = 0.0;
   int waves_per_block = 4;             // pair-kernel launch shape
-  int order_by_cost = 0;               // launch blocks longest list first
+  int order_by_cost = 1;               // launch blocks longest list first
   // device buffers
   double4* pos64 = nullptr;
   double* rhs64 = nullptr;
@@ -160,6 +160,7 @@ struct ds_problem {
   float4* loc32 = nullptr;             // sources: tile frame, .w = |X|^2
   float4* tgt32 = nullptr;             // targets: block frame
   bool f32_direct = false;             // FP32 kernel uses direct differences
+  int f32_kind = DS_F32_VALU;          // FP32 pair kernel: VALU or MFMA form
   int* tptr = nullptr;
   int* tidx = nullptr;
   float4* toff32 = nullptr;            // per list entry: centre offsets

@@ -21,6 +21,19 @@
 // cancel badly) use direct differences instead (12 operations).  The sum of
 // each lane is FP32; the epilogue (identity or noise term, dot products) is
 // FP64.
+//
+// k_matvec32m forms the same expanded argument on the matrix cores: per
+// source tile and 16-row sub-block s of the target block, one
+// v_mfma_f32_16x16x4f32 computes the 16 x 16 table arguments
+//   xx_ij = |l_i|^2 + [l_i, 1] . [-2 Y_j, |Y_j|^2]
+// with l_i the target in its block frame, Y_j = X_j - o the source moved
+// into that frame (o = the entry's frame offset), A = [l_i, 1] and the
+// accumulator input C = |l_i|^2 fixed per block, and B = one value per lane
+// and tile.  The VALU is left with the clamp, index, fraction, table
+// address and two fma per pair (6 operations).  Lane l = 16 g + c holds
+// A[i = c][k = g] and B[k = g][j = c]; the result D holds rows 4 g + r
+// (r = 0..3) of column c, so each lane accumulates 16 rows x 1 source per
+// tile and the 16 lanes of a group are summed once at the end.
 #include "cg_device.h"
 #include "ds_internal.h"
 
@@ -70,7 +83,7 @@ __global__ __launch_bounds__(256) void k_matvec32(
   const double var_s = a.sub_cv[sv].x;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
   const int row = I * DS_BS + lane;
   const bool real = lane < a.blk_nreal[I];
   const float4 li = a.tgt[row];
@@ -119,6 +132,122 @@ __global__ __launch_bounds__(256) void k_matvec32(
   if (CG) store_partials<1>(v, a.partials, a.nblk, I);
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int FORM, bool CG>
+__global__ __launch_bounds__(256) void k_matvec32m(
+    const float4* __restrict__ loc, const float* __restrict__ u32,
+    const int* __restrict__ tidx, const float4* __restrict__ toff,
+    Matvec32Args a) {
+  extern __shared__ float2 tab32[];
+  __shared__ float red[4 * 64];
+  const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
+  const int sv = a.blk_sub[I];
+  const ds_cg_scalars* sc = a.sc + sv;
+  if (CG && sc->done) return;
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
+    tab32[k] = a.table[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
+  const int g = lane >> 4, c = lane & 15;
+  const int row0 = I * DS_BS;
+  // A operands (target component g, or 1) and accumulator inputs |l|^2
+  float av[4];
+  v4f ci[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const float4 l = a.tgt[row0 + 16 * s + c];
+    av[s] = g == 0 ? l.x : (g == 1 ? l.y : (g == 2 ? l.z : 1.0f));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float4 m = a.tgt[row0 + 16 * s + 4 * g + r];
+      ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x));
+    }
+  }
+  // B = -2 Y_g (g < 3) or |Y|^2 (g = 3), formed without branches
+  const float cx = g == 0 ? -2.0f : 0.0f, cy = g == 1 ? -2.0f : 0.0f;
+  const float cz = g == 2 ? -2.0f : 0.0f, cw = g == 3 ? 1.0f : 0.0f;
+  const float tlast = a.tlast;
+  float acc[4][4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
+  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
+  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  // the next tile's source record and vector value are fetched while the
+  // current tile is processed
+  float4 Xn = make_float4(0.f, 0.f, 0.f, 0.f);
+  float un = 0.0f;
+  if (kb < ke) {
+    const int j0 = __builtin_amdgcn_readfirstlane(tidx[kb]) * DS_TS;
+    Xn = loc[j0 + c];
+    un = u32[j0 + c];
+  }
+  for (int k = kb; k < ke; k += nw) {
+    const float4 o = toff[k];
+    // a target in the tile frame is l + o, so the source in the block
+    // frame is Y = X - o
+    const float4 X = Xn;
+    const float uj = un;
+    if (k + nw < ke) {
+      const int j1 = __builtin_amdgcn_readfirstlane(tidx[k + nw]) * DS_TS;
+      Xn = loc[j1 + c];
+      un = u32[j1 + c];
+    }
+    const float yx = X.x - o.x, yy = X.y - o.y, yz = X.z - o.z;
+    const float y2 = fmaf(yz, yz, fmaf(yy, yy, yx * yx));
+    const float b = fmaf(cx, yx, fmaf(cy, yy, fmaf(cz, yz, cw * y2)));
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], b, ci[s], 0,
+                                                        0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float xx = __builtin_amdgcn_fmed3f(d[r], 0.0f, tlast);
+        const int ix = (int)xx;
+        const float t = __builtin_amdgcn_fractf(xx);
+        const float2 f = tab32[ix];
+        acc[s][r] = fmaf(fmaf(f.y, t, f.x), uj, acc[s][r]);
+      }
+    }
+  }
+  // sum over the 16 source columns of each lane group
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float v = acc[s][r];
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) v += __shfl_xor(v, m, 64);
+      acc[s][r] = v;
+    }
+  if (c == 0)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        red[wave * 64 + 16 * s + 4 * g + r] = acc[s][r];
+  __syncthreads();
+  if (wave != 0) return;
+  const int row = row0 + lane;
+  double y = 0.0;
+  for (int w = 0; w < nw; ++w) y += (double)red[w * 64 + lane];
+  const double var_s = a.sub_cv[sv].x;
+  const double ai = a.in[row];
+  const double wi = a.pos[row].w;
+  double bi = (FORM == DS_FORM_W) ? wi * (var_s * y) + ai
+                                  : var_s * y + wi * ai;
+  if (lane >= a.blk_nreal[I]) bi = 0.0;
+  a.out[row] = bi;
+  if (CG) {
+    double v[1] = {wave_sum(ai * bi)};
+    store_partials<1>(v, a.partials, a.nblk, I);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
@@ -152,7 +281,14 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   hipLaunchKernelGGL((k_matvec32<F, C, D>), grid, block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
   const bool d = p->f32_direct;
-  if (cg) {
+  if (!d && p->f32_kind == DS_F32_MFMA) {
+#define DS_L32M(F, C) \
+  hipLaunchKernelGGL((k_matvec32m<F, C>), grid, block, shm, st, a.loc, a.u32, \
+                     a.tidx, a.toff, a)
+    if (cg) { if (w) DS_L32M(DS_FORM_W, true); else DS_L32M(DS_FORM_SN, true); }
+    else { if (w) DS_L32M(DS_FORM_W, false); else DS_L32M(DS_FORM_SN, false); }
+#undef DS_L32M
+  } else if (cg) {
     if (w) { if (d) DS_L32(DS_FORM_W, true, true); else DS_L32(DS_FORM_W, true, false); }
     else { if (d) DS_L32(DS_FORM_SN, true, true); else DS_L32(DS_FORM_SN, true, false); }
   } else {

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   const double var_s = cvs.x;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
   const int row = I * DS_BS + lane;
   const bool real = lane < a.blk_nreal[I];
   const double4 pi = a.pos[row];
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_map64(MapArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
   const int T = blockIdx.x;
   const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
   const int ix = tix * DS_MAP_T + (lane >> 4);
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_points64(PointsArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
   const int I = blockIdx.x;
   const int slot = I * DS_BS + lane;
   const double4 ti = a.tgt[slot];

@@ -16,8 +16,9 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (DS_FORM_SN, DS_FORM_W, DS_PREC_F32, DS_PREC_F64, DsCovar,
-                   DsPcgParams, DsPcgResult, DsProblemStats, check, lib)
+from ._lib import (DS_F32_MFMA, DS_F32_VALU, DS_FORM_SN, DS_FORM_W,
+                   DS_PREC_F32, DS_PREC_F64, DsCovar, DsPcgParams,
+                   DsPcgResult, DsProblemStats, check, lib)
 
 DEFAULT_TABLE_X_MAX = 4.0   # lib/signal_covar.h:7
 DEFAULT_TABLE_N = 200       # lib/signal_covar.h:8
@@ -303,6 +304,10 @@ class _WFVector:
         check(lib().ds_problem_set_launch(self.h, waves, int(order_by_cost)),
               self.ctx.h)
 
+    def set_f32_kernel(self, kind):
+        """DS_F32_MFMA (default) or DS_F32_VALU form of the FP32 kernel."""
+        check(lib().ds_problem_set_f32_kernel(self.h, kind), self.ctx.h)
+
     def solve_resident(self, params: PCGParams, mixed=False,
                        refine_drop=1e-3):
         res = DsPcgResult()
@@ -389,6 +394,9 @@ class BatchWF(_WFVector):
     def set_launch(self, waves, order_by_cost):
         check(lib().ds_problem_set_launch(self.h, waves, int(order_by_cost)),
               self.ctx.h)
+
+    def set_f32_kernel(self, kind):
+        check(lib().ds_problem_set_f32_kernel(self.h, kind), self.ctx.h)
 
     def build_dense_bf16(self):
         """Explicit BF16 S for subvolumes sharing one geometry; the mixed

@@ -34,6 +34,10 @@ typedef enum {
 } ds_status;
 
 enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
+/* FP32 pair kernel: table argument expanded on the VALU (the default), or
+ * on the matrix cores (v_mfma_f32_16x16x4f32).  Problems with points far
+ * from their tile frame always use direct differences on the VALU. */
+enum { DS_F32_VALU = 0, DS_F32_MFMA = 1 };
 
 /* Arithmetic of the pair evaluation.
  * DS_PREC_F64: the reference's FP64 table interpolation; the table argument
@@ -137,12 +141,15 @@ int ds_batch_num_sub(const ds_problem* p);
  * a hipGraph and replay it (mixed solver). */
 int ds_batch_solve(ds_problem* p, const ds_pcg_params* params, int mixed,
                    double refine_drop, int use_graph, ds_pcg_result* res);
-/* Pair-kernel launch shape: waves per block (1, 2, 4) and whether blocks
- * are launched longest tile list first.  Changes only the order of sums. */
+/* Pair-kernel launch shape: waves per block (1, 2, 4; changes the order of
+ * a row's floating-point sums) and whether blocks are launched longest tile
+ * list first (the default; changes no result). */
 int ds_problem_set_launch(ds_problem* p, int waves_per_block,
                           int order_by_cost);
-/* Time every launch shape on this problem's pair kernel (prec) and keep
- * the fastest. */
+/* FP32 pair kernel variant (DS_F32_VALU or DS_F32_MFMA). */
+int ds_problem_set_f32_kernel(ds_problem* p, int kind);
+/* Time every launch shape (and, for FP32, both kernel variants) on this
+ * problem's pair kernel (prec) and keep the fastest. */
 int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
                         int* order_out, double* ms_out);
 

@@ -88,6 +88,49 @@ def test_matvec64_residual_precond(D, ora, case, form, params):
                   _cutoff_allowance(fp, form, params, a))
 
 
+@pytest.mark.parametrize("kind", ["valu", "mfma"])
+@pytest.mark.parametrize("form", ["w", "sn"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_matvec32_kernels(D, ora, case, form, kind):
+    """Both FP32 pair kernels (table argument expanded on the VALU or on the
+    matrix cores) meet the FP32 row bound against the oracle."""
+    pix = CASES[case]
+    f = FORMS[form]
+    params = (0.05, 2.5, 2.1)
+    fp = ora.to_form(pix, f)
+    c = ora.covar(*params)
+    wf, _ = _problem(D, form, pix, params)
+    wf.set_f32_kernel(D.DS_F32_VALU if kind == "valu" else D.DS_F32_MFMA)
+    a = np.random.default_rng(len(pix) + 7).standard_normal(len(pix))
+    bound = ora.A_product(f, fp, c, np.abs(a))
+    b32 = wf.A_product(a, prec=D.DS_PREC_F32)
+    ref = ora.A_product(f, fp, c, a)
+    assert np.all(np.abs(b32 - ref) <= 2e-5 * bound +
+                  _cutoff_allowance(fp, form, params, a))
+
+
+@pytest.mark.parametrize("kind", ["valu", "mfma"])
+def test_matvec32_per_pair(D, ora, kind):
+    """Per pair: A e_j is column j of A, so each entry is one pair's
+    covariance.  FP32 covariance error <= 2e-6 var_s per pair (SN form, so
+    the entry is S_ij itself) away from the cutoff, and the diagonal
+    var_s + N_j."""
+    pix = CASES["small"]
+    params = (0.05, 2.5, 2.1)
+    fp = ora.to_form(pix, 0)
+    c = ora.covar(*params)
+    wf, _ = _problem(D, "sn", pix, params)
+    wf.set_f32_kernel(D.DS_F32_VALU if kind == "valu" else D.DS_F32_MFMA)
+    n = len(pix)
+    for j in np.random.default_rng(3).choice(n, 6, replace=False):
+        e = np.zeros(n)
+        e[j] = 1.0
+        col = wf.A_product(e, prec=D.DS_PREC_F32)
+        ref = ora.A_product(0, fp, c, e)
+        allow = _cutoff_allowance(fp, "sn", params, e)
+        assert np.all(np.abs(col - ref) <= 2e-6 * params[0] + allow), j
+
+
 def _cutoff_allowance(fp, form, params, a):
     """Per row: var_s e^-8 * weight_i * sum over pairs within 1e-3 of the
     table cutoff (argument 199 in table units) of weight_j |a_j|."""
