@@ -45,6 +45,7 @@ class DsProblemStats(C.Structure):
     _fields_ = [("n", C.c_int), ("n_blocks", C.c_int), ("n_tiles", C.c_int),
                 ("tile_pairs", C.c_longlong),
                 ("pairs_per_matvec", C.c_longlong),
+                ("sub_pairs_per_matvec", C.c_longlong),
                 ("setup_seconds", C.c_double)]
 
 

@@ -294,6 +294,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->tgt32);
   dev_free(p->tptr);
   dev_free(p->tidx);
+  dev_free(p->tmask);
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
@@ -482,6 +483,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   p->sub_norm_b.assign(n_sub, 0.0);
   std::vector<Blocking> blks(n_sub);
   std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
+  std::vector<std::vector<unsigned char>> msks(n_sub);
   auto build = [&](int k) {
     const ds::CovarDev cv = ds::make_covar(&covars[k]);
     p->sub_cv_h[k] = cv;
@@ -489,8 +491,10 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     axis_scale(cv, sc);
     const double* pk = pixels + 5 * (size_t)p->sub_off[k];
     ds::build_blocking(pk, 5, sub_n[k], sc, blks[k]);
+    // a block's 16-row sub-blocks are its own tiles (targets = sources)
     ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
-                         blks[k], cv, ptrs[k], idxs[k]);
+                         blks[k], cv, ptrs[k], idxs[k], blks[k].tlo.data(),
+                         blks[k].thi.data(), &msks[k]);
     // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
     double sum_b2 = 0.0;
     for (int i = 0; i < sub_n[k]; ++i) {
@@ -545,13 +549,18 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       ptr[B0 + I + 1] = ptr[B0 + I] + (ptrs[k][I + 1] - ptrs[k][I]);
   }
   idx.resize(ptr[nblk]);
+  std::vector<unsigned char> msk(ptr[nblk]);
   for (int k = 0; k < n_sub; ++k) {
     const int B0 = p->sub_blk0_h[k];
     for (int I = 0; I < blks[k].nblk; ++I)
-      for (int e = ptrs[k][I]; e < ptrs[k][I + 1]; ++e)
+      for (int e = ptrs[k][I]; e < ptrs[k][I + 1]; ++e) {
         idx[ptr[B0 + I] + (e - ptrs[k][I])] = idxs[k][e] + B0 * DS_TPB;
+        msk[ptr[B0 + I] + (e - ptrs[k][I])] = msks[k][e];
+      }
   }
   p->ntile_pairs = (long long)idx.size();
+  p->nsub_pairs = 0;
+  for (unsigned char m : msk) p->nsub_pairs += __builtin_popcount(m);
   p->blk_len_h.resize(nblk);
   for (int I = 0; I < nblk; ++I) p->blk_len_h[I] = ptr[I + 1] - ptr[I];
 
@@ -668,6 +677,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->tgt32, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tptr, ptr.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tidx, idx.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tmask, std::max<size_t>(msk.size(), 1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table64, t64.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table32, t32.size()));
@@ -707,6 +717,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->tgt32, htgt.data(), sizeof(float4) * npad));
   HIP_OR_FAIL(ctx, up(p->tptr, ptr.data(), sizeof(int) * ptr.size()));
   HIP_OR_FAIL(ctx, up(p->tidx, idx.data(), sizeof(int) * idx.size()));
+  HIP_OR_FAIL(ctx, up(p->tmask, msk.data(), msk.size()));
   HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
@@ -763,6 +774,7 @@ int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out) {
   out->n_tiles = p->ntile;
   out->tile_pairs = p->ntile_pairs;
   out->pairs_per_matvec = p->ntile_pairs * (long long)(DS_BS * DS_TS);
+  out->sub_pairs_per_matvec = p->nsub_pairs * (long long)(DS_TS * DS_TS);
   out->setup_seconds = p->setup_seconds;
   return DS_OK;
 }

@@ -68,9 +68,15 @@ void build_blocking(const double* pts, int stride, int n, const double scale[3],
 
 // Exact culling: for each target box keep the source tiles that may hold a
 // pair with table index < table_n - 1 under the reference's arithmetic.
+// With sublo/subhi (DS_TPB sub-boxes per target, 3 x DS_TPB x ntgt) and
+// mask, each kept entry also gets the bit mask of the sub-boxes that pass
+// the same test, and entries no sub-box passes are dropped.
 void build_tile_lists(int ntgt, const double* tlo, const double* thi,
                       const Blocking& src, const CovarDev& cv,
-                      std::vector<int>& ptr, std::vector<int>& idx);
+                      std::vector<int>& ptr, std::vector<int>& idx,
+                      const double* sublo = nullptr,
+                      const double* subhi = nullptr,
+                      std::vector<unsigned char>* mask = nullptr);
 
 // Lower bound of the table argument xx for any pair drawn from two boxes,
 // with the operation order of lib/signal_covar.h:67-77 and
@@ -125,6 +131,7 @@ struct ds_problem {
   ds::Blocking blk;                    // one-subvolume problems: its blocking
   std::vector<int> perm;               // slot -> caller index, -1 for padding
   long long ntile_pairs = 0;           // (block, tile) pairs after culling
+  long long nsub_pairs = 0;            // (16-row sub-block, tile) pairs
   double norm_b = 0.0;                 // |b| of subvolume 0
   // subvolumes
   int n_sub = 1;
@@ -136,6 +143,7 @@ struct ds_problem {
   std::vector<int> blk_len_h;          // list length per block
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
+  unsigned char* tmask = nullptr;      // per list entry: live sub-blocks
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
   int* sub_blk0 = nullptr;             // first block per subvolume
@@ -160,7 +168,7 @@ struct ds_problem {
   float4* loc32 = nullptr;             // sources: tile frame, .w = |X|^2
   float4* tgt32 = nullptr;             // targets: block frame
   bool f32_direct = false;             // FP32 kernel uses direct differences
-  int f32_kind = DS_F32_VALU;          // FP32 pair kernel: VALU or MFMA form
+  int f32_kind = DS_F32_MFMA;          // FP32 pair kernel: MFMA or VALU form
   int* tptr = nullptr;
   int* tidx = nullptr;
   float4* toff32 = nullptr;            // per list entry: centre offsets

@@ -159,13 +159,17 @@ double box_gap_xx(const double* alo, const double* ahi, const double* blo,
 
 void build_tile_lists(int ntgt, const double* tlo, const double* thi,
                       const Blocking& src, const CovarDev& cv,
-                      std::vector<int>& ptr, std::vector<int>& idx) {
+                      std::vector<int>& ptr, std::vector<int>& idx,
+                      const double* sublo, const double* subhi,
+                      std::vector<unsigned char>* mask) {
   // The reference drops a pair when (int)xx > table_n - 2, i.e. when
   // xx >= table_n - 1.  The box gap is a lower bound of every pair's
   // coordinate differences; a relative margin of 1e-12 covers the few ulps
   // by which the bound's own rounding may exceed a pair's.
   const double limit = (double)(cv.table_n - 1) * (1.0 + 1e-12);
+  const bool sub = sublo && subhi && mask;
   std::vector<std::vector<int>> lists(ntgt);
+  std::vector<std::vector<unsigned char>> masks(sub ? ntgt : 0);
   parallel_for(ntgt, [&](int I) {
     std::vector<int>& L = lists[I];
     const double* alo = tlo + 3 * (size_t)I;
@@ -173,9 +177,24 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
     if (!(alo[0] <= ahi[0])) return;  // empty target
     for (int J = 0; J < src.ntile; ++J) {
       if (src.tcount[J] == 0) continue;
-      const double xx = box_gap_xx(alo, ahi, &src.tlo[3 * (size_t)J],
-                                   &src.thi[3 * (size_t)J], cv);
-      if (xx < limit) L.push_back(J);
+      const double* jlo = &src.tlo[3 * (size_t)J];
+      const double* jhi = &src.thi[3 * (size_t)J];
+      const double xx = box_gap_xx(alo, ahi, jlo, jhi, cv);
+      if (!(xx < limit)) continue;
+      if (sub) {
+        // the same test for each 16-row sub-block; an entry none of them
+        // passes holds only pairs beyond the cutoff
+        unsigned m = 0;
+        for (int s = 0; s < DS_TPB; ++s) {
+          const double* slo = sublo + 3 * ((size_t)I * DS_TPB + s);
+          const double* shi = subhi + 3 * ((size_t)I * DS_TPB + s);
+          if (!(slo[0] <= shi[0])) continue;   // empty sub-block
+          if (box_gap_xx(slo, shi, jlo, jhi, cv) < limit) m |= 1u << s;
+        }
+        if (!m) continue;
+        masks[I].push_back((unsigned char)m);
+      }
+      L.push_back(J);
     }
   });
   ptr.assign(ntgt + 1, 0);
@@ -183,6 +202,11 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
   idx.resize(ptr[ntgt]);
   for (int I = 0; I < ntgt; ++I)
     std::copy(lists[I].begin(), lists[I].end(), idx.begin() + ptr[I]);
+  if (sub) {
+    mask->resize(ptr[ntgt]);
+    for (int I = 0; I < ntgt; ++I)
+      std::copy(masks[I].begin(), masks[I].end(), mask->begin() + ptr[I]);
+  }
 }
 
 }  // namespace ds

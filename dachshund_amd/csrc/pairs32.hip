@@ -49,6 +49,7 @@ struct Matvec32Args {
   const double* in;       // FP64 input, own rows
   const int* tptr;
   const int* tidx;
+  const unsigned char* tmask;   // per list entry: live 16-row sub-blocks
   const float4* toff;     // per list entry: block frame - tile frame
   const float2* table;    // {f[k], f[k+1]-f[k]}, last = {0, 0}
   const int* blk_sub;
@@ -166,7 +167,6 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x));
     }
   }
-  // B = -2 Y_g (g < 3) or |Y|^2 (g = 3), formed without branches
   const float cx = g == 0 ? -2.0f : 0.0f, cy = g == 1 ? -2.0f : 0.0f;
   const float cz = g == 2 ? -2.0f : 0.0f, cw = g == 3 ? 1.0f : 0.0f;
   const float tlast = a.tlast;
@@ -177,8 +177,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
-  // the next tile's source record and vector value are fetched while the
-  // current tile is processed
+  // the next entry's source record and vector value are fetched while the
+  // current one is processed
   float4 Xn = make_float4(0.f, 0.f, 0.f, 0.f);
   float un = 0.0f;
   if (kb < ke) {
@@ -187,9 +187,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     un = u32[j0 + c];
   }
   for (int k = kb; k < ke; k += nw) {
+    const unsigned live = __builtin_amdgcn_readfirstlane(a.tmask[k]);
     const float4 o = toff[k];
-    // a target in the tile frame is l + o, so the source in the block
-    // frame is Y = X - o
     const float4 X = Xn;
     const float uj = un;
     if (k + nw < ke) {
@@ -197,11 +196,17 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       Xn = loc[j1 + c];
       un = u32[j1 + c];
     }
+    // a target in the tile frame is l + o, so the source in the block
+    // frame is Y = X - o; B = -2 Y_g (g < 3) or |Y|^2 (g = 3), formed
+    // without branches
     const float yx = X.x - o.x, yy = X.y - o.y, yz = X.z - o.z;
     const float y2 = fmaf(yz, yz, fmaf(yy, yy, yx * yx));
     const float b = fmaf(cx, yx, fmaf(cy, yy, fmaf(cz, yz, cw * y2)));
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
+      // sub-blocks whose box is beyond the cutoff of the tile's box hold
+      // only zero pairs (wave-uniform skip)
+      if (!(live & (1u << s))) continue;
       const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], b, ci[s], 0,
                                                         0, 0);
 #pragma unroll
@@ -261,6 +266,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.in = in;
   a.tptr = p->tptr;
   a.tidx = p->tidx;
+  a.tmask = p->tmask;
   a.toff = p->toff32;
   a.table = p->table32;
   a.out = out;

@@ -34,9 +34,10 @@ typedef enum {
 } ds_status;
 
 enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
-/* FP32 pair kernel: table argument expanded on the VALU (the default), or
- * on the matrix cores (v_mfma_f32_16x16x4f32).  Problems with points far
- * from their tile frame always use direct differences on the VALU. */
+/* FP32 pair kernel: table argument expanded on the VALU, or on the matrix
+ * cores (v_mfma_f32_16x16x4f32, with 16-row sub-block culling; the
+ * default).  Problems with points far from their tile frame always use
+ * direct differences on the VALU. */
 enum { DS_F32_VALU = 0, DS_F32_MFMA = 1 };
 
 /* Arithmetic of the pair evaluation.
@@ -89,6 +90,8 @@ typedef struct {
   int n_tiles;               /* source tiles of 16 slots */
   long long tile_pairs;      /* (block, tile) pairs kept after the cutoff */
   long long pairs_per_matvec;/* evaluated pairs per matvec = tile_pairs*1024 */
+  long long sub_pairs_per_matvec; /* the same with 16-row sub-block culling
+                                     (pairs the MFMA FP32 kernel evaluates) */
   double setup_seconds;      /* host time of ordering, culling and upload */
 } ds_problem_stats;
 
