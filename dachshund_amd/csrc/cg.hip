@@ -24,10 +24,11 @@ namespace ds {
 namespace {
 
 constexpr int kVecThreads = 256;
+constexpr int kVecWaves = 4;   // target blocks per CG vector workgroup
 
-// The CG vector kernels run one 64-thread block per target block, so every
-// block belongs to one subvolume and its reduction group is that
-// subvolume's blocks.
+// The CG vector kernels run one wave per target block (kVecWaves waves per
+// workgroup), so every wave belongs to one subvolume and stores the partial
+// sums of its target block.
 struct UpdArgs {
   const double* d;
   const double* q;
@@ -45,12 +46,13 @@ struct UpdArgs {
 };
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
-__global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
-  const int I = blockIdx.x;
+__global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
+  const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
+  if (I >= a.nblk) return;
   const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
   if (sc->done) return;
   const double alpha = sc->alpha;
-  const int i = I * DS_BS + threadIdx.x;
+  const int i = I * DS_BS + (threadIdx.x & 63);
   const double xi = a.x[i] + alpha * a.d[i];
   a.x[i] = xi;
   if (a.wvec) a.xu[i] = a.wvec[i] * xi;
@@ -60,7 +62,10 @@ __global__ __launch_bounds__(64) void k_cg_update(UpdArgs a) {
   const double si = ri / a.diag[i];
   a.s[i] = si;
   double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
-  store_partials<2>(v, a.partials, a.nblk, I);
+  if ((threadIdx.x & 63) == 0) {
+    a.partials[I] = v[0];
+    a.partials[(size_t)a.nblk + I] = v[1];
+  }
 }
 
 struct FinArgs {
@@ -78,8 +83,8 @@ struct FinArgs {
 // finalize its scalars (alpha for NV = 1, the step for NV = 2).  Skips the
 // subvolumes whose producing kernel skipped them.
 template <int NV>
-__global__ __launch_bounds__(256) void k_finalize(FinArgs a) {
-  __shared__ double red[NV * 4];
+__global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
+  __shared__ double red[NV * 16];
   const int sv = blockIdx.x;
   ds_cg_scalars* sc = a.sc + sv;
   if (sc->done) return;
@@ -105,6 +110,7 @@ struct DirArgs {
   float* du32;
   unsigned short* du16;   // BF16 copy for the explicit block path
   ds_cg_scalars* sc;
+  int nblk;
 };
 
 __device__ __forceinline__ unsigned short to_bf16(float f) {
@@ -114,12 +120,13 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
 }
 
 // lib/pcg.h:61-64: d = s + beta d (and its scaled copies).
-__global__ __launch_bounds__(64) void k_cg_dir(DirArgs a) {
-  const int I = blockIdx.x;
+__global__ __launch_bounds__(64 * kVecWaves) void k_cg_dir(DirArgs a) {
+  const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
+  if (I >= a.nblk) return;
   const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
   if (sc->done) return;
   const double beta = sc->beta;
-  const int i = I * DS_BS + threadIdx.x;
+  const int i = I * DS_BS + (threadIdx.x & 63);
   const double di = a.s[i] + beta * a.d[i];
   a.d[i] = di;
   const double ui = a.wvec ? a.wvec[i] * di : di;
@@ -198,10 +205,14 @@ hipError_t launch_finalize(ds_problem* p, const CgEpilogue& e,
   a.mixed = e.mixed;
   a.is_true = e.is_true;
   a.need_want = e.need_want;
+  // one block per subvolume, sized to its number of partials
+  const int threads = p->n_sub == 1 ? 1024 : 256;
   if (e.kind == EK_CG_DQ)
-    hipLaunchKernelGGL(k_finalize<1>, dim3(p->n_sub), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(p->n_sub), dim3(threads), 0, st,
+                       a);
   else
-    hipLaunchKernelGGL(k_finalize<2>, dim3(p->n_sub), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(p->n_sub), dim3(threads), 0, st,
+                       a);
   return hipGetLastError();
 }
 
@@ -229,7 +240,8 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.sc = p->sc;
   a.nblk = p->nblk;
   a.only_x = only_x;
-  hipLaunchKernelGGL(k_cg_update, dim3(p->nblk), dim3(DS_BS), 0, st, a);
+  const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
+  hipLaunchKernelGGL(k_cg_update, g, b, 0, st, a);
   DS_TRY(hipGetLastError());
   if (only_x) return hipSuccess;
   CgEpilogue e;
@@ -249,7 +261,9 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   a.du32 = mixed ? p->du32 : nullptr;
   a.du16 = mixed && p->use_dense ? p->du16 : nullptr;
   a.sc = p->sc;
-  hipLaunchKernelGGL(k_cg_dir, dim3(p->nblk), dim3(DS_BS), 0, st, a);
+  a.nblk = p->nblk;
+  const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
+  hipLaunchKernelGGL(k_cg_dir, g, b, 0, st, a);
   return hipGetLastError();
 }
 
