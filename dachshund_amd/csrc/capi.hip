@@ -1167,7 +1167,7 @@ int ds_problem_build_dense_bf16(ds_problem* p) {
     return fail(p->ctx, DS_E_STATE, "the explicit block path needs at most "
                 "%d subvolumes sharing one geometry", DS_DENSE_R);
   const int ld = p->npad / p->n_sub;
-  const int mpad = (ld + 127) / 128 * 128;
+  const int mpad = (ld + 255) / 256 * 256;   // GEMM workgroups: 256 rows
   const double bytes = 2.0 * (double)mpad * ld;
   if (bytes > 2.4e11)
     return fail(p->ctx, DS_E_NOMEM, "explicit S would need %.1f GB",

@@ -12,15 +12,14 @@
 // and alpha (k_dense_epi).  FP64 true residuals (the matrix-free parity
 // kernel) keep the stopping rule exact.
 //
-// k_gemm_bf16: 256 threads = 4 waves; a wave owns rows [row0, row0+32) x all
-// R = 32 columns of a K range (split K: blockIdx.y).  Per K-chunk of 64 a
-// lane of half h = lane/32 reads the contiguous run k0 + 32h .. k0 + 32h +
-// 31 of its row of S and of its column of U (the MFMA's k-slots of a half
-// are fed from that run; A and B use the same assignment, so every product
-// S[row][k] U[k][col] of the chunk is formed exactly once).  Each row's 128
-// bytes per chunk are read by two lanes.  Output tile layout of
-// v_mfma_f32_32x32x8bf16_1k: register j of lane l holds row
-// 8*(j/4) + 4*(l/32) + j%4, column l%32.
+// k_gemm_bf16: a wave owns rows [row0, row0+32) x all R = 32 columns of a K
+// range (split K: blockIdx.y).  Per K-chunk of 64 a lane of half h = lane/32
+// reads the contiguous run k0 + 32h .. k0 + 32h + 31 of its row of S and of
+// its column of U (the MFMA's k-slots of a half are fed from that run; A
+// and B use the same assignment, so every product S[row][k] U[k][col] of
+// the chunk is formed exactly once).  Each row's 128 bytes per chunk are
+// read by two lanes.  Output tile layout of v_mfma_f32_32x32x8bf16_1k:
+// register j of lane l holds row 8*(j/4) + 4*(l/32) + j%4, column l%32.
 // COMPILE WITH -ffp-contract=off (the S build is FP64 parity arithmetic).
 #include "cg_device.h"
 #include "ds_internal.h"
@@ -33,7 +32,6 @@ typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kR = DS_DENSE_R;     // right-hand sides per GEMM
-constexpr int kMT = 128;           // rows per workgroup
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
   unsigned u = __float_as_uint(f);
@@ -81,42 +79,82 @@ __global__ __launch_bounds__(256) void k_build_s(const double4* pos, int n,
 }
 
 // Yp[split][col][row] = sum over the split's K range of S[row][k] U[col][k]
-__global__ __launch_bounds__(256) void k_gemm_bf16(
+//
+// A workgroup of kW waves covers kW x 32 rows and one K range (split K:
+// blockIdx.y).  The 64-wide K chunks of U (32 columns x 64 k, 4 KB) are
+// staged once per workgroup in LDS (double-buffered, rows padded by 16 B
+// so the 32 column reads of a wave spread over the banks), so U is read
+// from L2 once per kW x 32 rows instead of once per wave; each wave streams
+// its 32 rows of S from HBM with the next chunk's loads in flight while the
+// current chunk's 8 MFMAs run.
+constexpr int kW = 8;                 // waves per workgroup
+constexpr int kUPitch = 64 * 2 + 16;  // LDS bytes per U column chunk
+
+__device__ __forceinline__ void mfma_chunk(const uint4 (&a)[4],
+                                           const uint4 (&b)[4], v16f& acc) {
+  const unsigned* au = reinterpret_cast<const unsigned*>(a);
+  const unsigned* bu = reinterpret_cast<const unsigned*>(b);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    v4s av, bv;
+    av[0] = (short)(au[2 * s] & 0xFFFFu);
+    av[1] = (short)(au[2 * s] >> 16);
+    av[2] = (short)(au[2 * s + 1] & 0xFFFFu);
+    av[3] = (short)(au[2 * s + 1] >> 16);
+    bv[0] = (short)(bu[2 * s] & 0xFFFFu);
+    bv[1] = (short)(bu[2 * s] >> 16);
+    bv[2] = (short)(bu[2 * s + 1] & 0xFFFFu);
+    bv[3] = (short)(bu[2 * s + 1] >> 16);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(64 * kW) void k_gemm_bf16(
     const unsigned short* __restrict__ S, const unsigned short* __restrict__ U,
     float* __restrict__ Yp, int ld, int mpad, int klen,
     const int* __restrict__ done_count, int n_sub) {
+  __shared__ __align__(16) unsigned char ub[2][kR * kUPitch];
   if (*done_count >= n_sub) return;   // every right-hand side converged
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 31, h = lane >> 5;
-  const int row0 = blockIdx.x * kMT + wave * 32;
+  const int row0 = blockIdx.x * (32 * kW) + wave * 32;
   const int split = blockIdx.y;
   const int kbeg = split * klen;
   const int kend = min(ld, kbeg + klen);
+  const int nch = (kend - kbeg + 63) / 64;   // klen and ld are multiples of 64
   const unsigned short* srow = S + (size_t)(row0 + r) * ld + 32 * h;
-  const unsigned short* ucol = U + (size_t)r * ld + 32 * h;
+  // U chunk loader: thread t < 256 moves 16 B (column t / 8, part t % 8)
+  const int ucol = threadIdx.x >> 3, upart = threadIdx.x & 7;
+  auto stage = [&](int c, int buf) {
+    if (threadIdx.x < kR * 8) {
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          U + (size_t)ucol * ld + kbeg + 64 * c + 8 * upart);
+      *reinterpret_cast<uint4*>(&ub[buf][ucol * kUPitch + 16 * upart]) = v;
+    }
+  };
   v16f acc = {};
-  for (int k0 = kbeg; k0 < kend; k0 += 64) {
-    uint4 a[4], b[4];
+  uint4 a[4], an[4];
+  if (nch > 0) {
+    stage(0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = reinterpret_cast<const uint4*>(srow + k0)[q];
-      b[q] = reinterpret_cast<const uint4*>(ucol + k0)[q];
-    }
-    const unsigned* au = reinterpret_cast<const unsigned*>(a);
-    const unsigned* bu = reinterpret_cast<const unsigned*>(b);
+    for (int q = 0; q < 4; ++q)
+      a[q] = reinterpret_cast<const uint4*>(srow + kbeg)[q];
+  }
+  for (int c = 0; c < nch; ++c) {
+    __syncthreads();   // chunk c staged; chunk c - 1's buffer is free
+    if (c + 1 < nch) {
+      stage(c + 1, (c + 1) & 1);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      v4s av, bv;
-      av[0] = (short)(au[2 * s] & 0xFFFFu);
-      av[1] = (short)(au[2 * s] >> 16);
-      av[2] = (short)(au[2 * s + 1] & 0xFFFFu);
-      av[3] = (short)(au[2 * s + 1] >> 16);
-      bv[0] = (short)(bu[2 * s] & 0xFFFFu);
-      bv[1] = (short)(bu[2 * s] >> 16);
-      bv[2] = (short)(bu[2 * s + 1] & 0xFFFFu);
-      bv[3] = (short)(bu[2 * s + 1] >> 16);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+      for (int q = 0; q < 4; ++q)
+        an[q] = reinterpret_cast<const uint4*>(srow + kbeg + 64 * (c + 1))[q];
     }
+    uint4 b[4];
+    const unsigned char* bp = &ub[c & 1][r * kUPitch + 64 * h];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = reinterpret_cast<const uint4*>(bp)[q];
+    mfma_chunk(a, b, acc);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = an[q];
   }
   float* y = Yp + ((size_t)split * kR + r) * mpad + row0;
 #pragma unroll
@@ -175,8 +213,8 @@ hipError_t dense_build(ds_problem* p, hipStream_t st) {
 hipError_t dense_gemm(ds_problem* p, hipStream_t st) {
   const int ld = p->dense_ld;
   const int klen = ((ld + p->dense_split - 1) / p->dense_split + 63) / 64 * 64;
-  const dim3 grid(p->dense_mpad / kMT, p->dense_split);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, p->dense_s,
+  const dim3 grid(p->dense_mpad / (32 * kW), p->dense_split);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(64 * kW), 0, st, p->dense_s,
                      p->du16, p->dense_y, ld, p->dense_mpad, klen,
                      p->done_count, p->n_sub);
   return hipGetLastError();
