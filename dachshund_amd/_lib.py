@@ -15,6 +15,7 @@ DS_OK, DS_E_ARG, DS_E_HIP, DS_E_NOMEM, DS_E_STATE = range(5)
 DS_FORM_SN, DS_FORM_W = 0, 1
 DS_PREC_F64, DS_PREC_F32 = 0, 1
 DS_F32_VALU, DS_F32_MFMA = 0, 1
+DS_PRECOND_JACOBI, DS_PRECOND_BLOCK = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -66,11 +67,14 @@ SIGNATURES = {
     "ds_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp,
                                   C.POINTER(DsCovar), C.POINTER(_vp)]),
     "ds_batch_num_sub": (C.c_int, [_vp]),
+    "ds_batch_sub_pairs": (C.c_int, [_vp, C.POINTER(C.c_longlong),
+                                     C.POINTER(C.c_longlong)]),
     "ds_batch_solve": (C.c_int, [_vp, C.POINTER(DsPcgParams), C.c_int,
                                  C.c_double, C.c_int,
                                  C.POINTER(DsPcgResult)]),
     "ds_problem_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ds_problem_set_f32_kernel": (C.c_int, [_vp, C.c_int]),
+    "ds_problem_set_precond": (C.c_int, [_vp, C.c_int]),
     "ds_problem_build_dense_bf16": (C.c_int, [_vp]),
     "ds_problem_set_dense": (C.c_int, [_vp, C.c_int]),
     "ds_problem_dense_product": (C.c_int, [_vp, _dp, _dp]),

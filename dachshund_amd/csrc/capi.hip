@@ -295,6 +295,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->tptr);
   dev_free(p->tidx);
   dev_free(p->tmask);
+  dev_free(p->minv);
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
@@ -561,6 +562,15 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   p->ntile_pairs = (long long)idx.size();
   p->nsub_pairs = 0;
   for (unsigned char m : msk) p->nsub_pairs += __builtin_popcount(m);
+  p->sub_pairs64.assign(n_sub, 0);
+  p->sub_pairs32.assign(n_sub, 0);
+  for (int k = 0; k < n_sub; ++k) {
+    const int B0 = p->sub_blk0_h[k], B1 = p->sub_blk0_h[k + 1];
+    for (int e = ptr[B0]; e < ptr[B1]; ++e)
+      p->sub_pairs32[k] += __builtin_popcount(msk[e]);
+    p->sub_pairs64[k] = (long long)(ptr[B1] - ptr[B0]) * (DS_BS * DS_TS);
+    p->sub_pairs32[k] *= DS_TS * DS_TS;
+  }
   p->blk_len_h.resize(nblk);
   for (int I = 0; I < nblk; ++I) p->blk_len_h[I] = ptr[I + 1] - ptr[I];
 
@@ -822,8 +832,7 @@ int ds_A_residual(ds_problem* p, const double* x, double* r) {
 int ds_precond(ds_problem* p, const double* b, double* x) {
   if (!p || !b || !x) return DS_E_ARG;
   if (int rc = upload_sorted(p, b, p->tmp)) return rc;
-  HIP_OR_FAIL(p->ctx, ds::launch_precond(p->tmp, p->diag, p->q, p->npad,
-                                         p->ctx->stream));
+  HIP_OR_FAIL(p->ctx, ds::launch_precond_p(p, p->tmp, p->q, p->ctx->stream));
   return download_sorted(p, p->q, x);
 }
 
@@ -1070,6 +1079,16 @@ int ds_problem_fill_x(ds_problem* p, double value) {
 
 int ds_batch_num_sub(const ds_problem* p) { return p ? p->n_sub : 0; }
 
+int ds_batch_sub_pairs(const ds_problem* p, long long* pairs64,
+                       long long* pairs32) {
+  if (!p || !pairs64 || !pairs32) return DS_E_ARG;
+  for (int k = 0; k < p->n_sub; ++k) {
+    pairs64[k] = p->sub_pairs64[k];
+    pairs32[k] = p->sub_pairs32[k];
+  }
+  return DS_OK;
+}
+
 int ds_batch_solve(ds_problem* p, const ds_pcg_params* prm, int mixed,
                    double drop, int use_graph, ds_pcg_result* res) {
   if (!p || !res) return DS_E_ARG;
@@ -1087,6 +1106,20 @@ int ds_problem_set_launch(ds_problem* p, int waves, int order_by_cost) {
     return fail(p->ctx, DS_E_ARG, "waves_per_block must be 1, 2 or 4");
   p->waves_per_block = waves;
   p->order_by_cost = order_by_cost ? 1 : 0;
+  return DS_OK;
+}
+
+int ds_problem_set_precond(ds_problem* p, int kind) {
+  if (!p) return DS_E_ARG;
+  if (kind != DS_PRECOND_JACOBI && kind != DS_PRECOND_BLOCK)
+    return fail(p->ctx, DS_E_ARG, "unknown preconditioner %d", kind);
+  if (kind == DS_PRECOND_BLOCK && !p->minv) {
+    hipStream_t st = p->ctx->stream;
+    HIP_OR_FAIL(p->ctx, dev_alloc(&p->minv, (size_t)p->nblk * DS_BS * DS_BS));
+    HIP_OR_FAIL(p->ctx, ds::bj_build(p, st));
+    HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+  }
+  p->precond_kind = kind;
   return DS_OK;
 }
 

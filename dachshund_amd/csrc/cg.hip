@@ -33,6 +33,7 @@ struct UpdArgs {
   const double* d;
   const double* q;
   const double* diag;
+  const float* minv;      // block-Jacobi inverses, or nullptr (Jacobi)
   const double* wvec;     // w (W form) or nullptr
   const int* blk_sub;
   double* x;
@@ -59,7 +60,8 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
   if (a.only_x) return;
   const double ri = a.r[i] - alpha * a.q[i];
   a.r[i] = ri;
-  const double si = ri / a.diag[i];
+  const double si = a.minv ? bj_apply(a.minv + (size_t)I * DS_BS * DS_BS, ri)
+                          : ri / a.diag[i];
   a.s[i] = si;
   double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
   if ((threadIdx.x & 63) == 0) {
@@ -230,6 +232,7 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.d = p->d;
   a.q = p->q;
   a.diag = p->diag;
+  a.minv = p->use_minv() ? p->minv : nullptr;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
   a.blk_sub = p->blk_sub;
   a.x = p->x;

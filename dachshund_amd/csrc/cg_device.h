@@ -88,6 +88,24 @@ __device__ __forceinline__ void gather_partials(double (&v)[NV],
   block_sum<NV>(v, red);
 }
 
+// Block-Jacobi application for one 64-row target block, one wave: lane i
+// returns sum_k M_ik r_k, with r_k (lane k's value) broadcast by readlane
+// and M stored as mt[k * 64 + i] (coalesced per k).  All 64 lanes must be
+// active.
+__device__ __forceinline__ double bj_apply(const float* __restrict__ mt,
+                                           double r) {
+  const int lane = threadIdx.x & 63;
+  const int lo = __double2loint(r), hi = __double2hiint(r);
+  double s = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < 64; ++k) {
+    const double rk = __hiloint2double(__builtin_amdgcn_readlane(hi, k),
+                                       __builtin_amdgcn_readlane(lo, k));
+    s = fma((double)mt[k * 64 + lane], rk, s);
+  }
+  return s;
+}
+
 // lib/pcg.h:35-37: alpha = delta / (d . q).
 __device__ __forceinline__ void finalize_alpha(ds_cg_scalars* sc, double dq) {
   sc->dq = dq;

@@ -132,6 +132,7 @@ struct ds_problem {
   std::vector<int> perm;               // slot -> caller index, -1 for padding
   long long ntile_pairs = 0;           // (block, tile) pairs after culling
   long long nsub_pairs = 0;            // (16-row sub-block, tile) pairs
+  std::vector<long long> sub_pairs64, sub_pairs32;   // per subvolume
   double norm_b = 0.0;                 // |b| of subvolume 0
   // subvolumes
   int n_sub = 1;
@@ -144,6 +145,9 @@ struct ds_problem {
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
   unsigned char* tmask = nullptr;      // per list entry: live sub-blocks
+  float* minv = nullptr;               // block-Jacobi inverses, nblk x 64 x 64
+  int precond_kind = DS_PRECOND_JACOBI;
+  bool use_minv() const { return precond_kind == DS_PRECOND_BLOCK && minv; }
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
   int* sub_blk0 = nullptr;             // first block per subvolume
@@ -193,6 +197,11 @@ struct ds_problem {
 namespace ds {
 hipError_t launch_scale(const double* v, const double* w, double* u,
                         float* u32, int npad, hipStream_t st);
+// Block-Jacobi: build the inverses (minv allocated); apply the problem's
+// preconditioner (Jacobi or block) to a device vector.
+hipError_t bj_build(ds_problem* p, hipStream_t st);
+hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
+                            hipStream_t st);
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st);
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,

@@ -65,6 +65,7 @@ struct MatvecArgs {
   const double* u;        // scaled input: w .* a (W form) or a (SN form)
   const double* rhs;      // wd or d (residual)
   const double* diag;     // Jacobi diagonal (CG residual epilogue)
+  const float* minv;      // block-Jacobi inverses, or nullptr
   const int* tptr;
   const int* tidx;
   const double2* table;
@@ -124,7 +125,9 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
     if (EPI == EPI_CG_DQ) {
       v[0] = ai * bi;  // d . q
     } else if (EPI == EPI_CG_RES) {
-      const double si = real ? bi / a.diag[row] : 0.0;
+      const double si =
+          a.minv ? bj_apply(a.minv + (size_t)I * DS_BS * DS_BS, bi)
+                 : (real ? bi / a.diag[row] : 0.0);
       a.out2[row] = si;
       v[0] = bi * si;  // r . M^-1 r
       v[1] = bi * bi;  // r . r
@@ -240,6 +243,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.u = u;
   a.rhs = p->rhs64;
   a.diag = p->diag;
+  a.minv = p->use_minv() ? p->minv : nullptr;
   a.tptr = p->tptr;
   a.tidx = p->tidx;
   a.table = p->table64;

@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DS_F32_MFMA, DS_F32_VALU, DS_FORM_SN, DS_FORM_W,
+                   DS_PRECOND_BLOCK, DS_PRECOND_JACOBI,
                    DS_PREC_F32, DS_PREC_F64, DsCovar, DsPcgParams,
                    DsPcgResult, DsProblemStats, check, lib)
 
@@ -308,6 +309,12 @@ class _WFVector:
         """DS_F32_MFMA (default) or DS_F32_VALU form of the FP32 kernel."""
         check(lib().ds_problem_set_f32_kernel(self.h, kind), self.ctx.h)
 
+    def set_precond(self, kind):
+        """DS_PRECOND_JACOBI (the reference's; default) or DS_PRECOND_BLOCK
+        (exact inverses of the 64-row target blocks) for this problem's
+        solves and preconditioner_solve."""
+        check(lib().ds_problem_set_precond(self.h, kind), self.ctx.h)
+
     def solve_resident(self, params: PCGParams, mixed=False,
                        refine_drop=1e-3):
         res = DsPcgResult()
@@ -397,6 +404,19 @@ class BatchWF(_WFVector):
 
     def set_f32_kernel(self, kind):
         check(lib().ds_problem_set_f32_kernel(self.h, kind), self.ctx.h)
+
+    def set_precond(self, kind):
+        check(lib().ds_problem_set_precond(self.h, kind), self.ctx.h)
+
+    def sub_pairs(self):
+        """Per subvolume: evaluated pairs per matvec of the FP64 kernel and
+        of the MFMA FP32 kernel."""
+        p64 = np.zeros(self.n_sub, dtype=np.int64)
+        p32 = np.zeros(self.n_sub, dtype=np.int64)
+        check(lib().ds_batch_sub_pairs(
+            self.h, p64.ctypes.data_as(C.POINTER(C.c_longlong)),
+            p32.ctypes.data_as(C.POINTER(C.c_longlong))), self.ctx.h)
+        return p64, p32
 
     def build_dense_bf16(self):
         """Explicit BF16 S for subvolumes sharing one geometry; the mixed

@@ -39,6 +39,10 @@ enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
  * default).  Problems with points far from their tile frame always use
  * direct differences on the VALU. */
 enum { DS_F32_VALU = 0, DS_F32_MFMA = 1 };
+/* Preconditioner of a problem's device solves: the diagonal of A (the
+ * reference's Jacobi; the default) or the exact inverse of each 64-row
+ * target block of A (block-Jacobi). */
+enum { DS_PRECOND_JACOBI = 0, DS_PRECOND_BLOCK = 1 };
 
 /* Arithmetic of the pair evaluation.
  * DS_PREC_F64: the reference's FP64 table interpolation; the table argument
@@ -138,6 +142,10 @@ int ds_batch_create(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
                     const double* pixels, const ds_covar* covars,
                     ds_problem** out);
 int ds_batch_num_sub(const ds_problem* p);
+/* Evaluated pairs per matvec of each subvolume: FP64 kernel (64-row
+ * blocks) and MFMA FP32 kernel (16-row sub-block culling). */
+int ds_batch_sub_pairs(const ds_problem* p, long long* pairs64,
+                       long long* pairs32);
 /* Solve every subvolume from the resident x (W space for the W form); res
  * gets one entry per subvolume.  mixed: the FP32/FP64 solver of
  * ds_solve_mixed with refine_drop.  use_graph: capture the iterations into
@@ -149,6 +157,11 @@ int ds_batch_solve(ds_problem* p, const ds_pcg_params* params, int mixed,
  * list first (the default; changes no result). */
 int ds_problem_set_launch(ds_problem* p, int waves_per_block,
                           int order_by_cost);
+/* Preconditioner of the problem's solves and of ds_precond (no reference
+ * counterpart for DS_PRECOND_BLOCK: the reference's preconditioner is
+ * Jacobi, lib/wf_vector.cc:42-49, :185-194).  The block inverses are built
+ * on the device on first selection (16 KB of FP32 per 64 rows). */
+int ds_problem_set_precond(ds_problem* p, int kind);
 /* FP32 pair kernel variant (DS_F32_VALU or DS_F32_MFMA). */
 int ds_problem_set_f32_kernel(ds_problem* p, int kind);
 /* Time every launch shape (and, for FP32, both kernel variants) on this
