@@ -45,6 +45,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 300k-pixel problem, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (null if that summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic_c5_300k.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if f and f[0] == kernel:
+            return float(f[1]) + float(f[2])
+    return None
+
+
 def cpu_baseline(pix_w, var_s, l_perp, l_para, n_full, n_mv_full,
                  budget_s=12.0):
     """The reference's own WFVectorW::A_product (lib/wf_vector.cc:196-220),
@@ -449,9 +463,13 @@ def main():
         "dense_equiv_pair_evals_per_s": n * n * (rmx.n_matvec32 +
                                                  rmx.n_matvec64) / t_mx,
         "roofline": {
-            "bound": "valu", "achieved": tf32, "peak": PEAK_TFLOPS["f32"],
+            "bound": "mfma", "achieved": tf32, "peak": PEAK_TFLOPS["f32"],
             "unit": "TFLOP/s", "frac": tf32 / PEAK_TFLOPS["f32"],
-            "traffic": None,
+            "traffic": pmc_traffic("k_matvec32m"),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + "
+                            "WRITE_SIZE, copy-calibrated)",
+            "limiter": "VALU issue: 6 VALU ops per pair beside the MFMA "
+                       "(PMC counters in profiles/)",
             "kernel": "k_matvec32m (FP32 pair matvec, MFMA table arguments)",
             "ms_per_launch": ms32, "flops_per_pair": FLOPS_PER_PAIR,
             "pairs_per_launch": pairs32,
