@@ -126,6 +126,8 @@ def bench_block(args):
         f" s")
     sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
     batch = D.BatchWF(sets, [sp] * R)
+    kind = (D.DS_PRECOND_BLOCK if args.precond == "block"
+            else D.DS_PRECOND_JACOBI)
     t = time.perf_counter()
     batch.build_dense_bf16()
     t_build = time.perf_counter() - t
@@ -139,11 +141,14 @@ def bench_block(args):
             t = time.perf_counter()
             if mode == "dense":
                 batch.set_dense(True)
+                batch.set_precond(kind)
                 res = batch.solve(prm, mixed=True, use_graph=True)
             elif mode == "mixed":
                 batch.set_dense(False)
+                batch.set_precond(kind)
                 res = batch.solve(prm, mixed=True, use_graph=True)
             else:
+                batch.set_precond(D.DS_PRECOND_JACOBI)
                 res = batch.solve(prm, mixed=False, use_graph=False)
             ts.append(time.perf_counter() - t)
             assert all(r.success for r in res), mode
@@ -186,7 +191,9 @@ def bench_block(args):
         "data": "synthetic (seeded GRF, 32 noise realizations)",
         "config": {"workload": f"{n} pixels (400 skewers x 250), box "
                                f"{w.box}, 32 RHS, seed {w.seed}",
-                   "tol": args.tol},
+                   "tol": args.tol,
+                   "precond": {"dense": args.precond, "mixed": args.precond,
+                               "fp64": "jacobi"}},
         "time_to_tol_s": {k: v["s"] for k, v in out.items()},
         "iterations_max": {k: v["iters_max"] for k, v in out.items()},
         "s_build_explicit_S": t_build,
@@ -216,6 +223,9 @@ def bench_batch(args):
                 f"{time.perf_counter() - t0:.1f} s")
     sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
     batch = D.BatchWF(pix, sps)
+    kind = (D.DS_PRECOND_BLOCK if args.precond == "block"
+            else D.DS_PRECOND_JACOBI)
+    batch.set_precond(kind)
     st = batch.stats()
     log(f"batch: {batch.n} pixels, setup {st.setup_seconds:.2f} s, "
         f"{st.pairs_per_matvec:.3e} pairs per batched matvec")
@@ -239,6 +249,8 @@ def bench_batch(args):
     t_seq = 0.0
     for p, sp in zip(pix, sps):
         wf = D.WFVectorW(p, sp)
+        wf.set_launch(waves, order)
+        wf.set_precond(kind)
         wf.fill_x(0.0)
         t = time.perf_counter()
         r = wf.solve_resident(D.PCGParams(wf.n, 2000, 50, args.tol),
@@ -261,7 +273,7 @@ def bench_batch(args):
         "dtype": "f32 pairs + f64 vectors",
         "data": "synthetic (seeded GRF subvolumes)",
         "config": {"workload": f"{len(subs)} subvolumes, {batch.n} pixels",
-                   "tol": args.tol},
+                   "tol": args.tol, "precond": args.precond},
         "time_to_tol_s": {"batched": t_batch, "one_by_one": t_seq},
         "speedup_vs_one_by_one": t_seq / t_batch,
         "autotuned": {"waves_per_block": waves, "longest_first": order,
@@ -281,6 +293,9 @@ def main():
     ap.add_argument("--warmstart", default=os.path.join(
         ROOT, "dachshund_amd", "warmstart_c5.npz"))
     ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--precond", choices=("jacobi", "block"), default="block",
+                    help="preconditioner of the mixed solves of the c3 and "
+                         "c4 workloads (plain FP64 keeps Jacobi)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the "
