@@ -143,8 +143,13 @@ def bench_block(args):
                 batch.set_dense(True)
                 batch.set_precond(kind)
                 res = batch.solve(prm, mixed=True, use_graph=True)
+            elif mode == "fused":
+                batch.set_fused(True)
+                batch.set_precond(kind)
+                res = batch.solve(prm, mixed=True, use_graph=True)
             elif mode == "mixed":
                 batch.set_dense(False)
+                batch.set_fused(False)
                 batch.set_precond(kind)
                 res = batch.solve(prm, mixed=True, use_graph=True)
             else:
@@ -155,23 +160,33 @@ def bench_block(args):
         return float(np.mean(ts)), res
 
     out = {}
-    for mode in ("dense", "mixed", "fp64"):
+    xs = {}
+    for mode in ("dense", "fused", "mixed", "fp64"):
         run(mode, args.warmup)
         tm, res = run(mode, args.steps)
         out[mode] = {"s": tm, "iters_max": max(r.num_iters for r in res),
                      "fp64_true_residuals": max(r.n_true for r in res)}
         log(f"{mode}: {tm * 1e3:.1f} ms for {R} RHS, iterations <= "
             f"{out[mode]['iters_max']}")
-        if mode == "dense":
-            x_dense = batch.get_x()
+        xs[mode] = batch.get_x()
     # accuracy against tight FP64 solves of every right-hand side
     batch.fill_x(0.0)
     batch.solve(D.PCGParams(batch.n, 4000, w.step_r, 1e-10), mixed=False,
                 use_graph=False)
     x_tight = batch.get_x()
-    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
-            for a, b in zip(batch.split(x_dense), batch.split(x_tight))]
+    errs = {m: max(float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                   for a, b in zip(batch.split(xs[m]), batch.split(x_tight)))
+            for m in ("dense", "fused", "mixed")}
+    batch.set_dense(True)
     ms = batch.time_dense(reps=10)
+    batch.set_fused(True)
+    ms_fused = batch.time_dense(reps=10)
+    batch.set_fused(False)
+    st = batch.stats()
+    pairs1 = st.pairs_per_matvec / R     # one geometry's 64 x 16 pairs
+    log(f"fused multi-RHS kernel {ms_fused:.3f} ms: {pairs1:.3e} S tiles "
+        f"pairs x {R} RHS ({2 * pairs1 * R / (ms_fused * 1e-3) / 1e12:.0f} "
+        f"MFMA TFLOP/s, {pairs1 / (ms_fused * 1e-3) / 1e9:.0f} Gpair/s)")
     # algorithmic bytes of one block product: S once (BF16), the R
     # directions (BF16) and the R results (FP32)
     bytes_launch = 2.0 * n * n + 2.0 * R * n + 4.0 * R * n
@@ -192,12 +207,17 @@ def bench_block(args):
         "config": {"workload": f"{n} pixels (400 skewers x 250), box "
                                f"{w.box}, 32 RHS, seed {w.seed}",
                    "tol": args.tol,
-                   "precond": {"dense": args.precond, "mixed": args.precond,
-                               "fp64": "jacobi"}},
+                   "precond": {"dense": args.precond, "fused": args.precond,
+                               "mixed": args.precond, "fp64": "jacobi"}},
         "time_to_tol_s": {k: v["s"] for k, v in out.items()},
         "iterations_max": {k: v["iters_max"] for k, v in out.items()},
         "s_build_explicit_S": t_build,
-        "x_rel_l2_vs_fp64_tight_max": max(errs),
+        "x_rel_l2_vs_fp64_tight_max": errs,
+        "fused_kernel": {"ms_per_launch": ms_fused,
+                         "pairs_per_launch": pairs1,
+                         "pair_evals_per_s": pairs1 / (ms_fused * 1e-3),
+                         "mfma_tflops": 2 * pairs1 * R / (ms_fused * 1e-3)
+                         / 1e12},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "k_gemm_bf16",

@@ -75,6 +75,7 @@ SIGNATURES = {
     "ds_problem_set_launch": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ds_problem_set_f32_kernel": (C.c_int, [_vp, C.c_int]),
     "ds_problem_set_precond": (C.c_int, [_vp, C.c_int]),
+    "ds_problem_set_fused": (C.c_int, [_vp, C.c_int]),
     "ds_problem_build_dense_bf16": (C.c_int, [_vp]),
     "ds_problem_set_dense": (C.c_int, [_vp, C.c_int]),
     "ds_problem_dense_product": (C.c_int, [_vp, _dp, _dp]),

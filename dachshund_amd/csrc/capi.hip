@@ -1229,12 +1229,34 @@ int ds_problem_set_dense(ds_problem* p, int on) {
   if (on && !p->dense_s)
     return fail(p->ctx, DS_E_STATE, "no explicit block built");
   p->use_dense = on != 0;
+  if (on) p->use_fused = false;
+  return DS_OK;
+}
+
+int ds_problem_set_fused(ds_problem* p, int on) {
+  if (!p) return DS_E_ARG;
+  if (on) {
+    if (!p->shared_geometry || p->n_sub > DS_DENSE_R)
+      return fail(p->ctx, DS_E_STATE, "the fused multi-RHS operator needs "
+                  "at most %d subvolumes sharing one geometry", DS_DENSE_R);
+    if (!p->du16) {
+      const size_t ld = (size_t)p->npad / p->n_sub;
+      HIP_OR_FAIL(p->ctx, dev_alloc(&p->du16, (size_t)DS_DENSE_R * ld));
+      // unused right-hand-side columns stay zero
+      HIP_OR_FAIL(p->ctx, hipMemsetAsync(p->du16, 0, sizeof(unsigned short) *
+                                         DS_DENSE_R * ld, p->ctx->stream));
+      HIP_OR_FAIL(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    }
+    p->use_dense = false;
+  }
+  p->use_fused = on != 0;
   return DS_OK;
 }
 
 int ds_problem_time_dense(ds_problem* p, int reps, double* ms_per_launch) {
   if (!p || !ms_per_launch || reps < 1) return DS_E_ARG;
-  if (!p->dense_s) return fail(p->ctx, DS_E_STATE, "no explicit block built");
+  if (!p->dense_s && !p->use_fused)
+    return fail(p->ctx, DS_E_STATE, "no multi-RHS operator selected");
   hipStream_t st = p->ctx->stream;
   // the timing input: bf16 copy of the resident x (any vector will do);
   // done_count must read "not finished" for the kernel to run
@@ -1247,9 +1269,12 @@ int ds_problem_time_dense(ds_problem* p, int reps, double* ms_per_launch) {
   hipEvent_t e0, e1;
   HIP_OR_FAIL(p->ctx, hipEventCreate(&e0));
   HIP_OR_FAIL(p->ctx, hipEventCreate(&e1));
-  hipError_t e = ds::dense_gemm(p, st);
+  auto op = [&]() {
+    return p->use_fused ? ds::fused_mrhs(p, st) : ds::dense_gemm(p, st);
+  };
+  hipError_t e = op();
   if (e == hipSuccess) e = hipEventRecord(e0, st);
-  for (int i = 0; i < reps && e == hipSuccess; ++i) e = ds::dense_gemm(p, st);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = op();
   if (e == hipSuccess) e = hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.f;
@@ -1266,7 +1291,8 @@ int ds_problem_dense_product(ds_problem* p, const double* a, double* b) {
   // the pair of kernels the mixed solver uses, without the CG epilogue's
   // scalar updates.
   if (!p || !a || !b) return DS_E_ARG;
-  if (!p->dense_s) return fail(p->ctx, DS_E_STATE, "no explicit block built");
+  if (!p->dense_s && !p->use_fused)
+    return fail(p->ctx, DS_E_STATE, "no multi-RHS operator selected");
   hipStream_t st = p->ctx->stream;
   if (int rc = upload_sorted(p, a, p->d)) return rc;
   const double* w = p->form == DS_FORM_W ? p->wvec : nullptr;
@@ -1282,8 +1308,12 @@ int ds_problem_dense_product(ds_problem* p, const double* a, double* b) {
                                      hipMemcpyHostToDevice, st));
   HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->done_count, &zero, sizeof(int),
                                      hipMemcpyHostToDevice, st));
-  HIP_OR_FAIL(p->ctx, ds::dense_gemm(p, st));
-  HIP_OR_FAIL(p->ctx, ds::dense_epilogue(p, st));
+  if (p->use_fused && !p->use_dense) {
+    HIP_OR_FAIL(p->ctx, ds::fused_mrhs(p, st));
+  } else {
+    HIP_OR_FAIL(p->ctx, ds::dense_gemm(p, st));
+    HIP_OR_FAIL(p->ctx, ds::dense_epilogue(p, st));
+  }
   return download_sorted(p, p->q, b);
 }
 

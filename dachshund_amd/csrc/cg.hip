@@ -262,7 +262,7 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   a.d = p->d;
   a.du = p->du;
   a.du32 = mixed ? p->du32 : nullptr;
-  a.du16 = mixed && p->use_dense ? p->du16 : nullptr;
+  a.du16 = mixed && (p->use_dense || p->use_fused) ? p->du16 : nullptr;
   a.sc = p->sc;
   a.nblk = p->nblk;
   const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
@@ -276,7 +276,10 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
   CgEpilogue dq;
   dq.kind = EK_CG_DQ;
   if (mixed) {
-    if (p->use_dense) {
+    if (p->use_fused) {
+      // shared-geometry batch: S tiles on the fly x all RHS on MFMA
+      DS_TRY(fused_mrhs(p, st));
+    } else if (p->use_dense) {
       // explicit BF16 block: q = w .* (S (w .* d)) + d for every RHS at once
       DS_TRY(dense_gemm(p, st));
       DS_TRY(dense_epilogue(p, st));
@@ -360,7 +363,7 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init, st));
     DS_TRY(launch_scale(p->d, w, w ? p->du : nullptr,
                         mixed ? p->du32 : nullptr, p->npad, st));
-    if (mixed && p->use_dense)
+    if (mixed && (p->use_dense || p->use_fused))
       DS_TRY(launch_to_bf16(p->du32, p->du16, p->npad, st));
     // Enqueue iterations in chunks; after each chunk the number of finished
     // subvolumes is copied to pinned memory (two slots, alternating) and the

@@ -20,6 +20,19 @@
 // the chunk is formed exactly once).  Each row's 128 bytes per chunk are
 // read by two lanes.  Output tile layout of v_mfma_f32_32x32x8bf16_1k:
 // register j of lane l holds row 8*(j/4) + 4*(l/32) + j%4, column l%32.
+//
+// k_fused_mrhs: the same multi-RHS product without materializing S.  A
+// batch of subvolumes sharing one geometry has one set of tile lists; per
+// (64-row target block, 16-source tile) entry the VALU evaluates the 64 x 16
+// interpolant values g_ij in FP32 (the expanded table argument of the FP32
+// pair kernel: lane = target row), rounds them to BF16, and two K-steps of
+// v_mfma_f32_32x32x8bf16_1k per 32-row half multiply them into all R right-
+// hand sides at once (A = g tile, B = the BF16 directions of the 16 sources
+// for the 32 RHS).  The MFMA A layout wants, in lane l, 4 consecutive
+// sources of row 32 rb + l%32; half of those values live in lane l ^ 32,
+// which sends them with one cross-half shuffle per K-step.  The epilogue
+// scales by var_s in FP64, adds the identity or noise term per RHS and
+// stores the d . q partial of each subvolume's block.
 // COMPILE WITH -ffp-contract=off (the S build is FP64 parity arithmetic).
 #include "cg_device.h"
 #include "ds_internal.h"
@@ -161,6 +174,132 @@ __global__ __launch_bounds__(64 * kW) void k_gemm_bf16(
   for (int j = 0; j < 16; ++j) y[8 * (j >> 2) + 4 * h + (j & 3)] = acc[j];
 }
 
+struct FusedArgs {
+  const float4* tgt;          // targets, block frames (subvolume 0)
+  const int* tptr;            // subvolume 0's lists
+  const double4* pos;         // batch records (w or N in .w)
+  const double* d;            // batch directions (FP64)
+  const unsigned short* U;    // BF16 scaled directions [R][ld]
+  const float2* table;        // {f, df}
+  const int* blk_nreal;
+  const ds_cg_scalars* sc;
+  const int* done_count;
+  double* q;
+  double* partials;
+  double var_s;
+  float tlast;
+  int nblk1, ld, n_sub, nblk, table_n, form;
+};
+
+__device__ __forceinline__ v4s as_v4s(unsigned lo, unsigned hi) {
+  v4s v;
+  v[0] = (short)(lo & 0xFFFFu);
+  v[1] = (short)(lo >> 16);
+  v[2] = (short)(hi & 0xFFFFu);
+  v[3] = (short)(hi >> 16);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_fused_mrhs(
+    const float4* __restrict__ loc, const int* __restrict__ tidx,
+    const float4* __restrict__ toff, FusedArgs a) {
+  extern __shared__ float2 tab32[];
+  __shared__ float part[3][2][16][64];   // waves 1..3: partial tiles
+  if (*a.done_count >= a.n_sub) return;   // every right-hand side converged
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
+    tab32[k] = a.table[k];
+  __syncthreads();
+  const int I = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
+  const int h = lane >> 5, r = lane & 31;
+  const float4 li = a.tgt[I * DS_BS + lane];
+  const float tlast = a.tlast;
+  v16f acc0 = {}, acc1 = {};
+  const unsigned short* ucol = a.U + (size_t)r * a.ld + 4 * h;
+  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
+  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  for (int k = kb; k < ke; k += nw) {
+    const int j0 = __builtin_amdgcn_readfirstlane(tidx[k]) * DS_TS;
+    const float4 o = toff[k];
+    const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
+    const float ri = fmaf(zi, zi, fmaf(yi, yi, xi * xi));
+    const float mx = -2.0f * xi, my = -2.0f * yi, mz = -2.0f * zi;
+    unsigned P[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      unsigned short gb[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const float4 pj = loc[j0 + 2 * m + e];   // {X, Y, Z, |X|^2}
+        float xx = fmaf(mz, pj.z, fmaf(my, pj.y, fmaf(mx, pj.x, ri + pj.w)));
+        xx = __builtin_amdgcn_fmed3f(xx, 0.0f, tlast);
+        const int ix = (int)xx;
+        const float t = __builtin_amdgcn_fractf(xx);
+        const float2 f = tab32[ix];
+        gb[e] = bf16_rne(fmaf(f.y, t, f.x));
+      }
+      P[m] = gb[0] | ((unsigned)gb[1] << 16);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const unsigned own0 = h ? P[4 * ks + 2] : P[4 * ks];
+      const unsigned own1 = h ? P[4 * ks + 3] : P[4 * ks + 1];
+      const unsigned snd0 = h ? P[4 * ks] : P[4 * ks + 2];
+      const unsigned snd1 = h ? P[4 * ks + 1] : P[4 * ks + 3];
+      const unsigned rcv0 = (unsigned)__shfl_xor((int)snd0, 32, 64);
+      const unsigned rcv1 = (unsigned)__shfl_xor((int)snd1, 32, 64);
+      const uint2 bu =
+          *reinterpret_cast<const uint2*>(ucol + j0 + 8 * ks);
+      const v4s bv = as_v4s(bu.x, bu.y);
+      const v4s a0 = h ? as_v4s(rcv0, rcv1) : as_v4s(own0, own1);
+      const v4s a1 = h ? as_v4s(own0, own1) : as_v4s(rcv0, rcv1);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, bv, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, bv, acc1, 0, 0, 0);
+    }
+  }
+  // combine the waves' partial tiles
+  if (wave > 0) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      part[wave - 1][0][j][lane] = acc0[j];
+      part[wave - 1][1][j][lane] = acc1[j];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  for (int w = 1; w < nw; ++w)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      acc0[j] += part[w - 1][0][j][lane];
+      acc1[j] += part[w - 1][1][j][lane];
+    }
+  // epilogue: lane holds rhs r, rows 32 rb + 8 (j / 4) + 4 h + j % 4
+  const bool live = r < a.n_sub && !a.sc[r].done;
+  const int nreal = a.blk_nreal[I];
+  double dot = 0.0;
+  if (live) {
+    const size_t g0 = ((size_t)r * a.nblk1 + I) * DS_BS;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int i = 32 * rb + 8 * (j >> 2) + 4 * h + (j & 3);
+        const size_t g = g0 + i;
+        const double y = a.var_s * (double)(rb ? acc1[j] : acc0[j]);
+        const double di = a.d[g];
+        const double wN = a.pos[g].w;
+        double qi = a.form == DS_FORM_W ? wN * y + di : y + wN * di;
+        if (i >= nreal) qi = 0.0;
+        a.q[g] = qi;
+        dot += di * qi;
+      }
+  }
+  dot += __shfl_xor(dot, 32, 64);
+  if (live && h == 0) a.partials[(size_t)r * a.nblk1 + I] = dot;
+}
+
 // One 64-thread block per target block of the batch: right-hand side
 // sv = blk_sub[I], row i = (I - sub_blk0[sv]) * 64 + lane of the shared
 // geometry.  q = w .* Y + d (W) or Y + N .* d (SN); d . q -> alpha.
@@ -218,6 +357,38 @@ hipError_t dense_gemm(ds_problem* p, hipStream_t st) {
                      p->du16, p->dense_y, ld, p->dense_mpad, klen,
                      p->done_count, p->n_sub);
   return hipGetLastError();
+}
+
+hipError_t fused_mrhs(ds_problem* p, hipStream_t st) {
+  FusedArgs a{};
+  const int nblk1 = p->nblk / p->n_sub;
+  a.tgt = p->tgt32;
+  a.tptr = p->tptr;
+  a.pos = p->pos64;
+  a.d = p->d;
+  a.U = p->du16;
+  a.table = p->table32;
+  a.blk_nreal = p->blk_nreal;
+  a.sc = p->sc;
+  a.done_count = p->done_count;
+  a.q = p->q;
+  a.partials = p->partials;
+  a.var_s = p->sub_cv_h[0].var_s;
+  a.tlast = (float)(p->cv.table_n - 1);
+  a.nblk1 = nblk1;
+  a.ld = p->npad / p->n_sub;
+  a.n_sub = p->n_sub;
+  a.nblk = p->nblk;
+  a.table_n = p->cv.table_n;
+  a.form = p->form;
+  hipLaunchKernelGGL(k_fused_mrhs, dim3(nblk1), dim3(64 * p->waves_per_block),
+                     sizeof(float2) * p->cv.table_n, st, p->loc32, p->tidx,
+                     p->toff32, a);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  CgEpilogue dq;
+  dq.kind = EK_CG_DQ;
+  return launch_finalize(p, dq, st);
 }
 
 hipError_t dense_epilogue(ds_problem* p, hipStream_t st) {

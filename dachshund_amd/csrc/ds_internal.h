@@ -158,6 +158,7 @@ struct ds_problem {
   bool shared_geometry = false;        // all subvolumes: same pixels/weights
   // explicit BF16 block path (shared geometry, <= DS_DENSE_R subvolumes)
   bool use_dense = false;
+  bool use_fused = false;              // fused S tiles x all RHS (MFMA)
   unsigned short* dense_s = nullptr;   // S, dense_mpad x dense_ld, BF16
   float* dense_y = nullptr;            // split-K partials of S U
   unsigned short* du16 = nullptr;      // BF16 scaled directions, R x ld
@@ -212,6 +213,9 @@ hipError_t launch_to_bf16(const float* src, unsigned short* dst, int n,
 hipError_t dense_build(ds_problem* p, hipStream_t st);
 hipError_t dense_gemm(ds_problem* p, hipStream_t st);
 hipError_t dense_epilogue(ds_problem* p, hipStream_t st);
+// Fused multi-RHS operator of a shared-geometry batch: q and the d . q
+// partials of every subvolume from one pass over subvolume 0's tile lists.
+hipError_t fused_mrhs(ds_problem* p, hipStream_t st);
 hipError_t launch_column(const ds_problem* p, const double* table_dev,
                          const double* map_pt, double* col, hipStream_t st);
 hipError_t launch_dot1(const double* a, const double* b, int n, double* out,

@@ -429,6 +429,12 @@ class BatchWF(_WFVector):
         check(lib().ds_problem_set_dense(self.h, int(bool(on))), self.ctx.h)
         self._dense = bool(on)
 
+    def set_fused(self, on):
+        """Fused multi-RHS operator (S tiles on the fly x all right-hand
+        sides on MFMA, BF16); switches the explicit block off."""
+        check(lib().ds_problem_set_fused(self.h, int(bool(on))), self.ctx.h)
+        self._dense = bool(on)
+
     def dense_product(self, a):
         a = self._vec(a)
         b = np.empty(self.n)

@@ -181,9 +181,17 @@ int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
  * a refine_drop of about 0.1 (1e-3 suits the FP32 operator). */
 int ds_problem_build_dense_bf16(ds_problem* p);
 int ds_problem_set_dense(ds_problem* p, int on);
-/* b = A a through the explicit block (all subvolumes). */
+/* The same multi-RHS operator without materializing S (no reference
+ * counterpart): per (64-row block, 16-source tile) of the shared geometry
+ * the interpolant tile is evaluated once in FP32 on the VALU, rounded to
+ * BF16 and multiplied into all right-hand sides by MFMA with FP32
+ * accumulation.  Needs no explicit block; switching it on switches the
+ * explicit block off. */
+int ds_problem_set_fused(ds_problem* p, int on);
+/* b = A a through the selected multi-RHS operator (all subvolumes). */
 int ds_problem_dense_product(ds_problem* p, const double* a, double* b);
-/* Average duration of the block GEMM (HIP events, problem's stream). */
+/* Average duration of the selected multi-RHS operator's kernel (the block
+ * GEMM, or the fused kernel with its finalize; HIP events). */
 int ds_problem_time_dense(ds_problem* p, int reps, double* ms_per_launch);
 
 /* WFVector*::A_product (lib/wf_vector.cc:51-74 SN, :196-220 W): b = A a. */

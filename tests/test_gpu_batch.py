@@ -96,7 +96,15 @@ def _realizations(R, seed=21):
     return w, sets
 
 
-def test_dense_bf16_block_product(ora):
+def _multi_rhs(batch, mode):
+    if mode == "explicit":
+        batch.build_dense_bf16()
+    else:
+        batch.set_fused(True)
+
+
+@pytest.mark.parametrize("mode", ["explicit", "fused"])
+def test_dense_bf16_block_product(ora, mode):
     """S built in BF16 (relative rounding 2^-9) times BF16 directions, FP32
     accumulation: each row within 8e-3 of sum_j |S_ij u_j| of the FP64
     product; the identity part is exact."""
@@ -104,7 +112,7 @@ def test_dense_bf16_block_product(ora):
     w, sets = _realizations(5)
     sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
     batch = D.BatchWF([D.pixel_to_wpixel(p) for p in sets], [sp] * len(sets))
-    batch.build_dense_bf16()
+    _multi_rhs(batch, mode)
     a = np.random.default_rng(1).standard_normal(batch.n)
     b = batch.split(batch.dense_product(a))
     c = ora.covar(w.var_s, w.l_perp, w.l_para)
@@ -118,7 +126,8 @@ def test_dense_bf16_block_product(ora):
     assert max(errs) < 3e-3, errs
 
 
-def test_dense_bf16_block_solve(ora):
+@pytest.mark.parametrize("mode", ["explicit", "fused"])
+def test_dense_bf16_block_solve(ora, mode):
     """Multi-RHS mixed solve with the explicit BF16 block as inner operator:
     every right-hand side meets the FP64 tolerance (oracle residual) and
     matches its own FP64 solve."""
@@ -126,7 +135,7 @@ def test_dense_bf16_block_solve(ora):
     w, sets = _realizations(6, seed=22)
     sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
     batch = D.BatchWF([D.pixel_to_wpixel(p) for p in sets], [sp] * len(sets))
-    batch.build_dense_bf16()
+    _multi_rhs(batch, mode)
     batch.fill_x(0.0)
     tol = 1e-6
     res = batch.solve(D.PCGParams(batch.n, 2000, 50, tol), mixed=True,
