@@ -386,6 +386,7 @@ def main():
         solve(False)
         solve(True)
         solve(False, block=True)
+        solve(True, block=True)
         solve(True, learned, True)
         if learned:
             solve(True, True)
@@ -399,6 +400,8 @@ def main():
     report("mixed + learned x0, Jacobi", tml, rml)
     tb64, rb64 = timed(False, args.steps, block=True)
     report("plain FP64 PCG, block-Jacobi", tb64, rb64)
+    tbm0, rbm0 = timed(True, args.steps, block=True)
+    report("mixed, x0 = 0, block-Jacobi", tbm0, rbm0)
     tmx, rmx = timed(True, args.steps, learned=learned, block=True)
     report("headline: mixed + learned x0 + block-Jacobi", tmx, rmx)
     t_pred = t_knn = None
@@ -480,12 +483,14 @@ def main():
                           "mixed_learned_jacobi": float(np.mean(tml)),
                           "mixed_x0_zero_jacobi": float(np.mean(tm0)),
                           "plain_fp64_block_jacobi": float(np.mean(tb64)),
+                          "mixed_x0_zero_block_jacobi": float(np.mean(tbm0)),
                           "plain_fp64": t_64},
         "speedup_vs_plain_fp64": t_64 / t_mx,
         "iterations": {"headline": rmx.num_iters, "headline_fp64_residuals":
                        rmx.n_true, "mixed_learned_jacobi": rml.num_iters,
                        "mixed_x0_zero_jacobi": rm0.num_iters,
                        "plain_fp64_block_jacobi": rb64.num_iters,
+                       "mixed_x0_zero_block_jacobi": rbm0.num_iters,
                        "plain_fp64": r64.num_iters},
         "learned_warm_start": bool(ws),
         "learned_predict_s": t_pred,

@@ -33,14 +33,23 @@ struct WsTrainer {
   float* dy = nullptr;
   float* loss_part = nullptr;
   int* iota = nullptr;      // 0..WS_MAX_BATCH-1
+  unsigned short* wsplit = nullptr;   // layers 1-3 as 3 BF16 parts (MFMA)
   long long step = 0;
 };
 
+// Padded input widths of layers 1-3 in the MFMA forward and the size of
+// the split-weight buffer (3 BF16 parts x 64 units x padded width).
+#define WS_K1P 168
+#define WS_SPLIT_ELEMS (3 * WS_HIDDEN * (WS_K1P + 2 * WS_HIDDEN))
+
 hipError_t ws_features(ds_problem* p, unsigned short* feat,
                        hipStream_t st);
-hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
-                      int npad, int quant, const ds_problem* p, double* x_out,
-                      float* t_out, hipStream_t st);
+// Inference; with quant (BF16 activations) and wsplit it runs on the
+// matrix cores, else on the VALU.
+hipError_t ws_forward(const float* P, unsigned short* wsplit,
+                      const unsigned short* feat, int n, int npad, int quant,
+                      const ds_problem* p, double* x_out, float* t_out,
+                      hipStream_t st);
 hipError_t ws_targets(const ds_problem* p, float* t, hipStream_t st);
 hipError_t ws_batch(WsTrainer& T, const unsigned short* feat,
                     const float* target, const int* idx, int B, int quant,

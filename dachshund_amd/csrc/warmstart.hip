@@ -415,6 +415,170 @@ __global__ __launch_bounds__(64) void k_fwd(const float* __restrict__ P,
   }
 }
 
+// ---- inference on the matrix cores ----------------------------------------
+//
+// Each FP32 weight is split into three BF16 parts, w = hi + mid + lo (8
+// mantissa bits each, so the sum carries the FP32 value); activations are
+// BF16 already (features, and every hidden layer with quant), so the
+// products x * part are exact in FP32 and v_mfma_f32_32x32x8bf16_1k sums
+// them with FP32 accumulation: the same arithmetic class as the VALU
+// forward, in another summation order.  One wave runs 64 pixels: A = the
+// activation rows (LDS), B = 4 consecutive inputs of one unit of one part
+// (global, L2-resident), D = 32 pixels x 32 units per MFMA; the epilogue
+// adds the bias, applies ReLU and rounds to BF16 into LDS for the next
+// layer.  The 64 -> 1 output layer is a VALU dot product.
+
+typedef short v4s_ws __attribute__((ext_vector_type(4)));
+typedef float v16f_ws __attribute__((ext_vector_type(16)));
+
+// Wsplit layout per layer L (inputs K_L padded to Kp_L): [part][unit][Kp_L].
+__global__ __launch_bounds__(256) void k_split_weights(const float* P,
+                                                       unsigned short* ws) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = H * WS_K1P, n2 = H * H;
+  if (t >= n1 + 2 * n2) return;
+  int layer, rem;
+  if (t < n1) { layer = 0; rem = t; }
+  else if (t < n1 + n2) { layer = 1; rem = t - n1; }
+  else { layer = 2; rem = t - n1 - n2; }
+  const int kp = layer == 0 ? WS_K1P : H;
+  const int kin = layer == 0 ? WS_FEAT : H;
+  const int j = rem / kp, k = rem % kp;
+  const float* W = P + (layer == 0 ? WS_OFF_W1 : layer == 1 ? WS_OFF_W2
+                                                            : WS_OFF_W3);
+  const float w = k < kin ? W[k * H + j] : 0.0f;
+  const unsigned short hi = f2bf(w);
+  const float r1 = w - bf2f(hi);
+  const unsigned short mid = f2bf(r1);
+  const float r2 = r1 - bf2f(mid);
+  const unsigned short lo = f2bf(r2);
+  const size_t base = layer == 0 ? 0 : (size_t)3 * n1 + (size_t)3 * n2 *
+                                                           (layer - 1);
+  const size_t plane = (size_t)H * kp;
+  ws[base + 0 * plane + rem] = hi;
+  ws[base + 1 * plane + rem] = mid;
+  ws[base + 2 * plane + rem] = lo;
+}
+
+__device__ __forceinline__ v4s_ws ld4(const unsigned short* p) {
+  const uint2 u = *reinterpret_cast<const uint2*>(p);
+  v4s_ws v;
+  v[0] = (short)(u.x & 0xFFFFu);
+  v[1] = (short)(u.x >> 16);
+  v[2] = (short)(u.y & 0xFFFFu);
+  v[3] = (short)(u.y >> 16);
+  return v;
+}
+
+// acc[mh][nh] += act[32 mh + r][k] * W[k][32 nh + r] over NKS K-steps of 8.
+template <int NKS>
+__device__ __forceinline__ void mfma_layer(const unsigned short* act,
+                                           int pitch,
+                                           const unsigned short* __restrict__ w,
+                                           int kp, v16f_ws (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+  const size_t plane = (size_t)H * kp;
+#pragma unroll 2
+  for (int ks = 0; ks < NKS; ++ks) {
+    const v4s_ws a0 = ld4(act + r * pitch + 8 * ks + 4 * h);
+    const v4s_ws a1 = ld4(act + (32 + r) * pitch + 8 * ks + 4 * h);
+#pragma unroll
+    for (int part = 0; part < 3; ++part)
+#pragma unroll
+      for (int nh = 0; nh < 2; ++nh) {
+        const v4s_ws b =
+            ld4(w + part * plane + (size_t)(32 * nh + r) * kp + 8 * ks + 4 * h);
+        acc[0][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, b,
+                                                             acc[0][nh], 0, 0,
+                                                             0);
+        acc[1][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b,
+                                                             acc[1][nh], 0, 0,
+                                                             0);
+      }
+  }
+}
+
+// bias + ReLU + BF16 into act[pixel][unit] (pitch), D layout of the MFMA:
+// register j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column
+// l % 32 (plus 32 mh / 32 nh).
+__device__ __forceinline__ void mfma_epilogue(const v16f_ws (&acc)[2][2],
+                                              const float* __restrict__ b,
+                                              unsigned short* act, int pitch) {
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+    for (int nh = 0; nh < 2; ++nh) {
+      const float bias = b[32 * nh + r];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int px = 32 * mh + 8 * (j >> 2) + 4 * h + (j & 3);
+        act[px * pitch + 32 * nh + r] =
+            f2bf(fmaxf(acc[mh][nh][j] + bias, 0.0f));
+      }
+    }
+}
+
+constexpr int kFP = 172;   // LDS pitch (BF16) of the feature rows
+constexpr int kAP = 68;    // LDS pitch (BF16) of the hidden activations
+
+__global__ __launch_bounds__(64) void k_fwd_mfma(
+    const float* __restrict__ P, const unsigned short* __restrict__ wsplit,
+    FwdArgs a) {
+  __shared__ __align__(16) unsigned short sf[64 * kFP];   // features / h2
+  __shared__ __align__(16) unsigned short sa[64 * kAP];   // h1 / h3
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * 64;
+  const int rows = min(64, a.npad - r0);
+  // feature rows (162 BF16 = 81 dwords) into pitch-172 rows, zero padded
+  const unsigned* src =
+      reinterpret_cast<const unsigned*>(a.feat + (size_t)r0 * WS_FEAT);
+  unsigned* dst = reinterpret_cast<unsigned*>(sf);
+  for (int k = lane; k < 64 * (kFP / 2); k += 64) {
+    const int row = k / (kFP / 2), c = k % (kFP / 2);
+    dst[k] = (row < rows && c < WS_FEAT / 2) ? src[row * (WS_FEAT / 2) + c]
+                                             : 0u;
+  }
+  __syncthreads();
+  const size_t n1 = (size_t)H * WS_K1P, n2 = (size_t)H * H;
+  v16f_ws acc[2][2];
+  auto zero = [&]() {
+#pragma unroll
+    for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+      for (int nh = 0; nh < 2; ++nh) acc[mh][nh] = (v16f_ws){};
+  };
+  zero();
+  mfma_layer<WS_K1P / 8>(sf, kFP, wsplit, WS_K1P, acc);
+  mfma_epilogue(acc, P + WS_OFF_B1, sa, kAP);
+  __syncthreads();
+  zero();
+  mfma_layer<H / 8>(sa, kAP, wsplit + 3 * n1, H, acc);
+  mfma_epilogue(acc, P + WS_OFF_B2, sf, kAP);
+  __syncthreads();
+  zero();
+  mfma_layer<H / 8>(sf, kAP, wsplit + 3 * n1 + 3 * n2, H, acc);
+  mfma_epilogue(acc, P + WS_OFF_B3, sa, kAP);
+  __syncthreads();
+  const int i = r0 + lane;
+  if (i >= a.npad) return;
+  if (i >= a.n) {
+    if (a.x_out) a.x_out[i] = 0.0;
+    if (a.t_out) a.t_out[i] = 0.0f;
+    return;
+  }
+  const float* W4 = P + WS_OFF_W4;
+  float t = P[WS_OFF_B4];
+#pragma unroll 8
+  for (int u = 0; u < H; ++u) t = fmaf(bf2f(sa[lane * kAP + u]), W4[u], t);
+  if (a.t_out) a.t_out[i] = t;
+  if (a.x_out) {
+    const double x0 = (double)t * a.inv_sqrt_vs;
+    const double w = a.pos[i].w;
+    a.x_out[i] = a.form == DS_FORM_W ? x0 / w : x0;
+  }
+}
+
 // Training: forward + back-propagation of the squared error of sample
 // idx[b] (b < B) into per-sample activations (BF16) and deltas (FP32).
 struct TrainArgs {
@@ -625,9 +789,10 @@ hipError_t ws_features(ds_problem* p, unsigned short* feat,
   return hipGetLastError();
 }
 
-hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
-                      int npad, int quant, const ds_problem* p, double* x_out,
-                      float* t_out, hipStream_t st) {
+hipError_t ws_forward(const float* P, unsigned short* wsplit,
+                      const unsigned short* feat, int n, int npad, int quant,
+                      const ds_problem* p, double* x_out, float* t_out,
+                      hipStream_t st) {
   FwdArgs a{};
   a.feat = feat;
   a.P = P;
@@ -640,8 +805,16 @@ hipError_t ws_forward(const float* P, const unsigned short* feat, int n,
   a.quant = quant;
   a.inv_sqrt_vs = p ? 1.0 / std::sqrt(p->cv.var_s) : 1.0;
   const dim3 g((npad + 63) / 64), b(64);
-  if (quant) hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, P, a);
-  else hipLaunchKernelGGL(k_fwd<false>, g, b, 0, st, P, a);
+  if (quant && wsplit) {
+    const int ns = H * (WS_K1P + 2 * H);
+    hipLaunchKernelGGL(k_split_weights, dim3((ns + 255) / 256), dim3(256), 0,
+                       st, P, wsplit);
+    hipLaunchKernelGGL(k_fwd_mfma, g, b, 0, st, P, wsplit, a);
+  } else if (quant) {
+    hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, P, a);
+  } else {
+    hipLaunchKernelGGL(k_fwd<false>, g, b, 0, st, P, a);
+  }
   return hipGetLastError();
 }
 

@@ -59,7 +59,7 @@ unsigned short host_f2bf(float f) {
 
 void free_trainer(ds::WsTrainer& T) {
   void* ps[] = {T.P, T.m, T.v, T.grad, T.part, T.act, T.delta, T.dy,
-                T.loss_part, T.iota};
+                T.loss_part, T.iota, T.wsplit};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   T = ds::WsTrainer();
@@ -112,6 +112,7 @@ int ds_ws_create(ds_ctx* ctx, unsigned long long seed, ds_warmstart** out) {
   WS_HIP(ws, hipMalloc(&T.dy, sizeof(float) * WS_MAX_BATCH));
   WS_HIP(ws, hipMalloc(&T.loss_part, sizeof(float) * (WS_MAX_BATCH / 256)));
   WS_HIP(ws, hipMalloc(&T.iota, sizeof(int) * WS_MAX_BATCH));
+  WS_HIP(ws, hipMalloc(&T.wsplit, sizeof(unsigned short) * WS_SPLIT_ELEMS));
   std::vector<int> iota(WS_MAX_BATCH);
   std::iota(iota.begin(), iota.end(), 0);
   // He-uniform weights, zero biases, from the seed
@@ -214,7 +215,8 @@ int ds_ws_predict(ds_warmstart* ws, ds_problem* p) {
   if (int rc = ensure_scratch(ws, p->npad)) return rc;
   hipStream_t st = ws->ctx->stream;
   WS_HIP(ws, ds::ws_features(p, ws->fscratch, st));
-  WS_HIP(ws, ds::ws_forward(ws->T.P, ws->fscratch, p->n, p->npad, ws->quant,
+  WS_HIP(ws, ds::ws_forward(ws->T.P, ws->T.wsplit, ws->fscratch, p->n,
+                            p->npad, ws->quant,
                             p, p->x, nullptr, st));
   WS_HIP(ws, hipStreamSynchronize(st));
   return DS_OK;
@@ -324,7 +326,8 @@ int ds_ws_eval_batch(ds_warmstart* ws, int n, const float* feat,
   float* t_out = nullptr;
   WS_HIP(ws, hipMalloc(&t_out, sizeof(float) * n));
   std::unique_ptr<float, decltype(&hipFree)> g1(t_out, hipFree);
-  WS_HIP(ws, ds::ws_forward(ws->T.P, ws->fscratch, n, n, ws->quant, nullptr,
+  WS_HIP(ws, ds::ws_forward(ws->T.P, ws->T.wsplit, ws->fscratch, n, n,
+                            ws->quant, nullptr,
                             nullptr, t_out, st));
   WS_HIP(ws, ds::ws_batch(ws->T, ws->fscratch, ws->tscratch, ws->T.iota, n,
                           ws->quant, ws->T.loss_part, st));
