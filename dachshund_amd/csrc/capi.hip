@@ -824,8 +824,20 @@ int ds_A_residual(ds_problem* p, const double* x, double* r) {
   hipStream_t st = p->ctx->stream;
   const double* w = p->form == DS_FORM_W ? p->wvec : nullptr;
   HIP_OR_FAIL(p->ctx, ds::launch_scale(p->tmp, w, p->r, nullptr, p->npad, st));
-  HIP_OR_FAIL(p->ctx, ds::launch_matvec64_ex(p, 1, p->tmp, p->r, p->q,
-                                             ds::CgEpilogue(), st));
+  if (p->shared_geometry && p->n_sub > 1 && (p->use_fused || p->use_dense)) {
+    // the multi-RHS residual pass the mixed solver uses on such batches;
+    // it acts on the subvolumes that want a true residual: all of them
+    std::vector<ds_cg_scalars> h(p->n_sub);
+    std::memset(h.data(), 0, sizeof(ds_cg_scalars) * p->n_sub);
+    for (auto& c : h) c.want_true = 1;
+    HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->sc, h.data(),
+                                       sizeof(ds_cg_scalars) * p->n_sub,
+                                       hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(p->ctx, ds::launch_res64_mrhs(p, p->tmp, p->r, p->q, st));
+  } else {
+    HIP_OR_FAIL(p->ctx, ds::launch_matvec64_ex(p, 1, p->tmp, p->r, p->q,
+                                               ds::CgEpilogue(), st));
+  }
   return download_sorted(p, p->q, r);
 }
 
@@ -1115,7 +1127,10 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
     return fail(p->ctx, DS_E_ARG, "unknown preconditioner %d", kind);
   if (kind == DS_PRECOND_BLOCK && !p->minv) {
     hipStream_t st = p->ctx->stream;
-    HIP_OR_FAIL(p->ctx, dev_alloc(&p->minv, (size_t)p->nblk * DS_BS * DS_BS));
+    // subvolumes sharing one geometry share one set of block inverses
+    p->minv_mod = p->shared_geometry ? p->nblk / p->n_sub : p->nblk;
+    HIP_OR_FAIL(p->ctx, dev_alloc(&p->minv,
+                                  (size_t)p->minv_mod * DS_BS * DS_BS));
     HIP_OR_FAIL(p->ctx, ds::bj_build(p, st));
     HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
   }

@@ -34,6 +34,7 @@ struct UpdArgs {
   const double* q;
   const double* diag;
   const float* minv;      // block-Jacobi inverses, or nullptr (Jacobi)
+  int minv_mod;
   const double* wvec;     // w (W form) or nullptr
   const int* blk_sub;
   double* x;
@@ -60,13 +61,50 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
   if (a.only_x) return;
   const double ri = a.r[i] - alpha * a.q[i];
   a.r[i] = ri;
-  const double si = a.minv ? bj_apply(a.minv + (size_t)I * DS_BS * DS_BS, ri)
-                          : ri / a.diag[i];
+  const double si =
+      a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS, ri)
+             : ri / a.diag[i];
   a.s[i] = si;
   double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
   if ((threadIdx.x & 63) == 0) {
     a.partials[I] = v[0];
     a.partials[(size_t)a.nblk + I] = v[1];
+  }
+}
+
+// The preconditioned true residual and its dot-product partials for the
+// subvolumes that asked for a true residual (after launch_res64_mrhs):
+// the per-subvolume residual epilogue's operations, one wave per block.
+struct ResPArgs {
+  const double* r;
+  const double* diag;
+  const float* minv;
+  int minv_mod;
+  const int* blk_sub;
+  const int* blk_nreal;
+  const ds_cg_scalars* sc;
+  double* s;
+  double* partials;
+  int nblk;
+};
+
+__global__ __launch_bounds__(64 * kVecWaves) void k_res_precond(ResPArgs a) {
+  const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
+  if (I >= a.nblk) return;
+  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
+  if (sc->done || !sc->want_true) return;
+  const int lane = threadIdx.x & 63;
+  const int i = I * DS_BS + lane;
+  const bool real = lane < a.blk_nreal[I];
+  const double bi = a.r[i];
+  const double si =
+      a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS, bi)
+             : (real ? bi / a.diag[i] : 0.0);
+  a.s[i] = si;
+  const double v0 = wave_sum(bi * si), v1 = wave_sum(bi * bi);
+  if (lane == 0) {
+    a.partials[I] = v0;
+    a.partials[(size_t)a.nblk + I] = v1;
   }
 }
 
@@ -192,6 +230,23 @@ hipError_t launch_precond(const double* b, const double* diag, double* x,
   return hipGetLastError();
 }
 
+hipError_t launch_res_precond(ds_problem* p, hipStream_t st) {
+  ResPArgs a{};
+  a.r = p->r;
+  a.diag = p->diag;
+  a.minv = p->use_minv() ? p->minv : nullptr;
+  a.minv_mod = p->minv_mod;
+  a.blk_sub = p->blk_sub;
+  a.blk_nreal = p->blk_nreal;
+  a.sc = p->sc;
+  a.s = p->s;
+  a.partials = p->partials;
+  a.nblk = p->nblk;
+  const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
+  hipLaunchKernelGGL(k_res_precond, g, b, 0, st, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_finalize(ds_problem* p, const CgEpilogue& e,
                            hipStream_t st) {
   FinArgs a{};
@@ -233,6 +288,7 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.q = p->q;
   a.diag = p->diag;
   a.minv = p->use_minv() ? p->minv : nullptr;
+  a.minv_mod = p->minv_mod;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
   a.blk_sub = p->blk_sub;
   a.x = p->x;
@@ -294,7 +350,15 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
     tr.is_true = 1;
     tr.need_want = 1;
     tr.drop = drop;
-    DS_TRY(launch_matvec64_ex(p, 1, p->x, p->xu, p->r, tr, st));
+    if (p->use_fused || p->use_dense) {
+      // shared geometry: one pass for every subvolume's true residual
+      DS_TRY(launch_res64_mrhs(p, p->x, p->form == DS_FORM_W ? p->xu : p->x,
+                               p->r, st));
+      DS_TRY(launch_res_precond(p, st));
+      DS_TRY(launch_finalize(p, tr, st));
+    } else {
+      DS_TRY(launch_matvec64_ex(p, 1, p->x, p->xu, p->r, tr, st));
+    }
   } else {
     DS_TRY(launch_matvec64_ex(p, 0, p->d, p->du, p->q, dq, st));
     if (iter % step_r == 0) {

@@ -145,7 +145,9 @@ struct ds_problem {
   int* blk_sub = nullptr;              // subvolume of each block
   int* blk_nreal = nullptr;            // real rows of each block
   unsigned char* tmask = nullptr;      // per list entry: live sub-blocks
-  float* minv = nullptr;               // block-Jacobi inverses, nblk x 64 x 64
+  float* minv = nullptr;               // block-Jacobi inverses, 64 x 64 each
+  int minv_mod = 1;                    // block I uses inverse I % minv_mod
+                                       // (shared geometry: one set)
   int precond_kind = DS_PRECOND_JACOBI;
   bool use_minv() const { return precond_kind == DS_PRECOND_BLOCK && minv; }
   int* knn_nbr = nullptr;              // learned warm start: K nearest
@@ -203,6 +205,12 @@ hipError_t launch_scale(const double* v, const double* w, double* u,
 hipError_t bj_build(ds_problem* p, hipStream_t st);
 hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
                             hipStream_t st);
+// Shared-geometry batches: the true residual of every subvolume that wants
+// one (one pass over the tile lists for all right-hand sides), then its
+// preconditioned residual and dot-product partials.
+hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
+                             const double* u, double* out, hipStream_t st);
+hipError_t launch_res_precond(ds_problem* p, hipStream_t st);
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st);
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,

@@ -66,6 +66,7 @@ struct MatvecArgs {
   const double* rhs;      // wd or d (residual)
   const double* diag;     // Jacobi diagonal (CG residual epilogue)
   const float* minv;      // block-Jacobi inverses, or nullptr
+  int minv_mod;
   const int* tptr;
   const int* tidx;
   const double2* table;
@@ -126,7 +127,8 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
       v[0] = ai * bi;  // d . q
     } else if (EPI == EPI_CG_RES) {
       const double si =
-          a.minv ? bj_apply(a.minv + (size_t)I * DS_BS * DS_BS, bi)
+          a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS,
+                            bi)
                  : (real ? bi / a.diag[row] : 0.0);
       a.out2[row] = si;
       v[0] = bi * si;  // r . M^-1 r
@@ -140,6 +142,109 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   } else {
     double p2[2] = {wave_sum(v[0]), wave_sum(v[1])};
     store_partials<2>(p2, a.partials, a.nblk, I);
+  }
+}
+
+// Multi-RHS FP64 residual of a shared-geometry batch (every subvolume the
+// same pixels, weights and covariance; subvolume r's vector at r * ld):
+// out_r = rhs_r - A x_r for every subvolume that wants a true residual,
+// from one pass over subvolume 0's tile lists.  Each pair's value is the
+// parity kernel's (same operations), and each right-hand side accumulates
+// it in the same source order with the same wave split, so every residual
+// equals the per-subvolume kernel's bit for bit.  The 16 sources x 32
+// right-hand sides of a tile are staged in LDS per wave and read as
+// broadcasts.
+constexpr int kMR = DS_DENSE_R;   // right-hand sides per pass
+
+struct Res64Args {
+  const double4* pos;
+  const double* x;        // x of every subvolume
+  const double* xu;       // w .* x (W) or x (SN)
+  const double* rhs;
+  const int* tptr;        // subvolume 0's lists
+  const double2* table;
+  const int* blk_nreal;
+  const ds_cg_scalars* sc;
+  double* out;
+  double var_s, c_perp, c_para, dx_inv, tlast;
+  int nblk1, ld, n_sub, table_n;
+};
+
+template <int FORM>
+__global__ __launch_bounds__(256) void k_res64_mrhs(
+    const double4* __restrict__ pos, const int* __restrict__ tidx,
+    Res64Args a) {
+  extern __shared__ double2 tab[];
+  __shared__ double us[4][DS_TS][kMR];
+  __shared__ double red[3][64];
+  bool want = false;
+  for (int r = 0; r < a.n_sub; ++r)
+    want = want || (!a.sc[r].done && a.sc[r].want_true);
+  if (!want) return;   // uniform: no subvolume asked for a true residual
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
+  __syncthreads();
+  const int I = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
+  const int row = I * DS_BS + lane;   // subvolume 0's slot
+  const double4 pi = pos[row];
+  double acc[kMR];
+#pragma unroll
+  for (int r = 0; r < kMR; ++r) acc[r] = 0.0;
+  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
+  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  for (int k = kb; k < ke; k += nw) {
+    const int j0 = __builtin_amdgcn_readfirstlane(tidx[k]) * DS_TS;
+    // stage u_r[j0 + jj] (lane: rhs lane % 32, sources lane / 32 + 2q)
+#pragma unroll
+    for (int q = 0; q < DS_TS / 2; ++q) {
+      const int jj = (lane >> 5) + 2 * q, r = lane & 31;
+      us[wave][jj][r] =
+          r < a.n_sub ? a.xu[(size_t)r * a.ld + j0 + jj] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 2
+    for (int jj = 0; jj < DS_TS; ++jj) {
+      const double4 pj = pos[j0 + jj];
+      const double dx = pi.x - pj.x;
+      const double dy = pi.y - pj.y;
+      const double dz = pi.z - pj.z;
+      const double x_perp_2 = dx * dx + dy * dy;
+      const double x_para_2 = dz * dz;
+      double xx = (a.c_perp * x_perp_2 + a.c_para * x_para_2) * a.dx_inv;
+      xx = fmin(xx, a.tlast);
+      const int ix = (int)xx;
+      const double t = xx - (double)ix;
+      const double2 f = tab[ix];
+      const double gp = fma(f.y, t, f.x);
+#pragma unroll
+      for (int r = 0; r < kMR; ++r) acc[r] = fma(gp, us[wave][jj][r], acc[r]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const bool real = lane < a.blk_nreal[I];
+  for (int r = 0; r < a.n_sub; ++r) {
+    if (a.sc[r].done || !a.sc[r].want_true) continue;   // uniform
+    double mine = acc[0];
+#pragma unroll
+    for (int q = 1; q < kMR; ++q)
+      if (q == r) mine = acc[q];
+    if (wave > 0) red[wave - 1][lane] = mine;
+    __syncthreads();
+    if (wave == 0) {
+      // rows in wave order, as the per-subvolume kernel sums them
+      double y = mine;
+      for (int w = 1; w < nw; ++w) y += red[w - 1][lane];
+      const size_t g = (size_t)r * a.ld + row;
+      const double ai = a.x[g];
+      double bi = (FORM == DS_FORM_W) ? pi.w * (a.var_s * y) + ai
+                                      : a.var_s * y + pi.w * ai;
+      bi = a.rhs[g] - bi;
+      if (!real) bi = 0.0;
+      a.out[g] = bi;
+    }
+    __syncthreads();
   }
 }
 
@@ -244,6 +349,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.rhs = p->rhs64;
   a.diag = p->diag;
   a.minv = p->use_minv() ? p->minv : nullptr;
+  a.minv_mod = p->minv_mod;
   a.tptr = p->tptr;
   a.tidx = p->tidx;
   a.table = p->table64;
@@ -278,6 +384,38 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess || e.kind == EPI_NONE) return err;
   return launch_finalize(p, e, st);
+}
+
+hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
+                             const double* u, double* out, hipStream_t st) {
+  Res64Args a{};
+  a.pos = p->pos64;
+  a.x = in;
+  a.xu = u;
+  a.rhs = p->rhs64;
+  a.tptr = p->tptr;
+  a.table = p->table64;
+  a.blk_nreal = p->blk_nreal;
+  a.sc = p->sc;
+  a.out = out;
+  a.var_s = p->sub_cv_h[0].var_s;
+  a.c_perp = p->sub_cv_h[0].c_perp;
+  a.c_para = p->sub_cv_h[0].c_para;
+  a.dx_inv = p->cv.dx_inv;
+  a.tlast = (double)(p->cv.table_n - 1);
+  a.nblk1 = p->nblk / p->n_sub;
+  a.ld = p->npad / p->n_sub;
+  a.n_sub = p->n_sub;
+  a.table_n = p->cv.table_n;
+  const dim3 grid(a.nblk1), block(64 * p->waves_per_block);
+  const size_t shm = sizeof(double2) * p->cv.table_n;
+  if (p->form == DS_FORM_W)
+    hipLaunchKernelGGL((k_res64_mrhs<DS_FORM_W>), grid, block, shm, st,
+                       p->pos64, p->tidx, a);
+  else
+    hipLaunchKernelGGL((k_res64_mrhs<DS_FORM_SN>), grid, block, shm, st,
+                       p->pos64, p->tidx, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,

@@ -84,11 +84,11 @@ __global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
 // x = M^-1 b for every target block (one wave per block).
 __global__ __launch_bounds__(256) void k_bj_apply(const double* b,
                                                   const float* minv, double* x,
-                                                  int nblk) {
+                                                  int nblk, int mod) {
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (I >= nblk) return;
   const int i = I * DS_BS + (threadIdx.x & 63);
-  x[i] = bj_apply(minv + (size_t)I * DS_BS * DS_BS, b[i]);
+  x[i] = bj_apply(minv + (size_t)(I % mod) * DS_BS * DS_BS, b[i]);
 }
 
 }  // namespace
@@ -104,7 +104,7 @@ hipError_t bj_build(ds_problem* p, hipStream_t st) {
   a.dx_inv = p->cv.dx_inv;
   a.table_n = p->cv.table_n;
   a.form = p->form;
-  hipLaunchKernelGGL(k_bj_build, dim3(p->nblk), dim3(DS_BS), 0, st, a);
+  hipLaunchKernelGGL(k_bj_build, dim3(p->minv_mod), dim3(DS_BS), 0, st, a);
   return hipGetLastError();
 }
 
@@ -112,7 +112,7 @@ hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
                             hipStream_t st) {
   if (!p->use_minv()) return launch_precond(b, p->diag, x, p->npad, st);
   hipLaunchKernelGGL(k_bj_apply, dim3((p->nblk + 3) / 4), dim3(256), 0, st, b,
-                     p->minv, x, p->nblk);
+                     p->minv, x, p->nblk, p->minv_mod);
   return hipGetLastError();
 }
 
