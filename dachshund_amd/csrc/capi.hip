@@ -326,6 +326,8 @@ void problem_free(ds_problem* p) {
   if (p->sc_host) (void)hipHostFree(p->sc_host);
   if (p->done_host) (void)hipHostFree(p->done_host);
   p->done_host = nullptr;
+  if (p->norm_host) (void)hipHostFree(p->norm_host);
+  p->norm_host = nullptr;
   if (p->host_tmp) (void)hipHostFree(p->host_tmp);
   p->sc_host = nullptr;
   p->host_tmp = nullptr;
@@ -715,6 +717,8 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
                                  sizeof(ds_cg_scalars) * n_sub,
                                  hipHostMallocDefault));
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->done_host, 2 * sizeof(int),
+                                 hipHostMallocDefault));
+  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->norm_host, 2 * sizeof(double),
                                  hipHostMallocDefault));
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->host_tmp, sizeof(double) * npad,
                                  hipHostMallocDefault));

@@ -440,6 +440,7 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     DS_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     DS_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int iter = 1, k = 0;
+    double norm_prev = 0.0;
     hipError_t err = hipSuccess;
     while (iter < max_iter && err == hipSuccess) {
       const int end = std::min(max_iter, iter + chunk);
@@ -462,10 +463,25 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
       if (err != hipSuccess) break;
       err = hipMemcpyAsync(&p->done_host[k & 1], p->done_count, sizeof(int),
                            hipMemcpyDeviceToHost, st);
+      if (err == hipSuccess && ns == 1)
+        err = hipMemcpyAsync(&p->norm_host[k & 1], &p->sc[0].norm_r,
+                             sizeof(double), hipMemcpyDeviceToHost, st);
       if (err == hipSuccess) err = hipEventRecord(ev[k & 1], st);
       if (err == hipSuccess && k > 0) {
         err = hipEventSynchronize(ev[(k - 1) & 1]);
         if (p->done_host[(k - 1) & 1] >= ns) break;
+        // Single volume near the end: when the decay over the last chunk
+        // predicts the tolerance within the chunk just enqueued, wait for
+        // it before enqueuing another, so that no whole chunk of launches
+        // runs after convergence.  Scheduling only; results are unchanged.
+        const double n2 = ns == 1 ? p->norm_host[(k - 1) & 1] : 0.0;
+        if (err == hipSuccess && ns == 1 && k > 1 && norm_prev > 0.0 &&
+            n2 > 0.0 && n2 < norm_prev &&
+            n2 * (n2 / norm_prev) <= 2.0 * h[0].tol_norm_b) {
+          err = hipEventSynchronize(ev[k & 1]);
+          if (err == hipSuccess && p->done_host[k & 1] >= ns) break;
+        }
+        norm_prev = n2;
       }
       ++k;
     }

@@ -194,6 +194,7 @@ struct ds_problem {
   int norms_cap = 0;
   ds_cg_scalars* sc_host = nullptr;    // pinned, n_sub entries (final)
   int* done_host = nullptr;            // pinned poll slots (2)
+  double* norm_host = nullptr;         // pinned: |r| at chunk ends (2)
   double* host_tmp = nullptr;          // pinned npad doubles
 };
 
