@@ -837,7 +837,7 @@ int ds_A_residual(ds_problem* p, const double* x, double* r) {
     HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->sc, h.data(),
                                        sizeof(ds_cg_scalars) * p->n_sub,
                                        hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(p->ctx, ds::launch_res64_mrhs(p, p->tmp, p->r, p->q, st));
+    HIP_OR_FAIL(p->ctx, ds::launch_res64_mrhs(p, p->tmp, p->r, p->q, 1, st));
   } else {
     HIP_OR_FAIL(p->ctx, ds::launch_matvec64_ex(p, 1, p->tmp, p->r, p->q,
                                                ds::CgEpilogue(), st));

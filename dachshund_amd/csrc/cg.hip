@@ -86,13 +86,14 @@ struct ResPArgs {
   double* s;
   double* partials;
   int nblk;
+  int all;   // every live subvolume (initial residual)
 };
 
 __global__ __launch_bounds__(64 * kVecWaves) void k_res_precond(ResPArgs a) {
   const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
   if (I >= a.nblk) return;
   const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
-  if (sc->done || !sc->want_true) return;
+  if (sc->done || !(a.all || sc->want_true)) return;
   const int lane = threadIdx.x & 63;
   const int i = I * DS_BS + lane;
   const bool real = lane < a.blk_nreal[I];
@@ -139,6 +140,42 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
     finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0],
                   v[NV - 1], a.init, a.mixed, a.drop, a.is_true,
                   a.done_count);
+}
+
+// Shared-geometry batches with block-Jacobi: the subvolumes share the
+// block inverses, so one workgroup per geometry block stages its 16 KB
+// inverse in LDS once and applies it to that block of every subvolume
+// (wave w takes subvolumes w, w + kVecWaves, ...).  Same operations as
+// k_cg_update, so the results are the same bit for bit.
+__global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
+    UpdArgs a, int nblk1, int n_sub) {
+  __shared__ float mt[DS_BS * DS_BS];
+  const int J = blockIdx.x;
+  const float* src = a.minv + (size_t)J * DS_BS * DS_BS;
+  for (int k = threadIdx.x; k < DS_BS * DS_BS; k += blockDim.x)
+    mt[k] = src[k];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int sv = wave; sv < n_sub; sv += kVecWaves) {
+    const ds_cg_scalars* sc = a.sc + sv;
+    if (sc->done) continue;
+    const double alpha = sc->alpha;
+    const int I = sv * nblk1 + J;
+    const int i = I * DS_BS + lane;
+    const double xi = a.x[i] + alpha * a.d[i];
+    a.x[i] = xi;
+    if (a.wvec) a.xu[i] = a.wvec[i] * xi;
+    if (a.only_x) continue;
+    const double ri = a.r[i] - alpha * a.q[i];
+    a.r[i] = ri;
+    const double si = bj_apply(mt, ri);
+    a.s[i] = si;
+    const double v0 = wave_sum(ri * si), v1 = wave_sum(ri * ri);
+    if (lane == 0) {
+      a.partials[I] = v0;
+      a.partials[(size_t)a.nblk + I] = v1;
+    }
+  }
 }
 
 struct DirArgs {
@@ -230,7 +267,8 @@ hipError_t launch_precond(const double* b, const double* diag, double* x,
   return hipGetLastError();
 }
 
-hipError_t launch_res_precond(ds_problem* p, hipStream_t st) {
+hipError_t launch_res_precond(ds_problem* p, double* out2, int all,
+                              hipStream_t st) {
   ResPArgs a{};
   a.r = p->r;
   a.diag = p->diag;
@@ -239,9 +277,10 @@ hipError_t launch_res_precond(ds_problem* p, hipStream_t st) {
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
   a.sc = p->sc;
-  a.s = p->s;
+  a.s = out2;
   a.partials = p->partials;
   a.nblk = p->nblk;
+  a.all = all;
   const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
   hipLaunchKernelGGL(k_res_precond, g, b, 0, st, a);
   return hipGetLastError();
@@ -299,8 +338,14 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.sc = p->sc;
   a.nblk = p->nblk;
   a.only_x = only_x;
-  const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
-  hipLaunchKernelGGL(k_cg_update, g, b, 0, st, a);
+  const int nblk1 = p->nblk / p->n_sub;
+  if (a.minv && p->shared_geometry && p->n_sub > 1 && p->minv_mod == nblk1) {
+    hipLaunchKernelGGL(k_cg_update_shared, dim3(nblk1), dim3(64 * kVecWaves),
+                       0, st, a, nblk1, p->n_sub);
+  } else {
+    const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
+    hipLaunchKernelGGL(k_cg_update, g, b, 0, st, a);
+  }
   DS_TRY(hipGetLastError());
   if (only_x) return hipSuccess;
   CgEpilogue e;
@@ -353,8 +398,8 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
     if (p->use_fused || p->use_dense) {
       // shared geometry: one pass for every subvolume's true residual
       DS_TRY(launch_res64_mrhs(p, p->x, p->form == DS_FORM_W ? p->xu : p->x,
-                               p->r, st));
-      DS_TRY(launch_res_precond(p, st));
+                               p->r, 0, st));
+      DS_TRY(launch_res_precond(p, p->s, 0, st));
       DS_TRY(launch_finalize(p, tr, st));
     } else {
       DS_TRY(launch_matvec64_ex(p, 1, p->x, p->xu, p->r, tr, st));
@@ -424,7 +469,15 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     init.out2 = p->d;
     init.init = 1;
     init.mixed = mixed;
-    DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init, st));
+    if (mixed && (p->use_fused || p->use_dense)) {
+      // shared geometry: every subvolume's initial residual in one pass
+      DS_TRY(launch_res64_mrhs(p, p->x, w ? p->xu : p->x, p->r, 1, st));
+      DS_TRY(launch_res_precond(p, p->d, 1, st));
+      DS_TRY(launch_finalize(p, init, st));
+    } else {
+      DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init,
+                                st));
+    }
     DS_TRY(launch_scale(p->d, w, w ? p->du : nullptr,
                         mixed ? p->du32 : nullptr, p->npad, st));
     if (mixed && (p->use_dense || p->use_fused))

@@ -210,8 +210,10 @@ hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
 // one (one pass over the tile lists for all right-hand sides), then its
 // preconditioned residual and dot-product partials.
 hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
-                             const double* u, double* out, hipStream_t st);
-hipError_t launch_res_precond(ds_problem* p, hipStream_t st);
+                             const double* u, double* out, int all,
+                             hipStream_t st);
+hipError_t launch_res_precond(ds_problem* p, double* out2, int all,
+                              hipStream_t st);
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st);
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,

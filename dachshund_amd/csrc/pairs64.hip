@@ -168,6 +168,7 @@ struct Res64Args {
   double* out;
   double var_s, c_perp, c_para, dx_inv, tlast;
   int nblk1, ld, n_sub, table_n;
+  int all;                // every live subvolume (initial residual)
 };
 
 template <int FORM>
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
   __shared__ double red[3][64];
   bool want = false;
   for (int r = 0; r < a.n_sub; ++r)
-    want = want || (!a.sc[r].done && a.sc[r].want_true);
+    want = want || (!a.sc[r].done && (a.all || a.sc[r].want_true));
   if (!want) return;   // uniform: no subvolume asked for a true residual
   for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
   __syncthreads();
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
   }
   const bool real = lane < a.blk_nreal[I];
   for (int r = 0; r < a.n_sub; ++r) {
-    if (a.sc[r].done || !a.sc[r].want_true) continue;   // uniform
+    if (a.sc[r].done || !(a.all || a.sc[r].want_true)) continue;  // uniform
     double mine = acc[0];
 #pragma unroll
     for (int q = 1; q < kMR; ++q)
@@ -387,7 +388,8 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
 }
 
 hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
-                             const double* u, double* out, hipStream_t st) {
+                             const double* u, double* out, int all,
+                             hipStream_t st) {
   Res64Args a{};
   a.pos = p->pos64;
   a.x = in;
@@ -407,6 +409,7 @@ hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
   a.ld = p->npad / p->n_sub;
   a.n_sub = p->n_sub;
   a.table_n = p->cv.table_n;
+  a.all = all;
   const dim3 grid(a.nblk1), block(64 * p->waves_per_block);
   const size_t shm = sizeof(double2) * p->cv.table_n;
   if (p->form == DS_FORM_W)
