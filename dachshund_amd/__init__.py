@@ -4,7 +4,8 @@ The product is libdachshund_gpu.so (HIP kernels behind the C ABI of
 include/dachshund_gpu.h) and the C++ drop-in of the reference's library API
 built on it; this package is the Python mirror of the same interface.
 """
-from .wiener import (DS_F32_MFMA, DS_F32_VALU, DS_PREC_F32, DS_PREC_F64,
+from .wiener import (DS_F32_MFMA, DS_F32_MFMA_CLAMPED, DS_F32_VALU,
+                     DS_PREC_F32, DS_PREC_F64,
                      DS_PRECOND_BLOCK, DS_PRECOND_JACOBI,
                      BatchWF, Context, PCGParams,
                      PCGResult,
@@ -16,7 +17,7 @@ from .wiener import (DS_F32_MFMA, DS_F32_VALU, DS_PREC_F32, DS_PREC_F64,
                      wpixel_to_pixel)
 
 __all__ = [
-    "DS_F32_MFMA", "DS_F32_VALU", "DS_PRECOND_BLOCK", "DS_PRECOND_JACOBI", "DS_PREC_F32", "DS_PREC_F64", "BatchWF", "Context", "PCGParams", "PCGResult",
+    "DS_F32_MFMA", "DS_F32_MFMA_CLAMPED", "DS_F32_VALU", "DS_PRECOND_BLOCK", "DS_PRECOND_JACOBI", "DS_PREC_F32", "DS_PREC_F64", "BatchWF", "Context", "PCGParams", "PCGResult",
     "SignalCovarParams", "WFVectorSN", "WFVectorW", "default_context",
     "init_uniform_map_coords", "map_covar_diag", "npixel_to_pixel", "pixel_to_npixel",
     "pixel_to_wpixel", "plan_tiles", "points_from_pixels", "read_pixel_data",

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libdachshund_gpu.so")
 DS_OK, DS_E_ARG, DS_E_HIP, DS_E_NOMEM, DS_E_STATE = range(5)
 DS_FORM_SN, DS_FORM_W = 0, 1
 DS_PREC_F64, DS_PREC_F32 = 0, 1
-DS_F32_VALU, DS_F32_MFMA = 0, 1
+DS_F32_VALU, DS_F32_MFMA, DS_F32_MFMA_CLAMPED = 0, 1, 2
 DS_PRECOND_JACOBI, DS_PRECOND_BLOCK = 0, 1
 
 _dp = C.POINTER(C.c_double)

@@ -299,6 +299,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
+  dev_free(p->table32x);
   dev_free(p->table_raw);
   dev_free(p->blk_sub);
   dev_free(p->blk_nreal);
@@ -487,6 +488,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::vector<Blocking> blks(n_sub);
   std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
   std::vector<std::vector<unsigned char>> msks(n_sub);
+  std::vector<double> far(n_sub, 0.0);
   auto build = [&](int k) {
     const ds::CovarDev cv = ds::make_covar(&covars[k]);
     p->sub_cv_h[k] = cv;
@@ -497,7 +499,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     // a block's 16-row sub-blocks are its own tiles (targets = sources)
     ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
                          blks[k], cv, ptrs[k], idxs[k], blks[k].tlo.data(),
-                         blks[k].thi.data(), &msks[k]);
+                         blks[k].thi.data(), &msks[k], &far[k]);
     // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
     double sum_b2 = 0.0;
     for (int i = 0; i < sub_n[k]; ++i) {
@@ -675,6 +677,22 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::vector<double2> t64;
   std::vector<float2> t32;
   make_tables(&covars[0], t64, t32);
+  // The clamp-free FP32 MFMA kernel reads entry (int)(xx + 1): entry 0
+  // holds f_0 (arguments rounded below 0), entries 1..table_n the table,
+  // zeros up past the largest argument any listed pair can reach (box
+  // bound, plus room for FP32 rounding of frames and products).
+  std::vector<float2> t32x;
+  {
+    double far_max = 0.0;
+    for (double v : far) far_max = std::max(far_max, v);
+    const double need = far_max * 1.01 + 4.0;
+    if (need < (double)DS_TABLE_X_MAX) {
+      const int len = std::max((int)need + 1, (int)t32.size() + 1);
+      t32x.assign(len, make_float2(0.0f, 0.0f));
+      t32x[0] = make_float2(t32[0].x, 0.0f);
+      for (size_t k = 0; k < t32.size(); ++k) t32x[k + 1] = t32[k];
+    }
+  }
 
   hipStream_t st = ctx->stream;
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -735,6 +753,11 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
+  if (!t32x.empty()) {
+    HIP_OR_FAIL(ctx, dev_alloc(&p->table32x, t32x.size()));
+    HIP_OR_FAIL(ctx, up(p->table32x, t32x.data(), sizeof(float2) * t32x.size()));
+    p->tlen_x = (int)t32x.size();
+  }
   HIP_OR_FAIL(ctx, up(p->table_raw, p->table.data(),
                       sizeof(double) * p->table.size()));
   HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
@@ -1144,7 +1167,7 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
 
 int ds_problem_set_f32_kernel(ds_problem* p, int kind) {
   if (!p) return DS_E_ARG;
-  if (kind != DS_F32_VALU && kind != DS_F32_MFMA)
+  if (kind != DS_F32_VALU && kind != DS_F32_MFMA && kind != DS_F32_MFMA_CLAMPED)
     return fail(p->ctx, DS_E_ARG, "unknown FP32 kernel kind %d", kind);
   p->f32_kind = kind;
   return DS_OK;

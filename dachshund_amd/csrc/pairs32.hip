@@ -52,6 +52,8 @@ struct Matvec32Args {
   const unsigned char* tmask;   // per list entry: live 16-row sub-blocks
   const float4* toff;     // per list entry: block frame - tile frame
   const float2* table;    // {f[k], f[k+1]-f[k]}, last = {0, 0}
+  const float2* table_x;  // shifted, zero-extended copy (clamp-free kernel)
+  int tlen_x;
   const int* blk_sub;
   const int* blk_nreal;
   const int* sub_blk0;
@@ -135,7 +137,12 @@ __global__ __launch_bounds__(256) void k_matvec32(
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int FORM, bool CG>
+// NOCLAMP: the accumulator input carries |l|^2 + 1, so the MFMA yields
+// xx + 1 > 0 for every pair and (int) truncation needs no clamp below; the
+// shifted table (entry 0 = {f_0, 0}: arguments rounded below 0) is
+// zero-extended past the largest argument the problem's lists can reach,
+// so none above either.  5 VALU operations per pair.
+template <int FORM, bool CG, bool NOCLAMP>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
     const int* __restrict__ tidx, const float4* __restrict__ toff,
@@ -146,8 +153,11 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   const int sv = a.blk_sub[I];
   const ds_cg_scalars* sc = a.sc + sv;
   if (CG && sc->done) return;
-  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
-    tab32[k] = a.table[k];
+  {
+    const float2* tsrc = NOCLAMP ? a.table_x : a.table;
+    const int tlen = NOCLAMP ? a.tlen_x : a.table_n;
+    for (int k = threadIdx.x; k < tlen; k += blockDim.x) tab32[k] = tsrc[k];
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -164,7 +174,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float4 m = a.tgt[row0 + 16 * s + 4 * g + r];
-      ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x));
+      ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x)) +
+                 (NOCLAMP ? 1.0f : 0.0f);
     }
   }
   const float cx = g == 0 ? -2.0f : 0.0f, cy = g == 1 ? -2.0f : 0.0f;
@@ -211,7 +222,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
                                                         0, 0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float xx = __builtin_amdgcn_fmed3f(d[r], 0.0f, tlast);
+        const float xx =
+            NOCLAMP ? d[r] : __builtin_amdgcn_fmed3f(d[r], 0.0f, tlast);
         const int ix = (int)xx;
         const float t = __builtin_amdgcn_fractf(xx);
         const float2 f = tab32[ix];
@@ -269,6 +281,8 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.tmask = p->tmask;
   a.toff = p->toff32;
   a.table = p->table32;
+  a.table_x = p->table32x;
+  a.tlen_x = p->tlen_x;
   a.out = out;
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
@@ -287,12 +301,19 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   hipLaunchKernelGGL((k_matvec32<F, C, D>), grid, block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
   const bool d = p->f32_direct;
-  if (!d && p->f32_kind == DS_F32_MFMA) {
-#define DS_L32M(F, C) \
-  hipLaunchKernelGGL((k_matvec32m<F, C>), grid, block, shm, st, a.loc, a.u32, \
-                     a.tidx, a.toff, a)
-    if (cg) { if (w) DS_L32M(DS_FORM_W, true); else DS_L32M(DS_FORM_SN, true); }
-    else { if (w) DS_L32M(DS_FORM_W, false); else DS_L32M(DS_FORM_SN, false); }
+  if (!d && p->f32_kind != DS_F32_VALU) {
+    const bool nc = p->f32_kind == DS_F32_MFMA && p->tlen_x > 0;
+    const size_t shm_m = sizeof(float2) * (nc ? p->tlen_x : p->cv.table_n);
+#define DS_L32M(F, C, N) \
+  hipLaunchKernelGGL((k_matvec32m<F, C, N>), grid, block, shm_m, st, a.loc, \
+                     a.u32, a.tidx, a.toff, a)
+    if (nc) {
+      if (cg) { if (w) DS_L32M(DS_FORM_W, true, true); else DS_L32M(DS_FORM_SN, true, true); }
+      else { if (w) DS_L32M(DS_FORM_W, false, true); else DS_L32M(DS_FORM_SN, false, true); }
+    } else {
+      if (cg) { if (w) DS_L32M(DS_FORM_W, true, false); else DS_L32M(DS_FORM_SN, true, false); }
+      else { if (w) DS_L32M(DS_FORM_W, false, false); else DS_L32M(DS_FORM_SN, false, false); }
+    }
 #undef DS_L32M
   } else if (cg) {
     if (w) { if (d) DS_L32(DS_FORM_W, true, true); else DS_L32(DS_FORM_W, true, false); }

@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (DS_F32_MFMA, DS_F32_VALU, DS_FORM_SN, DS_FORM_W,
+from ._lib import (DS_F32_MFMA, DS_F32_MFMA_CLAMPED, DS_F32_VALU, DS_FORM_SN,
+                   DS_FORM_W,
                    DS_PRECOND_BLOCK, DS_PRECOND_JACOBI,
                    DS_PREC_F32, DS_PREC_F64, DsCovar, DsPcgParams,
                    DsPcgResult, DsProblemStats, check, lib)

@@ -88,7 +88,10 @@ def test_matvec64_residual_precond(D, ora, case, form, params):
                   _cutoff_allowance(fp, form, params, a))
 
 
-@pytest.mark.parametrize("kind", ["valu", "mfma"])
+F32_KINDS = {"valu": 0, "mfma": 1, "mfma_clamped": 2}
+
+
+@pytest.mark.parametrize("kind", sorted(F32_KINDS))
 @pytest.mark.parametrize("form", ["w", "sn"])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_matvec32_kernels(D, ora, case, form, kind):
@@ -100,7 +103,7 @@ def test_matvec32_kernels(D, ora, case, form, kind):
     fp = ora.to_form(pix, f)
     c = ora.covar(*params)
     wf, _ = _problem(D, form, pix, params)
-    wf.set_f32_kernel(D.DS_F32_VALU if kind == "valu" else D.DS_F32_MFMA)
+    wf.set_f32_kernel(F32_KINDS[kind])
     a = np.random.default_rng(len(pix) + 7).standard_normal(len(pix))
     bound = ora.A_product(f, fp, c, np.abs(a))
     b32 = wf.A_product(a, prec=D.DS_PREC_F32)
@@ -109,7 +112,7 @@ def test_matvec32_kernels(D, ora, case, form, kind):
                   _cutoff_allowance(fp, form, params, a))
 
 
-@pytest.mark.parametrize("kind", ["valu", "mfma"])
+@pytest.mark.parametrize("kind", sorted(F32_KINDS))
 def test_matvec32_per_pair(D, ora, kind):
     """Per pair: A e_j is column j of A, so each entry is one pair's
     covariance.  FP32 covariance error <= 2e-6 var_s per pair (SN form, so
@@ -120,7 +123,7 @@ def test_matvec32_per_pair(D, ora, kind):
     fp = ora.to_form(pix, 0)
     c = ora.covar(*params)
     wf, _ = _problem(D, "sn", pix, params)
-    wf.set_f32_kernel(D.DS_F32_VALU if kind == "valu" else D.DS_F32_MFMA)
+    wf.set_f32_kernel(F32_KINDS[kind])
     n = len(pix)
     for j in np.random.default_rng(3).choice(n, 6, replace=False):
         e = np.zeros(n)
