@@ -226,6 +226,80 @@ def bench_block(args):
     }), flush=True)
 
 
+def bench_chunk(args):
+    """76,800 pixels (240 skewers x 320, lognormal noise) and the 60 x 48 x
+    880 map grid (2.53M cells): plain FP64 PCG and the mixed solve to
+    1e-6, then the S_mp x projection onto the grid (lib/map.cc:20-43), the
+    step that dominates this configuration in the reference (1.95e11 pairs
+    dense)."""
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    w = synth.workload("c2")
+    pix = synth.make_pixels(w)
+    n = len(pix)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    st = wf.stats()
+    prm = D.PCGParams(n, w.max_iter, w.step_r, w.tol)
+    log(f"c2: n={n}, setup {st.setup_seconds:.2f} s, {st.pairs_per_matvec:.3e}"
+        f" pairs per FP64 matvec")
+
+    def solve(mixed, block):
+        wf.set_precond(D.DS_PRECOND_BLOCK if block else D.DS_PRECOND_JACOBI)
+        wf.fill_x(0.0)
+        return (wf.solve_resident(prm, mixed=True) if mixed
+                else wf.solve_resident(prm))
+
+    out = {}
+    for name, mixed, block in (("plain_fp64", False, False),
+                               ("mixed_block_jacobi", True, True)):
+        for _ in range(args.warmup):
+            solve(mixed, block)
+        ts = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            r = solve(mixed, block)
+            ts.append(time.perf_counter() - t)
+            assert r.success, name
+        out[name] = {"s": float(np.mean(ts)), "iters": r.num_iters}
+        log(f"{name}: {np.mean(ts) * 1e3:.1f} ms, {r.num_iters} iters")
+    ms64 = wf.time_matvec(D.DS_PREC_F64, reps=20)
+    ms32 = wf.time_matvec(D.DS_PREC_F32, reps=20)
+    log(f"matvec: FP64 {ms64:.3f} ms ({st.pairs_per_matvec / ms64 * 1e3:.3e} "
+        f"pairs/s), FP32 {ms32:.3f} ms "
+        f"({st.sub_pairs_per_matvec / ms32 * 1e3:.3e} pairs/s)")
+    box, shape = w.box, w.map_shape
+    cells = int(np.prod(shape))
+    wf.smp_grid(*box, *shape)
+    ts = []
+    for _ in range(max(args.steps, 1)):
+        t = time.perf_counter()
+        m = wf.smp_grid(*box, *shape)
+        ts.append(time.perf_counter() - t)
+    t_map = float(np.mean(ts))
+    log(f"S_mp x onto {shape} ({cells} cells): {t_map * 1e3:.1f} ms "
+        f"(host copy of the map included), {n * cells / t_map:.3e} "
+        f"dense-equivalent pairs/s")
+    print(json.dumps({
+        "metric": "survey-chunk solve + map projection (76,800 px, 2.53M "
+                  "cells)",
+        "value": n * cells / t_map,
+        "unit": "dense-equivalent map pair-evals/s (S_mp x)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_map * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded GRF skewers, lognormal noise)",
+        "config": {"workload": f"{n} pixels (240 skewers x 320), box "
+                               f"{w.box}, map {shape}", "tol": w.tol},
+        "time_to_tol_s": {k: v["s"] for k, v in out.items()},
+        "iterations": {k: v["iters"] for k, v in out.items()},
+        "matvec_ms": {"f64": ms64, "f32": ms32},
+        "pairs_per_matvec": {"f64": st.pairs_per_matvec,
+                             "f32": st.sub_pairs_per_matvec},
+        "map_s": t_map, "map_norm": float(np.linalg.norm(m)),
+    }), flush=True)
+
+
 def bench_batch(args):
     """512 subvolumes with log-uniform pixel counts in [2000, 40000] and
     per-subvolume correlation lengths: one batched mixed solve (every
@@ -326,6 +400,8 @@ def main():
         return bench_block(args)
     if args.workload == "c4":
         return bench_batch(args)
+    if args.workload == "c2":
+        return bench_chunk(args)
 
     import dachshund_amd as D
     from dachshund_amd import synth

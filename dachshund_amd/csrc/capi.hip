@@ -7,9 +7,11 @@
 // the context; nothing here exits or falls back to host arithmetic.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -202,24 +204,61 @@ int check_grid(ds_ctx* ctx, double lx, double ly, double lz, int nx, int ny,
 int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g,
                  const Blocking& blk, const double4* pos, const double* u,
                  const double2* table64, const CovarDev& cv, double* m_host) {
-  std::vector<double> lo, hi;
-  grid_tiles(g, lo, hi);
   const int nt = g.tx * g.ty * g.tz;
   std::vector<int> ptr, idx;
-  ds::build_tile_lists(nt, lo.data(), hi.data(), blk, cv, ptr, idx);
   DevBuf<int> dptr, didx;
   DevBuf<double> dm;
   const size_t n_map = (size_t)g.nx * g.ny * g.nz;
-  HIP_OR_FAIL(ctx, dptr.alloc(ptr.size()));
-  HIP_OR_FAIL(ctx, didx.alloc(idx.size()));
-  HIP_OR_FAIL(ctx, dm.alloc(n_map));
   hipStream_t st = ctx->stream;
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(dptr.p, ptr.data(), sizeof(int) * ptr.size(),
-                                  hipMemcpyHostToDevice, st));
-  if (!idx.empty())
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(didx.p, idx.data(),
-                                    sizeof(int) * idx.size(),
+  HIP_OR_FAIL(ctx, dptr.alloc((size_t)nt + 1));
+  const char* host_lists = std::getenv("DS_HOST_GRID_LISTS");
+  if (host_lists && host_lists[0] == '1') {
+    // diagnostic: the host's binned builder (the lists are identical)
+    std::vector<double> lo, hi;
+    grid_tiles(g, lo, hi);
+    ds::build_tile_lists(nt, lo.data(), hi.data(), blk, cv, ptr, idx);
+    HIP_OR_FAIL(ctx, didx.alloc(std::max<size_t>(idx.size(), 1)));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(dptr.p, ptr.data(),
+                                    sizeof(int) * ptr.size(),
                                     hipMemcpyHostToDevice, st));
+    if (!idx.empty())
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(didx.p, idx.data(),
+                                      sizeof(int) * idx.size(),
+                                      hipMemcpyHostToDevice, st));
+  } else {
+    // full scan on the device: count, scan on the host, fill
+    DevBuf<double> dlo, dhi;
+    DevBuf<int> dcnt;
+    const size_t nb = 3 * (size_t)blk.ntile;
+    HIP_OR_FAIL(ctx, dlo.alloc(nb));
+    HIP_OR_FAIL(ctx, dhi.alloc(nb));
+    HIP_OR_FAIL(ctx, dcnt.alloc(nt));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(dlo.p, blk.tlo.data(), sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(dhi.p, blk.thi.data(), sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo.p, dhi.p, blk.ntile, cv,
+                                           dcnt.p, nullptr, nullptr, st));
+    std::vector<int> cnt(nt);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * nt,
+                                    hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+    ptr.assign((size_t)nt + 1, 0);
+    for (int T = 0; T < nt; ++T) {
+      if ((long long)ptr[T] + cnt[T] > INT_MAX)
+        return fail(ctx, DS_E_ARG, "map grid: too many tile-list entries");
+      ptr[T + 1] = ptr[T] + cnt[T];
+    }
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(dptr.p, ptr.data(),
+                                    sizeof(int) * ptr.size(),
+                                    hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(ctx, didx.alloc(std::max(ptr[nt], 1)));
+    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo.p, dhi.p, blk.ntile, cv,
+                                           nullptr, dptr.p, didx.p, st));
+    // the boxes and counts are freed on return: finish with them first
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+  }
+  HIP_OR_FAIL(ctx, dm.alloc(n_map));
   HIP_OR_FAIL(ctx, ds::launch_map64(mode, g, pos, u, dptr.p, didx.p, table64,
                                     cv, dm.p, st));
   HIP_OR_FAIL(ctx, hipMemcpyAsync(m_host, dm.p, sizeof(double) * n_map,

@@ -87,6 +87,8 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
 // lib/wf_vector.cc:62-66.
 double box_gap_xx(const double* alo, const double* ahi, const double* blo,
                   const double* bhi, const CovarDev& cv);
+// Culling threshold on box_gap_xx: an entry is kept when xx < cull_limit.
+double cull_limit(const CovarDev& cv);
 
 CovarDev make_covar(const ds_covar* c);
 
@@ -282,6 +284,12 @@ hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
                         const double* u, const int* tptr, const int* tidx,
                         const double2* table64, const CovarDev& cv,
                         double* m, hipStream_t st);
+// Source-tile lists of the grid's tiles on the device: count[T] when idx is
+// null, else idx at the offsets ptr; equal to build_tile_lists' lists.
+hipError_t launch_grid_lists(const MapGrid& g, const double* slo,
+                             const double* shi, int ntile,
+                             const CovarDev& cv, int* count, const int* ptr,
+                             int* idx, hipStream_t st);
 // Arbitrary target points (Point records), targets sorted by a blocking.
 hipError_t launch_points64(int mode, int n_tgt, const double4* tgt,
                            const int* tgt_perm, const double4* pos,

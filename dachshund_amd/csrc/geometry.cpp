@@ -157,6 +157,14 @@ double box_gap_xx(const double* alo, const double* ahi, const double* blo,
   return (ga_perp + ga_para) * cv.dx_inv;
 }
 
+double cull_limit(const CovarDev& cv) {
+  // The reference drops a pair when (int)xx > table_n - 2, i.e. when
+  // xx >= table_n - 1.  The box gap is a lower bound of every pair's
+  // coordinate differences; a relative margin of 1e-12 covers the few ulps
+  // by which the bound's own rounding may exceed a pair's.
+  return (double)(cv.table_n - 1) * (1.0 + 1e-12);
+}
+
 // Largest table argument of any pair of points in the two boxes.
 double box_max_xx(const double* alo, const double* ahi, const double* blo,
                   const double* bhi, const CovarDev& cv) {
@@ -167,26 +175,128 @@ double box_max_xx(const double* alo, const double* ahi, const double* blo,
          cv.dx_inv;
 }
 
+namespace {
+
+// Source tiles binned on a uniform grid by box centre, so that a target
+// box visits only the tiles whose centres can lie within the cutoff of it
+// instead of all of them.  The search is a conservative prefilter: the
+// exact box test below still decides every entry, and the entries kept are
+// put in ascending tile order, so the lists equal those of a full scan.
+struct TileBins {
+  bool ok = false;
+  double cmin[3], inv[3], reach[3];
+  int nb[3];
+  std::vector<int> start, tiles;
+
+  TileBins(const Blocking& src, const CovarDev& cv, double limit) {
+    // A pair passes the cutoff test only if each axis' own term does:
+    // c_perp g^2 dx_inv < limit for x and y, c_para g^2 dx_inv < limit for z.
+    const double kperp = cv.c_perp * cv.dx_inv, kpara = cv.c_para * cv.dx_inv;
+    if (!(kperp > 0.0) || !(kpara > 0.0)) return;
+    const double r[3] = {std::sqrt(limit / kperp), std::sqrt(limit / kperp),
+                         std::sqrt(limit / kpara)};
+    double cmax[3], h[3] = {0.0, 0.0, 0.0};
+    int live = 0;
+    for (int a = 0; a < 3; ++a) { cmin[a] = 1e300; cmax[a] = -1e300; }
+    for (int J = 0; J < src.ntile; ++J) {
+      if (src.tcount[J] == 0) continue;
+      ++live;
+      for (int a = 0; a < 3; ++a) {
+        const double lo = src.tlo[3 * (size_t)J + a];
+        const double hi = src.thi[3 * (size_t)J + a];
+        const double c = 0.5 * (lo + hi);
+        cmin[a] = std::min(cmin[a], c);
+        cmax[a] = std::max(cmax[a], c);
+        h[a] = std::max(h[a], 0.5 * (hi - lo));
+      }
+    }
+    if (live == 0) return;
+    size_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(r[a]) || !std::isfinite(cmax[a] - cmin[a])) return;
+      // centres farther than the cutoff plus the widest tile's half-extent
+      // from a target box hold no pair within the cutoff; the relative
+      // margin covers the rounding of centres and half-extents
+      reach[a] = (r[a] + h[a]) * (1.0 + 1e-9) +
+                 1e-12 * (std::fabs(cmin[a]) + std::fabs(cmax[a]));
+      const double span = cmax[a] - cmin[a];
+      const double w = std::max(reach[a], span / 1024.0);
+      nb[a] = (w > 0.0) ? std::max(1, (int)std::min(1024.0, span / w) + 1) : 1;
+      inv[a] = (span > 0.0) ? nb[a] / (span * (1.0 + 1e-12)) : 0.0;
+      total *= (size_t)nb[a];
+    }
+    start.assign(total + 1, 0);
+    std::vector<int> bin(src.ntile, -1);
+    for (int J = 0; J < src.ntile; ++J) {
+      if (src.tcount[J] == 0) continue;
+      int b[3];
+      for (int a = 0; a < 3; ++a)
+        b[a] = index(a, 0.5 * (src.tlo[3 * (size_t)J + a] +
+                               src.thi[3 * (size_t)J + a]));
+      bin[J] = (b[0] * nb[1] + b[1]) * nb[2] + b[2];
+      ++start[bin[J] + 1];
+    }
+    for (size_t k = 0; k < total; ++k) start[k + 1] += start[k];
+    tiles.resize(live);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int J = 0; J < src.ntile; ++J)
+      if (bin[J] >= 0) tiles[fill[bin[J]]++] = J;
+    ok = true;
+  }
+
+  // bin of a coordinate: monotone in x, clamped to the grid
+  int index(int a, double x) const {
+    const double t = (x - cmin[a]) * inv[a];
+    if (!(t > 0.0)) return 0;
+    return (int)std::min<double>(nb[a] - 1, std::floor(t));
+  }
+
+  // tiles whose centres may lie within reach of [lo, hi], bin by bin
+  void candidates(const double* lo, const double* hi,
+                  std::vector<int>& out) const {
+    out.clear();
+    int b0[3], b1[3];
+    for (int a = 0; a < 3; ++a) {
+      b0[a] = index(a, lo[a] - reach[a]);
+      b1[a] = index(a, hi[a] + reach[a]);
+    }
+    for (int x = b0[0]; x <= b1[0]; ++x)
+      for (int y = b0[1]; y <= b1[1]; ++y)
+        for (int z = b0[2]; z <= b1[2]; ++z) {
+          const int k = (x * nb[1] + y) * nb[2] + z;
+          out.insert(out.end(), tiles.begin() + start[k],
+                     tiles.begin() + start[k + 1]);
+        }
+  }
+};
+
+}  // namespace
+
 void build_tile_lists(int ntgt, const double* tlo, const double* thi,
                       const Blocking& src, const CovarDev& cv,
                       std::vector<int>& ptr, std::vector<int>& idx,
                       const double* sublo, const double* subhi,
                       std::vector<unsigned char>* mask, double* max_xx) {
-  // The reference drops a pair when (int)xx > table_n - 2, i.e. when
-  // xx >= table_n - 1.  The box gap is a lower bound of every pair's
-  // coordinate differences; a relative margin of 1e-12 covers the few ulps
-  // by which the bound's own rounding may exceed a pair's.
-  const double limit = (double)(cv.table_n - 1) * (1.0 + 1e-12);
+  const double limit = cull_limit(cv);
   const bool sub = sublo && subhi && mask;
+  const TileBins bins(src, cv, limit);
   std::vector<std::vector<int>> lists(ntgt);
   std::vector<std::vector<unsigned char>> masks(sub ? ntgt : 0);
   std::vector<double> far(sub ? ntgt : 0, 0.0);
+  std::vector<int> all;
+  if (!bins.ok) {
+    all.resize(src.ntile);
+    std::iota(all.begin(), all.end(), 0);
+  }
   parallel_for(ntgt, [&](int I) {
     std::vector<int>& L = lists[I];
     const double* alo = tlo + 3 * (size_t)I;
     const double* ahi = thi + 3 * (size_t)I;
     if (!(alo[0] <= ahi[0])) return;  // empty target
-    for (int J = 0; J < src.ntile; ++J) {
+    std::vector<int> cand;
+    if (bins.ok) bins.candidates(alo, ahi, cand);
+    const std::vector<int>& C = bins.ok ? cand : all;
+    for (int J : C) {
       if (src.tcount[J] == 0) continue;
       const double* jlo = &src.tlo[3 * (size_t)J];
       const double* jhi = &src.thi[3 * (size_t)J];
@@ -209,6 +319,20 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
         masks[I].push_back((unsigned char)m);
       }
       L.push_back(J);
+    }
+    if (bins.ok && !std::is_sorted(L.begin(), L.end())) {
+      // ascending tile order, as a full scan would give
+      std::vector<int> o(L.size());
+      std::iota(o.begin(), o.end(), 0);
+      std::sort(o.begin(), o.end(), [&](int a, int b) { return L[a] < L[b]; });
+      std::vector<int> L2(L.size());
+      for (size_t k = 0; k < o.size(); ++k) L2[k] = L[o[k]];
+      L.swap(L2);
+      if (sub) {
+        std::vector<unsigned char> M2(o.size());
+        for (size_t k = 0; k < o.size(); ++k) M2[k] = masks[I][o[k]];
+        masks[I].swap(M2);
+      }
     }
   });
   ptr.assign(ntgt + 1, 0);

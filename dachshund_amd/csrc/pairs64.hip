@@ -297,6 +297,77 @@ __global__ __launch_bounds__(256) void k_map64(MapArgs a) {
   }
 }
 
+// Source-tile lists of the map grid's 4x4x4 tiles, built on the device by
+// a full scan: one wavefront per grid tile tests every source tile box with
+// the cutoff test of build_tile_lists (same operations, no contraction), so
+// the lists, ascending in tile index, equal the host's.  Pass 0 counts each
+// tile's entries, pass 1 writes them at the offsets the host scanned.
+struct GridListArgs {
+  const double* slo;      // source tile boxes, 3 per tile (empty: lo > hi)
+  const double* shi;
+  int* count;
+  const int* ptr;
+  int* idx;
+  double lx, ly, lz;
+  int nx, ny, nz, tx, ty, tz;
+  double c_perp, c_para, dx_inv, limit;
+  int ntile;
+};
+
+__device__ inline double axis_gap(double alo, double ahi, double blo,
+                                  double bhi) {
+  if (blo > ahi) return blo - ahi;
+  if (alo > bhi) return alo - bhi;
+  return 0.0;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_grid_lists(GridListArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int T = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 +
+                                               (threadIdx.x >> 6));
+  if (T >= a.tx * a.ty * a.tz) return;
+  const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
+  // the tile's cell-centre box (lib/util.cc:30-39 for the centres)
+  const double dx = a.lx / a.nx, dy = a.ly / a.ny, dz = a.lz / a.nz;
+  const int x0 = tix * DS_MAP_T, x1 = min(a.nx - 1, x0 + DS_MAP_T - 1);
+  const int y0 = tiy * DS_MAP_T, y1 = min(a.ny - 1, y0 + DS_MAP_T - 1);
+  const int z0 = tiz * DS_MAP_T, z1 = min(a.nz - 1, z0 + DS_MAP_T - 1);
+  const double alo0 = dx * (x0 + 0.5), ahi0 = dx * (x1 + 0.5);
+  const double alo1 = dy * (y0 + 0.5), ahi1 = dy * (y1 + 0.5);
+  const double alo2 = dz * (z0 + 0.5), ahi2 = dz * (z1 + 0.5);
+  const int base = FILL ? a.ptr[T] : 0;
+  int cnt = 0;
+  for (int J0 = 0; J0 < a.ntile; J0 += 64) {
+    const int J = J0 + lane;
+    bool keep = false;
+    if (J < a.ntile) {
+      const double* lo = a.slo + 3 * (size_t)J;
+      const double* hi = a.shi + 3 * (size_t)J;
+      const double l0 = lo[0], l1 = lo[1], l2 = lo[2];
+      const double h0 = hi[0], h1 = hi[1], h2 = hi[2];
+      if (l0 <= h0) {
+        const double g0 = axis_gap(alo0, ahi0, l0, h0);
+        const double g1 = axis_gap(alo1, ahi1, l1, h1);
+        const double g2 = axis_gap(alo2, ahi2, l2, h2);
+        const double x_perp_2 = (g0 * g0 + g1 * g1);
+        const double x_para_2 = g2 * g2;
+        const double ga_perp = a.c_perp * x_perp_2;
+        const double ga_para = a.c_para * x_para_2;
+        keep = (ga_perp + ga_para) * a.dx_inv < a.limit;
+      }
+    }
+    const unsigned long long b = __ballot(keep);
+    if (FILL && keep) {
+      const int r = __builtin_amdgcn_mbcnt_hi(
+          (unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+      a.idx[base + cnt + r] = J;
+    }
+    cnt += __popcll(b);
+  }
+  if (!FILL && lane == 0) a.count[T] = cnt;
+}
+
 // Arbitrary target points, sorted into blocks of 64 by their own blocking.
 struct PointsArgs {
   const double4* tgt;     // sorted targets (npad_t)
@@ -447,6 +518,33 @@ hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
     hipLaunchKernelGGL(k_map64<true>, grid, block, shm, st, a);
   else
     hipLaunchKernelGGL(k_map64<false>, grid, block, shm, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_lists(const MapGrid& g, const double* slo,
+                             const double* shi, int ntile,
+                             const CovarDev& cv, int* count, const int* ptr,
+                             int* idx, hipStream_t st) {
+  GridListArgs a{};
+  a.slo = slo;
+  a.shi = shi;
+  a.count = count;
+  a.ptr = ptr;
+  a.idx = idx;
+  a.lx = g.lx; a.ly = g.ly; a.lz = g.lz;
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+  a.tx = g.tx; a.ty = g.ty; a.tz = g.tz;
+  a.c_perp = cv.c_perp;
+  a.c_para = cv.c_para;
+  a.dx_inv = cv.dx_inv;
+  a.limit = cull_limit(cv);
+  a.ntile = ntile;
+  const size_t nt = (size_t)g.tx * g.ty * g.tz;
+  const dim3 grid((unsigned)((nt + 3) / 4)), block(256);
+  if (idx)
+    hipLaunchKernelGGL(k_grid_lists<true>, grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL(k_grid_lists<false>, grid, block, 0, st, a);
   return hipGetLastError();
 }
 
