@@ -339,6 +339,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->table64);
   dev_free(p->table32);
   dev_free(p->table32x);
+  dev_free(p->table32g);
   dev_free(p->table_raw);
   dev_free(p->blk_sub);
   dev_free(p->blk_nreal);
@@ -704,7 +705,12 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       float o[3];
       for (int a = 0; a < 3; ++a)
         o[a] = (float)((cen[3 * (size_t)I + a] - tcen[3 * T + a]) * s3[a]);
-      hoff[e] = make_float4(o[0], o[1], o[2], 0.0f);
+      // .w carries the bits of (live sub-block mask | tile << 4), so the
+      // MFMA kernel reads an entry in one 16-byte load
+      const unsigned bits = (unsigned)msk[e] | ((unsigned)T << 4);
+      float wbits;
+      std::memcpy(&wbits, &bits, sizeof(float));
+      hoff[e] = make_float4(o[0], o[1], o[2], wbits);
     }
   }
   // launch order: longest tile list first (for the autotuner)
@@ -716,20 +722,30 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::vector<double2> t64;
   std::vector<float2> t32;
   make_tables(&covars[0], t64, t32);
-  // The clamp-free FP32 MFMA kernel reads entry (int)(xx + 1): entry 0
-  // holds f_0 (arguments rounded below 0), entries 1..table_n the table,
-  // zeros up past the largest argument any listed pair can reach (box
-  // bound, plus room for FP32 rounding of frames and products).
+  // The FP32 MFMA kernels store each interval's interpolant
+  // f_k + df_k (xx - k) as the line g_k + df_k xx (g_k = f_k - k df_k,
+  // formed in FP64), one fma per pair with no fraction.
+  std::vector<float2> t32g(t64.size());
+  for (size_t k = 0; k < t64.size(); ++k)
+    t32g[k] = make_float2((float)(t64[k].x - (double)k * t64[k].y),
+                          (float)t64[k].y);
+  // The clamp-free FP32 MFMA kernel reads entry (int)(xx + 1) at xx + 1:
+  // entry 0 holds f_0 (arguments rounded below 0), entries 1..table_n the
+  // table's lines in the shifted argument, zeros up past the largest
+  // argument any listed pair can reach (box bound, plus room for FP32
+  // rounding of frames and products).
   std::vector<float2> t32x;
   {
     double far_max = 0.0;
     for (double v : far) far_max = std::max(far_max, v);
     const double need = far_max * 1.01 + 4.0;
     if (need < (double)DS_TABLE_X_MAX) {
-      const int len = std::max((int)need + 1, (int)t32.size() + 1);
+      const int len = std::max((int)need + 1, (int)t64.size() + 1);
       t32x.assign(len, make_float2(0.0f, 0.0f));
-      t32x[0] = make_float2(t32[0].x, 0.0f);
-      for (size_t k = 0; k < t32.size(); ++k) t32x[k + 1] = t32[k];
+      t32x[0] = make_float2((float)t64[0].x, 0.0f);
+      for (size_t k = 0; k < t64.size(); ++k)
+        t32x[k + 1] = make_float2(
+            (float)(t64[k].x - (double)(k + 1) * t64[k].y), (float)t64[k].y);
     }
   }
 
@@ -750,6 +766,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table64, t64.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table32, t32.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->table32g, t32g.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table_raw, p->table.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
@@ -792,6 +809,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
+  HIP_OR_FAIL(ctx, up(p->table32g, t32g.data(), sizeof(float2) * t32g.size()));
   if (!t32x.empty()) {
     HIP_OR_FAIL(ctx, dev_alloc(&p->table32x, t32x.size()));
     HIP_OR_FAIL(ctx, up(p->table32x, t32x.data(), sizeof(float2) * t32x.size()));

@@ -187,8 +187,10 @@ struct ds_problem {
   float4* toff32 = nullptr;            // per list entry: centre offsets
   double2* table64 = nullptr;          // {T[k], T[k+1]-T[k]}, last = {0, 0}
   float2* table32 = nullptr;
-  // FP32 MFMA kernel without the clamp: table shifted by one and
-  // zero-extended past the largest pair argument of the lists
+  // FP32 MFMA kernels: {g_k, df_k}, the interval's interpolant as the line
+  // g_k + df_k xx; without the clamp: shifted by one and zero-extended
+  // past the largest pair argument of the lists
+  float2* table32g = nullptr;
   float2* table32x = nullptr;
   int tlen_x = 0;                      // entries of table32x (0: not built)
   double* table_raw = nullptr;         // T[k] as given (dense/column paths)

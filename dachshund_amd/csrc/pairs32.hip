@@ -29,8 +29,10 @@
 // with l_i the target in its block frame, Y_j = X_j - o the source moved
 // into that frame (o = the entry's frame offset), A = [l_i, 1] and the
 // accumulator input C = |l_i|^2 fixed per block, and B = one value per lane
-// and tile.  The VALU is left with the clamp, index, fraction, table
-// address and two fma per pair (6 operations).  Lane l = 16 g + c holds
+// and tile.  Its table stores each interval's interpolant as the line
+// g_k + df_k xx (g_k = f_k - k df_k, formed in FP64 on the host), so the
+// VALU is left with the clamp, index, table address and two fma per pair
+// (5 operations; 4 without the clamp).  Lane l = 16 g + c holds
 // A[i = c][k = g] and B[k = g][j = c]; the result D holds rows 4 g + r
 // (r = 0..3) of column c, so each lane accumulates 16 rows x 1 source per
 // tile and the 16 lanes of a group are summed once at the end.
@@ -52,7 +54,8 @@ struct Matvec32Args {
   const unsigned char* tmask;   // per list entry: live 16-row sub-blocks
   const float4* toff;     // per list entry: block frame - tile frame
   const float2* table;    // {f[k], f[k+1]-f[k]}, last = {0, 0}
-  const float2* table_x;  // shifted, zero-extended copy (clamp-free kernel)
+  const float2* table_g;  // {g, df}: the interpolant as g + df xx (MFMA)
+  const float2* table_x;  // shifted, zero-extended lines (clamp-free MFMA)
   int tlen_x;
   const int* blk_sub;
   const int* blk_nreal;
@@ -64,6 +67,7 @@ struct Matvec32Args {
   const ds_cg_scalars* sc;
   float tlast;
   int nblk, table_n;
+  int lane_zero;          // 0; lane & lane_zero makes an index lane-varying
 };
 
 // The per-tile data (list entry, frame offset, source records and vector)
@@ -141,7 +145,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // xx + 1 > 0 for every pair and (int) truncation needs no clamp below; the
 // shifted table (entry 0 = {f_0, 0}: arguments rounded below 0) is
 // zero-extended past the largest argument the problem's lists can reach,
-// so none above either.  5 VALU operations per pair.
+// so none above either.  4 VALU operations per pair.
 template <int FORM, bool CG, bool NOCLAMP>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   const ds_cg_scalars* sc = a.sc + sv;
   if (CG && sc->done) return;
   {
-    const float2* tsrc = NOCLAMP ? a.table_x : a.table;
+    const float2* tsrc = NOCLAMP ? a.table_x : a.table_g;
     const int tlen = NOCLAMP ? a.tlen_x : a.table_n;
     for (int k = threadIdx.x; k < tlen; k += blockDim.x) tab32[k] = tsrc[k];
   }
@@ -188,48 +192,65 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
-  // the next entry's source record and vector value are fetched while the
-  // current one is processed
-  float4 Xn = make_float4(0.f, 0.f, 0.f, 0.f);
-  float un = 0.0f;
+  // Entry records (frame offset, and in .w the bits of live mask | tile
+  // << 4) are fetched two entries ahead and the source record and vector
+  // value one entry ahead, all by vector loads issued at the top of an
+  // entry and first read after its arithmetic (indices past the list are
+  // clamped to its last entry: duplicate loads, never used).
+  // (lz is 0, but not known to be: a lane-varying index keeps the records
+  // on vector loads, whose completion is waited for where they are used.)
+  const int lz = lane & a.lane_zero;
+  const int kl = ke - 1;
+  float4 Rc = make_float4(0.f, 0.f, 0.f, 0.f), Rn = Rc, Xc = Rc;
+  float uc = 0.0f;
   if (kb < ke) {
-    const int j0 = __builtin_amdgcn_readfirstlane(tidx[kb]) * DS_TS;
-    Xn = loc[j0 + c];
-    un = u32[j0 + c];
+    Rc = toff[kb + lz];
+    Rn = toff[min(kb + nw, kl) + lz];
+    const int j0 =
+        (__builtin_amdgcn_readfirstlane(__float_as_int(Rc.w)) >> 4) * DS_TS;
+    Xc = loc[j0 + c];
+    uc = u32[j0 + c];
   }
   for (int k = kb; k < ke; k += nw) {
-    const unsigned live = __builtin_amdgcn_readfirstlane(a.tmask[k]);
-    const float4 o = toff[k];
-    const float4 X = Xn;
-    const float uj = un;
-    if (k + nw < ke) {
-      const int j1 = __builtin_amdgcn_readfirstlane(tidx[k + nw]) * DS_TS;
-      Xn = loc[j1 + c];
-      un = u32[j1 + c];
-    }
+    const int j1 =
+        (__builtin_amdgcn_readfirstlane(__float_as_int(Rn.w)) >> 4) * DS_TS;
+    const float4 Xn = loc[j1 + c];
+    const float un = u32[j1 + c];
+    const float4 Rnn = toff[min(k + 2 * nw, kl) + lz];
+    const float4 o = Rc;
+    const float4 X = Xc;
+    const float uj = uc;
+    const unsigned live =
+        __builtin_amdgcn_readfirstlane(__float_as_int(o.w)) & 15u;
     // a target in the tile frame is l + o, so the source in the block
     // frame is Y = X - o; B = -2 Y_g (g < 3) or |Y|^2 (g = 3), formed
     // without branches
     const float yx = X.x - o.x, yy = X.y - o.y, yz = X.z - o.z;
     const float y2 = fmaf(yz, yz, fmaf(yy, yy, yx * yx));
     const float b = fmaf(cx, yx, fmaf(cy, yy, fmaf(cz, yz, cw * y2)));
+    // sub-blocks whose box is beyond the cutoff of the tile's box hold
+    // only zero pairs (wave-uniform skips); the live sub-blocks' MFMAs are
+    // issued together, ahead of the VALU work that reads them
+    v4f d[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (live & (1u << s))
+        d[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], b, ci[s], 0, 0, 0);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      // sub-blocks whose box is beyond the cutoff of the tile's box hold
-      // only zero pairs (wave-uniform skip)
       if (!(live & (1u << s))) continue;
-      const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], b, ci[s], 0,
-                                                        0, 0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float xx =
-            NOCLAMP ? d[r] : __builtin_amdgcn_fmed3f(d[r], 0.0f, tlast);
-        const int ix = (int)xx;
-        const float t = __builtin_amdgcn_fractf(xx);
-        const float2 f = tab32[ix];
-        acc[s][r] = fmaf(fmaf(f.y, t, f.x), uj, acc[s][r]);
+            NOCLAMP ? d[s][r] : __builtin_amdgcn_fmed3f(d[s][r], 0.0f, tlast);
+        const float2 f = tab32[(int)xx];
+        acc[s][r] = fmaf(fmaf(f.y, xx, f.x), uj, acc[s][r]);
       }
     }
+    Rc = Rn;
+    Rn = Rnn;
+    Xc = Xn;
+    uc = un;
   }
   // sum over the 16 source columns of each lane group
 #pragma unroll
@@ -281,6 +302,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.tmask = p->tmask;
   a.toff = p->toff32;
   a.table = p->table32;
+  a.table_g = p->table32g;
   a.table_x = p->table32x;
   a.tlen_x = p->tlen_x;
   a.out = out;

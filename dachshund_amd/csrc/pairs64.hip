@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_map64(MapArgs a) {
 }
 
 // Source-tile lists of the map grid's 4x4x4 tiles, built on the device by
-// a full scan: one wavefront per grid tile tests every source tile box with
+// a full scan: one wavefront per 4 grid tiles tests every source tile box with
 // the cutoff test of build_tile_lists (same operations, no contraction), so
 // the lists, ascending in tile index, equal the host's.  Pass 0 counts each
 // tile's entries, pass 1 writes them at the offsets the host scanned.
@@ -321,51 +321,74 @@ __device__ inline double axis_gap(double alo, double ahi, double blo,
   return 0.0;
 }
 
+constexpr int kGridTilesPerWave = 4;
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_grid_lists(GridListArgs a) {
+  constexpr int kT = kGridTilesPerWave;
   const int lane = threadIdx.x & 63;
-  const int T = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 +
-                                               (threadIdx.x >> 6));
-  if (T >= a.tx * a.ty * a.tz) return;
-  const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
-  // the tile's cell-centre box (lib/util.cc:30-39 for the centres)
+  const int nt = a.tx * a.ty * a.tz;
+  const int T0 = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * 4 + (threadIdx.x >> 6)) * kT);
+  if (T0 >= nt) return;
+  // the tiles' cell-centre boxes (lib/util.cc:30-39 for the centres)
   const double dx = a.lx / a.nx, dy = a.ly / a.ny, dz = a.lz / a.nz;
-  const int x0 = tix * DS_MAP_T, x1 = min(a.nx - 1, x0 + DS_MAP_T - 1);
-  const int y0 = tiy * DS_MAP_T, y1 = min(a.ny - 1, y0 + DS_MAP_T - 1);
-  const int z0 = tiz * DS_MAP_T, z1 = min(a.nz - 1, z0 + DS_MAP_T - 1);
-  const double alo0 = dx * (x0 + 0.5), ahi0 = dx * (x1 + 0.5);
-  const double alo1 = dy * (y0 + 0.5), ahi1 = dy * (y1 + 0.5);
-  const double alo2 = dz * (z0 + 0.5), ahi2 = dz * (z1 + 0.5);
-  const int base = FILL ? a.ptr[T] : 0;
-  int cnt = 0;
+  double alo[kT][3], ahi[kT][3];
+  int base[kT], cnt[kT];
+#pragma unroll
+  for (int t = 0; t < kT; ++t) {
+    const int T = min(T0 + t, nt - 1);
+    const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
+    const int x0 = tix * DS_MAP_T, x1 = min(a.nx - 1, x0 + DS_MAP_T - 1);
+    const int y0 = tiy * DS_MAP_T, y1 = min(a.ny - 1, y0 + DS_MAP_T - 1);
+    const int z0 = tiz * DS_MAP_T, z1 = min(a.nz - 1, z0 + DS_MAP_T - 1);
+    alo[t][0] = dx * (x0 + 0.5);
+    ahi[t][0] = dx * (x1 + 0.5);
+    alo[t][1] = dy * (y0 + 0.5);
+    ahi[t][1] = dy * (y1 + 0.5);
+    alo[t][2] = dz * (z0 + 0.5);
+    ahi[t][2] = dz * (z1 + 0.5);
+    base[t] = (FILL && T0 + t < nt) ? a.ptr[T0 + t] : 0;
+    cnt[t] = 0;
+  }
   for (int J0 = 0; J0 < a.ntile; J0 += 64) {
     const int J = J0 + lane;
-    bool keep = false;
+    bool live = false;
+    double l[3] = {0.0, 0.0, 0.0}, h[3] = {0.0, 0.0, 0.0};
     if (J < a.ntile) {
       const double* lo = a.slo + 3 * (size_t)J;
       const double* hi = a.shi + 3 * (size_t)J;
-      const double l0 = lo[0], l1 = lo[1], l2 = lo[2];
-      const double h0 = hi[0], h1 = hi[1], h2 = hi[2];
-      if (l0 <= h0) {
-        const double g0 = axis_gap(alo0, ahi0, l0, h0);
-        const double g1 = axis_gap(alo1, ahi1, l1, h1);
-        const double g2 = axis_gap(alo2, ahi2, l2, h2);
+      l[0] = lo[0]; l[1] = lo[1]; l[2] = lo[2];
+      h[0] = hi[0]; h[1] = hi[1]; h[2] = hi[2];
+      live = l[0] <= h[0];
+    }
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+      bool keep = false;
+      if (live && T0 + t < nt) {
+        const double g0 = axis_gap(alo[t][0], ahi[t][0], l[0], h[0]);
+        const double g1 = axis_gap(alo[t][1], ahi[t][1], l[1], h[1]);
+        const double g2 = axis_gap(alo[t][2], ahi[t][2], l[2], h[2]);
         const double x_perp_2 = (g0 * g0 + g1 * g1);
         const double x_para_2 = g2 * g2;
         const double ga_perp = a.c_perp * x_perp_2;
         const double ga_para = a.c_para * x_para_2;
         keep = (ga_perp + ga_para) * a.dx_inv < a.limit;
       }
+      const unsigned long long b = __ballot(keep);
+      if (FILL && keep) {
+        const int r = __builtin_amdgcn_mbcnt_hi(
+            (unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+        a.idx[base[t] + cnt[t] + r] = J;
+      }
+      cnt[t] += __popcll(b);
     }
-    const unsigned long long b = __ballot(keep);
-    if (FILL && keep) {
-      const int r = __builtin_amdgcn_mbcnt_hi(
-          (unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
-      a.idx[base + cnt + r] = J;
-    }
-    cnt += __popcll(b);
   }
-  if (!FILL && lane == 0) a.count[T] = cnt;
+  if (!FILL && lane == 0) {
+#pragma unroll
+    for (int t = 0; t < kT; ++t)
+      if (T0 + t < nt) a.count[T0 + t] = cnt[t];
+  }
 }
 
 // Arbitrary target points, sorted into blocks of 64 by their own blocking.
@@ -540,7 +563,8 @@ hipError_t launch_grid_lists(const MapGrid& g, const double* slo,
   a.limit = cull_limit(cv);
   a.ntile = ntile;
   const size_t nt = (size_t)g.tx * g.ty * g.tz;
-  const dim3 grid((unsigned)((nt + 3) / 4)), block(256);
+  const size_t waves = (nt + kGridTilesPerWave - 1) / kGridTilesPerWave;
+  const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
   if (idx)
     hipLaunchKernelGGL(k_grid_lists<true>, grid, block, 0, st, a);
   else
