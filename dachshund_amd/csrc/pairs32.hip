@@ -192,34 +192,13 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
-  // Entry records (frame offset, and in .w the bits of live mask | tile
-  // << 4) are fetched two entries ahead and the source record and vector
-  // value one entry ahead, all by vector loads issued at the top of an
-  // entry and first read after its arithmetic (indices past the list are
-  // clamped to its last entry: duplicate loads, never used).
-  // (lz is 0, but not known to be: a lane-varying index keeps the records
-  // on vector loads, whose completion is waited for where they are used.)
-  const int lz = lane & a.lane_zero;
-  const int kl = ke - 1;
-  float4 Rc = make_float4(0.f, 0.f, 0.f, 0.f), Rn = Rc, Xc = Rc;
-  float uc = 0.0f;
-  if (kb < ke) {
-    Rc = toff[kb + lz];
-    Rn = toff[min(kb + nw, kl) + lz];
-    const int j0 =
-        (__builtin_amdgcn_readfirstlane(__float_as_int(Rc.w)) >> 4) * DS_TS;
-    Xc = loc[j0 + c];
-    uc = u32[j0 + c];
-  }
-  for (int k = kb; k < ke; k += nw) {
-    const int j1 =
-        (__builtin_amdgcn_readfirstlane(__float_as_int(Rn.w)) >> 4) * DS_TS;
-    const float4 Xn = loc[j1 + c];
-    const float un = u32[j1 + c];
-    const float4 Rnn = toff[min(k + 2 * nw, kl) + lz];
-    const float4 o = Rc;
-    const float4 X = Xc;
-    const float uj = uc;
+  // One entry: its record o (frame offset; .w holds the bits of live mask
+  // | tile << 4), the tile's source record X and vector value uj.  (The
+  // MFMA results live across entries: a sub-block an entry skips keeps
+  // the last value, never read.)
+  v4f d[4];
+  auto entry = [&](const float4& o, const float4& X, float uj)
+      __attribute__((always_inline)) {
     const unsigned live =
         __builtin_amdgcn_readfirstlane(__float_as_int(o.w)) & 15u;
     // a target in the tile frame is l + o, so the source in the block
@@ -231,7 +210,6 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     // sub-blocks whose box is beyond the cutoff of the tile's box hold
     // only zero pairs (wave-uniform skips); the live sub-blocks' MFMAs are
     // issued together, ahead of the VALU work that reads them
-    v4f d[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
       if (live & (1u << s))
@@ -247,10 +225,44 @@ __global__ __launch_bounds__(256) void k_matvec32m(
         acc[s][r] = fmaf(fmaf(f.y, xx, f.x), uj, acc[s][r]);
       }
     }
-    Rc = Rn;
-    Rn = Rnn;
-    Xc = Xn;
-    uc = un;
+  };
+  // Records are fetched two entries ahead, source records and vector
+  // values one entry ahead, by vector loads from a wave-uniform base (held
+  // in scalar registers) plus a fixed per-lane offset; each load is first
+  // read after the next entry's arithmetic.  Indices past the list are
+  // clamped to its last entry (duplicate loads, never used).  lz is 0 but
+  // not known to be, which keeps the records on vector loads.
+  const unsigned cu = (unsigned)c;
+  const unsigned lz = (unsigned)(lane & a.lane_zero);
+  const int kl = ke - 1;
+  auto tile_of = [&](const float4& R) __attribute__((always_inline)) {
+    return (__builtin_amdgcn_readfirstlane(__float_as_int(R.w)) >> 4) * DS_TS;
+  };
+  if (kb < ke) {
+    float4 R0 = (toff + kb)[lz];
+    float4 R1 = (toff + min(kb + nw, kl))[lz];
+    int j = tile_of(R0);
+    float4 X0 = (loc + j)[cu];
+    float u0 = (u32 + j)[cu];
+    for (int k = kb;; k += 2 * nw) {
+      // entry k; prefetch the sources of k + nw, the record of k + 2 nw
+      j = tile_of(R1);
+      const float4 X1 = (loc + j)[cu];
+      const float u1 = (u32 + j)[cu];
+      const float4 R2 = (toff + min(k + 2 * nw, kl))[lz];
+      entry(R0, X0, u0);
+      if (k + nw >= ke) break;
+      // entry k + nw; prefetch the sources of k + 2 nw, the record of
+      // k + 3 nw
+      j = tile_of(R2);
+      X0 = (loc + j)[cu];
+      u0 = (u32 + j)[cu];
+      const float4 R3 = (toff + min(k + 3 * nw, kl))[lz];
+      entry(R1, X1, u1);
+      if (k + 2 * nw >= ke) break;
+      R0 = R2;
+      R1 = R3;
+    }
   }
   // sum over the 16 source columns of each lane group
 #pragma unroll
