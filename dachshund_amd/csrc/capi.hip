@@ -338,7 +338,6 @@ void problem_free(ds_problem* p) {
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
-  dev_free(p->table32x);
   dev_free(p->table32g);
   dev_free(p->table_raw);
   dev_free(p->blk_sub);
@@ -528,7 +527,6 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::vector<Blocking> blks(n_sub);
   std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
   std::vector<std::vector<unsigned char>> msks(n_sub);
-  std::vector<double> far(n_sub, 0.0);
   auto build = [&](int k) {
     const ds::CovarDev cv = ds::make_covar(&covars[k]);
     p->sub_cv_h[k] = cv;
@@ -539,7 +537,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     // a block's 16-row sub-blocks are its own tiles (targets = sources)
     ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
                          blks[k], cv, ptrs[k], idxs[k], blks[k].tlo.data(),
-                         blks[k].thi.data(), &msks[k], &far[k]);
+                         blks[k].thi.data(), &msks[k]);
     // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
     double sum_b2 = 0.0;
     for (int i = 0; i < sub_n[k]; ++i) {
@@ -722,32 +720,13 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::vector<double2> t64;
   std::vector<float2> t32;
   make_tables(&covars[0], t64, t32);
-  // The FP32 MFMA kernels store each interval's interpolant
+  // The FP32 MFMA kernel stores each interval's interpolant
   // f_k + df_k (xx - k) as the line g_k + df_k xx (g_k = f_k - k df_k,
   // formed in FP64), one fma per pair with no fraction.
   std::vector<float2> t32g(t64.size());
   for (size_t k = 0; k < t64.size(); ++k)
     t32g[k] = make_float2((float)(t64[k].x - (double)k * t64[k].y),
                           (float)t64[k].y);
-  // The clamp-free FP32 MFMA kernel reads entry (int)(xx + 1) at xx + 1:
-  // entry 0 holds f_0 (arguments rounded below 0), entries 1..table_n the
-  // table's lines in the shifted argument, zeros up past the largest
-  // argument any listed pair can reach (box bound, plus room for FP32
-  // rounding of frames and products).
-  std::vector<float2> t32x;
-  {
-    double far_max = 0.0;
-    for (double v : far) far_max = std::max(far_max, v);
-    const double need = far_max * 1.01 + 4.0;
-    if (need < (double)DS_TABLE_X_MAX) {
-      const int len = std::max((int)need + 1, (int)t64.size() + 1);
-      t32x.assign(len, make_float2(0.0f, 0.0f));
-      t32x[0] = make_float2((float)t64[0].x, 0.0f);
-      for (size_t k = 0; k < t64.size(); ++k)
-        t32x[k + 1] = make_float2(
-            (float)(t64[k].x - (double)(k + 1) * t64[k].y), (float)t64[k].y);
-    }
-  }
 
   hipStream_t st = ctx->stream;
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -810,11 +789,6 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
   HIP_OR_FAIL(ctx, up(p->table32g, t32g.data(), sizeof(float2) * t32g.size()));
-  if (!t32x.empty()) {
-    HIP_OR_FAIL(ctx, dev_alloc(&p->table32x, t32x.size()));
-    HIP_OR_FAIL(ctx, up(p->table32x, t32x.data(), sizeof(float2) * t32x.size()));
-    p->tlen_x = (int)t32x.size();
-  }
   HIP_OR_FAIL(ctx, up(p->table_raw, p->table.data(),
                       sizeof(double) * p->table.size()));
   HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));

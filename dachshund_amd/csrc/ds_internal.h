@@ -31,7 +31,6 @@
 #define DS_BS 64          // target rows per block (one wavefront)
 #define DS_TS 16          // source slots per tile
 #define DS_TPB (DS_BS / DS_TS)
-constexpr int DS_TABLE_X_MAX = 4096;   // entries of the extended FP32 table
 #define DS_MAX_TABLE 1024
 #define DS_MAP_T 4        // map tiles are 4x4x4 cells
 #define DS_DENSE_R 32     // right-hand sides of the explicit BF16 block path
@@ -71,16 +70,13 @@ void build_blocking(const double* pts, int stride, int n, const double scale[3],
 // pair with table index < table_n - 1 under the reference's arithmetic.
 // With sublo/subhi (DS_TPB sub-boxes per target, 3 x DS_TPB x ntgt) and
 // mask, each kept entry also gets the bit mask of the sub-boxes that pass
-// the same test, and entries no sub-box passes are dropped; max_xx then
-// receives the largest table argument any pair of a kept (sub-box, tile)
-// can have.
+// the same test, and entries no sub-box passes are dropped.
 void build_tile_lists(int ntgt, const double* tlo, const double* thi,
                       const Blocking& src, const CovarDev& cv,
                       std::vector<int>& ptr, std::vector<int>& idx,
                       const double* sublo = nullptr,
                       const double* subhi = nullptr,
-                      std::vector<unsigned char>* mask = nullptr,
-                      double* max_xx = nullptr);
+                      std::vector<unsigned char>* mask = nullptr);
 
 // Lower bound of the table argument xx for any pair drawn from two boxes,
 // with the operation order of lib/signal_covar.h:67-77 and
@@ -187,12 +183,9 @@ struct ds_problem {
   float4* toff32 = nullptr;            // per list entry: centre offsets
   double2* table64 = nullptr;          // {T[k], T[k+1]-T[k]}, last = {0, 0}
   float2* table32 = nullptr;
-  // FP32 MFMA kernels: {g_k, df_k}, the interval's interpolant as the line
-  // g_k + df_k xx; without the clamp: shifted by one and zero-extended
-  // past the largest pair argument of the lists
+  // FP32 MFMA kernel: {g_k, df_k}, the interval's interpolant as the line
+  // g_k + df_k xx
   float2* table32g = nullptr;
-  float2* table32x = nullptr;
-  int tlen_x = 0;                      // entries of table32x (0: not built)
   double* table_raw = nullptr;         // T[k] as given (dense/column paths)
   // CG work vectors (npad each)
   double *x = nullptr, *r = nullptr, *d = nullptr, *s = nullptr, *q = nullptr,

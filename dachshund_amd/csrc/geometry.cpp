@@ -165,16 +165,6 @@ double cull_limit(const CovarDev& cv) {
   return (double)(cv.table_n - 1) * (1.0 + 1e-12);
 }
 
-// Largest table argument of any pair of points in the two boxes.
-double box_max_xx(const double* alo, const double* ahi, const double* blo,
-                  const double* bhi, const CovarDev& cv) {
-  double g[3];
-  for (int a = 0; a < 3; ++a)
-    g[a] = std::max(std::fabs(bhi[a] - alo[a]), std::fabs(ahi[a] - blo[a]));
-  return (cv.c_perp * (g[0] * g[0] + g[1] * g[1]) + cv.c_para * g[2] * g[2]) *
-         cv.dx_inv;
-}
-
 namespace {
 
 // Source tiles binned on a uniform grid by box centre, so that a target
@@ -276,13 +266,12 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
                       const Blocking& src, const CovarDev& cv,
                       std::vector<int>& ptr, std::vector<int>& idx,
                       const double* sublo, const double* subhi,
-                      std::vector<unsigned char>* mask, double* max_xx) {
+                      std::vector<unsigned char>* mask) {
   const double limit = cull_limit(cv);
   const bool sub = sublo && subhi && mask;
   const TileBins bins(src, cv, limit);
   std::vector<std::vector<int>> lists(ntgt);
   std::vector<std::vector<unsigned char>> masks(sub ? ntgt : 0);
-  std::vector<double> far(sub ? ntgt : 0, 0.0);
   std::vector<int> all;
   if (!bins.ok) {
     all.resize(src.ntile);
@@ -310,10 +299,7 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
           const double* slo = sublo + 3 * ((size_t)I * DS_TPB + s);
           const double* shi = subhi + 3 * ((size_t)I * DS_TPB + s);
           if (!(slo[0] <= shi[0])) continue;   // empty sub-block
-          if (box_gap_xx(slo, shi, jlo, jhi, cv) < limit) {
-            m |= 1u << s;
-            far[I] = std::max(far[I], box_max_xx(slo, shi, jlo, jhi, cv));
-          }
+          if (box_gap_xx(slo, shi, jlo, jhi, cv) < limit) m |= 1u << s;
         }
         if (!m) continue;
         masks[I].push_back((unsigned char)m);
@@ -344,11 +330,6 @@ void build_tile_lists(int ntgt, const double* tlo, const double* thi,
     mask->resize(ptr[ntgt]);
     for (int I = 0; I < ntgt; ++I)
       std::copy(masks[I].begin(), masks[I].end(), mask->begin() + ptr[I]);
-    if (max_xx) {
-      double m = 0.0;
-      for (double v : far) m = std::max(m, v);
-      *max_xx = m;
-    }
   }
 }
 

@@ -32,8 +32,8 @@
 // and tile.  Its table stores each interval's interpolant as the line
 // g_k + df_k xx (g_k = f_k - k df_k, formed in FP64 on the host), so the
 // VALU is left with the clamp, index, table address and two fma per pair
-// (5 operations; 4 without the clamp).  Lane l = 16 g + c holds
-// A[i = c][k = g] and B[k = g][j = c]; the result D holds rows 4 g + r
+// (5 operations).  Lane l = 16 g + c holds A[i = c][k = g] and
+// B[k = g][j = c]; the result D holds rows 4 g + r
 // (r = 0..3) of column c, so each lane accumulates 16 rows x 1 source per
 // tile and the 16 lanes of a group are summed once at the end.
 #include "cg_device.h"
@@ -55,8 +55,6 @@ struct Matvec32Args {
   const float4* toff;     // per list entry: block frame - tile frame
   const float2* table;    // {f[k], f[k+1]-f[k]}, last = {0, 0}
   const float2* table_g;  // {g, df}: the interpolant as g + df xx (MFMA)
-  const float2* table_x;  // shifted, zero-extended lines (clamp-free MFMA)
-  int tlen_x;
   const int* blk_sub;
   const int* blk_nreal;
   const int* sub_blk0;
@@ -141,12 +139,7 @@ __global__ __launch_bounds__(256) void k_matvec32(
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// NOCLAMP: the accumulator input carries |l|^2 + 1, so the MFMA yields
-// xx + 1 > 0 for every pair and (int) truncation needs no clamp below; the
-// shifted table (entry 0 = {f_0, 0}: arguments rounded below 0) is
-// zero-extended past the largest argument the problem's lists can reach,
-// so none above either.  4 VALU operations per pair.
-template <int FORM, bool CG, bool NOCLAMP>
+template <int FORM, bool CG>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
     const int* __restrict__ tidx, const float4* __restrict__ toff,
@@ -158,9 +151,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   const ds_cg_scalars* sc = a.sc + sv;
   if (CG && sc->done) return;
   {
-    const float2* tsrc = NOCLAMP ? a.table_x : a.table_g;
-    const int tlen = NOCLAMP ? a.tlen_x : a.table_n;
-    for (int k = threadIdx.x; k < tlen; k += blockDim.x) tab32[k] = tsrc[k];
+    for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
+      tab32[k] = a.table_g[k];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -178,8 +170,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float4 m = a.tgt[row0 + 16 * s + 4 * g + r];
-      ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x)) +
-                 (NOCLAMP ? 1.0f : 0.0f);
+      ci[s][r] = fmaf(m.z, m.z, fmaf(m.y, m.y, m.x * m.x));
     }
   }
   const float cx = g == 0 ? -2.0f : 0.0f, cy = g == 1 ? -2.0f : 0.0f;
@@ -219,8 +210,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       if (!(live & (1u << s))) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float xx =
-            NOCLAMP ? d[s][r] : __builtin_amdgcn_fmed3f(d[s][r], 0.0f, tlast);
+        const float xx = __builtin_amdgcn_fmed3f(d[s][r], 0.0f, tlast);
         const float2 f = tab32[(int)xx];
         acc[s][r] = fmaf(fmaf(f.y, xx, f.x), uj, acc[s][r]);
       }
@@ -315,8 +305,6 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.toff = p->toff32;
   a.table = p->table32;
   a.table_g = p->table32g;
-  a.table_x = p->table32x;
-  a.tlen_x = p->tlen_x;
   a.out = out;
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
@@ -336,18 +324,11 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                      a.u32, a.tidx, a.toff, a)
   const bool d = p->f32_direct;
   if (!d && p->f32_kind != DS_F32_VALU) {
-    const bool nc = p->f32_kind == DS_F32_MFMA && p->tlen_x > 0;
-    const size_t shm_m = sizeof(float2) * (nc ? p->tlen_x : p->cv.table_n);
-#define DS_L32M(F, C, N) \
-  hipLaunchKernelGGL((k_matvec32m<F, C, N>), grid, block, shm_m, st, a.loc, \
+#define DS_L32M(F, C) \
+  hipLaunchKernelGGL((k_matvec32m<F, C>), grid, block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
-    if (nc) {
-      if (cg) { if (w) DS_L32M(DS_FORM_W, true, true); else DS_L32M(DS_FORM_SN, true, true); }
-      else { if (w) DS_L32M(DS_FORM_W, false, true); else DS_L32M(DS_FORM_SN, false, true); }
-    } else {
-      if (cg) { if (w) DS_L32M(DS_FORM_W, true, false); else DS_L32M(DS_FORM_SN, true, false); }
-      else { if (w) DS_L32M(DS_FORM_W, false, false); else DS_L32M(DS_FORM_SN, false, false); }
-    }
+    if (cg) { if (w) DS_L32M(DS_FORM_W, true); else DS_L32M(DS_FORM_SN, true); }
+    else { if (w) DS_L32M(DS_FORM_W, false); else DS_L32M(DS_FORM_SN, false); }
 #undef DS_L32M
   } else if (cg) {
     if (w) { if (d) DS_L32(DS_FORM_W, true, true); else DS_L32(DS_FORM_W, true, false); }

@@ -36,9 +36,7 @@ typedef enum {
 enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
 /* FP32 pair kernel: table argument expanded on the VALU, or on the matrix
  * cores (v_mfma_f32_16x16x4f32, with 16-row sub-block culling; the
- * default).  The MFMA form runs without the argument clamp when the
- * problem's largest listed pair argument fits an extended table (decided at
- * creation); DS_F32_MFMA_CLAMPED forces the clamped form.  Problems with
+ * default; DS_F32_MFMA_CLAMPED is accepted as the same kernel).  Problems with
  * points far from their tile frame always use direct differences on the
  * VALU. */
 enum { DS_F32_VALU = 0, DS_F32_MFMA = 1, DS_F32_MFMA_CLAMPED = 2 };
