@@ -45,6 +45,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def kernel_ms(wf, prec):
+    """Average launch duration (ms, HIP events on the problem's stream) of
+    the pair kernel of precision `prec`: 200 launches to reach the steady
+    clock of a running solve, then the median of 5 averages of 25."""
+    wf.time_matvec(prec, reps=200)
+    ts = sorted(wf.time_matvec(prec, reps=25) for _ in range(5))
+    return ts[2]
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` on the 300k-pixel problem, from the
     calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
@@ -263,8 +272,8 @@ def bench_chunk(args):
             assert r.success, name
         out[name] = {"s": float(np.mean(ts)), "iters": r.num_iters}
         log(f"{name}: {np.mean(ts) * 1e3:.1f} ms, {r.num_iters} iters")
-    ms64 = wf.time_matvec(D.DS_PREC_F64, reps=20)
-    ms32 = wf.time_matvec(D.DS_PREC_F32, reps=20)
+    ms64 = kernel_ms(wf, D.DS_PREC_F64)
+    ms32 = kernel_ms(wf, D.DS_PREC_F32)
     log(f"matvec: FP64 {ms64:.3f} ms ({st.pairs_per_matvec / ms64 * 1e3:.3e} "
         f"pairs/s), FP32 {ms32:.3f} ms "
         f"({st.sub_pairs_per_matvec / ms32 * 1e3:.3e} pairs/s)")
@@ -297,6 +306,55 @@ def bench_chunk(args):
         "pairs_per_matvec": {"f64": st.pairs_per_matvec,
                              "f32": st.sub_pairs_per_matvec},
         "map_s": t_map, "map_norm": float(np.linalg.norm(m)),
+    }), flush=True)
+
+
+def bench_kernels(args):
+    """Pair kernels alone on the 300,000-pixel problem, for hardware-counter
+    runs: first a 1 GiB device-to-device copy (twice; the runtime's copy
+    kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the byte counters), then
+    `--steps` launches each of the VALU FP32, the MFMA FP32 and the FP64
+    pair kernel (4 waves per block, longest list first)."""
+    import ctypes
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    nbytes = ctypes.c_size_t(1 << 30)
+    assert hip.hipMalloc(ctypes.byref(a), nbytes) == 0
+    assert hip.hipMalloc(ctypes.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, nbytes, 3) == 0      # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    w = synth.workload("c5")
+    pix = synth.make_pixels(w)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    wf.set_launch(4, True)
+    st = wf.stats()
+    out = {}
+    for name, kind in (("k_matvec32", D.DS_F32_VALU),
+                       ("k_matvec32m", D.DS_F32_MFMA)):
+        wf.set_f32_kernel(kind)
+        out[name] = wf.time_matvec(D.DS_PREC_F32, reps=args.steps)
+    out["k_matvec64"] = wf.time_matvec(D.DS_PREC_F64, reps=args.steps)
+    for k, ms in out.items():
+        log(f"{k}: {ms * 1e3:.1f} us per launch")
+    print(json.dumps({
+        "metric": "pair kernel launch time (300,000 pixels)",
+        "value": out["k_matvec32m"] * 1e3, "unit": "us per launch",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": out["k_matvec32m"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 / f64",
+        "data": "synthetic (seeded GRF skewers)",
+        "config": {"workload": "300000 pixels, W form, pair kernels alone"},
+        "us_per_launch": {k: v * 1e3 for k, v in out.items()},
+        "pairs_per_launch": {"k_matvec32": st.pairs_per_matvec,
+                             "k_matvec32m": st.sub_pairs_per_matvec,
+                             "k_matvec64": st.pairs_per_matvec},
     }), flush=True)
 
 
@@ -380,7 +438,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c5")
+    ap.add_argument("--workload", default="c5",
+                    choices=("c5", "c1", "c2", "c3", "c4", "kernels"),
+                    help="c5: the 300k-pixel headline (default; c1 runs the "
+                         "same comparison on 8,192 pixels); c2: 76,800 "
+                         "pixels and the 2.53M-cell map; c3: 100k pixels x "
+                         "32 realizations; c4: 512 subvolumes; kernels: the "
+                         "pair kernels alone (for counter runs)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--drop", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -402,6 +466,8 @@ def main():
         return bench_batch(args)
     if args.workload == "c2":
         return bench_chunk(args)
+    if args.workload == "kernels":
+        return bench_kernels(args)
 
     import dachshund_amd as D
     from dachshund_amd import synth
@@ -519,8 +585,8 @@ def main():
         f"{rel_mx:.2e}, plain FP64 {rel_64:.2e}")
 
     # dominant kernel: FP32 pair matvec (the mixed solver's inner operator)
-    ms32 = wf.time_matvec(D.DS_PREC_F32, reps=20)
-    ms64 = wf.time_matvec(D.DS_PREC_F64, reps=20)
+    ms32 = kernel_ms(wf, D.DS_PREC_F32)
+    ms64 = kernel_ms(wf, D.DS_PREC_F64)
     pairs = st.pairs_per_matvec          # FP64 kernel (64-row blocks)
     pairs32 = st.sub_pairs_per_matvec    # FP32 MFMA kernel (16-row culling)
     tf32 = FLOPS_PER_PAIR * pairs32 / (ms32 * 1e-3) / 1e12
@@ -584,8 +650,10 @@ def main():
             "traffic": pmc_traffic("k_matvec32m"),
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + "
                             "WRITE_SIZE, copy-calibrated)",
-            "limiter": "VALU issue: 6 VALU ops per pair beside the MFMA "
-                       "(PMC counters in profiles/)",
+            "limiter": "VALU issue (5 VALU ops per live pair plus ~1.6 per "
+                       "pair of per-entry work) serialized with the MFMA "
+                       "(no VALU/MFMA co-execution measured; PMC counters "
+                       "in profiles/)",
             "kernel": "k_matvec32m (FP32 pair matvec, MFMA table arguments)",
             "ms_per_launch": ms32, "flops_per_pair": FLOPS_PER_PAIR,
             "pairs_per_launch": pairs32,
