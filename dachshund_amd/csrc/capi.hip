@@ -1286,12 +1286,44 @@ int ds_problem_build_dense_bf16(ds_problem* p) {
   p->dense_mpad = mpad;
   HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_s, (size_t)mpad * ld));
   HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_y,
-                                (size_t)p->dense_split * DS_DENSE_R * mpad));
+                                (size_t)DS_DENSE_SPLIT_MAX * DS_DENSE_R * mpad));
   HIP_OR_FAIL(p->ctx, dev_alloc(&p->du16, (size_t)DS_DENSE_R * ld));
   // unused right-hand-side columns stay zero
   HIP_OR_FAIL(p->ctx, hipMemsetAsync(p->du16, 0, sizeof(unsigned short) *
                                      (size_t)DS_DENSE_R * ld, st));
   HIP_OR_FAIL(p->ctx, ds::dense_build(p, st));
+  // Split-K count of the GEMM: how evenly its workgroups fall on the CUs
+  // decides the rate, so time each candidate on the problem itself (the
+  // rate does not depend on the data) and keep the fastest.
+  const int zero = 0;
+  HIP_OR_FAIL(p->ctx, hipMemcpyAsync(p->done_count, &zero, sizeof(int),
+                                     hipMemcpyHostToDevice, st));
+  hipEvent_t e0, e1;
+  HIP_OR_FAIL(p->ctx, hipEventCreate(&e0));
+  HIP_OR_FAIL(p->ctx, hipEventCreate(&e1));
+  hipError_t e = hipSuccess;
+  float best = 1e30f;
+  int best_split = p->dense_split;
+  for (int split : {2, 3, 4, 5, 6, 8}) {
+    static_assert(DS_DENSE_SPLIT_MAX >= 8, "dense_y holds 8 splits");
+    p->dense_split = split;
+    e = ds::dense_gemm(p, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = ds::dense_gemm(p, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    if (ms < best) {
+      best = ms;
+      best_split = split;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_OR_FAIL(p->ctx, e);
+  p->dense_split = best_split;
   HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
   p->use_dense = true;
   return DS_OK;

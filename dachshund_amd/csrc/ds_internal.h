@@ -34,6 +34,7 @@
 #define DS_MAX_TABLE 1024
 #define DS_MAP_T 4        // map tiles are 4x4x4 cells
 #define DS_DENSE_R 32     // right-hand sides of the explicit BF16 block path
+#define DS_DENSE_SPLIT_MAX 8   // largest split-K count of its GEMM
 
 namespace ds {
 
