@@ -397,11 +397,12 @@ def bench_batch(args):
     t_batch = float(np.mean(ts))
     log(f"batched: {t_batch * 1e3:.1f} ms, iterations <= "
         f"{max(r.num_iters for r in res)}")
-    # the same subvolumes one after another
+    # the same subvolumes one after another, each with the library's
+    # default launch shape (4 waves per block, longest list first; the
+    # batch's tuned shape suits a 6.7M-pixel launch, not a 2k-pixel one)
     t_seq = 0.0
     for p, sp in zip(pix, sps):
         wf = D.WFVectorW(p, sp)
-        wf.set_launch(waves, order)
         wf.set_precond(kind)
         wf.fill_x(0.0)
         t = time.perf_counter()

@@ -343,6 +343,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->blk_sub);
   dev_free(p->blk_nreal);
   dev_free(p->blk_order);
+  dev_free(p->ord32);
   dev_free(p->sub_blk0);
   dev_free(p->sub_cv);
   dev_free(p->tcount);
@@ -631,7 +632,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   // points are then near the source frame, which keeps the FP32 expansion
   // |Xi|^2 + |Xj|^2 - 2 Xi.Xj well conditioned.
   std::vector<double> cen(3 * (size_t)nblk), tcen(3 * (size_t)p->ntile);
-  double max_src = 0.0;
+  std::vector<double> max_src(n_sub, 0.0);
   for (int k = 0; k < n_sub; ++k) {
     const Blocking& b = blks[k];
     const ds::CovarDev& cv = p->sub_cv_h[k];
@@ -683,7 +684,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       for (int a = 0; a < 3; ++a) {
         l[a] = (float)((Pj[a] - tcen[3 * T + a]) * s3[a]);
         t[a] = (float)((Pj[a] - cen[3 * B + a]) * s3[a]);
-        if (real) max_src = std::max(max_src, std::fabs((double)l[a]));
+        if (real) max_src[k] = std::max(max_src[k], std::fabs((double)l[a]));
       }
       hloc[g] = make_float4(l[0], l[1], l[2],
                             l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
@@ -691,8 +692,9 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     }
   }
   // A tile holding a point far from its median (isolated outliers) would
-  // make the expansion lose its accuracy: use direct differences then.
-  p->f32_direct = max_src > 16.0;
+  // make the expansion lose its accuracy: that subvolume's blocks use
+  // direct differences (decided per subvolume, so a batch runs each
+  // subvolume's blocks with the kernel it would get alone).
   // per list entry: target block frame minus source tile frame (pre-scaled)
   std::vector<float4> hoff(idx.size());
   for (int I = 0; I < nblk; ++I) {
@@ -717,6 +719,23 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return p->blk_len_h[a] > p->blk_len_h[b];
   });
+  // FP32 launch lists: the same two orders, each split into the blocks of
+  // subvolumes without far points, then the others
+  std::vector<int> ord32(2 * (size_t)nblk);
+  {
+    auto direct = [&](int I) { return max_src[hblk_sub[I]] > 16.0; };
+    int nf = 0;
+    for (int I : order) nf += !direct(I);
+    p->n32_fast = nf;
+    for (int part = 0; part < 2; ++part) {
+      int* dst = ord32.data() + (size_t)part * nblk;
+      int f = 0, d = nf;
+      for (int k = 0; k < nblk; ++k) {
+        const int I = part == 0 ? order[k] : k;
+        if (direct(I)) dst[d++] = I; else dst[f++] = I;
+      }
+    }
+  }
   std::vector<double2> t64;
   std::vector<float2> t32;
   make_tables(&covars[0], t64, t32);
@@ -750,6 +769,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_order, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->ord32, 2 * (size_t)nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
@@ -794,6 +814,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->blk_order, order.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->ord32, ord32.data(), sizeof(int) * 2 * (size_t)nblk));
   HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
                       sizeof(int) * (n_sub + 1)));
   HIP_OR_FAIL(ctx, up(p->sub_cv, hsubcv.data(), sizeof(double4) * n_sub));

@@ -177,7 +177,12 @@ struct ds_problem {
   double* diag = nullptr;              // Jacobi diagonal of A
   float4* loc32 = nullptr;             // sources: tile frame, .w = |X|^2
   float4* tgt32 = nullptr;             // targets: block frame
-  bool f32_direct = false;             // FP32 kernel uses direct differences
+  // FP32 kernel launch lists: blocks of subvolumes whose points all sit
+  // near their tile frames first (MFMA or expanded VALU form), then those
+  // of the rest (direct differences); longest list first in [0, nblk),
+  // block order in [nblk, 2 nblk)
+  int* ord32 = nullptr;
+  int n32_fast = 0;                    // blocks in the first part
   int f32_kind = DS_F32_MFMA;          // FP32 pair kernel: MFMA or VALU form
   int* tptr = nullptr;
   int* tidx = nullptr;

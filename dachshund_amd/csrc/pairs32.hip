@@ -59,7 +59,7 @@ struct Matvec32Args {
   const int* blk_nreal;
   const int* sub_blk0;
   const double4* sub_cv;
-  const int* blk_order;
+  const int* blk_order;   // launch slot -> block
   double* out;
   double* partials;
   const ds_cg_scalars* sc;
@@ -310,33 +310,40 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.blk_nreal = p->blk_nreal;
   a.sub_blk0 = p->sub_blk0;
   a.sub_cv = p->sub_cv;
-  a.blk_order = p->order_by_cost ? p->blk_order : nullptr;
   a.partials = p->partials;
   a.sc = p->sc;
   a.tlast = (float)(p->cv.table_n - 1);
   a.nblk = p->nblk;
   a.table_n = p->cv.table_n;
-  const dim3 grid(p->nblk), block(64 * p->waves_per_block);
+  const dim3 block(64 * p->waves_per_block);
   const size_t shm = sizeof(float2) * p->cv.table_n;
   const bool w = p->form == DS_FORM_W;
-#define DS_L32(F, C, D) \
-  hipLaunchKernelGGL((k_matvec32<F, C, D>), grid, block, shm, st, a.loc, \
+  // blocks of subvolumes without far points: MFMA (or expanded VALU) form;
+  // the others: direct differences
+  const int* ord = p->ord32 + (p->order_by_cost ? 0 : p->nblk);
+  const int nf = p->n32_fast, nd = p->nblk - p->n32_fast;
+#define DS_L32(F, C, D, G) \
+  hipLaunchKernelGGL((k_matvec32<F, C, D>), dim3(G), block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
-  const bool d = p->f32_direct;
-  if (!d && p->f32_kind != DS_F32_VALU) {
-#define DS_L32M(F, C) \
-  hipLaunchKernelGGL((k_matvec32m<F, C>), grid, block, shm, st, a.loc, \
+#define DS_L32M(F, C, G) \
+  hipLaunchKernelGGL((k_matvec32m<F, C>), dim3(G), block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
-    if (cg) { if (w) DS_L32M(DS_FORM_W, true); else DS_L32M(DS_FORM_SN, true); }
-    else { if (w) DS_L32M(DS_FORM_W, false); else DS_L32M(DS_FORM_SN, false); }
-#undef DS_L32M
-  } else if (cg) {
-    if (w) { if (d) DS_L32(DS_FORM_W, true, true); else DS_L32(DS_FORM_W, true, false); }
-    else { if (d) DS_L32(DS_FORM_SN, true, true); else DS_L32(DS_FORM_SN, true, false); }
-  } else {
-    if (w) { if (d) DS_L32(DS_FORM_W, false, true); else DS_L32(DS_FORM_W, false, false); }
-    else { if (d) DS_L32(DS_FORM_SN, false, true); else DS_L32(DS_FORM_SN, false, false); }
+  if (nf > 0) {
+    a.blk_order = ord;
+    if (p->f32_kind != DS_F32_VALU) {
+      if (cg) { if (w) DS_L32M(DS_FORM_W, true, nf); else DS_L32M(DS_FORM_SN, true, nf); }
+      else { if (w) DS_L32M(DS_FORM_W, false, nf); else DS_L32M(DS_FORM_SN, false, nf); }
+    } else {
+      if (cg) { if (w) DS_L32(DS_FORM_W, true, false, nf); else DS_L32(DS_FORM_SN, true, false, nf); }
+      else { if (w) DS_L32(DS_FORM_W, false, false, nf); else DS_L32(DS_FORM_SN, false, false, nf); }
+    }
   }
+  if (nd > 0) {
+    a.blk_order = ord + nf;
+    if (cg) { if (w) DS_L32(DS_FORM_W, true, true, nd); else DS_L32(DS_FORM_SN, true, true, nd); }
+    else { if (w) DS_L32(DS_FORM_W, false, true, nd); else DS_L32(DS_FORM_SN, false, true, nd); }
+  }
+#undef DS_L32M
 #undef DS_L32
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess || !cg) return err;

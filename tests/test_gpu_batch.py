@@ -20,10 +20,18 @@ def _subvolumes(count=6, seed=4):
     return subs, [synth.make_pixels(w) for w in subs]
 
 
+@pytest.mark.parametrize("far", [False, True])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_batch_bit_identical_to_single(mixed):
+def test_batch_bit_identical_to_single(mixed, far):
+    """far: one subvolume gets a pixel 60 Mpc/h away from the others, so
+    its FP32 kernel is the direct-difference form while the rest of the
+    batch keeps the MFMA form; each subvolume must still match its own
+    single-volume solve bit for bit."""
     import dachshund_amd as D
     subs, pix = _subvolumes()
+    if far:
+        pix[2] = pix[2].copy()
+        pix[2][-1, 0] += 60.0
     sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
     wpix = [D.pixel_to_wpixel(p) for p in pix]
     batch = D.BatchWF(wpix, sps)
@@ -43,6 +51,14 @@ def test_batch_bit_identical_to_single(mixed):
         assert res[k].success and r.success
         assert res[k].num_iters == r.num_iters, k
         assert np.array_equal(xb[k], x), k
+    if far:
+        # the FP32 operator of the mixed batch against the FP64 one
+        rng = np.random.default_rng(3)
+        a = rng.standard_normal(n)
+        b64 = batch.split(batch.A_product(a, D.DS_PREC_F64))
+        b32 = batch.split(batch.A_product(a, D.DS_PREC_F32))
+        for u, v in zip(b32, b64):
+            assert np.linalg.norm(u - v) <= 1e-5 * np.linalg.norm(v)
     # graph replay of the same iterations
     if mixed:
         batch.fill_x(0.0)
