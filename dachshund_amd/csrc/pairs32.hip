@@ -139,6 +139,22 @@ __global__ __launch_bounds__(256) void k_matvec32(
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// One butterfly step over H pairs of values: the lane whose bit is clear
+// keeps v[0..H) and sends v[H..2H), its partner (DPP control CTRL, a
+// permutation within 16-lane rows) the reverse; v[0..H) becomes kept +
+// received.
+template <int H, int CTRL>
+__device__ __forceinline__ void butterfly_step(float* v, int upper) {
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const float keep = upper ? v[j + H] : v[j];
+    const float send = upper ? v[j] : v[j + H];
+    const float recv = __int_as_float(__builtin_amdgcn_update_dpp(
+        0, __float_as_int(send), CTRL, 0xF, 0xF, false));
+    v[j] = keep + recv;
+  }
+}
+
 template <int FORM, bool CG>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
@@ -254,22 +270,24 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       R1 = R3;
     }
   }
-  // sum over the 16 source columns of each lane group
+  // Sum over the 16 source columns (lanes c) of each lane group: a
+  // butterfly that halves the 16 values at every step.  The pair (c, c')
+  // of a step differs in one bit b of c (and maybe lower ones); the lane
+  // with b clear keeps the lower half of its values and sends the upper,
+  // its partner the reverse, and each adds what it receives.  Partners:
+  // c ^ 15 (row mirror), c ^ 7 (half-row mirror), c ^ 2, c ^ 1 (quad
+  // permutes); these span all 16 lanes, and lane c ends with the total of
+  // value index c = 4 s + r.
+  float v[16];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float v = acc[s][r];
-#pragma unroll
-      for (int m = 1; m < 16; m <<= 1) v += __shfl_xor(v, m, 64);
-      acc[s][r] = v;
-    }
-  if (c == 0)
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        red[wave * 64 + 16 * s + 4 * g + r] = acc[s][r];
+    for (int r = 0; r < 4; ++r) v[4 * s + r] = acc[s][r];
+  butterfly_step<8, 0x140>(v, c & 8);
+  butterfly_step<4, 0x141>(v, c & 4);
+  butterfly_step<2, 0x4E>(v, c & 2);
+  butterfly_step<1, 0xB1>(v, c & 1);
+  red[wave * 64 + 16 * (c >> 2) + 4 * g + (c & 3)] = v[0];
   __syncthreads();
   if (wave != 0) return;
   const int row = row0 + lane;

@@ -36,9 +36,9 @@ typedef enum {
 enum { DS_FORM_SN = 0, DS_FORM_W = 1 };
 /* FP32 pair kernel: table argument expanded on the VALU, or on the matrix
  * cores (v_mfma_f32_16x16x4f32, with 16-row sub-block culling; the
- * default; DS_F32_MFMA_CLAMPED is accepted as the same kernel).  Problems with
- * points far from their tile frame always use direct differences on the
- * VALU. */
+ * default; DS_F32_MFMA_CLAMPED is accepted as the same kernel).  A
+ * subvolume with a point far from its tile frame always uses direct
+ * differences on the VALU for its own blocks (decided per subvolume). */
 enum { DS_F32_VALU = 0, DS_F32_MFMA = 1, DS_F32_MFMA_CLAMPED = 2 };
 /* Preconditioner of a problem's device solves: the diagonal of A (the
  * reference's Jacobi; the default) or the exact inverse of each 64-row
