@@ -185,23 +185,32 @@ struct TileBins {
     if (!(kperp > 0.0) || !(kpara > 0.0)) return;
     const double r[3] = {std::sqrt(limit / kperp), std::sqrt(limit / kperp),
                          std::sqrt(limit / kpara)};
+    // The bins span the 1st to 99th percentile of the tile centres per
+    // axis (outliers would stretch a min-max grid until the bulk shares a
+    // few bins); centres outside fall into the edge bins, since the bin
+    // index is clamped, and a clamped monotone index keeps every query a
+    // superset of the tiles within reach.
     double cmax[3], h[3] = {0.0, 0.0, 0.0};
-    int live = 0;
-    for (int a = 0; a < 3; ++a) { cmin[a] = 1e300; cmax[a] = -1e300; }
+    std::vector<double> cs[3];
     for (int J = 0; J < src.ntile; ++J) {
       if (src.tcount[J] == 0) continue;
-      ++live;
       for (int a = 0; a < 3; ++a) {
         const double lo = src.tlo[3 * (size_t)J + a];
         const double hi = src.thi[3 * (size_t)J + a];
-        const double c = 0.5 * (lo + hi);
-        cmin[a] = std::min(cmin[a], c);
-        cmax[a] = std::max(cmax[a], c);
+        cs[a].push_back(0.5 * (lo + hi));
         h[a] = std::max(h[a], 0.5 * (hi - lo));
       }
     }
+    const int live = (int)cs[0].size();
     if (live == 0) return;
-    size_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double>& v = cs[a];
+      const size_t q0 = (size_t)(0.01 * (live - 1)), q1 = live - 1 - q0;
+      std::nth_element(v.begin(), v.begin() + q0, v.end());
+      cmin[a] = v[q0];
+      std::nth_element(v.begin(), v.begin() + q1, v.end());
+      cmax[a] = v[q1];
+    }
     for (int a = 0; a < 3; ++a) {
       if (!std::isfinite(r[a]) || !std::isfinite(cmax[a] - cmin[a])) return;
       // centres farther than the cutoff plus the widest tile's half-extent
@@ -212,8 +221,21 @@ struct TileBins {
       const double span = cmax[a] - cmin[a];
       const double w = std::max(reach[a], span / 1024.0);
       nb[a] = (w > 0.0) ? std::max(1, (int)std::min(1024.0, span / w) + 1) : 1;
+    }
+    // any bin grid gives the same candidates' superset; keep the grid
+    // within a few bins per tile (outliers stretch the span, not the work)
+    const size_t cap = std::max<size_t>(4096, 2 * (size_t)live);
+    auto count = [&]() { return (size_t)nb[0] * nb[1] * nb[2]; };
+    while (count() > cap) {
+      int a = 0;
+      for (int q = 1; q < 3; ++q)
+        if (nb[q] > nb[a]) a = q;
+      nb[a] = (nb[a] + 1) / 2;
+    }
+    const size_t total = count();
+    for (int a = 0; a < 3; ++a) {
+      const double span = cmax[a] - cmin[a];
       inv[a] = (span > 0.0) ? nb[a] / (span * (1.0 + 1e-12)) : 0.0;
-      total *= (size_t)nb[a];
     }
     start.assign(total + 1, 0);
     std::vector<int> bin(src.ntile, -1);
