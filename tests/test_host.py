@@ -43,7 +43,8 @@ def _nonzero_pairs(ora, pts, c):
     return np.argwhere(S != 0.0)
 
 
-@pytest.mark.parametrize("case", ["fixture", "tiny", "small", "clustered"])
+@pytest.mark.parametrize("case", ["fixture", "tiny", "small", "clustered",
+                                  "outliers"])
 def test_culling_keeps_every_nonzero_pair(ora, case):
     import dachshund_amd as D
     from dachshund_amd import synth
@@ -56,6 +57,15 @@ def test_culling_keeps_every_nonzero_pair(ora, case):
         pts = np.repeat(base, 20, axis=0)
         pts[::3] += rng.normal(0, 3.0, (len(pts[::3]), 3))
         lp, ll = 1.7, 2.9
+    elif case == "outliers":
+        # far-away points stretch the tile-centre range (the binned
+        # candidate search must still find every listed tile)
+        w = synth.workload("small")
+        pts = synth.make_pixels(w)[:, :3].copy()
+        pts[5] = [1e4, -3e3, 5e5]
+        pts[700] = [-2e3, 40.0, 7.0]
+        pts[1300:1310] = pts[1300] + np.array([0.0, 0.0, 1e3])
+        lp, ll = w.l_perp, w.l_para
     else:
         w = synth.workload(case)
         pts, (lp, ll) = synth.make_pixels(w)[:, :3], (w.l_perp, w.l_para)
