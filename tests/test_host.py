@@ -118,3 +118,16 @@ def test_read_pixel_data_roundtrip(tmp_path):
         D.read_pixel_data(str(tmp_path / "nope.bin"), 3)
     with pytest.raises(ValueError):
         D.read_pixel_data(str(p), len(pix) + 1)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU's share; asked for more it must say so and
+    exit before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
