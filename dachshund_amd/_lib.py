@@ -47,7 +47,8 @@ class DsProblemStats(C.Structure):
                 ("tile_pairs", C.c_longlong),
                 ("pairs_per_matvec", C.c_longlong),
                 ("sub_pairs_per_matvec", C.c_longlong),
-                ("setup_seconds", C.c_double)]
+                ("setup_seconds", C.c_double),
+                ("sub_pairs_evaluated", C.c_longlong)]
 
 
 # name -> (restype, argtypes); every symbol of include/dachshund_gpu.h

@@ -344,6 +344,12 @@ void problem_free(ds_problem* p) {
   dev_free(p->blk_nreal);
   dev_free(p->blk_order);
   dev_free(p->ord32);
+  dev_free(p->hptr);
+  dev_free(p->hrec);
+  dev_free(p->pbuf);
+  dev_free(p->ydir);
+  dev_free(p->gptr);
+  dev_free(p->glist);
   dev_free(p->sub_blk0);
   dev_free(p->sub_cv);
   dev_free(p->tcount);
@@ -713,26 +719,78 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       hoff[e] = make_float4(o[0], o[1], o[2], wbits);
     }
   }
+  // Half lists of the symmetric MFMA kernel.  The 16-row sub-blocks of a
+  // block are its own tiles (tile 4 B + s), and the sub-block test is the
+  // tile-pair test, so the pair {P, Q} of tiles is live in P's block's
+  // entry for Q exactly when it is live in Q's block's entry for P: keep
+  // it once, in the block of the smaller tile (both products from one
+  // evaluation), and a tile with itself once (its own product only).
+  std::vector<int> hptr(nblk + 1, 0);
+  std::vector<float4> hrec;
+  std::vector<int> gcnt(p->ntile + 1, 0), gsrc;   // (entry, tile) pairs
+  hrec.reserve(idx.size() / 2 + nblk);
+  p->nsub_eval = 0;
+  for (int B = 0; B < nblk; ++B) {
+    for (int e = ptr[B]; e < ptr[B + 1]; ++e) {
+      const int T = idx[e], dT = T - DS_TPB * B;
+      if (dT < 0) continue;
+      // sub-blocks s <= dT evaluate, s < dT also give the transposed part
+      const unsigned keep = dT >= DS_TPB - 1 ? 15u : (2u << dT) - 1u;
+      const unsigned m = msk[e] & keep;
+      if (!m) continue;
+      const unsigned tr = m & (dT >= DS_TPB ? 15u : (1u << dT) - 1u);
+      const unsigned bits = m | ((unsigned)T << 4);
+      float wbits;
+      std::memcpy(&wbits, &bits, sizeof(float));
+      if (tr) {
+        gsrc.push_back((int)hrec.size());
+        gsrc.push_back(T);
+        ++gcnt[T + 1];
+      }
+      hrec.push_back(make_float4(hoff[e].x, hoff[e].y, hoff[e].z, wbits));
+      p->nsub_eval += __builtin_popcount(m);
+    }
+    hptr[B + 1] = (int)hrec.size();
+  }
+  std::vector<int> gptr(p->ntile + 1, 0), glist(gsrc.size() / 2);
+  for (int T = 0; T < p->ntile; ++T) gptr[T + 1] = gptr[T] + gcnt[T + 1];
+  {
+    // entries in block order within each tile (gsrc is in block order)
+    std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+    for (size_t k = 0; k < gsrc.size(); k += 2)
+      glist[fill[gsrc[k + 1]]++] = gsrc[k];
+  }
   // launch order: longest tile list first (for the autotuner)
   std::vector<int> order(nblk);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return p->blk_len_h[a] > p->blk_len_h[b];
   });
-  // FP32 launch lists: the same two orders, each split into the blocks of
-  // subvolumes without far points, then the others
+  // FP32 launch lists: longest list first and block order, each split into
+  // the blocks of subvolumes without far points (lengths of their half
+  // lists, which the MFMA kernel runs), then the others
   std::vector<int> ord32(2 * (size_t)nblk);
   {
     auto direct = [&](int I) { return max_src[hblk_sub[I]] > 16.0; };
     int nf = 0;
     for (int I : order) nf += !direct(I);
     p->n32_fast = nf;
+    // the MFMA kernel runs the half lists: longest half list first
+    std::vector<int> horder(nblk);
+    std::iota(horder.begin(), horder.end(), 0);
+    std::stable_sort(horder.begin(), horder.end(), [&](int a, int b) {
+      return hptr[a + 1] - hptr[a] > hptr[b + 1] - hptr[b];
+    });
     for (int part = 0; part < 2; ++part) {
       int* dst = ord32.data() + (size_t)part * nblk;
       int f = 0, d = nf;
       for (int k = 0; k < nblk; ++k) {
-        const int I = part == 0 ? order[k] : k;
-        if (direct(I)) dst[d++] = I; else dst[f++] = I;
+        if (part == 0) {
+          if (!direct(horder[k])) dst[f++] = horder[k];
+          if (direct(order[k])) dst[d++] = order[k];
+        } else {
+          if (direct(k)) dst[d++] = k; else dst[f++] = k;
+        }
       }
     }
   }
@@ -770,6 +828,12 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_order, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->ord32, 2 * (size_t)nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->hptr, nblk + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->hrec, std::max<size_t>(hrec.size(), 1)));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * hrec.size(), 1)));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->ydir, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, p->ntile + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->glist, std::max<size_t>(glist.size(), 1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
@@ -815,6 +879,12 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->blk_order, order.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->ord32, ord32.data(), sizeof(int) * 2 * (size_t)nblk));
+  HIP_OR_FAIL(ctx, up(p->hptr, hptr.data(), sizeof(int) * (nblk + 1)));
+  if (!hrec.empty())
+    HIP_OR_FAIL(ctx, up(p->hrec, hrec.data(), sizeof(float4) * hrec.size()));
+  HIP_OR_FAIL(ctx, up(p->gptr, gptr.data(), sizeof(int) * (p->ntile + 1)));
+  if (!glist.empty())
+    HIP_OR_FAIL(ctx, up(p->glist, glist.data(), sizeof(int) * glist.size()));
   HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
                       sizeof(int) * (n_sub + 1)));
   HIP_OR_FAIL(ctx, up(p->sub_cv, hsubcv.data(), sizeof(double4) * n_sub));
@@ -865,6 +935,7 @@ int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out) {
   out->pairs_per_matvec = p->ntile_pairs * (long long)(DS_BS * DS_TS);
   out->sub_pairs_per_matvec = p->nsub_pairs * (long long)(DS_TS * DS_TS);
   out->setup_seconds = p->setup_seconds;
+  out->sub_pairs_evaluated = p->nsub_eval * (long long)(DS_TS * DS_TS);
   return DS_OK;
 }
 

@@ -183,6 +183,19 @@ struct ds_problem {
   // block order in [nblk, 2 nblk)
   int* ord32 = nullptr;
   int n32_fast = 0;                    // blocks in the first part
+  // Symmetric MFMA FP32 kernel: each unordered pair of 16-slot tiles is
+  // evaluated once.  Block B keeps the entries of tiles T >= 4 B, live
+  // bits restricted to its sub-blocks s with 4 B + s <= T (records as
+  // toff32); sub-blocks with 4 B + s < T also give the transposed product,
+  // whose 16 column sums per entry go to pbuf and are gathered per tile
+  // (gptr / glist, entries in block order) into that tile's rows.
+  int* hptr = nullptr;
+  float4* hrec = nullptr;
+  float* pbuf = nullptr;               // 16 floats per half-list entry
+  double* ydir = nullptr;              // per row: the block's own sums
+  int* gptr = nullptr;
+  int* glist = nullptr;
+  long long nsub_eval = 0;             // live (entry, sub-block) evaluated
   int f32_kind = DS_F32_MFMA;          // FP32 pair kernel: MFMA or VALU form
   int* tptr = nullptr;
   int* tidx = nullptr;

@@ -66,6 +66,12 @@ struct Matvec32Args {
   float tlast;
   int nblk, table_n;
   int lane_zero;          // 0; lane & lane_zero makes an index lane-varying
+  // symmetric MFMA kernel
+  const int* hptr;
+  float* pbuf;
+  double* ydir;
+  const int* gptr;
+  const int* glist;
 };
 
 // The per-tile data (list entry, frame offset, source records and vector)
@@ -155,27 +161,37 @@ __device__ __forceinline__ void butterfly_step(float* v, int upper) {
   }
 }
 
-template <int FORM, bool CG>
+// Symmetric form: block I runs its half list (tiles T >= 4 I).  Every
+// live (entry, sub-block s) 16 x 16 tile of values S_ij is evaluated once
+// and used twice: y_i += S_ij u_j for the block's rows (summed in
+// registers, written per row to ydir), and, when 4 I + s < T, the
+// transposed y_j += S_ij u_i for the tile's rows, whose 16 column sums per
+// entry go to pbuf.  k_sym_epi then adds each tile's incoming column sums
+// and applies the epilogue.
+template <bool CG>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
-    const int* __restrict__ tidx, const float4* __restrict__ toff,
-    Matvec32Args a) {
+    const float4* __restrict__ toff, Matvec32Args a) {
   extern __shared__ float2 tab32[];
   __shared__ float red[4 * 64];
-  const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
+  // the block's own vector values, then 16 zeros (the transposed factor
+  // of sub-blocks without a transposed product)
+  __shared__ __align__(16) float uown[64 + 16];
+  const int I = a.blk_order[blockIdx.x];
   const int sv = a.blk_sub[I];
   const ds_cg_scalars* sc = a.sc + sv;
   if (CG && sc->done) return;
-  {
-    for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
-      tab32[k] = a.table_g[k];
-  }
+  for (int k = threadIdx.x; k < a.table_n; k += blockDim.x)
+    tab32[k] = a.table_g[k];
+  if (threadIdx.x < 64) uown[threadIdx.x] = u32[I * DS_BS + threadIdx.x];
+  if (threadIdx.x < 16) uown[64 + threadIdx.x] = 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
   const int g = lane >> 4, c = lane & 15;
   const int row0 = I * DS_BS;
+  const int T0 = DS_TPB * I;    // the block's first tile
   // A operands (target component g, or 1) and accumulator inputs |l|^2
   float av[4];
   v4f ci[4];
@@ -197,17 +213,19 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
-  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
-  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
-  // One entry: its record o (frame offset; .w holds the bits of live mask
-  // | tile << 4), the tile's source record X and vector value uj.  (The
-  // MFMA results live across entries: a sub-block an entry skips keeps
-  // the last value, never read.)
+  const int kb = __builtin_amdgcn_readfirstlane(a.hptr[I]) + wave;
+  const int ke = __builtin_amdgcn_readfirstlane(a.hptr[I + 1]);
+  // One entry e: its record o (frame offset; .w holds the bits of live
+  // mask | tile << 4), the tile's source record X and vector value uj.
+  // (The MFMA results live across entries: a sub-block an entry skips
+  // keeps the last value, never read.)
   v4f d[4];
-  auto entry = [&](const float4& o, const float4& X, float uj)
+  auto entry = [&](int e, const float4& o, const float4& X, float uj)
       __attribute__((always_inline)) {
-    const unsigned live =
-        __builtin_amdgcn_readfirstlane(__float_as_int(o.w)) & 15u;
+    const unsigned bits = __builtin_amdgcn_readfirstlane(__float_as_int(o.w));
+    const unsigned live = bits & 15u;
+    const int dT = (int)(bits >> 4) - T0;   // >= 0 in a half list
+    const unsigned tr = live & (dT >= 4 ? 15u : (1u << dT) - 1u);
     // a target in the tile frame is l + o, so the source in the block
     // frame is Y = X - o; B = -2 Y_g (g < 3) or |Y|^2 (g = 3), formed
     // without branches
@@ -221,15 +239,33 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int s = 0; s < 4; ++s)
       if (live & (1u << s))
         d[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], b, ci[s], 0, 0, 0);
+    float at = 0.0f;   // transposed: sum over this lane's 4 rows
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       if (!(live & (1u << s))) continue;
+      const float4 uo = *reinterpret_cast<const float4*>(
+          &uown[(tr & (1u << s)) ? 16 * s + 4 * g : 64]);
+      const float uor[4] = {uo.x, uo.y, uo.z, uo.w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float xx = __builtin_amdgcn_fmed3f(d[s][r], 0.0f, tlast);
         const float2 f = tab32[(int)xx];
-        acc[s][r] = fmaf(fmaf(f.y, xx, f.x), uj, acc[s][r]);
+        const float sij = fmaf(f.y, xx, f.x);
+        acc[s][r] = fmaf(sij, uj, acc[s][r]);
+        at = fmaf(sij, uor[r], at);
       }
+    }
+    if (tr) {
+      // the column sums: add the 4 lane groups of each column c (lanes
+      // c + 16 g) with two half-swaps (32-lane halves, then 16-lane rows),
+      // each of which leaves the pairwise sum in both halves
+      const auto h32 = __builtin_amdgcn_permlane32_swap(
+          __float_as_uint(at), __float_as_uint(at), false, false);
+      at = __uint_as_float(h32[0]) + __uint_as_float(h32[1]);
+      const auto h16 = __builtin_amdgcn_permlane16_swap(
+          __float_as_uint(at), __float_as_uint(at), false, false);
+      at = __uint_as_float(h16[0]) + __uint_as_float(h16[1]);
+      if (g == 0) a.pbuf[16 * (size_t)e + c] = at;
     }
   };
   // Records are fetched two entries ahead, source records and vector
@@ -256,7 +292,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       const float4 X1 = (loc + j)[cu];
       const float u1 = (u32 + j)[cu];
       const float4 R2 = (toff + min(k + 2 * nw, kl))[lz];
-      entry(R0, X0, u0);
+      entry(k, R0, X0, u0);
       if (k + nw >= ke) break;
       // entry k + nw; prefetch the sources of k + 2 nw, the record of
       // k + 3 nw
@@ -264,7 +300,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       X0 = (loc + j)[cu];
       u0 = (u32 + j)[cu];
       const float4 R3 = (toff + min(k + 3 * nw, kl))[lz];
-      entry(R1, X1, u1);
+      entry(k + nw, R1, X1, u1);
       if (k + 2 * nw >= ke) break;
       R0 = R2;
       R1 = R3;
@@ -290,9 +326,37 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   red[wave * 64 + 16 * (c >> 2) + 4 * g + (c & 3)] = v[0];
   __syncthreads();
   if (wave != 0) return;
-  const int row = row0 + lane;
   double y = 0.0;
   for (int w = 0; w < nw; ++w) y += (double)red[w * 64 + lane];
+  a.ydir[row0 + lane] = y;
+}
+
+// Rows of block I: the block's own sums plus the column sums of every
+// entry (in block order) whose transposed product lands in the row's tile,
+// then the epilogue (identity or noise term; d . q partial for CG).
+template <int FORM, bool CG>
+__global__ __launch_bounds__(64) void k_sym_epi(Matvec32Args a) {
+  const int I = a.blk_order[blockIdx.x];
+  const int sv = a.blk_sub[I];
+  if (CG && a.sc[sv].done) return;
+  const int lane = threadIdx.x;
+  const int row = I * DS_BS + lane;
+  const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
+  double y = a.ydir[row];
+  const int gb = a.gptr[T], ge = a.gptr[T + 1];
+  // eight entries' loads in flight at a time, summed in list order
+  int k = gb;
+  for (; k + 8 <= ge; k += 8) {
+    int id[8];
+    float pv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) id[q] = a.glist[k + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pv[q] = a.pbuf[16 * (size_t)id[q] + c];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) y += (double)pv[q];
+  }
+  for (; k < ge; ++k) y += (double)a.pbuf[16 * (size_t)a.glist[k] + c];
   const double var_s = a.sub_cv[sv].x;
   const double ai = a.in[row];
   const double wi = a.pos[row].w;
@@ -324,6 +388,11 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.table = p->table32;
   a.table_g = p->table32g;
   a.out = out;
+  a.hptr = p->hptr;
+  a.pbuf = p->pbuf;
+  a.ydir = p->ydir;
+  a.gptr = p->gptr;
+  a.glist = p->glist;
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
   a.sub_blk0 = p->sub_blk0;
@@ -343,9 +412,12 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
 #define DS_L32(F, C, D, G) \
   hipLaunchKernelGGL((k_matvec32<F, C, D>), dim3(G), block, shm, st, a.loc, \
                      a.u32, a.tidx, a.toff, a)
-#define DS_L32M(F, C, G) \
-  hipLaunchKernelGGL((k_matvec32m<F, C>), dim3(G), block, shm, st, a.loc, \
-                     a.u32, a.tidx, a.toff, a)
+#define DS_L32M(F, C, G)                                                  \
+  do {                                                                    \
+    hipLaunchKernelGGL((k_matvec32m<C>), dim3(G), block, shm, st, a.loc,  \
+                       a.u32, p->hrec, a);                                \
+    hipLaunchKernelGGL((k_sym_epi<F, C>), dim3(G), dim3(64), 0, st, a);  \
+  } while (0)
   if (nf > 0) {
     a.blk_order = ord;
     if (p->f32_kind != DS_F32_VALU) {

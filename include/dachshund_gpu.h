@@ -96,8 +96,12 @@ typedef struct {
   long long tile_pairs;      /* (block, tile) pairs kept after the cutoff */
   long long pairs_per_matvec;/* evaluated pairs per matvec = tile_pairs*1024 */
   long long sub_pairs_per_matvec; /* the same with 16-row sub-block culling
-                                     (pairs the MFMA FP32 kernel evaluates) */
+                                     (pair products the MFMA FP32 kernel
+                                     delivers) */
   double setup_seconds;      /* host time of ordering, culling and upload */
+  long long sub_pairs_evaluated;  /* of those, evaluated by the MFMA FP32
+                                     kernel (each tile pair once, used for
+                                     both products) */
 } ds_problem_stats;
 
 /* ---- host-only planning (no device needed) ---------------------------- */
