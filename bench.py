@@ -651,13 +651,16 @@ def main():
             "traffic": pmc_traffic("k_matvec32m"),
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + "
                             "WRITE_SIZE, copy-calibrated)",
-            "limiter": "VALU issue (5 VALU ops per live pair plus ~1.6 per "
-                       "pair of per-entry work) serialized with the MFMA "
-                       "(no VALU/MFMA co-execution measured; PMC counters "
-                       "in profiles/)",
-            "kernel": "k_matvec32m (FP32 pair matvec, MFMA table arguments)",
+            "limiter": "VALU issue serialized with the MFMA (no VALU/MFMA "
+                       "co-execution measured) and, at half the work per "
+                       "block, per-block start-up (PMC counters in "
+                       "profiles/)",
+            "kernel": "k_matvec32m + k_sym_epi (symmetric FP32 operator: "
+                      "each tile pair evaluated once, used for both "
+                      "products; then the per-tile gather and epilogue)",
             "ms_per_launch": ms32, "flops_per_pair": FLOPS_PER_PAIR,
             "pairs_per_launch": pairs32,
+            "pairs_evaluated_per_launch": st.sub_pairs_evaluated,
         },
         "fp64_kernel": {"achieved": tf64, "peak": PEAK_TFLOPS["f64"],
                         "unit": "TFLOP/s", "frac": tf64 / PEAK_TFLOPS["f64"],

@@ -334,12 +334,15 @@ __global__ __launch_bounds__(256) void k_matvec32m(
 // Rows of block I: the block's own sums plus the column sums of every
 // entry (in block order) whose transposed product lands in the row's tile,
 // then the epilogue (identity or noise term; d . q partial for CG).
+// One wave per target block, four per workgroup.
 template <int FORM, bool CG>
-__global__ __launch_bounds__(64) void k_sym_epi(Matvec32Args a) {
-  const int I = a.blk_order[blockIdx.x];
+__global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= nb) return;
+  const int I = a.blk_order[slot];
   const int sv = a.blk_sub[I];
   if (CG && a.sc[sv].done) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int row = I * DS_BS + lane;
   const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
   double y = a.ydir[row];
@@ -365,8 +368,8 @@ __global__ __launch_bounds__(64) void k_sym_epi(Matvec32Args a) {
   if (lane >= a.blk_nreal[I]) bi = 0.0;
   a.out[row] = bi;
   if (CG) {
-    double v[1] = {wave_sum(ai * bi)};
-    store_partials<1>(v, a.partials, a.nblk, I);
+    const double v = wave_sum(ai * bi);
+    if (lane == 0) a.partials[I] = v;
   }
 }
 
@@ -416,7 +419,8 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   do {                                                                    \
     hipLaunchKernelGGL((k_matvec32m<C>), dim3(G), block, shm, st, a.loc,  \
                        a.u32, p->hrec, a);                                \
-    hipLaunchKernelGGL((k_sym_epi<F, C>), dim3(G), dim3(64), 0, st, a);  \
+    hipLaunchKernelGGL((k_sym_epi<F, C>), dim3(((G) + 3) / 4), dim3(256), \
+                       0, st, a, (G));                                     \
   } while (0)
   if (nf > 0) {
     a.blk_order = ord;
