@@ -54,18 +54,19 @@ def kernel_ms(wf, prec):
     return ts[2]
 
 
-def pmc_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the 300k-pixel problem, from the
-    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-    profiles/ (null if that summary is absent)."""
+def pmc_traffic(*kernels):
+    """HBM bytes per launch of the kernels together on the 300k-pixel
+    problem, from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/ (null if that summary lacks one of them)."""
     path = os.path.join(ROOT, "profiles", "pair_kernel_traffic_c5_300k.txt")
     if not os.path.exists(path):
         return None
+    found = {}
     for line in open(path):
         f = line.split()
-        if f and f[0] == kernel:
-            return float(f[1]) + float(f[2])
-    return None
+        if f and f[0] in kernels:
+            found[f[0]] = float(f[1]) + float(f[2])
+    return sum(found.values()) if len(found) == len(kernels) else None
 
 
 def cpu_baseline(pix_w, var_s, l_perp, l_para, n_full, n_mv_full,
@@ -648,7 +649,7 @@ def main():
         "roofline": {
             "bound": "mfma", "achieved": tf32, "peak": PEAK_TFLOPS["f32"],
             "unit": "TFLOP/s", "frac": tf32 / PEAK_TFLOPS["f32"],
-            "traffic": pmc_traffic("k_matvec32m"),
+            "traffic": pmc_traffic("k_matvec32m", "k_sym_epi"),
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + "
                             "WRITE_SIZE, copy-calibrated)",
             "limiter": "VALU issue serialized with the MFMA (no VALU/MFMA "
