@@ -349,7 +349,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->pbuf);
   dev_free(p->ydir);
   dev_free(p->gptr);
-  dev_free(p->glist);
+  dev_free(p->gslot);
   dev_free(p->sub_blk0);
   dev_free(p->sub_cv);
   dev_free(p->tcount);
@@ -752,14 +752,15 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     }
     hptr[B + 1] = (int)hrec.size();
   }
-  std::vector<int> gptr(p->ntile + 1, 0), glist(gsrc.size() / 2);
+  std::vector<int> gptr(p->ntile + 1, 0), gslot(hrec.size(), -1);
   for (int T = 0; T < p->ntile; ++T) gptr[T + 1] = gptr[T] + gcnt[T + 1];
   {
-    // entries in block order within each tile (gsrc is in block order)
+    // slots in block order within each tile (gsrc is in block order)
     std::vector<int> fill(gptr.begin(), gptr.end() - 1);
     for (size_t k = 0; k < gsrc.size(); k += 2)
-      glist[fill[gsrc[k + 1]]++] = gsrc[k];
+      gslot[gsrc[k]] = fill[gsrc[k + 1]]++;
   }
+  const size_t nslot = gsrc.size() / 2;
   // launch order: longest tile list first (for the autotuner)
   std::vector<int> order(nblk);
   std::iota(order.begin(), order.end(), 0);
@@ -830,10 +831,10 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->ord32, 2 * (size_t)nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->hptr, nblk + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->hrec, std::max<size_t>(hrec.size(), 1)));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * hrec.size(), 1)));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gslot, std::max<size_t>(gslot.size(), 1)));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * nslot, 1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->ydir, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, p->ntile + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->glist, std::max<size_t>(glist.size(), 1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
@@ -883,8 +884,8 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   if (!hrec.empty())
     HIP_OR_FAIL(ctx, up(p->hrec, hrec.data(), sizeof(float4) * hrec.size()));
   HIP_OR_FAIL(ctx, up(p->gptr, gptr.data(), sizeof(int) * (p->ntile + 1)));
-  if (!glist.empty())
-    HIP_OR_FAIL(ctx, up(p->glist, glist.data(), sizeof(int) * glist.size()));
+  if (!gslot.empty())
+    HIP_OR_FAIL(ctx, up(p->gslot, gslot.data(), sizeof(int) * gslot.size()));
   HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
                       sizeof(int) * (n_sub + 1)));
   HIP_OR_FAIL(ctx, up(p->sub_cv, hsubcv.data(), sizeof(double4) * n_sub));

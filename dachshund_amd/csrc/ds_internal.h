@@ -187,14 +187,15 @@ struct ds_problem {
   // evaluated once.  Block B keeps the entries of tiles T >= 4 B, live
   // bits restricted to its sub-blocks s with 4 B + s <= T (records as
   // toff32); sub-blocks with 4 B + s < T also give the transposed product,
-  // whose 16 column sums per entry go to pbuf and are gathered per tile
-  // (gptr / glist, entries in block order) into that tile's rows.
+  // whose 16 column sums per entry go to slot gslot[e] of pbuf.  The slots
+  // of a tile are consecutive (gptr[T] .. gptr[T + 1]), in block order, so
+  // the gather reads one contiguous run per tile.
   int* hptr = nullptr;
   float4* hrec = nullptr;
-  float* pbuf = nullptr;               // 16 floats per half-list entry
+  int* gslot = nullptr;                // per half-list entry (-1: none)
+  float* pbuf = nullptr;               // 16 floats per slot
   double* ydir = nullptr;              // per row: the block's own sums
   int* gptr = nullptr;
-  int* glist = nullptr;
   long long nsub_eval = 0;             // live (entry, sub-block) evaluated
   int f32_kind = DS_F32_MFMA;          // FP32 pair kernel: MFMA or VALU form
   int* tptr = nullptr;

@@ -71,7 +71,7 @@ struct Matvec32Args {
   float* pbuf;
   double* ydir;
   const int* gptr;
-  const int* glist;
+  const int* gslot;
 };
 
 // The per-tile data (list entry, frame offset, source records and vector)
@@ -215,12 +215,16 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int r = 0; r < 4; ++r) acc[s][r] = 0.0f;
   const int kb = __builtin_amdgcn_readfirstlane(a.hptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.hptr[I + 1]);
+  // (lz is 0 but not known to be, which keeps uniform-address loads on
+  // vector loads, waited for where they are used)
+  const unsigned cu = (unsigned)c;
+  const unsigned lz = (unsigned)(lane & a.lane_zero);
   // One entry e: its record o (frame offset; .w holds the bits of live
   // mask | tile << 4), the tile's source record X and vector value uj.
   // (The MFMA results live across entries: a sub-block an entry skips
   // keeps the last value, never read.)
   v4f d[4];
-  auto entry = [&](int e, const float4& o, const float4& X, float uj)
+  auto entry = [&](const float4& o, const float4& X, float uj, int slot)
       __attribute__((always_inline)) {
     const unsigned bits = __builtin_amdgcn_readfirstlane(__float_as_int(o.w));
     const unsigned live = bits & 15u;
@@ -265,24 +269,24 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       const auto h16 = __builtin_amdgcn_permlane16_swap(
           __float_as_uint(at), __float_as_uint(at), false, false);
       at = __uint_as_float(h16[0]) + __uint_as_float(h16[1]);
-      if (g == 0) a.pbuf[16 * (size_t)e + c] = at;
+      if (g == 0) a.pbuf[16 * (size_t)slot + c] = at;
     }
   };
   // Records are fetched two entries ahead, source records and vector
   // values one entry ahead, by vector loads from a wave-uniform base (held
   // in scalar registers) plus a fixed per-lane offset; each load is first
   // read after the next entry's arithmetic.  Indices past the list are
-  // clamped to its last entry (duplicate loads, never used).  lz is 0 but
-  // not known to be, which keeps the records on vector loads.
-  const unsigned cu = (unsigned)c;
-  const unsigned lz = (unsigned)(lane & a.lane_zero);
+  // clamped to its last entry (duplicate loads, never used).
   const int kl = ke - 1;
   auto tile_of = [&](const float4& R) __attribute__((always_inline)) {
     return (__builtin_amdgcn_readfirstlane(__float_as_int(R.w)) >> 4) * DS_TS;
   };
   if (kb < ke) {
+    // records and column-sum slots two entries ahead
     float4 R0 = (toff + kb)[lz];
+    int G0 = (a.gslot + kb)[lz];
     float4 R1 = (toff + min(kb + nw, kl))[lz];
+    int G1 = (a.gslot + min(kb + nw, kl))[lz];
     int j = tile_of(R0);
     float4 X0 = (loc + j)[cu];
     float u0 = (u32 + j)[cu];
@@ -292,7 +296,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       const float4 X1 = (loc + j)[cu];
       const float u1 = (u32 + j)[cu];
       const float4 R2 = (toff + min(k + 2 * nw, kl))[lz];
-      entry(k, R0, X0, u0);
+      const int G2 = (a.gslot + min(k + 2 * nw, kl))[lz];
+      entry(R0, X0, u0, G0);
       if (k + nw >= ke) break;
       // entry k + nw; prefetch the sources of k + 2 nw, the record of
       // k + 3 nw
@@ -300,10 +305,13 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       X0 = (loc + j)[cu];
       u0 = (u32 + j)[cu];
       const float4 R3 = (toff + min(k + 3 * nw, kl))[lz];
-      entry(k + nw, R1, X1, u1);
+      const int G3 = (a.gslot + min(k + 3 * nw, kl))[lz];
+      entry(R1, X1, u1, G1);
       if (k + 2 * nw >= ke) break;
       R0 = R2;
       R1 = R3;
+      G0 = G2;
+      G1 = G3;
     }
   }
   // Sum over the 16 source columns (lanes c) of each lane group: a
@@ -332,8 +340,9 @@ __global__ __launch_bounds__(256) void k_matvec32m(
 }
 
 // Rows of block I: the block's own sums plus the column sums of every
-// entry (in block order) whose transposed product lands in the row's tile,
-// then the epilogue (identity or noise term; d . q partial for CG).
+// entry (in block order: the tile's consecutive slots) whose transposed
+// product lands in the row's tile, then the epilogue (identity or noise
+// term; d . q partial for CG).
 // One wave per target block, four per workgroup.
 template <int FORM, bool CG>
 __global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
@@ -347,19 +356,17 @@ __global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
   const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
   double y = a.ydir[row];
   const int gb = a.gptr[T], ge = a.gptr[T + 1];
-  // eight entries' loads in flight at a time, summed in list order
+  // eight slots' loads in flight at a time, summed in slot order
+  const float* pb = a.pbuf + c;
   int k = gb;
   for (; k + 8 <= ge; k += 8) {
-    int id[8];
     float pv[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) id[q] = a.glist[k + q];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) pv[q] = a.pbuf[16 * (size_t)id[q] + c];
+    for (int q = 0; q < 8; ++q) pv[q] = pb[16 * (size_t)(k + q)];
 #pragma unroll
     for (int q = 0; q < 8; ++q) y += (double)pv[q];
   }
-  for (; k < ge; ++k) y += (double)a.pbuf[16 * (size_t)a.glist[k] + c];
+  for (; k < ge; ++k) y += (double)pb[16 * (size_t)k];
   const double var_s = a.sub_cv[sv].x;
   const double ai = a.in[row];
   const double wi = a.pos[row].w;
@@ -395,7 +402,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.pbuf = p->pbuf;
   a.ydir = p->ydir;
   a.gptr = p->gptr;
-  a.glist = p->glist;
+  a.gslot = p->gslot;
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
   a.sub_blk0 = p->sub_blk0;
