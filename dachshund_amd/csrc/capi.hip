@@ -349,6 +349,7 @@ void problem_free(ds_problem* p) {
   dev_free(p->pbuf);
   dev_free(p->ydir);
   dev_free(p->gptr);
+  dev_free(p->sub_sym);
   dev_free(p->gslot);
   dev_free(p->sub_blk0);
   dev_free(p->sub_cv);
@@ -835,6 +836,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * nslot, 1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->ydir, npad));
   HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, p->ntile + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_sym, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
   HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
@@ -884,6 +886,11 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   if (!hrec.empty())
     HIP_OR_FAIL(ctx, up(p->hrec, hrec.data(), sizeof(float4) * hrec.size()));
   HIP_OR_FAIL(ctx, up(p->gptr, gptr.data(), sizeof(int) * (p->ntile + 1)));
+  {
+    std::vector<int> sym(n_sub);
+    for (int k = 0; k < n_sub; ++k) sym[k] = max_src[k] > 16.0 ? 0 : 1;
+    HIP_OR_FAIL(ctx, up(p->sub_sym, sym.data(), sizeof(int) * n_sub));
+  }
   if (!gslot.empty())
     HIP_OR_FAIL(ctx, up(p->gslot, gslot.data(), sizeof(int) * gslot.size()));
   HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),

@@ -31,7 +31,7 @@ constexpr int kVecWaves = 4;   // target blocks per CG vector workgroup
 // sums of its target block.
 struct UpdArgs {
   const double* d;
-  const double* q;
+  double* q;       // written where the update kernel forms q (sym)
   const double* diag;
   const float* minv;      // block-Jacobi inverses, or nullptr (Jacobi)
   int minv_mod;
@@ -45,13 +45,30 @@ struct UpdArgs {
   const ds_cg_scalars* sc;
   int nblk;
   int only_x;
+  // q of the subvolumes whose rows the symmetric FP32 operator left to
+  // this kernel (sub_sym non-null): formed here from ydir and the slots
+  SymRows sym;
+  const int* sub_sym;
+  const double4* sub_cv;
 };
+
+// q_i of row `lane` of block I, subvolume sv (see UpdArgs::sym).
+__device__ __forceinline__ double update_q(const UpdArgs& a, int I, int sv,
+                                           int lane, int i) {
+  if (a.sub_sym && a.sub_sym[sv]) {
+    const double q = sym_row(a.sym, I, lane, a.sub_cv[sv].x, a.d[i]);
+    a.q[i] = q;
+    return q;
+  }
+  return a.q[i];
+}
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
   const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
   if (I >= a.nblk) return;
-  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
+  const int sv = a.blk_sub[I];
+  const ds_cg_scalars* sc = a.sc + sv;
   if (sc->done) return;
   const double alpha = sc->alpha;
   const int i = I * DS_BS + (threadIdx.x & 63);
@@ -59,7 +76,7 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
   a.x[i] = xi;
   if (a.wvec) a.xu[i] = a.wvec[i] * xi;
   if (a.only_x) return;
-  const double ri = a.r[i] - alpha * a.q[i];
+  const double ri = a.r[i] - alpha * update_q(a, I, sv, threadIdx.x & 63, i);
   a.r[i] = ri;
   const double si =
       a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS, ri)
@@ -166,7 +183,7 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
     a.x[i] = xi;
     if (a.wvec) a.xu[i] = a.wvec[i] * xi;
     if (a.only_x) continue;
-    const double ri = a.r[i] - alpha * a.q[i];
+    const double ri = a.r[i] - alpha * update_q(a, I, sv, lane, i);
     a.r[i] = ri;
     const double si = bj_apply(mt, ri);
     a.s[i] = si;
@@ -320,8 +337,10 @@ namespace {
     if (e_ != hipSuccess) return e_;     \
   } while (0)
 
+// sym: the FP32 MFMA operator produced q's sums only (its symmetric form
+// on the subvolumes flagged in sub_sym); the update kernel forms q there.
 hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
-                         hipStream_t st) {
+                         hipStream_t st, int sym = 0) {
   UpdArgs a{};
   a.d = p->d;
   a.q = p->q;
@@ -338,6 +357,12 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.sc = p->sc;
   a.nblk = p->nblk;
   a.only_x = only_x;
+  if (sym) {
+    a.sym = SymRows{p->ydir, p->pbuf, p->gptr, p->pos64, p->blk_nreal,
+                    p->form};
+    a.sub_sym = p->sub_sym;
+    a.sub_cv = p->sub_cv;
+  }
   const int nblk1 = p->nblk / p->n_sub;
   if (a.minv && p->shared_geometry && p->n_sub > 1 && p->minv_mod == nblk1) {
     hipLaunchKernelGGL(k_cg_update_shared, dim3(nblk1), dim3(64 * kVecWaves),
@@ -387,7 +412,9 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
     } else {
       DS_TRY(launch_matvec32_ex(p, p->du32, p->d, p->q, 1, st));
     }
-    DS_TRY(launch_update(p, 0, 1, drop, st));
+    const int sym = !p->use_fused && !p->use_dense &&
+                    p->f32_kind != DS_F32_VALU && p->n32_fast > 0;
+    DS_TRY(launch_update(p, 0, 1, drop, st, sym));
     CgEpilogue tr;
     tr.kind = EK_CG_RES;
     tr.out2 = p->s;

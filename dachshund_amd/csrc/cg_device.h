@@ -16,6 +16,42 @@
 
 namespace ds {
 
+// Rows of the symmetric FP32 operator (MFMA kernel): the block's own sums
+// plus the column sums in its tile's consecutive slots (slot order), then
+// the epilogue q = w (var_s y) + a (W form) or var_s y + N a (SN form), 0
+// on padding rows.  Row `lane` of target block I, `a` its input value.
+struct SymRows {
+  const double* ydir;
+  const float* pbuf;
+  const int* gptr;
+  const double4* pos;
+  const int* blk_nreal;
+  int form;
+};
+
+__device__ __forceinline__ double sym_row(const SymRows& s, int I, int lane,
+                                          double var_s, double a) {
+  const int row = I * DS_BS + lane;
+  const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
+  double y = s.ydir[row];
+  const int gb = s.gptr[T], ge = s.gptr[T + 1];
+  // eight slots' loads in flight at a time, summed in slot order
+  const float* pb = s.pbuf + c;
+  int k = gb;
+  for (; k + 8 <= ge; k += 8) {
+    float pv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pv[q] = pb[16 * (size_t)(k + q)];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) y += (double)pv[q];
+  }
+  for (; k < ge; ++k) y += (double)pb[16 * (size_t)k];
+  const double wn = s.pos[row].w;
+  double q = s.form == DS_FORM_W ? wn * (var_s * y) + a : var_s * y + wn * a;
+  if (lane >= s.blk_nreal[I]) q = 0.0;
+  return q;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -196,6 +196,8 @@ struct ds_problem {
   float* pbuf = nullptr;               // 16 floats per slot
   double* ydir = nullptr;              // per row: the block's own sums
   int* gptr = nullptr;
+  int* sub_sym = nullptr;              // per subvolume: blocks run the
+                                       // symmetric kernel (no far points)
   long long nsub_eval = 0;             // live (entry, sub-block) evaluated
   int f32_kind = DS_F32_MFMA;          // FP32 pair kernel: MFMA or VALU form
   int* tptr = nullptr;
@@ -277,6 +279,10 @@ struct CgEpilogue {
 hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
                               const double* u, double* out,
                               const CgEpilogue& e, hipStream_t st);
+// FP32 A in.  With cg, in = the CG direction and the d . q partials are
+// stored for launch_finalize; the symmetric MFMA kernel then leaves `out`
+// of its subvolumes (sub_sym) to the update kernel, which forms each row
+// from ydir and the column-sum slots (launch_update with sym).
 hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                               const double* in, double* out, int cg,
                               hipStream_t st);

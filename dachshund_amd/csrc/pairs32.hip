@@ -166,14 +166,19 @@ __device__ __forceinline__ void butterfly_step(float* v, int upper) {
 // and used twice: y_i += S_ij u_j for the block's rows (summed in
 // registers, written per row to ydir), and, when 4 I + s < T, the
 // transposed y_j += S_ij u_i for the tile's rows, whose 16 column sums per
-// entry go to pbuf.  k_sym_epi then adds each tile's incoming column sums
-// and applies the epilogue.
-template <bool CG>
+// entry go to pbuf.  For A x, k_sym_epi then adds each tile's incoming
+// column sums and applies the epilogue; in a CG iteration the update kernel
+// does that per row, and this kernel stores the d . q partial of its block,
+// var_s (sum_i u_i y_i + sum over its transposed entries of u_j times the
+// column sums) + sum_i a_i^2 (W form; SN: + N_i a_i^2), with u_i = w_i a_i
+// (W) or a_i (SN) and u_j the sources' FP32 values.
+template <int FORM, bool CG>
 __global__ __launch_bounds__(256) void k_matvec32m(
     const float4* __restrict__ loc, const float* __restrict__ u32,
     const float4* __restrict__ toff, Matvec32Args a) {
   extern __shared__ float2 tab32[];
   __shared__ float red[4 * 64];
+  __shared__ double dq_red[4];
   // the block's own vector values, then 16 zeros (the transposed factor
   // of sub-blocks without a transposed product)
   __shared__ __align__(16) float uown[64 + 16];
@@ -224,6 +229,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   // (The MFMA results live across entries: a sub-block an entry skips
   // keeps the last value, never read.)
   v4f d[4];
+  double dqt = 0.0;   // CG: sum of u_j times this lane's column sums
   auto entry = [&](const float4& o, const float4& X, float uj, int slot)
       __attribute__((always_inline)) {
     const unsigned bits = __builtin_amdgcn_readfirstlane(__float_as_int(o.w));
@@ -269,7 +275,10 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       const auto h16 = __builtin_amdgcn_permlane16_swap(
           __float_as_uint(at), __float_as_uint(at), false, false);
       at = __uint_as_float(h16[0]) + __uint_as_float(h16[1]);
-      if (g == 0) a.pbuf[16 * (size_t)slot + c] = at;
+      if (g == 0) {
+        a.pbuf[16 * (size_t)slot + c] = at;
+        if (CG) dqt = fma((double)at, (double)uj, dqt);
+      }
     }
   };
   // Records are fetched two entries ahead, source records and vector
@@ -332,52 +341,43 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   butterfly_step<2, 0x4E>(v, c & 2);
   butterfly_step<1, 0xB1>(v, c & 1);
   red[wave * 64 + 16 * (c >> 2) + 4 * g + (c & 3)] = v[0];
+  if (CG) {
+    const double t = wave_sum(dqt);
+    if (lane == 0) dq_red[wave] = t;
+  }
   __syncthreads();
   if (wave != 0) return;
   double y = 0.0;
   for (int w = 0; w < nw; ++w) y += (double)red[w * 64 + lane];
-  a.ydir[row0 + lane] = y;
+  const int row = row0 + lane;
+  a.ydir[row] = y;
+  if (CG) {
+    const double var_s = a.sub_cv[sv].x;
+    const double ai = a.in[row];
+    const double wn = a.pos[row].w;
+    const double ui = FORM == DS_FORM_W ? wn * ai : ai;
+    const double ex = FORM == DS_FORM_W ? ai * ai : wn * ai * ai;
+    const bool real = lane < a.blk_nreal[I];
+    double tr = 0.0;
+    for (int w = 0; w < nw; ++w) tr += dq_red[w];
+    double v[1] = {wave_sum(real ? var_s * (ui * y) + ex : 0.0) +
+                   var_s * tr};
+    store_partials<1>(v, a.partials, a.nblk, I);
+  }
 }
 
-// Rows of block I: the block's own sums plus the column sums of every
-// entry (in block order: the tile's consecutive slots) whose transposed
-// product lands in the row's tile, then the epilogue (identity or noise
-// term; d . q partial for CG).
-// One wave per target block, four per workgroup.
-template <int FORM, bool CG>
+// A x: rows of block I (sym_row: the block's own sums plus its tiles'
+// column sums, then the epilogue), one wave per target block, four per
+// workgroup.
+template <int FORM>
 __global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
   const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (slot >= nb) return;
   const int I = a.blk_order[slot];
-  const int sv = a.blk_sub[I];
-  if (CG && a.sc[sv].done) return;
   const int lane = threadIdx.x & 63;
   const int row = I * DS_BS + lane;
-  const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
-  double y = a.ydir[row];
-  const int gb = a.gptr[T], ge = a.gptr[T + 1];
-  // eight slots' loads in flight at a time, summed in slot order
-  const float* pb = a.pbuf + c;
-  int k = gb;
-  for (; k + 8 <= ge; k += 8) {
-    float pv[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) pv[q] = pb[16 * (size_t)(k + q)];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) y += (double)pv[q];
-  }
-  for (; k < ge; ++k) y += (double)pb[16 * (size_t)k];
-  const double var_s = a.sub_cv[sv].x;
-  const double ai = a.in[row];
-  const double wi = a.pos[row].w;
-  double bi = (FORM == DS_FORM_W) ? wi * (var_s * y) + ai
-                                  : var_s * y + wi * ai;
-  if (lane >= a.blk_nreal[I]) bi = 0.0;
-  a.out[row] = bi;
-  if (CG) {
-    const double v = wave_sum(ai * bi);
-    if (lane == 0) a.partials[I] = v;
-  }
+  const SymRows rows{a.ydir, a.pbuf, a.gptr, a.pos, a.blk_nreal, FORM};
+  a.out[row] = sym_row(rows, I, lane, a.sub_cv[a.blk_sub[I]].x, a.in[row]);
 }
 
 }  // namespace
@@ -424,10 +424,11 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                      a.u32, a.tidx, a.toff, a)
 #define DS_L32M(F, C, G)                                                  \
   do {                                                                    \
-    hipLaunchKernelGGL((k_matvec32m<C>), dim3(G), block, shm, st, a.loc,  \
-                       a.u32, p->hrec, a);                                \
-    hipLaunchKernelGGL((k_sym_epi<F, C>), dim3(((G) + 3) / 4), dim3(256), \
-                       0, st, a, (G));                                     \
+    hipLaunchKernelGGL((k_matvec32m<F, C>), dim3(G), block, shm, st,      \
+                       a.loc, a.u32, p->hrec, a);                         \
+    if (!(C))                                                             \
+      hipLaunchKernelGGL((k_sym_epi<F>), dim3(((G) + 3) / 4), dim3(256),  \
+                         0, st, a, (G));                                  \
   } while (0)
   if (nf > 0) {
     a.blk_order = ord;
