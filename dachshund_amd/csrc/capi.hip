@@ -762,6 +762,10 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       gslot[gsrc[k]] = fill[gsrc[k + 1]]++;
   }
   const size_t nslot = gsrc.size() / 2;
+  // the CG update kernel addresses the slots by 32-bit byte offsets
+  if (nslot > (size_t)UINT32_MAX / (16 * sizeof(float)))
+    return fail(ctx, DS_E_ARG, "%zu column-sum slots exceed the 32-bit "
+                "offsets of the update kernel", nslot);
   // launch order: longest tile list first (for the autotuner)
   std::vector<int> order(nblk);
   std::iota(order.begin(), order.end(), 0);
@@ -1288,7 +1292,7 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
     // subvolumes sharing one geometry share one set of block inverses
     p->minv_mod = p->shared_geometry ? p->nblk / p->n_sub : p->nblk;
     HIP_OR_FAIL(p->ctx, dev_alloc(&p->minv,
-                                  (size_t)p->minv_mod * DS_BS * DS_BS));
+                                  (size_t)p->minv_mod * DS_BJ_SIZE));
     HIP_OR_FAIL(p->ctx, ds::bj_build(p, st));
     HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
   }

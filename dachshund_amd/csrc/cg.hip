@@ -31,7 +31,7 @@ constexpr int kVecWaves = 4;   // target blocks per CG vector workgroup
 // sums of its target block.
 struct UpdArgs {
   const double* d;
-  double* q;       // written where the update kernel forms q (sym)
+  const double* q;
   const double* diag;
   const float* minv;      // block-Jacobi inverses, or nullptr (Jacobi)
   int minv_mod;
@@ -55,32 +55,85 @@ struct UpdArgs {
 // q_i of row `lane` of block I, subvolume sv (see UpdArgs::sym).
 __device__ __forceinline__ double update_q(const UpdArgs& a, int I, int sv,
                                            int lane, int i) {
-  if (a.sub_sym && a.sub_sym[sv]) {
-    const double q = sym_row(a.sym, I, lane, a.sub_cv[sv].x, a.d[i]);
-    a.q[i] = q;
-    return q;
-  }
+  if (a.sub_sym && a.sub_sym[sv])
+    return sym_row(a.sym, I, lane, a.sub_cv[sv].x, a.d[i]);
   return a.q[i];
 }
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
-__global__ __launch_bounds__(64 * kVecWaves) void k_cg_update(UpdArgs a) {
-  const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
+// BJ: block-Jacobi (a.minv set, a.only_x clear), else Jacobi or x only;
+// W: a.wvec set; SYM: a.sub_sym set (q of the flagged subvolumes formed
+// here, as sym_row does, from the block's own sums and the tile's
+// column-sum slots, in slot order).  The kernel is bound by the latency of
+// its dependent loads (one wave per block, a few waves per SIMD), so every
+// load that depends only on the row is issued first, the block's inverse
+// in one batch, and the register budget keeps every block resident.
+template <bool BJ, bool W, bool SYM>
+__global__ __launch_bounds__(64 * kVecWaves)
+__attribute__((amdgpu_waves_per_eu(BJ ? 5 : 1))) void k_cg_update(
+    UpdArgs a) {
+  // wave-uniform: the per-block scalars come by scalar loads
+  const int I = __builtin_amdgcn_readfirstlane(blockIdx.x * kVecWaves +
+                                               (threadIdx.x >> 6));
   if (I >= a.nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int i = I * DS_BS + lane;
+  // The row's loads go out before the subvolume's state is known (its
+  // scalars are a chain of dependent loads).  No branch on that state: a
+  // finished subvolume's rows store x, xu and r back unchanged (its s and
+  // partials are read by nobody).
+  int gb = 0, ge = 0;
+  double ys = 0.0;
+  if (SYM) {
+    const int T = DS_TPB * I + (lane >> 4);
+    gb = a.sym.gptr[T];
+    ge = a.sym.gptr[T + 1];
+    ys = a.sym.ydir[i];
+  }
+  const double di = a.d[i], x0 = a.x[i];
+  const double wi = W ? a.wvec[i] : 0.0;
+  const double qd = a.q[i];
+  const double r0 = a.r[i];
+  // keep these loads ahead of the inverse's (loads complete in order)
+  __builtin_amdgcn_sched_barrier(0);
+  float m[DS_BJ_ROWS];
+  if (BJ)
+    load_bj_diagonals(a.minv + (size_t)(I % a.minv_mod) * DS_BJ_SIZE, m);
   const int sv = a.blk_sub[I];
   const ds_cg_scalars* sc = a.sc + sv;
-  if (sc->done) return;
+  const int symq = SYM ? a.sub_sym[sv] : 0;
+  const double var_s = SYM ? a.sub_cv[sv].x : 0.0;
+  const int nreal = SYM ? a.sym.blk_nreal[I] : 0;
+  const bool live = !sc->done;
   const double alpha = sc->alpha;
-  const int i = I * DS_BS + (threadIdx.x & 63);
-  const double xi = a.x[i] + alpha * a.d[i];
-  a.x[i] = xi;
-  if (a.wvec) a.xu[i] = a.wvec[i] * xi;
-  if (a.only_x) return;
-  const double ri = a.r[i] - alpha * update_q(a, I, sv, threadIdx.x & 63, i);
-  a.r[i] = ri;
-  const double si =
-      a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS, ri)
-             : ri / a.diag[i];
+  const double xi = x0 + alpha * di;
+  a.x[i] = live ? xi : x0;
+  if (W) a.xu[i] = wi * (live ? xi : x0);
+  if (!BJ && a.only_x) return;
+  double qi = qd;
+  if (SYM) {
+    double y = ys;
+    const unsigned c = lane & 15;
+    const char* pb = reinterpret_cast<const char*>(a.sym.pbuf);
+    for (int k = gb; k < ge; k += 16) {
+      float pv[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const unsigned off = 4u * (16u * (unsigned)min(k + q, ge - 1) + c);
+        pv[q] = *reinterpret_cast<const float*>(pb + off);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (k + q < ge) y += (double)pv[q];
+    }
+    const double wn = W ? wi : a.sym.pos[i].w;
+    double q = W ? wn * (var_s * y) + di : var_s * y + wn * di;
+    if (lane >= nreal) q = 0.0;
+    if (symq) qi = q;
+  }
+  const double ri = r0 - alpha * qi;
+  a.r[i] = live ? ri : r0;
+  const double si = BJ ? bj_apply_regs(m, ri) : ri / a.diag[i];
   a.s[i] = si;
   double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
   if ((threadIdx.x & 63) == 0) {
@@ -116,7 +169,7 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_res_precond(ResPArgs a) {
   const bool real = lane < a.blk_nreal[I];
   const double bi = a.r[i];
   const double si =
-      a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS, bi)
+      a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BJ_SIZE, bi)
              : (real ? bi / a.diag[i] : 0.0);
   a.s[i] = si;
   const double v0 = wave_sum(bi * si), v1 = wave_sum(bi * bi);
@@ -166,10 +219,10 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
 // k_cg_update, so the results are the same bit for bit.
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
     UpdArgs a, int nblk1, int n_sub) {
-  __shared__ float mt[DS_BS * DS_BS];
+  __shared__ float mt[DS_BJ_SIZE];
   const int J = blockIdx.x;
-  const float* src = a.minv + (size_t)J * DS_BS * DS_BS;
-  for (int k = threadIdx.x; k < DS_BS * DS_BS; k += blockDim.x)
+  const float* src = a.minv + (size_t)J * DS_BJ_SIZE;
+  for (int k = threadIdx.x; k < DS_BJ_SIZE; k += blockDim.x)
     mt[k] = src[k];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -358,8 +411,8 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   a.nblk = p->nblk;
   a.only_x = only_x;
   if (sym) {
-    a.sym = SymRows{p->ydir, p->pbuf, p->gptr, p->pos64, p->blk_nreal,
-                    p->form};
+    a.sym = SymRows{p->ydir, p->pbuf, p->gptr, p->pos64, p->wvec,
+                    p->blk_nreal, p->form};
     a.sub_sym = p->sub_sym;
     a.sub_cv = p->sub_cv;
   }
@@ -369,7 +422,17 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
                        0, st, a, nblk1, p->n_sub);
   } else {
     const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
-    hipLaunchKernelGGL(k_cg_update, g, b, 0, st, a);
+    const bool bj = a.minv && !only_x, w = a.wvec, sy = a.sub_sym;
+#define DS_UPD(B, W_, S) \
+  hipLaunchKernelGGL((k_cg_update<B, W_, S>), g, b, 0, st, a)
+    if (bj) {
+      if (w) { if (sy) DS_UPD(true, true, true); else DS_UPD(true, true, false); }
+      else { if (sy) DS_UPD(true, false, true); else DS_UPD(true, false, false); }
+    } else {
+      if (w) { if (sy) DS_UPD(false, true, true); else DS_UPD(false, true, false); }
+      else { if (sy) DS_UPD(false, false, true); else DS_UPD(false, false, false); }
+    }
+#undef DS_UPD
   }
   DS_TRY(hipGetLastError());
   if (only_x) return hipSuccess;

@@ -24,7 +24,8 @@ struct SymRows {
   const double* ydir;
   const float* pbuf;
   const int* gptr;
-  const double4* pos;
+  const double4* pos;    // SN form: N = pos.w
+  const double* wvec;    // W form: w
   const int* blk_nreal;
   int form;
 };
@@ -35,18 +36,19 @@ __device__ __forceinline__ double sym_row(const SymRows& s, int I, int lane,
   const int T = DS_TPB * I + (lane >> 4), c = lane & 15;
   double y = s.ydir[row];
   const int gb = s.gptr[T], ge = s.gptr[T + 1];
-  // eight slots' loads in flight at a time, summed in slot order
+  // sixteen slots' loads in flight at a time (indices clamped to the last
+  // slot, so every load is in bounds), summed in slot order
   const float* pb = s.pbuf + c;
-  int k = gb;
-  for (; k + 8 <= ge; k += 8) {
-    float pv[8];
+  for (int k = gb; k < ge; k += 16) {
+    float pv[16];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) pv[q] = pb[16 * (size_t)(k + q)];
+    for (int q = 0; q < 16; ++q)
+      pv[q] = pb[16 * (size_t)min(k + q, ge - 1)];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) y += (double)pv[q];
+    for (int q = 0; q < 16; ++q)
+      if (k + q < ge) y += (double)pv[q];
   }
-  for (; k < ge; ++k) y += (double)pb[16 * (size_t)k];
-  const double wn = s.pos[row].w;
+  const double wn = s.form == DS_FORM_W ? s.wvec[row] : s.pos[row].w;
   double q = s.form == DS_FORM_W ? wn * (var_s * y) + a : var_s * y + wn * a;
   if (lane >= s.blk_nreal[I]) q = 0.0;
   return q;
@@ -125,19 +127,45 @@ __device__ __forceinline__ void gather_partials(double (&v)[NV],
 }
 
 // Block-Jacobi application for one 64-row target block, one wave: lane i
-// returns sum_k M_ik r_k, with r_k (lane k's value) broadcast by readlane
-// and M stored as mt[k * 64 + i] (coalesced per k).  All 64 lanes must be
-// active.
+// returns sum_k M_{i,j} r_j, j = (i + k) mod 64, k = 0..63 in order.  The
+// symmetric M is stored by diagonals, DS_BJ_ROWS rows of 64 floats:
+// mt[k * 64 + i] = M_{i,(i+k) mod 64} for k = 0..32; for k > 32 the element
+// is M_{j,i} = mt[(64 - k) * 64 + j].  Every load is one 256-byte row (the
+// rows k > 32 permuted within it, and already read for 64 - k); r_j comes
+// from lane j by a shuffle.  All 64 lanes must be active.
 __device__ __forceinline__ double bj_apply(const float* __restrict__ mt,
                                            double r) {
   const int lane = threadIdx.x & 63;
-  const int lo = __double2loint(r), hi = __double2hiint(r);
   double s = 0.0;
 #pragma unroll 8
-  for (int k = 0; k < 64; ++k) {
-    const double rk = __hiloint2double(__builtin_amdgcn_readlane(hi, k),
-                                       __builtin_amdgcn_readlane(lo, k));
-    s = fma((double)mt[k * 64 + lane], rk, s);
+  for (int k = 0; k < DS_BS; ++k) {
+    const int j = (lane + k) & (DS_BS - 1);
+    const float m = k <= DS_BS / 2 ? mt[k * DS_BS + lane]
+                                   : mt[(DS_BS - k) * DS_BS + j];
+    s = fma((double)m, __shfl(r, j, 64), s);
+  }
+  return s;
+}
+
+// bj_apply with the lane's 33 stored diagonal values in registers (loaded
+// by load_bj_diagonals; the elements of rows k > 32 come from lane j by a
+// shuffle): the same values in the same order, so the same result.
+__device__ __forceinline__ void load_bj_diagonals(
+    const float* __restrict__ mt, float (&m)[DS_BJ_ROWS]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < DS_BJ_ROWS; ++k) m[k] = mt[k * DS_BS + lane];
+}
+
+__device__ __forceinline__ double bj_apply_regs(const float (&m)[DS_BJ_ROWS],
+                                                double r) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < DS_BS; ++k) {
+    const int j = (lane + k) & (DS_BS - 1);
+    const float mk = k <= DS_BS / 2 ? m[k] : __shfl(m[DS_BS - k], j, 64);
+    s = fma((double)mk, __shfl(r, j, 64), s);
   }
   return s;
 }

@@ -31,6 +31,10 @@
 #define DS_BS 64          // target rows per block (one wavefront)
 #define DS_TS 16          // source slots per tile
 #define DS_TPB (DS_BS / DS_TS)
+// block-Jacobi inverse of one block, stored by diagonals (symmetric):
+// DS_BJ_ROWS rows of DS_BS floats
+#define DS_BJ_ROWS (DS_BS / 2 + 1)
+#define DS_BJ_SIZE (DS_BJ_ROWS * DS_BS)
 #define DS_MAX_TABLE 1024
 #define DS_MAP_T 4        // map tiles are 4x4x4 cells
 #define DS_DENSE_R 32     // right-hand sides of the explicit BF16 block path

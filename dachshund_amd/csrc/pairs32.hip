@@ -48,6 +48,7 @@ struct Matvec32Args {
   const float4* tgt;      // targets: block-frame coordinates
   const float* u32;       // scaled input (w .* a, or a) in FP32
   const double4* pos;     // w (W form) or N (SN form) in .w
+  const double* wvec;     // w (W form)
   const double* in;       // FP64 input, own rows
   const int* tptr;
   const int* tidx;
@@ -376,7 +377,8 @@ __global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
   const int I = a.blk_order[slot];
   const int lane = threadIdx.x & 63;
   const int row = I * DS_BS + lane;
-  const SymRows rows{a.ydir, a.pbuf, a.gptr, a.pos, a.blk_nreal, FORM};
+  const SymRows rows{a.ydir, a.pbuf, a.gptr, a.pos, a.wvec, a.blk_nreal,
+                     FORM};
   a.out[row] = sym_row(rows, I, lane, a.sub_cv[a.blk_sub[I]].x, a.in[row]);
 }
 
@@ -390,6 +392,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.tgt = p->tgt32;
   a.u32 = u32;
   a.pos = p->pos64;
+  a.wvec = p->wvec;
   a.in = in;
   a.tptr = p->tptr;
   a.tidx = p->tidx;

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
       v[0] = ai * bi;  // d . q
     } else if (EPI == EPI_CG_RES) {
       const double si =
-          a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BS * DS_BS,
+          a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BJ_SIZE,
                             bi)
                  : (real ? bi / a.diag[row] : 0.0);
       a.out2[row] = si;

@@ -16,7 +16,8 @@
 // delta_ij (SN form) with the FP64 table interpolant of
 // lib/signal_covar.h:53-77 (padding rows: identity), then the block is
 // inverted in LDS by in-place Gauss-Jordan elimination (no pivoting: the
-// block is SPD) and stored as minv[I][k][i] = M_ik.
+// block is SPD), symmetrised and stored by diagonals (bj_apply's layout):
+// minv[I][k][i] = M_{i,(i+k) mod 64}, k = 0..32.
 // COMPILE WITH -ffp-contract=off.
 #include "cg_device.h"
 #include "ds_internal.h"
@@ -76,9 +77,11 @@ __global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
     }
     __syncthreads();
   }
-  float* out = a.minv + (size_t)I * DS_BS * DS_BS;
-  for (int k = 0; k < DS_BS; ++k)
-    out[k * DS_BS + i] = (float)(0.5 * (m[i][k] + m[k][i]));
+  float* out = a.minv + (size_t)I * DS_BJ_SIZE;
+  for (int k = 0; k < DS_BJ_ROWS; ++k) {
+    const int j = (i + k) & (DS_BS - 1);
+    out[k * DS_BS + i] = (float)(0.5 * (m[i][j] + m[j][i]));
+  }
 }
 
 // x = M^-1 b for every target block (one wave per block).
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void k_bj_apply(const double* b,
   const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (I >= nblk) return;
   const int i = I * DS_BS + (threadIdx.x & 63);
-  x[i] = bj_apply(minv + (size_t)(I % mod) * DS_BS * DS_BS, b[i]);
+  x[i] = bj_apply(minv + (size_t)(I % mod) * DS_BJ_SIZE, b[i]);
 }
 
 }  // namespace
