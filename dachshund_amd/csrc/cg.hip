@@ -266,18 +266,24 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
-// lib/pcg.h:61-64: d = s + beta d (and its scaled copies).
+// lib/pcg.h:61-64: d = s + beta d (and its scaled copies).  As in
+// k_cg_update, the row's loads go out before the subvolume's scalars and
+// a finished subvolume's rows store their values back unchanged.
+template <bool W>
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_dir(DirArgs a) {
-  const int I = blockIdx.x * kVecWaves + (threadIdx.x >> 6);
+  const int I = __builtin_amdgcn_readfirstlane(blockIdx.x * kVecWaves +
+                                               (threadIdx.x >> 6));
   if (I >= a.nblk) return;
-  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
-  if (sc->done) return;
-  const double beta = sc->beta;
   const int i = I * DS_BS + (threadIdx.x & 63);
-  const double di = a.s[i] + beta * a.d[i];
+  const double si = a.s[i], d0 = a.d[i];
+  const double wi = W ? a.wvec[i] : 1.0;
+  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
+  const bool live = !sc->done;
+  const double beta = sc->beta;
+  const double di = live ? si + beta * d0 : d0;
   a.d[i] = di;
-  const double ui = a.wvec ? a.wvec[i] * di : di;
-  if (a.wvec) a.du[i] = ui;
+  const double ui = W ? wi * di : di;
+  if (W) a.du[i] = ui;
   if (a.du32) a.du32[i] = (float)ui;
   if (a.du16) a.du16[i] = to_bf16((float)ui);
 }
@@ -455,7 +461,8 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   a.sc = p->sc;
   a.nblk = p->nblk;
   const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
-  hipLaunchKernelGGL(k_cg_dir, g, b, 0, st, a);
+  if (a.wvec) hipLaunchKernelGGL(k_cg_dir<true>, g, b, 0, st, a);
+  else hipLaunchKernelGGL(k_cg_dir<false>, g, b, 0, st, a);
   return hipGetLastError();
 }
 

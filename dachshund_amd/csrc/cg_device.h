@@ -120,7 +120,16 @@ __device__ __forceinline__ void gather_partials(double (&v)[NV],
 #pragma unroll
       for (int u = 0; u < 8; ++u) s += t[u];
     }
-    for (; b < count; b += step) s += pk[b];
+    // the rest (fewer than 8 per thread) as one batch of loads, indices
+    // clamped in bounds, summed in the same order
+    if (b < count) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = pk[min(b + u * step, count - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (b + u * step < count) s += t[u];
+    }
     v[k] = s;
   }
   block_sum<NV>(v, red);
