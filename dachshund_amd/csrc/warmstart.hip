@@ -272,21 +272,45 @@ __global__ __launch_bounds__(64) void k_ws_features(KnnArgs a) {
     auto datum = [&](int j, double wN) {
       return a.form == DS_FORM_W ? a.rhs[j] / wN : a.rhs[j];
     };
-    const int* nb = a.nbr + (size_t)row * K;
-#pragma unroll 4
-    for (int k = 0; k < K; ++k) {
-      float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-      const int j = nb[k];
-      if (j >= 0) {
-        const double4 pj = a.pos[j];
-        v[0] = (float)(pj.x - pi.x) * a.inv_lp;
-        v[1] = (float)(pj.y - pi.y) * a.inv_lp;
-        v[2] = (float)(pj.z - pi.z) * a.inv_ll;
-        v[3] = (float)noise(pj.w) * a.inv_sqrt_vs;
-        v[4] = (float)datum(j, pj.w) * a.inv_sqrt_vs;
+    // the row's K neighbour indices in 16-byte loads, then their records
+    // gathered eight neighbours at a time (absent neighbours read the row
+    // itself and give zeros)
+    static_assert(K % 8 == 0, "neighbour batches of 8");
+    const int4* nb4 = reinterpret_cast<const int4*>(a.nbr + (size_t)row * K);
+    int nb[K];
+#pragma unroll
+    for (int q = 0; q < K / 4; ++q) {
+      const int4 t = nb4[q];
+      nb[4 * q] = t.x;
+      nb[4 * q + 1] = t.y;
+      nb[4 * q + 2] = t.z;
+      nb[4 * q + 3] = t.w;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      double4 pj[8];
+      double dj[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = nb[k0 + u] >= 0 ? nb[k0 + u] : row;
+        pj[u] = a.pos[j];
+        dj[u] = a.rhs[j];
       }
 #pragma unroll
-      for (int c = 0; c < 5; ++c) f[5 * k + c] = f2bf(v[c]);
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        if (nb[k] >= 0) {
+          v[0] = (float)(pj[u].x - pi.x) * a.inv_lp;
+          v[1] = (float)(pj[u].y - pi.y) * a.inv_lp;
+          v[2] = (float)(pj[u].z - pi.z) * a.inv_ll;
+          v[3] = (float)noise(pj[u].w) * a.inv_sqrt_vs;
+          v[4] = (float)(a.form == DS_FORM_W ? dj[u] / pj[u].w : dj[u]) *
+                 a.inv_sqrt_vs;
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) f[5 * k + c] = f2bf(v[c]);
+      }
     }
     f[5 * K] = f2bf((float)noise(pi.w) * a.inv_sqrt_vs);
     f[5 * K + 1] = f2bf((float)datum(row, pi.w) * a.inv_sqrt_vs);
@@ -478,23 +502,41 @@ __device__ __forceinline__ void mfma_layer(const unsigned short* act,
                                            int kp, v16f_ws (&acc)[2][2]) {
   const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
   const size_t plane = (size_t)H * kp;
-#pragma unroll 2
-  for (int ks = 0; ks < NKS; ++ks) {
-    const v4s_ws a0 = ld4(act + r * pitch + 8 * ks + 4 * h);
-    const v4s_ws a1 = ld4(act + (32 + r) * pitch + 8 * ks + 4 * h);
+  // The weights (the same for every wave, cache-resident) arrive in
+  // batches of up to 8 k-steps, every load of a batch issued before its
+  // first MFMA; the MFMAs run in k-step order as before.
+  constexpr int CH = NKS <= 8 ? NKS : (NKS + 2) / 3;
 #pragma unroll
-    for (int part = 0; part < 3; ++part)
+  for (int c0 = 0; c0 < NKS; c0 += CH) {
+    v4s_ws b[CH][3][2];
 #pragma unroll
-      for (int nh = 0; nh < 2; ++nh) {
-        const v4s_ws b =
-            ld4(w + part * plane + (size_t)(32 * nh + r) * kp + 8 * ks + 4 * h);
-        acc[0][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, b,
-                                                             acc[0][nh], 0, 0,
-                                                             0);
-        acc[1][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b,
-                                                             acc[1][nh], 0, 0,
-                                                             0);
-      }
+    for (int u = 0; u < CH; ++u)
+      if (c0 + u < NKS)
+#pragma unroll
+        for (int part = 0; part < 3; ++part)
+#pragma unroll
+          for (int nh = 0; nh < 2; ++nh)
+            b[u][part][nh] = ld4(w + part * plane + (size_t)(32 * nh + r) * kp +
+                                 8 * (c0 + u) + 4 * h);
+    // keep the batch together (the scheduler would otherwise move each
+    // load next to its MFMA and expose its latency)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (c0 + u >= NKS) break;
+      const int ks = c0 + u;
+      const v4s_ws a0 = ld4(act + r * pitch + 8 * ks + 4 * h);
+      const v4s_ws a1 = ld4(act + (32 + r) * pitch + 8 * ks + 4 * h);
+#pragma unroll
+      for (int part = 0; part < 3; ++part)
+#pragma unroll
+        for (int nh = 0; nh < 2; ++nh) {
+          acc[0][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+              a0, b[u][part][nh], acc[0][nh], 0, 0, 0);
+          acc[1][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+              a1, b[u][part][nh], acc[1][nh], 0, 0, 0);
+        }
+    }
   }
 }
 
@@ -529,15 +571,38 @@ __global__ __launch_bounds__(64) void k_fwd_mfma(
   __shared__ __align__(16) unsigned short sa[64 * kAP];   // h1 / h3
   const int lane = threadIdx.x;
   const int r0 = blockIdx.x * 64;
-  const int rows = min(64, a.npad - r0);
-  // feature rows (162 BF16 = 81 dwords) into pitch-172 rows, zero padded
-  const unsigned* src =
-      reinterpret_cast<const unsigned*>(a.feat + (size_t)r0 * WS_FEAT);
+  // The block's 64 feature rows (162 BF16 = 81 dwords each; the feature
+  // buffers hold whole 64-row blocks, rows past npad only feed outputs
+  // that are not stored) are one contiguous, 16-byte aligned run: every
+  // 16-byte load is issued before the first is used, then the dwords go
+  // to pitch-172 rows whose pad columns are zero.
+  constexpr int kRowD = WS_FEAT / 2, kPitchD = kFP / 2;
+  constexpr int kQ = 64 * kRowD / 4, kIt = (kQ + 63) / 64;
+  static_assert(64 * kRowD % 4 == 0, "feature block is whole uint4s");
+  const uint4* src4 =
+      reinterpret_cast<const uint4*>(a.feat + (size_t)r0 * WS_FEAT);
   unsigned* dst = reinterpret_cast<unsigned*>(sf);
-  for (int k = lane; k < 64 * (kFP / 2); k += 64) {
-    const int row = k / (kFP / 2), c = k % (kFP / 2);
-    dst[k] = (row < rows && c < WS_FEAT / 2) ? src[row * (WS_FEAT / 2) + c]
-                                             : 0u;
+  uint4 q[kIt];
+#pragma unroll
+  for (int t = 0; t < kIt; ++t) {
+    const int k = lane + 64 * t;
+    q[t] = src4[min(k, kQ - 1)];
+  }
+#pragma unroll
+  for (int t = 0; t < kIt; ++t) {
+    const int k = lane + 64 * t;
+    if (k < kQ) {
+      const unsigned v[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = 4 * k + e, row = idx / kRowD;
+        dst[row * kPitchD + idx - row * kRowD] = v[e];
+      }
+    }
+  }
+  for (int k = lane; k < 64 * (kPitchD - kRowD); k += 64) {
+    const int row = k / (kPitchD - kRowD);
+    dst[row * kPitchD + kRowD + k - row * (kPitchD - kRowD)] = 0u;
   }
   __syncthreads();
   const size_t n1 = (size_t)H * WS_K1P, n2 = (size_t)H * H;

@@ -69,9 +69,12 @@ int ensure_scratch(ds_warmstart* ws, size_t rows) {
   if (rows > ws->fscratch_rows) {
     if (ws->fscratch) (void)hipFree(ws->fscratch);
     ws->fscratch = nullptr;
-    WS_HIP(ws, hipMalloc(&ws->fscratch, sizeof(unsigned short) * rows *
+    // whole 64-row blocks: the MFMA forward reads its 64 rows at once (a
+    // partial block's extra rows only feed outputs that are not stored)
+    const size_t frows = (rows + 63) / 64 * 64;
+    WS_HIP(ws, hipMalloc(&ws->fscratch, sizeof(unsigned short) * frows *
                                             WS_FEAT));
-    ws->fscratch_rows = rows;
+    ws->fscratch_rows = frows;
   }
   if (rows > ws->tscratch_n) {
     if (ws->tscratch) (void)hipFree(ws->tscratch);
