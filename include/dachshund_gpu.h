@@ -165,7 +165,8 @@ int ds_problem_set_launch(ds_problem* p, int waves_per_block,
 /* Preconditioner of the problem's solves and of ds_precond (no reference
  * counterpart for DS_PRECOND_BLOCK: the reference's preconditioner is
  * Jacobi, lib/wf_vector.cc:42-49, :185-194).  The block inverses are built
- * on the device on first selection (16 KB of FP32 per 64 rows). */
+ * on the device on first selection (8.4 KB of FP32 per 64 rows: the
+ * symmetric inverse stored by diagonals). */
 int ds_problem_set_precond(ds_problem* p, int kind);
 /* FP32 pair kernel variant (DS_F32_VALU or DS_F32_MFMA). */
 int ds_problem_set_f32_kernel(ds_problem* p, int kind);
