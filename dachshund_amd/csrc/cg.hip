@@ -157,6 +157,10 @@ struct ResPArgs {
   double* partials;
   int nblk;
   int all;   // every live subvolume (initial residual)
+  // set: the residual is formed here, r = rhs - q, and stored to r_out
+  const double* q;
+  const double* rhs;
+  double* r_out;
 };
 
 __global__ __launch_bounds__(64 * kVecWaves) void k_res_precond(ResPArgs a) {
@@ -167,7 +171,8 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_res_precond(ResPArgs a) {
   const int lane = threadIdx.x & 63;
   const int i = I * DS_BS + lane;
   const bool real = lane < a.blk_nreal[I];
-  const double bi = a.r[i];
+  const double bi = a.q ? a.rhs[i] - a.q[i] : a.r[i];
+  if (a.q) a.r_out[i] = bi;
   const double si =
       a.minv ? bj_apply(a.minv + (size_t)(I % a.minv_mod) * DS_BJ_SIZE, bi)
              : (real ? bi / a.diag[i] : 0.0);
@@ -344,8 +349,13 @@ hipError_t launch_precond(const double* b, const double* diag, double* x,
 }
 
 hipError_t launch_res_precond(ds_problem* p, double* out2, int all,
-                              hipStream_t st) {
+                              hipStream_t st, const double* q) {
   ResPArgs a{};
+  if (q) {
+    a.q = q;
+    a.rhs = p->rhs64;
+    a.r_out = p->r;
+  }
   a.r = p->r;
   a.diag = p->diag;
   a.minv = p->use_minv() ? p->minv : nullptr;
@@ -553,6 +563,8 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     else ++live;
   }
   const double* w = p->form == DS_FORM_W ? p->wvec : nullptr;
+  // the matrix-free mixed solver starts from an FP32 residual
+  const bool init32 = mixed && !p->use_fused && !p->use_dense;
   p->done_host[0] = ns - live;
   DS_TRY(hipMemcpyAsync(p->done_count, &p->done_host[0], sizeof(int),
                         hipMemcpyHostToDevice, st));
@@ -560,7 +572,8 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
                         hipMemcpyHostToDevice, st));
   if (live > 0) {
     // lib/pcg.h:97-110: r = b - A x; d = M^-1 r; delta = r . d; |r|.
-    DS_TRY(launch_scale(p->x, w, w ? p->xu : nullptr, nullptr, p->npad, st));
+    DS_TRY(launch_scale(p->x, w, w ? p->xu : nullptr,
+                        init32 ? p->du32 : nullptr, p->npad, st));
     CgEpilogue init;
     init.kind = EK_CG_RES;
     init.out2 = p->d;
@@ -570,6 +583,14 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
       // shared geometry: every subvolume's initial residual in one pass
       DS_TRY(launch_res64_mrhs(p, p->x, w ? p->xu : p->x, p->r, 1, st));
       DS_TRY(launch_res_precond(p, p->d, 1, st));
+      DS_TRY(launch_finalize(p, init, st));
+    } else if (init32) {
+      // The mixed solver's initial residual with the FP32 operator: its
+      // recurrence starts from b - A32 x0 (error ~1e-6 |A x0|, far above
+      // the point where the first FP64 true residual replaces it), and no
+      // subvolume stops before an FP64 true residual meets the tolerance.
+      DS_TRY(launch_matvec32_ex(p, p->du32, p->x, p->q, 0, st));
+      DS_TRY(launch_res_precond(p, p->d, 1, st, p->q));
       DS_TRY(launch_finalize(p, init, st));
     } else {
       DS_TRY(launch_matvec64_ex(p, 1, p->x, w ? p->xu : p->x, p->r, init,
@@ -677,8 +698,8 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     if (r.norm_b > 0.0) {
       const int steps = std::max(0, std::min(h[s].num_iters, max_iter - 1));
       if (mixed) {
-        r.n_matvec32 = steps;
-        r.n_matvec64 = 1 + h[s].n_true;
+        r.n_matvec32 = steps + (init32 ? 1 : 0);
+        r.n_matvec64 = (init32 ? 0 : 1) + h[s].n_true;
       } else {
         r.n_matvec64 = 1 + steps + steps / std::max(prm->step_r, 1);
       }

@@ -243,8 +243,11 @@ hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
 hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
                              const double* u, double* out, int all,
                              hipStream_t st);
+// s = M^-1 r and the r.s, r.r partials of the subvolumes that asked for a
+// true residual (all: every live one); with q, r = rhs - q is formed first
+// and stored to r.
 hipError_t launch_res_precond(ds_problem* p, double* out2, int all,
-                              hipStream_t st);
+                              hipStream_t st, const double* q = nullptr);
 hipError_t launch_precond(const double* b, const double* diag, double* x,
                           int npad, hipStream_t st);
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
