@@ -220,11 +220,13 @@ int ds_solve_pcg(ds_problem* p, const ds_pcg_params* params, double* x,
                  double* residual_norms, int norms_cap, ds_pcg_result* res);
 
 /* Mixed-precision solve of the same system (no reference counterpart):
- * FP32 pair evaluation inside PCG with FP64 vectors and dot products; when
- * the recurrence residual has fallen by `refine_drop` since the last FP64
- * true residual, or meets the tolerance, or step_r iterations have passed
- * since that one, an FP64 true residual replaces it.  Stops only when the
- * FP64 true residual meets |r| <= tol |b|.  Same x
+ * FP32 pair evaluation inside PCG with FP64 vectors and dot products; the
+ * initial residual b - A x0 is formed with the FP32 operator (matrix-free
+ * problems; the shared-geometry block paths form it in FP64); when the
+ * recurrence residual has fallen by `refine_drop` since the last
+ * (initial or FP64 true) residual, or meets the tolerance, or step_r
+ * iterations have passed since that one, an FP64 true residual replaces
+ * it.  Stops only when the FP64 true residual meets |r| <= tol |b|.  Same x
  * in/out convention as ds_solve_pcg. */
 int ds_solve_mixed(ds_problem* p, const ds_pcg_params* params,
                    double refine_drop, double* x, double* residual_norms,
