@@ -282,6 +282,32 @@ def test_mixed_solver_meets_fp64_tolerance(D, ora, form):
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("form", ["w", "sn"])
+def test_mixed_solver_from_converged_start(D, ora, form):
+    """The mixed solver's initial residual comes from the FP32 operator; a
+    start that already meets the tolerance must still end on an FP64 true
+    residual (at least one step, one true residual) and stay a solution."""
+    pix = CASES["small"]
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(0.05, 2.5, 2.1)
+    wf, _ = _problem(D, form, pix, (0.05, 2.5, 2.1))
+    n = len(pix)
+    tol = 1e-6
+    x0, ok, _, _ = ora.solve_pcg(f, fp, c, 5000, 50, 1e-12)
+    assert ok
+    # solve_pcg's x: W form y in, x = w .* y out (lib/wf_vector.cc:268-271)
+    x = x0 / fp[:, 3] if form == "w" else x0.copy()
+    res = wf.solve_mixed(D.PCGParams(n, 2000, 50, tol), x, refine_drop=1e-3)
+    assert res.success and res.n_true >= 1 and res.num_iters >= 1
+    assert res.n_matvec32 == res.num_iters + 1
+    assert res.n_matvec64 == res.n_true
+    y = x / fp[:, 3] if form == "w" else x
+    r = ora.A_residual(f, fp, c, y)
+    assert np.linalg.norm(r) <= tol * ora.norm_b(f, fp) * 1.0001
+    assert np.linalg.norm(x - x0) <= 1e-5 * np.linalg.norm(x0)
+
+
 def test_resident_solution_and_map(D, ora):
     """app/dachshund.cc:218-264 end to end on the 8,192-pixel workload:
     W form from x = 0, then smp_product onto the 16x16x64 grid."""
