@@ -218,8 +218,8 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
 }
 
 // Shared-geometry batches with block-Jacobi: the subvolumes share the
-// block inverses, so one workgroup per geometry block stages its 16 KB
-// inverse in LDS once and applies it to that block of every subvolume
+// block inverses, so one workgroup per geometry block stages its 8.4 KB
+// inverse (33 stored diagonals) in LDS once and applies it to that block of every subvolume
 // (wave w takes subvolumes w, w + kVecWaves, ...).  Same operations as
 // k_cg_update, so the results are the same bit for bit.
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
@@ -694,7 +694,8 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
     r.true_norm = mixed ? h[s].true_norm : h[s].norm_r;
     r.n_true = h[s].n_true;
     r.seconds = ms * 1e-3;
-    // pair-evaluation passes: initial residual + one per step (+ refreshes)
+    // pair-evaluation passes: initial residual (FP32 in the matrix-free
+    // mixed solver) + one per step (+ refreshes)
     if (r.norm_b > 0.0) {
       const int steps = std::max(0, std::min(h[s].num_iters, max_iter - 1));
       if (mixed) {

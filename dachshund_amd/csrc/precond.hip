@@ -4,8 +4,8 @@
 // lib/wf_vector.cc:42-49, :185-194), and the parity solver keeps it.  A
 // problem may instead select the exact inverse of each 64-row target block
 // of A (the k-d blocks of the culling, spatially compact): the local
-// couplings the diagonal misses, for one extra 16 KB read per block and
-// application.  At the 300k-pixel skewer density it cuts the iterations to
+// couplings the diagonal misses, for one extra 8.4 KB read per block and
+// application (the symmetric inverse stored by diagonals).  At the 300k-pixel skewer density it cuts the iterations to
 // 1e-6 by about a third.  The preconditioner is fixed and symmetric
 // positive definite (the inverse of SPD blocks, symmetrized and stored in
 // FP32), so PCG's theory is unchanged; solves stop on the same FP64 true
