@@ -1044,7 +1044,11 @@ int ds_solve_mixed_resident(ds_problem* p, const ds_pcg_params* prm,
     return fail(p->ctx, DS_E_ARG, "refine_drop must lie in (0, 1)");
   if (p->n_sub != 1)
     return fail(p->ctx, DS_E_STATE, "batch problem: use ds_batch_solve");
-  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 1, drop, res, 1));
+  // Plain launches: a single volume's iterations keep the GPU busy while
+  // the host enqueues, and capturing plus instantiating the chunk's graph
+  // on every call cost more than its replay saved (7.78 vs 7.38 ms at 300k
+  // pixels).  ds_batch_solve keeps the choice.
+  HIP_OR_FAIL(p->ctx, ds::run_solver(p, prm, 1, drop, res, 0));
   return DS_OK;
 }
 
