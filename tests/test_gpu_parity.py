@@ -308,6 +308,32 @@ def test_mixed_solver_from_converged_start(D, ora, form):
     assert np.linalg.norm(x - x0) <= 1e-5 * np.linalg.norm(x0)
 
 
+@pytest.mark.parametrize("precond", ["jacobi", "block"])
+@pytest.mark.parametrize("form", ["w", "sn"])
+@pytest.mark.parametrize("case", ["n1", "n2", "n63", "n65", "n129", "dups"])
+def test_mixed_solver_edge_cases(D, ora, case, form, precond):
+    """Sizes around the 64-row block (padding rows in the last block), one
+    and two pixels, and coincident pixels with a far-away loner (whose
+    problem runs the direct-difference FP32 form): the mixed solver with
+    either preconditioner meets the FP64 tolerance the oracle recomputes."""
+    pix = CASES[case]
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(0.05, 2.5, 2.1)
+    wf, _ = _problem(D, form, pix, (0.05, 2.5, 2.1))
+    wf.set_precond(D.DS_PRECOND_BLOCK if precond == "block"
+                   else D.DS_PRECOND_JACOBI)
+    n = len(pix)
+    tol = 1e-8
+    x = np.zeros(n)
+    res = wf.solve_mixed(D.PCGParams(n, 2000, 50, tol), x, refine_drop=1e-3)
+    assert res.success and res.n_true >= 1
+    assert np.all(np.isfinite(x))
+    y = x / fp[:, 3] if form == "w" else x
+    r = ora.A_residual(f, fp, c, y)
+    assert np.linalg.norm(r) <= tol * ora.norm_b(f, fp) * 1.0001
+
+
 def test_resident_solution_and_map(D, ora):
     """app/dachshund.cc:218-264 end to end on the 8,192-pixel workload:
     W form from x = 0, then smp_product onto the 16x16x64 grid."""
