@@ -46,6 +46,11 @@ hipError_t ws_features(ds_problem* p, unsigned short* feat,
                        hipStream_t st);
 // Inference; with quant (BF16 activations) and wsplit it runs on the
 // matrix cores, else on the VALU.
+// Prediction of the problem's x: with quant and wsplit one fused kernel
+// (features in LDS, network on the matrix cores), else features into feat
+// and the VALU forward.
+hipError_t ws_predict(const float* P, unsigned short* wsplit, int quant,
+                      ds_problem* p, unsigned short* feat, hipStream_t st);
 hipError_t ws_forward(const float* P, unsigned short* wsplit,
                       const unsigned short* feat, int n, int npad, int quant,
                       const ds_problem* p, double* x_out, float* t_out,

@@ -252,16 +252,11 @@ __global__ __launch_bounds__(64) void k_knn_select(KnnArgs a) {
   for (int k = 0; k < K; ++k) o[k] = bi[k];
 }
 
-// Features of each row from its neighbour list and the current data,
-// staged in LDS so that the block's 64 contiguous feature rows are written
-// out in coalesced dwords.
+// The WS_FEAT features of one row from its neighbour list and the current
+// data, as BF16 into f[0, WS_FEAT).
 template <int K>
-__global__ __launch_bounds__(64) void k_ws_features(KnnArgs a) {
-  __shared__ unsigned short stage[DS_BS * WS_FEAT];   // 20.25 KB
-  const int lane = threadIdx.x;
-  const int I = blockIdx.x;
-  const int row = I * DS_BS + lane;
-  unsigned short* f = stage + lane * WS_FEAT;
+__device__ __forceinline__ void ws_feature_row(const KnnArgs& a, int row,
+                                               unsigned short* f) {
   if (row >= a.n) {
     for (int c = 0; c < WS_FEAT; ++c) f[c] = 0;
   } else {
@@ -315,6 +310,16 @@ __global__ __launch_bounds__(64) void k_ws_features(KnnArgs a) {
     f[5 * K] = f2bf((float)noise(pi.w) * a.inv_sqrt_vs);
     f[5 * K + 1] = f2bf((float)datum(row, pi.w) * a.inv_sqrt_vs);
   }
+}
+
+// Features of each row, staged in LDS so that the block's 64 contiguous
+// feature rows are written out in coalesced dwords.
+template <int K>
+__global__ __launch_bounds__(64) void k_ws_features(KnnArgs a) {
+  __shared__ unsigned short stage[DS_BS * WS_FEAT];   // 20.25 KB
+  const int lane = threadIdx.x;
+  const int I = blockIdx.x;
+  ws_feature_row<K>(a, I * DS_BS + lane, stage + lane * WS_FEAT);
   __syncthreads();
   const unsigned* src = reinterpret_cast<const unsigned*>(stage);
   unsigned* dst = reinterpret_cast<unsigned*>(a.feat + (size_t)I * DS_BS *
@@ -564,6 +569,52 @@ __device__ __forceinline__ void mfma_epilogue(const v16f_ws (&acc)[2][2],
 constexpr int kFP = 172;   // LDS pitch (BF16) of the feature rows
 constexpr int kAP = 68;    // LDS pitch (BF16) of the hidden activations
 
+// The network on a block's 64 feature rows staged in sf (pitch kFP, pad
+// columns zero): layers on the matrix cores, activations through LDS, the
+// output layer on the VALU, the prediction stored for rows r0 + lane.
+__device__ __forceinline__ void fwd_mfma_rows(
+    const float* __restrict__ P, const unsigned short* __restrict__ wsplit,
+    const FwdArgs& a, unsigned short* sf, unsigned short* sa, int r0) {
+  const int lane = threadIdx.x;
+  const size_t n1 = (size_t)H * WS_K1P, n2 = (size_t)H * H;
+  v16f_ws acc[2][2];
+  auto zero = [&]() {
+#pragma unroll
+    for (int mh = 0; mh < 2; ++mh)
+#pragma unroll
+      for (int nh = 0; nh < 2; ++nh) acc[mh][nh] = (v16f_ws){};
+  };
+  zero();
+  mfma_layer<WS_K1P / 8>(sf, kFP, wsplit, WS_K1P, acc);
+  mfma_epilogue(acc, P + WS_OFF_B1, sa, kAP);
+  __syncthreads();
+  zero();
+  mfma_layer<H / 8>(sa, kAP, wsplit + 3 * n1, H, acc);
+  mfma_epilogue(acc, P + WS_OFF_B2, sf, kAP);
+  __syncthreads();
+  zero();
+  mfma_layer<H / 8>(sf, kAP, wsplit + 3 * n1 + 3 * n2, H, acc);
+  mfma_epilogue(acc, P + WS_OFF_B3, sa, kAP);
+  __syncthreads();
+  const int i = r0 + lane;
+  if (i >= a.npad) return;
+  if (i >= a.n) {
+    if (a.x_out) a.x_out[i] = 0.0;
+    if (a.t_out) a.t_out[i] = 0.0f;
+    return;
+  }
+  const float* W4 = P + WS_OFF_W4;
+  float t = P[WS_OFF_B4];
+#pragma unroll 8
+  for (int u = 0; u < H; ++u) t = fmaf(bf2f(sa[lane * kAP + u]), W4[u], t);
+  if (a.t_out) a.t_out[i] = t;
+  if (a.x_out) {
+    const double x0 = (double)t * a.inv_sqrt_vs;
+    const double w = a.pos[i].w;
+    a.x_out[i] = a.form == DS_FORM_W ? x0 / w : x0;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_fwd_mfma(
     const float* __restrict__ P, const unsigned short* __restrict__ wsplit,
     FwdArgs a) {
@@ -605,43 +656,25 @@ __global__ __launch_bounds__(64) void k_fwd_mfma(
     dst[row * kPitchD + kRowD + k - row * (kPitchD - kRowD)] = 0u;
   }
   __syncthreads();
-  const size_t n1 = (size_t)H * WS_K1P, n2 = (size_t)H * H;
-  v16f_ws acc[2][2];
-  auto zero = [&]() {
-#pragma unroll
-    for (int mh = 0; mh < 2; ++mh)
-#pragma unroll
-      for (int nh = 0; nh < 2; ++nh) acc[mh][nh] = (v16f_ws){};
-  };
-  zero();
-  mfma_layer<WS_K1P / 8>(sf, kFP, wsplit, WS_K1P, acc);
-  mfma_epilogue(acc, P + WS_OFF_B1, sa, kAP);
+  fwd_mfma_rows(P, wsplit, a, sf, sa, r0);
+}
+
+// Prediction in one pass: the block's features go straight into the
+// network's LDS rows (the same values as k_ws_features writes), then
+// fwd_mfma_rows.  No feature array in HBM.
+template <int K>
+__global__ __launch_bounds__(64) void k_predict_mfma(
+    const float* __restrict__ P, const unsigned short* __restrict__ wsplit,
+    KnnArgs ka, FwdArgs a) {
+  __shared__ __align__(16) unsigned short sf[64 * kFP];   // features / h2
+  __shared__ __align__(16) unsigned short sa[64 * kAP];   // h1 / h3
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * 64;
+  unsigned short* f = sf + lane * kFP;
+  ws_feature_row<K>(ka, r0 + lane, f);
+  for (int c = WS_FEAT; c < kFP; ++c) f[c] = 0;
   __syncthreads();
-  zero();
-  mfma_layer<H / 8>(sa, kAP, wsplit + 3 * n1, H, acc);
-  mfma_epilogue(acc, P + WS_OFF_B2, sf, kAP);
-  __syncthreads();
-  zero();
-  mfma_layer<H / 8>(sf, kAP, wsplit + 3 * n1 + 3 * n2, H, acc);
-  mfma_epilogue(acc, P + WS_OFF_B3, sa, kAP);
-  __syncthreads();
-  const int i = r0 + lane;
-  if (i >= a.npad) return;
-  if (i >= a.n) {
-    if (a.x_out) a.x_out[i] = 0.0;
-    if (a.t_out) a.t_out[i] = 0.0f;
-    return;
-  }
-  const float* W4 = P + WS_OFF_W4;
-  float t = P[WS_OFF_B4];
-#pragma unroll 8
-  for (int u = 0; u < H; ++u) t = fmaf(bf2f(sa[lane * kAP + u]), W4[u], t);
-  if (a.t_out) a.t_out[i] = t;
-  if (a.x_out) {
-    const double x0 = (double)t * a.inv_sqrt_vs;
-    const double w = a.pos[i].w;
-    a.x_out[i] = a.form == DS_FORM_W ? x0 / w : x0;
-  }
+  fwd_mfma_rows(P, wsplit, a, sf, sa, r0);
 }
 
 // Training: forward + back-propagation of the squared error of sample
@@ -804,9 +837,13 @@ __global__ __launch_bounds__(256) void k_targets(const double* x, int n,
     if (e_ != hipSuccess) return e_;  \
   } while (0)
 
-hipError_t ws_features(ds_problem* p, unsigned short* feat,
-                       hipStream_t st) {
-  KnnArgs a{};
+namespace {
+
+// The feature kernels' arguments; builds the problem's neighbour lists on
+// first use.
+hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
+                    KnnArgs& a) {
+  a = KnnArgs{};
   a.pos = p->pos64;
   a.rhs = p->rhs64;
   a.tgt = p->tgt32;
@@ -850,14 +887,12 @@ hipError_t ws_features(ds_problem* p, unsigned short* feat,
     p->knn_nbr = nbr;
   }
   a.nbr = p->knn_nbr;
-  hipLaunchKernelGGL(k_ws_features<WS_K>, dim3(p->nblk), dim3(64), 0, st, a);
-  return hipGetLastError();
+  return hipSuccess;
 }
 
-hipError_t ws_forward(const float* P, unsigned short* wsplit,
-                      const unsigned short* feat, int n, int npad, int quant,
-                      const ds_problem* p, double* x_out, float* t_out,
-                      hipStream_t st) {
+FwdArgs fwd_args(const float* P, const unsigned short* feat, int n, int npad,
+                 int quant, const ds_problem* p, double* x_out,
+                 float* t_out) {
   FwdArgs a{};
   a.feat = feat;
   a.P = P;
@@ -869,11 +904,57 @@ hipError_t ws_forward(const float* P, unsigned short* wsplit,
   a.form = p ? p->form : DS_FORM_SN;
   a.quant = quant;
   a.inv_sqrt_vs = p ? 1.0 / std::sqrt(p->cv.var_s) : 1.0;
+  return a;
+}
+
+hipError_t split_weights(const float* P, unsigned short* wsplit,
+                         hipStream_t st) {
+  const int ns = H * (WS_K1P + 2 * H);
+  hipLaunchKernelGGL(k_split_weights, dim3((ns + 255) / 256), dim3(256), 0,
+                     st, P, wsplit);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t ws_features(ds_problem* p, unsigned short* feat,
+                       hipStream_t st) {
+  KnnArgs a;
+  const hipError_t e = knn_args(p, feat, st, a);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ws_features<WS_K>, dim3(p->nblk), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t ws_predict(const float* P, unsigned short* wsplit, int quant,
+                      ds_problem* p, unsigned short* feat, hipStream_t st) {
+  if (!(quant && wsplit)) {
+    // VALU inference reads the feature array
+    const hipError_t e = ws_features(p, feat, st);
+    if (e != hipSuccess) return e;
+    return ws_forward(P, wsplit, feat, p->n, p->npad, quant, p, p->x,
+                      nullptr, st);
+  }
+  KnnArgs ka;
+  hipError_t e = knn_args(p, nullptr, st, ka);
+  if (e == hipSuccess) e = split_weights(P, wsplit, st);
+  if (e != hipSuccess) return e;
+  const FwdArgs a = fwd_args(P, nullptr, p->n, p->npad, quant, p, p->x,
+                             nullptr);
+  hipLaunchKernelGGL(k_predict_mfma<WS_K>, dim3(p->nblk), dim3(64), 0, st, P,
+                     wsplit, ka, a);
+  return hipGetLastError();
+}
+
+hipError_t ws_forward(const float* P, unsigned short* wsplit,
+                      const unsigned short* feat, int n, int npad, int quant,
+                      const ds_problem* p, double* x_out, float* t_out,
+                      hipStream_t st) {
+  const FwdArgs a = fwd_args(P, feat, n, npad, quant, p, x_out, t_out);
   const dim3 g((npad + 63) / 64), b(64);
   if (quant && wsplit) {
-    const int ns = H * (WS_K1P + 2 * H);
-    hipLaunchKernelGGL(k_split_weights, dim3((ns + 255) / 256), dim3(256), 0,
-                       st, P, wsplit);
+    const hipError_t e = split_weights(P, wsplit, st);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fwd_mfma, g, b, 0, st, P, wsplit, a);
   } else if (quant) {
     hipLaunchKernelGGL(k_fwd<true>, g, b, 0, st, P, a);

@@ -217,11 +217,10 @@ int ds_ws_predict(ds_warmstart* ws, ds_problem* p) {
     return ws_fail(ws->ctx, DS_E_STATE, "warm start needs a single volume");
   if (int rc = ensure_scratch(ws, p->npad)) return rc;
   hipStream_t st = ws->ctx->stream;
-  WS_HIP(ws, ds::ws_features(p, ws->fscratch, st));
-  WS_HIP(ws, ds::ws_forward(ws->T.P, ws->T.wsplit, ws->fscratch, p->n,
-                            p->npad, ws->quant,
-                            p, p->x, nullptr, st));
-  WS_HIP(ws, hipStreamSynchronize(st));
+  // stream-ordered: the prediction is in place for every later call on
+  // the context (no host wait here)
+  WS_HIP(ws, ds::ws_predict(ws->T.P, ws->T.wsplit, ws->quant, p,
+                            ws->fscratch, st));
   return DS_OK;
 }
 

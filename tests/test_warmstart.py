@@ -122,6 +122,36 @@ def test_gpu_features_vs_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["w", "sn"])
+def test_gpu_predict_equals_features_then_forward(form):
+    """The one-pass prediction (features built in LDS, network on the matrix
+    cores) gives the same initial guess, bit for bit, as the feature array
+    followed by the separate forward kernel."""
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    from dachshund_amd.warmstart import WarmStart
+    w = synth.workload("small")
+    pix = synth.make_pixels(w)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    if form == "w":
+        fp = D.pixel_to_wpixel(pix)
+        wf = D.WFVectorW(fp, sp)
+    else:
+        fp = D.pixel_to_npixel(pix)
+        wf = D.WFVectorSN(fp, sp)
+    ws = WarmStart(seed=3)
+    wf.fill_x(0.0)
+    ws.predict_into(wf)
+    y = wf.get_x()
+    f = ws.features(wf)
+    t, _, _ = ws.eval_batch(f, np.zeros(len(pix), dtype=np.float32))
+    x0 = t.astype(np.float64) * (1.0 / np.sqrt(w.var_s))
+    y_ref = x0 / fp[:, 3] if form == "w" else x0
+    assert np.any(y != 0.0)
+    assert np.array_equal(y, y_ref)
+
+
+@pytest.mark.gpu
 def test_trained_warm_start_cuts_iterations():
     """Train on solutions of 12 seeded problems, evaluate on 4 held-out
     seeds against x0 = 0, same FP64 stopping rule."""
