@@ -552,9 +552,12 @@ def main():
     xmx = wf.get_x()
     if ws:
         tp = []
+        ctx = D.default_context()
         for _ in range(args.steps):
+            ctx.synchronize()
             t = time.perf_counter()
             ws.predict_into(wf)
+            ctx.synchronize()
             tp.append(time.perf_counter() - t)
         t_pred = float(np.mean(tp))
         log(f"learned prediction alone (features + network): "
@@ -564,6 +567,7 @@ def main():
         wf2 = D.WFVectorW(pw, sp)
         t = time.perf_counter()
         ws.predict_into(wf2)
+        ctx.synchronize()
         t_knn = time.perf_counter() - t - t_pred
         wf2.close()
         log(f"one-time neighbour search (geometry setup): "

@@ -73,7 +73,9 @@ class WarmStart:
         return out
 
     def predict_into(self, wf):
-        """Write the predicted initial guess into wf's resident x."""
+        """Write the predicted initial guess into wf's resident x (enqueued
+        on the context's stream, ahead of any later call on it; the host
+        does not wait: ctx.synchronize() does)."""
         check(lib().ds_ws_predict(self.h, wf.h), self.ctx.h, "ws_predict")
 
     def add_sample(self, wf, seed: int = 0):

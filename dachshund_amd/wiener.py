@@ -169,6 +169,10 @@ class Context:
     def num_cu(self):
         return lib().ds_ctx_num_cu(self.h)
 
+    def synchronize(self):
+        """Wait for the work enqueued on the context's stream."""
+        check(lib().ds_ctx_synchronize(self.h), self.h, "synchronize")
+
     def close(self):
         if self.h:
             lib().ds_ctx_destroy(self.h)
