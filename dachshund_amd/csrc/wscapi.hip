@@ -211,16 +211,31 @@ int ds_ws_features(ds_warmstart* ws, ds_problem* p, float* feat_out) {
   return DS_OK;
 }
 
+// Work enqueued on `later` from now on runs after what `earlier` holds.
+static hipError_t order_after(hipStream_t later, hipStream_t earlier) {
+  hipEvent_t ev;
+  hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(ev, earlier);
+  if (e == hipSuccess) e = hipStreamWaitEvent(later, ev, 0);
+  const hipError_t d = hipEventDestroy(ev);
+  return e != hipSuccess ? e : d;
+}
+
 int ds_ws_predict(ds_warmstart* ws, ds_problem* p) {
   if (!ws || !p) return DS_E_ARG;
   if (p->n_sub != 1)
     return ws_fail(ws->ctx, DS_E_STATE, "warm start needs a single volume");
   if (int rc = ensure_scratch(ws, p->npad)) return rc;
   hipStream_t st = ws->ctx->stream;
-  // stream-ordered: the prediction is in place for every later call on
-  // the context (no host wait here)
+  // Stream-ordered, no host wait: the prediction is in place for every
+  // later call on the problem's context.  A warm start on another context
+  // (another stream) is ordered against the problem's stream both ways.
+  hipStream_t ps = p->ctx->stream;
+  if (ps != st) WS_HIP(ws, order_after(st, ps));
   WS_HIP(ws, ds::ws_predict(ws->T.P, ws->T.wsplit, ws->quant, p,
                             ws->fscratch, st));
+  if (ps != st) WS_HIP(ws, order_after(ps, st));
   return DS_OK;
 }
 

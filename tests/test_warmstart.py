@@ -152,6 +152,35 @@ def test_gpu_predict_equals_features_then_forward(form):
 
 
 @pytest.mark.gpu
+def test_gpu_predict_from_another_context():
+    """The prediction does not wait on the host; a warm start on another
+    context (another stream) is ordered against the problem's stream both
+    ways, so it overwrites the x just filled and is in place for the read
+    that follows: the same guess as from the problem's own context."""
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    from dachshund_amd.warmstart import WarmStart
+    w = synth.workload("small")
+    pix = synth.make_pixels(w)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    ctx2 = D.Context()
+    ws_a = WarmStart(seed=5)
+    ws_b = WarmStart(seed=5, ctx=ctx2)
+    wf.fill_x(7.0)
+    ws_a.predict_into(wf)
+    ya = wf.get_x()
+    for _ in range(3):
+        wf.fill_x(7.0)
+        ws_b.predict_into(wf)
+        yb = wf.get_x()
+        assert np.array_equal(ya, yb)
+    assert not np.any(ya == 7.0)
+    ws_b.close()
+    ctx2.close()
+
+
+@pytest.mark.gpu
 def test_trained_warm_start_cuts_iterations():
     """Train on solutions of 12 seeded problems, evaluate on 4 held-out
     seeds against x0 = 0, same FP64 stopping rule."""
