@@ -312,7 +312,10 @@ int ds_ws_get_params(ds_warmstart* ws, float* params, int n);
 int ds_ws_set_quantized(ds_warmstart* ws, int on);
 /* Features of every pixel of p, host order, n x ds_ws_feature_dim(). */
 int ds_ws_features(ds_warmstart* ws, ds_problem* p, float* feat_out);
-/* Predict the initial guess into p's resident x (W space for the W form). */
+/* Predict the initial guess into p's resident x (W space for the W form).
+ * Returns once the work is enqueued: it is ordered before every later call
+ * on p's context (also when ws lives on another context); errors of the
+ * kernels surface at the next call that waits for the device. */
 int ds_ws_predict(ds_warmstart* ws, ds_problem* p);
 /* Add p as a training sample: its features and, as target, its resident x,
  * which must hold a solution (x = w .* y after a W-form solve).  Samples
