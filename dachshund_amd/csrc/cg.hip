@@ -263,6 +263,13 @@ struct DirArgs {
   unsigned short* du16;   // BF16 copy for the explicit block path
   ds_cg_scalars* sc;
   int nblk;
+  // FOLD: the true residual's finalize (mixed solver) done here
+  const double* partials;
+  const int* sub_blk0;
+  double* norms;
+  int* done_count;
+  double drop;
+  int norms_cap;
 };
 
 __device__ __forceinline__ unsigned short to_bf16(float f) {
@@ -274,7 +281,12 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
 // lib/pcg.h:61-64: d = s + beta d (and its scaled copies).  As in
 // k_cg_update, the row's loads go out before the subvolume's scalars and
 // a finished subvolume's rows store their values back unchanged.
-template <bool W>
+// FOLD (mixed solver): when the subvolume took an FP64 true residual in
+// this step (want_true), each wave sums its r.s and r.r partials itself,
+// in one fixed order, and takes beta from them; the wave of the
+// subvolume's first block also records the step (finalize_step with
+// is_true), which leaves want_true set for the other waves.
+template <bool W, bool FOLD>
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_dir(DirArgs a) {
   const int I = __builtin_amdgcn_readfirstlane(blockIdx.x * kVecWaves +
                                                (threadIdx.x >> 6));
@@ -282,9 +294,25 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_dir(DirArgs a) {
   const int i = I * DS_BS + (threadIdx.x & 63);
   const double si = a.s[i], d0 = a.d[i];
   const double wi = W ? a.wvec[i] : 1.0;
-  const ds_cg_scalars* sc = a.sc + a.blk_sub[I];
+  const int sv = a.blk_sub[I];
+  ds_cg_scalars* sc = a.sc + sv;
   const bool live = !sc->done;
-  const double beta = sc->beta;
+  double beta = sc->beta;
+  if (FOLD && live && sc->want_true) {
+    const int b0 = a.sub_blk0[sv], nb = a.sub_blk0[sv + 1] - b0;
+    const int lane = threadIdx.x & 63;
+    double rs = 0.0, rr = 0.0;
+    for (int b = lane; b < nb; b += 64) {
+      rs += a.partials[b0 + b];
+      rr += a.partials[(size_t)a.nblk + b0 + b];
+    }
+    rs = wave_sum(rs);
+    rr = wave_sum(rr);
+    beta = rs / sc->saved_delta_old;
+    if (I == b0 && lane == 0)
+      finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, rs, rr, 0, 1,
+                    a.drop, 1, a.done_count);
+  }
   const double di = live ? si + beta * d0 : d0;
   a.d[i] = di;
   const double ui = W ? wi * di : di;
@@ -459,8 +487,15 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   return launch_finalize(p, e, st);
 }
 
-hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
+hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st,
+                      int fold = 0, double drop = 0.0) {
   DirArgs a{};
+  a.partials = p->partials;
+  a.sub_blk0 = p->sub_blk0;
+  a.norms = p->norms;
+  a.done_count = p->done_count;
+  a.drop = drop;
+  a.norms_cap = p->norms_cap;
   a.s = p->s;
   a.wvec = p->form == DS_FORM_W ? p->wvec : nullptr;
   a.blk_sub = p->blk_sub;
@@ -471,8 +506,13 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st) {
   a.sc = p->sc;
   a.nblk = p->nblk;
   const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
-  if (a.wvec) hipLaunchKernelGGL(k_cg_dir<true>, g, b, 0, st, a);
-  else hipLaunchKernelGGL(k_cg_dir<false>, g, b, 0, st, a);
+  if (fold) {
+    if (a.wvec) hipLaunchKernelGGL((k_cg_dir<true, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_dir<false, true>), g, b, 0, st, a);
+  } else {
+    if (a.wvec) hipLaunchKernelGGL((k_cg_dir<true, false>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_cg_dir<false, false>), g, b, 0, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -509,7 +549,10 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
       DS_TRY(launch_res_precond(p, p->s, 0, st));
       DS_TRY(launch_finalize(p, tr, st));
     } else {
+      // the true residual's finalize is folded into the direction kernel
+      tr.fold = 1;
       DS_TRY(launch_matvec64_ex(p, 1, p->x, p->xu, p->r, tr, st));
+      return launch_dir(p, mixed, st, 1, drop);
     }
   } else {
     DS_TRY(launch_matvec64_ex(p, 0, p->d, p->du, p->q, dq, st));

@@ -180,9 +180,12 @@ __device__ __forceinline__ double bj_apply_regs(const float (&m)[DS_BJ_ROWS],
 }
 
 // lib/pcg.h:35-37: alpha = delta / (d . q).
+// (Also ends a mixed solver's true-residual request: every reader of
+// want_true in the step that asked for it has run by now.)
 __device__ __forceinline__ void finalize_alpha(ds_cg_scalars* sc, double dq) {
   sc->dq = dq;
   sc->alpha = sc->delta / dq;
+  sc->want_true = 0;
 }
 
 __device__ __forceinline__ void mark_done(ds_cg_scalars* sc, int success,
@@ -242,7 +245,8 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
     sc->n_true += 1;
     sc->true_norm = norm_r;
     sc->ref_norm = norm_r;
-    sc->want_true = 0;
+    // want_true stays set until the next step's finalize_alpha: the
+    // direction kernel's waves of this subvolume still read it
     sc->last_true = iter;
     if (norm_r <= sc->tol_norm_b) {
       mark_done(sc, 1, iter, done_count);

@@ -282,6 +282,9 @@ struct CgEpilogue {
   double* out2 = nullptr;   // EK_CG_RES: M^-1 r
   int init = 0, mixed = 0, is_true = 0, need_want = 0;
   double drop = 0.0;
+  // the launcher skips its finalize: the direction kernel that follows
+  // folds it in (launch_dir with fold)
+  int fold = 0;
 };
 hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
                               const double* u, double* out,

@@ -477,7 +477,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   }
 #undef DS_L
   const hipError_t err = hipGetLastError();
-  if (err != hipSuccess || e.kind == EPI_NONE) return err;
+  if (err != hipSuccess || e.kind == EPI_NONE || e.fold) return err;
   return launch_finalize(p, e, st);
 }
 
