@@ -659,7 +659,9 @@ def main():
             "limiter": "VALU issue serialized with the MFMA (no VALU/MFMA "
                        "co-execution measured) and, at half the work per "
                        "block, per-block start-up (PMC counters in "
-                       "profiles/)",
+                       "profiles/); algorithmically, the 16 x 16 culling "
+                       "granularity: ~24 % of the pair products lie "
+                       "inside the cutoff",
             "kernel": "k_matvec32m + k_sym_epi (symmetric FP32 operator: "
                       "each tile pair evaluated once, used for both "
                       "products; then the per-tile gather and epilogue)",
