@@ -61,14 +61,15 @@ __device__ __forceinline__ double update_q(const UpdArgs& a, int I, int sv,
 }
 
 // lib/pcg.h:37-58: x += alpha d; r -= alpha q; s = M^-1 r; r.s and r.r.
-// BJ: block-Jacobi (a.minv set, a.only_x clear), else Jacobi or x only;
+// BJ: block-Jacobi (a.minv set, a.only_x clear; 1: r broadcast in FP64,
+// 2: in FP32, for the mixed solver), 0: Jacobi or x only;
 // W: a.wvec set; SYM: a.sub_sym set (q of the flagged subvolumes formed
 // here, as sym_row does, from the block's own sums and the tile's
 // column-sum slots, in slot order).  The kernel is bound by the latency of
 // its dependent loads (one wave per block, a few waves per SIMD), so every
 // load that depends only on the row is issued first, the block's inverse
 // in one batch, and the register budget keeps every block resident.
-template <bool BJ, bool W, bool SYM>
+template <int BJ, bool W, bool SYM>
 __global__ __launch_bounds__(64 * kVecWaves)
 __attribute__((amdgpu_waves_per_eu(BJ ? 5 : 1))) void k_cg_update(
     UpdArgs a) {
@@ -133,7 +134,9 @@ __attribute__((amdgpu_waves_per_eu(BJ ? 5 : 1))) void k_cg_update(
   }
   const double ri = r0 - alpha * qi;
   a.r[i] = live ? ri : r0;
-  const double si = BJ ? bj_apply_regs(m, ri) : ri / a.diag[i];
+  const double si = BJ == 2   ? bj_apply_regs_r32(m, ri)
+                    : BJ == 1 ? bj_apply_regs(m, ri)
+                              : ri / a.diag[i];
   a.s[i] = si;
   double v[2] = {wave_sum(ri * si), wave_sum(ri * ri)};
   if ((threadIdx.x & 63) == 0) {
@@ -222,6 +225,9 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
 // inverse (33 stored diagonals) in LDS once and applies it to that block of every subvolume
 // (wave w takes subvolumes w, w + kVecWaves, ...).  Same operations as
 // k_cg_update, so the results are the same bit for bit.
+// R32: the mixed solver's block-Jacobi (r broadcast in FP32), as
+// k_cg_update<2>.
+template <bool R32>
 __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
     UpdArgs a, int nblk1, int n_sub) {
   __shared__ float mt[DS_BJ_SIZE];
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
     if (a.only_x) continue;
     const double ri = a.r[i] - alpha * update_q(a, I, sv, lane, i);
     a.r[i] = ri;
-    const double si = bj_apply(mt, ri);
+    const double si = (R32 ? bj_apply_r32(mt, ri) : bj_apply(mt, ri));
     a.s[i] = si;
     const double v0 = wave_sum(ri * si), v1 = wave_sum(ri * ri);
     if (lane == 0) {
@@ -462,20 +468,29 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
   }
   const int nblk1 = p->nblk / p->n_sub;
   if (a.minv && p->shared_geometry && p->n_sub > 1 && p->minv_mod == nblk1) {
-    hipLaunchKernelGGL(k_cg_update_shared, dim3(nblk1), dim3(64 * kVecWaves),
-                       0, st, a, nblk1, p->n_sub);
+    if (mixed)
+      hipLaunchKernelGGL(k_cg_update_shared<true>, dim3(nblk1),
+                         dim3(64 * kVecWaves), 0, st, a, nblk1, p->n_sub);
+    else
+      hipLaunchKernelGGL(k_cg_update_shared<false>, dim3(nblk1),
+                         dim3(64 * kVecWaves), 0, st, a, nblk1, p->n_sub);
   } else {
     const dim3 g((p->nblk + kVecWaves - 1) / kVecWaves), b(64 * kVecWaves);
-    const bool bj = a.minv && !only_x, w = a.wvec, sy = a.sub_sym;
+    // block-Jacobi mode: 0 Jacobi (or x only), 1 FP64 broadcast of r,
+    // 2 FP32 broadcast of r (mixed solver)
+    const int bj = (a.minv && !only_x) ? (mixed ? 2 : 1) : 0;
+    const bool w = a.wvec, sy = a.sub_sym;
 #define DS_UPD(B, W_, S) \
   hipLaunchKernelGGL((k_cg_update<B, W_, S>), g, b, 0, st, a)
-    if (bj) {
-      if (w) { if (sy) DS_UPD(true, true, true); else DS_UPD(true, true, false); }
-      else { if (sy) DS_UPD(true, false, true); else DS_UPD(true, false, false); }
-    } else {
-      if (w) { if (sy) DS_UPD(false, true, true); else DS_UPD(false, true, false); }
-      else { if (sy) DS_UPD(false, false, true); else DS_UPD(false, false, false); }
-    }
+#define DS_UPD_B(B)                                        \
+  do {                                                     \
+    if (w) { if (sy) DS_UPD(B, true, true); else DS_UPD(B, true, false); } \
+    else { if (sy) DS_UPD(B, false, true); else DS_UPD(B, false, false); } \
+  } while (0)
+    if (bj == 2) DS_UPD_B(2);
+    else if (bj == 1) DS_UPD_B(1);
+    else DS_UPD_B(0);
+#undef DS_UPD_B
 #undef DS_UPD
   }
   DS_TRY(hipGetLastError());

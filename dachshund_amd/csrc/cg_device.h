@@ -179,6 +179,39 @@ __device__ __forceinline__ double bj_apply_regs(const float (&m)[DS_BJ_ROWS],
   return s;
 }
 
+// bj_apply with r rounded to FP32 before the shuffles (one 32-bit shuffle
+// per term instead of two; products and sum in FP64, same order): for the
+// mixed solver, whose operator is FP32 already.
+__device__ __forceinline__ double bj_apply_r32(const float* __restrict__ mt,
+                                               double r) {
+  const int lane = threadIdx.x & 63;
+  const float rf = (float)r;
+  double s = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < DS_BS; ++k) {
+    const int j = (lane + k) & (DS_BS - 1);
+    const float m = k <= DS_BS / 2 ? mt[k * DS_BS + lane]
+                                   : mt[(DS_BS - k) * DS_BS + j];
+    s = fma((double)m, (double)__shfl(rf, j, 64), s);
+  }
+  return s;
+}
+
+// The same with the lane's 33 stored values in registers.
+__device__ __forceinline__ double bj_apply_regs_r32(
+    const float (&m)[DS_BJ_ROWS], double r) {
+  const int lane = threadIdx.x & 63;
+  const float rf = (float)r;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < DS_BS; ++k) {
+    const int j = (lane + k) & (DS_BS - 1);
+    const float mk = k <= DS_BS / 2 ? m[k] : __shfl(m[DS_BS - k], j, 64);
+    s = fma((double)mk, (double)__shfl(rf, j, 64), s);
+  }
+  return s;
+}
+
 // lib/pcg.h:35-37: alpha = delta / (d . q).
 // (Also ends a mixed solver's true-residual request: every reader of
 // want_true in the step that asked for it has run by now.)
