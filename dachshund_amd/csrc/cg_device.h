@@ -179,37 +179,62 @@ __device__ __forceinline__ double bj_apply_regs(const float (&m)[DS_BJ_ROWS],
   return s;
 }
 
-// bj_apply with r rounded to FP32 before the shuffles (one 32-bit shuffle
-// per term instead of two; products and sum in FP64, same order): for the
-// mixed solver, whose operator is FP32 already.
+// Whole-wave rotations by one lane (DPP wave_rol:1 / wave_ror:1): lane i
+// receives lane i + 1, resp. i - 1 (mod 64).  All 64 lanes must be active.
+__device__ __forceinline__ float wave_rol1(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF,
+                                  false));
+}
+__device__ __forceinline__ double wave_ror1(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x13C, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x13C, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// bj_apply with r rounded to FP32 (the mixed solver, whose operator is FP32
+// already), the same terms in the same order (k = 0..63), products and sum
+// in FP64, the cross-lane moves on the VALU instead of LDS shuffles:
+// k <= 32: the stored row k times r_{i+k}, from a copy of r rotated one
+// lane per step; k > 32: the term is P_{k'}(j) = M_{j,(j+k') mod 64} r_j of
+// lane j = i - k' (k' = 64 - k), accumulated Horner-fashion over lanes:
+// A_31 = (the k <= 32 sum, rotated left by 31) + P_31, then
+// A_{k'} = ror1(A_{k'+1}) + P_{k'}, and lane i's result is ror1(A_1).
+// (Every product of two floats is exact in FP64, so each step rounds once,
+// as the fma of the sequential form does.)  m(k): the lane's stored value of
+// diagonal row k.
+template <class Row>
+__device__ __forceinline__ double bj_apply_r32_rot(Row m, double r) {
+  const int lane = threadIdx.x & 63;
+  const float rf = (float)r;
+  float rr = rf;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k <= DS_BS / 2; ++k) {
+    s = fma((double)m(k), (double)rr, s);
+    if (k < DS_BS / 2) rr = wave_rol1(rr);
+  }
+  double acc = __shfl(s, (lane + DS_BS / 2 - 1) & (DS_BS - 1), 64) +
+               (double)m(DS_BS / 2 - 1) * (double)rf;
+#pragma unroll
+  for (int kp = DS_BS / 2 - 2; kp >= 1; --kp)
+    acc = wave_ror1(acc) + (double)m(kp) * (double)rf;
+  return wave_ror1(acc);
+}
+
+// ... with the stored rows read from mt (LDS or global), or held in
+// registers (loaded by load_bj_diagonals).
 __device__ __forceinline__ double bj_apply_r32(const float* __restrict__ mt,
                                                double r) {
   const int lane = threadIdx.x & 63;
-  const float rf = (float)r;
-  double s = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < DS_BS; ++k) {
-    const int j = (lane + k) & (DS_BS - 1);
-    const float m = k <= DS_BS / 2 ? mt[k * DS_BS + lane]
-                                   : mt[(DS_BS - k) * DS_BS + j];
-    s = fma((double)m, (double)__shfl(rf, j, 64), s);
-  }
-  return s;
+  return bj_apply_r32_rot([&](int k) { return mt[k * DS_BS + lane]; }, r);
 }
 
-// The same with the lane's 33 stored values in registers.
 __device__ __forceinline__ double bj_apply_regs_r32(
     const float (&m)[DS_BJ_ROWS], double r) {
-  const int lane = threadIdx.x & 63;
-  const float rf = (float)r;
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < DS_BS; ++k) {
-    const int j = (lane + k) & (DS_BS - 1);
-    const float mk = k <= DS_BS / 2 ? m[k] : __shfl(m[DS_BS - k], j, 64);
-    s = fma((double)mk, (double)__shfl(rf, j, 64), s);
-  }
-  return s;
+  return bj_apply_r32_rot([&](int k) { return m[k]; }, r);
 }
 
 // lib/pcg.h:35-37: alpha = delta / (d . q).
