@@ -11,12 +11,18 @@
 // FP32), so PCG's theory is unchanged; solves stop on the same FP64 true
 // residual.
 //
-// k_bj_build: one 64-thread workgroup per target block; thread i builds row
-// i of the block A_ij = w_i S_ij w_j + delta_ij (W form) or S_ij + N_i
-// delta_ij (SN form) with the FP64 table interpolant of
-// lib/signal_covar.h:53-77 (padding rows: identity), then the block is
-// inverted in LDS by in-place Gauss-Jordan elimination (no pivoting: the
-// block is SPD), symmetrised and stored by diagonals (bj_apply's layout):
+// k_bj_build: one wave per target block; lane i builds row i of the block
+// A_ij = w_i S_ij w_j + delta_ij (W form) or S_ij + N_i delta_ij (SN form)
+// with the FP64 table interpolant of lib/signal_covar.h:53-77 (padding
+// rows: identity) and keeps it in registers.  The block is inverted by the
+// in-place Gauss-Jordan sweep (no pivoting: the block is SPD).  Step k:
+// lane k publishes its row and the scaled pivot row (row k / a_kk with
+// entry k = 1 / a_kk) in LDS; every lane i != k updates its row with the
+// factor a_ik, which the sweep's structure gives from the published row
+// (a_ik = a_ki while both are unswept, a_ik = -a_ki once i < k is swept),
+// so no lane indexes its own registers by k.  Lane k takes the scaled row.
+// The inverse goes through LDS once as FP32 to be symmetrised, rounded to
+// FP32 and stored by diagonals (bj_apply's layout):
 // minv[I][k][i] = M_{i,(i+k) mod 64}, k = 0..32.
 // COMPILE WITH -ffp-contract=off.
 #include "cg_device.h"
@@ -38,11 +44,15 @@ struct BjArgs {
 };
 
 __global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
-  __shared__ double m[DS_BS][DS_BS + 1];
+  __shared__ double raw[DS_BS];        // step k: row k as it stands
+  __shared__ double piv[DS_BS];        // step k: the scaled pivot row
+  __shared__ float mt[DS_BS][DS_BS + 1];
   const int I = blockIdx.x, i = threadIdx.x;
   const double4 cv = a.sub_cv[a.blk_sub[I]];
   const int nreal = a.blk_nreal[I];
   const double4 pi = a.pos[I * DS_BS + i];
+  double m[DS_BS];
+#pragma unroll
   for (int j = 0; j < DS_BS; ++j) {
     double v = (i == j) ? 1.0 : 0.0;
     if (i < nreal && j < nreal) {
@@ -60,27 +70,37 @@ __global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
       if (a.form == DS_FORM_W) v = pi.w * S * pj.w + v;
       else v = S + (i == j ? pi.w : 0.0);
     }
-    m[i][j] = v;
+    m[j] = v;
   }
-  __syncthreads();
   for (int k = 0; k < DS_BS; ++k) {
     if (i == k) {
-      const double p = 1.0 / m[k][k];
-      m[k][k] = 1.0;
-      for (int j = 0; j < DS_BS; ++j) m[k][j] *= p;
+#pragma unroll
+      for (int j = 0; j < DS_BS; ++j) raw[j] = m[j];
+      const double p = 1.0 / raw[k];
+#pragma unroll
+      for (int j = 0; j < DS_BS; ++j) piv[j] = m[j] * p;
+      piv[k] = p;
     }
     __syncthreads();
-    if (i != k) {
-      const double f = m[i][k];
-      m[i][k] = 0.0;
-      for (int j = 0; j < DS_BS; ++j) m[i][j] -= f * m[k][j];
+    // a_ik from the published row k; lane k: factor -1 on a zeroed row,
+    // which leaves it the scaled pivot row exactly
+    const double f = i == k ? -1.0 : (i < k ? -raw[i] : raw[i]);
+    const bool own = i == k;
+#pragma unroll
+    for (int j = 0; j < DS_BS; ++j) {
+      const double base = (own || j == k) ? 0.0 : m[j];
+      m[j] = fma(-f, piv[j], base);
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int j = 0; j < DS_BS; ++j) mt[i][j] = (float)m[j];
+  __syncthreads();
   float* out = a.minv + (size_t)I * DS_BJ_SIZE;
   for (int k = 0; k < DS_BJ_ROWS; ++k) {
     const int j = (i + k) & (DS_BS - 1);
-    out[k * DS_BS + i] = (float)(0.5 * (m[i][j] + m[j][i]));
+    out[k * DS_BS + i] =
+        (float)(0.5 * ((double)mt[i][j] + (double)mt[j][i]));
   }
 }
 

@@ -16,8 +16,8 @@
 // (layout [in][out]), activations rounded to BF16 (optional), output
 // t = x_i sqrt(var_s).
 //
-// Kernels: k_knn_scan / k_knn_select (neighbour search over the problem's
-// culled tile lists, once per problem), k_ws_features, k_fwd (inference), k_fwd_bwd (training forward + per-pixel
+// Kernels: k_knn (neighbour search over the problem's culled tile lists,
+// once per problem), k_ws_features, k_fwd (inference), k_fwd_bwd (training forward + per-pixel
 // back-propagated deltas), k_wgrad (deterministic chunked weight-gradient
 // sums), k_reduce_chunks, k_adam, k_gather_pred.
 #include <cmath>
@@ -49,8 +49,6 @@ struct KnnArgs {
   const float4* tgt;
   const int* tptr;
   const int* tidx;
-  float2* cand;           // npad x KNN_CAP kept candidates {d2, j}
-  int* cnt;               // kept candidates per row
   int* nbr;               // npad x K neighbour rows, -1 = none
   unsigned short* feat;   // npad x WS_FEAT
   int n, form;
@@ -61,12 +59,22 @@ constexpr int KNN_BINS = 64;    // histogram bins per refinement level
 constexpr int KNN_LEVELS = 3;   // refinement levels
 constexpr int KNN_CAP = 64;     // candidates kept per row for the selection
 
+// d2 of row (frame-relative li) and source slot j of list entry e: the
+// table argument in FP32 with direct differences in the tile frame.  The
+// scan and the selection both call this, so a candidate's d2 is the same
+// number wherever it is evaluated.
+__device__ __forceinline__ float knn_d2(const float4& li, const float4& o,
+                                        const float4& pj) {
+  const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
+  const float dX = xi - pj.x, dY = yi - pj.y, dZ = zi - pj.z;
+  return fmaf(dZ, dZ, fmaf(dY, dY, dX * dX));
+}
+
 // Visit every candidate neighbour of target `row` (block I): slots of the
-// block's listed source tiles below n, other than row itself, whose table
-// argument d2 (FP32, direct differences in the tile frame) is inside the
-// cutoff.  Every pass evaluates d2 with the same explicit operations.  The
-// tile data is wave-uniform and read-only here (restrict: scalar loads);
-// tile slots exist up to npad, those past n are padding.
+// block's listed source tiles below n, other than row itself, whose d2 is
+// inside the cutoff; visit(d2, e, jj) gets the list entry and the slot in
+// its tile.  The tile data is wave-uniform and read-only here (restrict:
+// scalar loads); tile slots exist up to npad, those past n are padding.
 template <class F>
 __device__ __forceinline__ void knn_scan(const KnnArgs& a,
                                          const float4* __restrict__ loc,
@@ -78,18 +86,26 @@ __device__ __forceinline__ void knn_scan(const KnnArgs& a,
   for (int e = kb; e < ke; ++e) {
     const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
     const float4 o = toff[e];
-    const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
 #pragma unroll
     for (int jj = 0; jj < DS_TS; ++jj) {
       const int j = j0 + jj;
-      const float4 pj = loc[j];
-      const float dX = xi - pj.x, dY = yi - pj.y, dZ = zi - pj.z;
-      const float d2 = fmaf(dZ, dZ, fmaf(dY, dY, dX * dX));
-      if (j < a.n && j != row && d2 < a.cutoff) visit(d2, j);
+      const float d2 = knn_d2(li, o, loc[j]);
+      if (j < a.n && j != row && d2 < a.cutoff) visit(d2, e, jj);
     }
   }
 }
 
+// Level 0 bins d2 logarithmically, eight bins per octave over [1, 256)
+// (the cutoff is 199): the bin is the float's exponent and top three
+// mantissa bits; d2 < 1 falls in bin 0.  Levels 1 and 2 split the crossing
+// bin of the level above into KNN_BINS equal bins.
+__device__ __forceinline__ int knn_bin0(float d2) {
+  const int b = (int)(__float_as_uint(d2) >> 20) - (127 << 3);
+  return min(max(b, 0), KNN_BINS - 1);
+}
+__device__ __forceinline__ float knn_bin0_lower(int b) {
+  return b <= 0 ? 0.0f : __uint_as_float((unsigned)(b + (127 << 3)) << 20);
+}
 __device__ __forceinline__ int knn_bin(float d2, float lo, float scale) {
   const int b = (int)((d2 - lo) * scale);
   return min(max(b, 0), KNN_BINS - 1);
@@ -100,22 +116,27 @@ __device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
   return d < e || (d == e && (unsigned)j < (unsigned)k);
 }
 
-// Neighbour search, part 1: keep, per target row, a small candidate set
-// that surely holds its K nearest.  A lane never inserts into a top-K list
+// The K nearest neighbours of each row in (d2, index) order, one wave per
+// target block, one lane per row.  A lane never inserts into a top-K list
 // while scanning (the whole wave would run the insertion whenever any lane
-// accepts a candidate).  Instead histogram passes find a threshold: level 0
-// bins d2 over [0, cutoff); the bin where the cumulative count reaches K is
-// split into KNN_BINS sub-bins at the next level while the candidates below
-// the threshold would exceed KNN_CAP.  The accepted set {bin path <=
-// crossing path} is closed downward in d2 and holds at least K candidates
-// (or all of them), ties at the threshold included; an append pass stores
-// it.  More than KNN_CAP survive only when more than about KNN_CAP - K
-// candidates share one FP32 distance after three refinements; then the
-// first KNN_CAP in scan order are kept.
-__global__ __launch_bounds__(64) void k_knn_scan(
+// accepts a candidate).  Histogram passes find a threshold instead: level 0
+// bins d2 logarithmically; the bin where the cumulative count reaches K is
+// split into KNN_BINS bins at the next level while the candidates up to it
+// would exceed KNN_CAP.  The accepted set {bin path <= crossing path} is
+// closed downward in d2 and holds at least K candidates (or all of them),
+// ties at the threshold included.  An append pass keeps it in LDS (the
+// list entry and tile slot of each candidate, so d2 is recomputed rather
+// than stored), then each lane selects its K nearest (replace-worst, then
+// an odd-even transposition sort).  More than KNN_CAP survive only when
+// more than about KNN_CAP - K candidates share one FP32 distance after
+// three refinements; then the first KNN_CAP in scan order are kept.
+template <int K>
+__global__ __launch_bounds__(64) void k_knn(
     const float4* __restrict__ loc, const float4* __restrict__ toff,
-    KnnArgs a, int K) {
-  __shared__ unsigned hist[KNN_BINS * 64];   // [bin][lane], 16 KB
+    KnnArgs a) {
+  // [bin][lane] histograms, then [slot][lane] candidates (e * 16 + jj)
+  __shared__ unsigned buf[KNN_BINS * 64];   // 16 KB
+  static_assert(KNN_CAP <= KNN_BINS, "candidates reuse the histogram");
   const int lane = threadIdx.x;
   const int I = blockIdx.x;
   const int row = I * DS_BS + lane;
@@ -129,18 +150,20 @@ __global__ __launch_bounds__(64) void k_knn_scan(
     sc[L] = 0.0f;
     cb[L] = 0;
   }
-  sc[0] = (float)KNN_BINS / a.cutoff;
-  int nlev = 0;          // levels whose crossing bin is fixed
-  bool settled = false;
-  int below = 0;         // candidates strictly below the current domain
+  int nlev = 0;              // levels whose crossing bin is fixed
+  bool settled = row >= a.n; // padding rows keep no neighbours
+  int below = 0;             // candidates strictly below the current domain
 
+  auto bin_at = [&](float d2, int L) {
+    return L == 0 ? knn_bin0(d2) : knn_bin(d2, lo[L], sc[L]);
+  };
   // -1: below the domain of level p (accepted), 0: inside it, 1: above.
   auto classify = [&](float d2, int p) {
     int c = 0;
 #pragma unroll
     for (int L = 0; L < KNN_LEVELS; ++L) {
       if (L < p && c == 0) {
-        const int b = knn_bin(d2, lo[L], sc[L]);
+        const int b = bin_at(d2, L);
         c = b < cb[L] ? -1 : (b > cb[L] ? 1 : 0);
       }
     }
@@ -149,15 +172,15 @@ __global__ __launch_bounds__(64) void k_knn_scan(
 
   for (int p = 0; p < KNN_LEVELS; ++p) {
     if (!__any(!settled)) break;
-    for (int b = 0; b < KNN_BINS; ++b) hist[b * 64 + lane] = 0u;
-    knn_scan(a, loc, toff, I, row, li, [&](float d2, int j) {
-      if (!settled && classify(d2, p) == 0)   // lane-private column: the
-        atomicAdd(&hist[knn_bin(d2, lo[p], sc[p]) * 64 + lane], 1u);
-    });                                       // atomic returns nothing
+    for (int b = 0; b < KNN_BINS; ++b) buf[b * 64 + lane] = 0u;
+    knn_scan(a, loc, toff, I, row, li, [&](float d2, int, int) {
+      if (!settled && classify(d2, p) == 0)   // lane-private column
+        atomicAdd(&buf[bin_at(d2, p) * 64 + lane], 1u);
+    });
     if (settled) continue;
     int cum = below, cross = -1, before = below;
     for (int b = 0; b < KNN_BINS; ++b) {
-      const int h = (int)hist[b * 64 + lane];
+      const int h = (int)buf[b * 64 + lane];
       if (cross < 0 && cum + h >= K) {
         cross = b;
         before = cum;
@@ -171,37 +194,35 @@ __global__ __launch_bounds__(64) void k_knn_scan(
     }
     cb[p] = cross;
     nlev = p + 1;
-    const int take = before + (int)hist[cross * 64 + lane];
+    const int take = before + (int)buf[cross * 64 + lane];
     if (take <= KNN_CAP || p + 1 == KNN_LEVELS) {
       settled = true;
       continue;
     }
     below = before;
-    const float w = 1.0f / sc[p];
-    lo[p + 1] = lo[p] + (float)cross * w;
-    sc[p + 1] = (float)KNN_BINS / w;
+    float l0, l1;
+    if (p == 0) {
+      l0 = knn_bin0_lower(cross);
+      l1 = knn_bin0_lower(cross + 1);
+    } else {
+      const float w = 1.0f / sc[p];
+      l0 = lo[p] + (float)cross * w;
+      l1 = l0 + w;
+    }
+    lo[p + 1] = l0;
+    sc[p + 1] = (float)KNN_BINS / (l1 - l0);
   }
 
   // append: below the crossing path, or inside the final crossing bin (or
   // the whole final domain when it holds fewer than K)
-  float2* out = a.cand + (size_t)row * KNN_CAP;
   int cnt = 0;
-  knn_scan(a, loc, toff, I, row, li, [&](float d2, int j) {
-    if (classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
-      out[cnt] = make_float2(d2, __int_as_float(j));
+  knn_scan(a, loc, toff, I, row, li, [&](float d2, int e, int jj) {
+    if (row < a.n && classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
+      buf[cnt * 64 + lane] = ((unsigned)e << 4) | (unsigned)jj;
       ++cnt;
     }
   });
-  a.cnt[row] = row < a.n ? cnt : 0;
-}
 
-// Part 2: the K nearest kept candidates of each row in (d2, index) order
-// (replace-worst selection, then an odd-even transposition sort).
-template <int K>
-__global__ __launch_bounds__(64) void k_knn_select(KnnArgs a) {
-  const int row = blockIdx.x * DS_BS + threadIdx.x;
-  const int cnt = a.cnt[row];
-  const float2* c = a.cand + (size_t)row * KNN_CAP;
   float bd[K];
   int bi[K];
 #pragma unroll
@@ -212,9 +233,10 @@ __global__ __launch_bounds__(64) void k_knn_select(KnnArgs a) {
   float wd = __builtin_inff();
   int wj = -1;
   for (int s = 0; s < cnt; ++s) {
-    const float2 e = c[s];
-    const float d2 = e.x;
-    const int j = __float_as_int(e.y);
+    const unsigned c = buf[s * 64 + lane];
+    const int e = (int)(c >> 4);
+    const int j = a.tidx[e] * DS_TS + (int)(c & 15u);
+    const float d2 = knn_d2(li, toff[e], loc[j]);
     if (!knn_less(d2, j, wd, wj)) continue;
     bool done = false;
     float nd = -1.0f;
@@ -858,28 +880,16 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
   a.cutoff = (float)(p->cv.table_n - 1);
   if (!p->knn_nbr) {
     // The neighbour lists depend on the geometry only: built once per
-    // problem, like its tile lists.
+    // problem, like its tile lists.  Candidates are kept as list entry * 16
+    // + tile slot in 32 bits.
+    if (p->ntile_pairs >= (1LL << 28)) return hipErrorInvalidValue;
     int* nbr = nullptr;
     hipError_t e = hipMalloc(&nbr, sizeof(int) * (size_t)p->npad * WS_K);
     if (e != hipSuccess) return e;
-    e = hipMalloc(&a.cand, sizeof(float2) * (size_t)p->npad * KNN_CAP);
-    if (e == hipSuccess) e = hipMalloc(&a.cnt, sizeof(int) * p->npad);
     a.nbr = nbr;
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_knn_scan, dim3(p->nblk), dim3(64), 0, st,
-                         p->loc32, p->toff32, a, WS_K);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_knn_select<WS_K>, dim3(p->nblk), dim3(64), 0, st,
-                         a);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (a.cand) (void)hipFree(a.cand);
-    if (a.cnt) (void)hipFree(a.cnt);
-    a.cand = nullptr;
-    a.cnt = nullptr;
+    hipLaunchKernelGGL(k_knn<WS_K>, dim3(p->nblk), dim3(64), 0, st,
+                       p->loc32, p->toff32, a);
+    e = hipGetLastError();
     if (e != hipSuccess) {
       (void)hipFree(nbr);
       return e;

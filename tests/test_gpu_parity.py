@@ -447,3 +447,69 @@ def test_map_points_edge_cases(D, ora, case):
     s = D.sigma_m_noise_dom(npx, mc, sp)
     so = ora.sigma_m_noise_dom(npx, mc, c)
     assert np.all(np.abs(s - so) <= 1e-12 * so)
+
+
+def _cutoff_pairs(lp, ll, span=40):
+    """Pixel offsets (dy, dz) whose table argument, computed in the
+    reference's operation order (lib/wf_vector.cc:62-66,
+    lib/signal_covar.h:53-77; dx = 0), lies within 199 (1 +- span 2^-52):
+    found by stepping one coordinate an ulp at a time along the pure
+    perpendicular, pure parallel and a mixed direction."""
+    cp, cl = 1 / (2 * lp * lp), 1 / (2 * ll * ll)
+    dx_inv = 1.0 / (8.0 / 199)
+    lo, hi = 199.0 * (1 - span * 2.0 ** -52), 199.0 * (1 + span * 2.0 ** -52)
+
+    def xx_of(dy, dz):
+        return (cp * (0.0 * 0.0 + dy * dy) + cl * (dz * dz)) * dx_inv
+
+    out = set()
+    for frac in (0.0, 0.37, 1.0):
+        for sign in (1.0, -1.0):
+            dy, dz = np.sqrt(frac * 8.0 / cp), np.sqrt((1 - frac) * 8.0 / cl)
+            while True:
+                x = xx_of(dy, dz)
+                if (sign > 0 and x > hi) or (sign < 0 and x < lo):
+                    break
+                if lo <= x <= hi:
+                    out.add((dy, dz, x))
+                if frac == 1.0:
+                    dy = np.nextafter(dy, sign * np.inf)
+                else:
+                    dz = np.nextafter(dz, sign * np.inf)
+    return sorted(out, key=lambda t: t[2])
+
+
+@pytest.mark.parametrize("form", ["sn", "w"])
+@pytest.mark.parametrize("lp,ll", [(2.5, 2.1), (1.0, 1.0), (3.3, 1.7)])
+def test_matvec64_cutoff_decision(D, ora, form, lp, ll):
+    """Pairs whose table argument straddles the cutoff (argument 8, table
+    index 199) within +-62 ulps: the FP64 kernel's off-diagonal entries
+    (columns A e_j, read through one product since every pair is isolated
+    25 Mpc/h from the next) have the oracle's zero pattern exactly and its
+    values to 8 ulps.  The oracle's own decision is pinned by the
+    reference's rule: zero exactly when the argument reaches 199."""
+    pairs = _cutoff_pairs(lp, ll)
+    m = len(pairs)
+    pix = np.zeros((2 * m, 5))
+    for p, (dy, dz, _) in enumerate(pairs):
+        pix[2 * p, 0] = pix[2 * p + 1, 0] = 25.0 * p
+        pix[2 * p + 1, 1] = dy
+        pix[2 * p + 1, 2] = dz
+    rng = np.random.default_rng(m)
+    pix[:, 3] = rng.uniform(0.1, 0.3, 2 * m)
+    pix[:, 4] = rng.standard_normal(2 * m)
+    f = FORMS[form]
+    fp = ora.to_form(pix, f)
+    c = ora.covar(0.05, lp, ll)
+    wf, _ = _problem(D, form, pix, (0.05, lp, ll))
+    xs = np.array([t[2] for t in pairs])
+    for side in (0, 1):
+        a = np.zeros(2 * m)
+        a[side::2] = 1.0
+        rows = slice(1 - side, None, 2)
+        ref = ora.A_product(f, fp, c, a)[rows]
+        got = wf.A_product(a)[rows]
+        assert np.array_equal(ref == 0.0, xs >= 199.0)
+        assert np.array_equal(got == 0.0, ref == 0.0)
+        assert np.all(np.abs(got - ref) <= 8 * np.spacing(np.abs(ref)))
+    assert 0 < np.sum(xs >= 199.0) < m
