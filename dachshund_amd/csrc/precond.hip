@@ -11,18 +11,19 @@
 // FP32), so PCG's theory is unchanged; solves stop on the same FP64 true
 // residual.
 //
-// k_bj_build: one wave per target block; lane i builds row i of the block
-// A_ij = w_i S_ij w_j + delta_ij (W form) or S_ij + N_i delta_ij (SN form)
-// with the FP64 table interpolant of lib/signal_covar.h:53-77 (padding
-// rows: identity) and keeps it in registers.  The block is inverted by the
-// in-place Gauss-Jordan sweep (no pivoting: the block is SPD).  Step k:
-// lane k publishes its row and the scaled pivot row (row k / a_kk with
-// entry k = 1 / a_kk) in LDS; every lane i != k updates its row with the
-// factor a_ik, which the sweep's structure gives from the published row
-// (a_ik = a_ki while both are unswept, a_ik = -a_ki once i < k is swept),
-// so no lane indexes its own registers by k.  Lane k takes the scaled row.
-// The inverse goes through LDS once as FP32 to be symmetrised, rounded to
-// FP32 and stored by diagonals (bj_apply's layout):
+// k_bj_build: one 128-thread workgroup per target block; row i of the
+// block is held in registers by threads i (columns 0..31) and 64 + i
+// (columns 32..63), built as A_ij = w_i S_ij w_j + delta_ij (W form) or
+// S_ij + N_i delta_ij (SN form) with the FP64 table interpolant of
+// lib/signal_covar.h:53-77 (padding rows: identity).  The block is inverted
+// by the in-place Gauss-Jordan sweep (no pivoting: the block is SPD).  Step
+// k: the two threads of row k publish it in LDS (entry k replaced by 1, the
+// pivot kept aside), and every row i != k is updated with the factor a_ik,
+// which the sweep's structure gives from the published row (a_ik = a_ki
+// while both are unswept, a_ik = -a_ki once i < k is swept), so no thread
+// indexes its registers by k; row k becomes the scaled pivot row.  The
+// inverse goes through LDS once as FP32 to be symmetrised, rounded to FP32
+// and stored by diagonals (bj_apply's layout):
 // minv[I][k][i] = M_{i,(i+k) mod 64}, k = 0..32.
 // COMPILE WITH -ffp-contract=off.
 #include "cg_device.h"
@@ -43,17 +44,19 @@ struct BjArgs {
   int table_n, form;
 };
 
-__global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
-  __shared__ double raw[DS_BS];        // step k: row k as it stands
-  __shared__ double piv[DS_BS];        // step k: the scaled pivot row
+__global__ __launch_bounds__(128) void k_bj_build(BjArgs a) {
+  constexpr int HC = DS_BS / 2;        // columns per thread
+  __shared__ double raw[DS_BS + 1];    // step k: row k (entry k = 1), pivot
   __shared__ float mt[DS_BS][DS_BS + 1];
-  const int I = blockIdx.x, i = threadIdx.x;
+  const int I = blockIdx.x, t = threadIdx.x;
+  const int i = t & (DS_BS - 1), h = t >> 6, c0 = h * HC;
   const double4 cv = a.sub_cv[a.blk_sub[I]];
   const int nreal = a.blk_nreal[I];
   const double4 pi = a.pos[I * DS_BS + i];
-  double m[DS_BS];
+  double m[HC];
 #pragma unroll
-  for (int j = 0; j < DS_BS; ++j) {
+  for (int c = 0; c < HC; ++c) {
+    const int j = c0 + c;
     double v = (i == j) ? 1.0 : 0.0;
     if (i < nreal && j < nreal) {
       const double4 pj = a.pos[I * DS_BS + j];
@@ -70,34 +73,37 @@ __global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
       if (a.form == DS_FORM_W) v = pi.w * S * pj.w + v;
       else v = S + (i == j ? pi.w : 0.0);
     }
-    m[j] = v;
+    m[c] = v;
   }
   for (int k = 0; k < DS_BS; ++k) {
     if (i == k) {
 #pragma unroll
-      for (int j = 0; j < DS_BS; ++j) raw[j] = m[j];
-      const double p = 1.0 / raw[k];
-#pragma unroll
-      for (int j = 0; j < DS_BS; ++j) piv[j] = m[j] * p;
-      piv[k] = p;
+      for (int c = 0; c < HC; ++c) raw[c0 + c] = m[c];
+      if ((k >> 5) == h) {   // this thread holds the pivot
+        raw[DS_BS] = raw[k];
+        raw[k] = 1.0;
+      }
     }
     __syncthreads();
-    // a_ik from the published row k; lane k: factor -1 on a zeroed row,
-    // which leaves it the scaled pivot row exactly
-    const double f = i == k ? -1.0 : (i < k ? -raw[i] : raw[i]);
-    const bool own = i == k;
+    const double p = 1.0 / raw[DS_BS];
+    const double f = i < k ? -raw[i] : raw[i];
 #pragma unroll
-    for (int j = 0; j < DS_BS; ++j) {
-      const double base = (own || j == k) ? 0.0 : m[j];
-      m[j] = fma(-f, piv[j], base);
+    for (int c = 0; c < HC; ++c) {
+      const double pr = raw[c0 + c] * p;
+      if (c0 + c == k) m[c] = -f * pr;   // wave-uniform
+      else m[c] = fma(-f, pr, m[c]);
+    }
+    if (i == k) {
+#pragma unroll
+      for (int c = 0; c < HC; ++c) m[c] = raw[c0 + c] * p;
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < DS_BS; ++j) mt[i][j] = (float)m[j];
+  for (int c = 0; c < HC; ++c) mt[i][c0 + c] = (float)m[c];
   __syncthreads();
   float* out = a.minv + (size_t)I * DS_BJ_SIZE;
-  for (int k = 0; k < DS_BJ_ROWS; ++k) {
+  for (int k = h; k < DS_BJ_ROWS; k += 2) {
     const int j = (i + k) & (DS_BS - 1);
     out[k * DS_BS + i] =
         (float)(0.5 * ((double)mt[i][j] + (double)mt[j][i]));
@@ -127,7 +133,8 @@ hipError_t bj_build(ds_problem* p, hipStream_t st) {
   a.dx_inv = p->cv.dx_inv;
   a.table_n = p->cv.table_n;
   a.form = p->form;
-  hipLaunchKernelGGL(k_bj_build, dim3(p->minv_mod), dim3(DS_BS), 0, st, a);
+  hipLaunchKernelGGL(k_bj_build, dim3(p->minv_mod), dim3(2 * DS_BS), 0, st,
+                     a);
   return hipGetLastError();
 }
 

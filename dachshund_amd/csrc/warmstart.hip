@@ -57,12 +57,11 @@ struct KnnArgs {
 
 constexpr int KNN_BINS = 64;    // histogram bins per refinement level
 constexpr int KNN_LEVELS = 3;   // refinement levels
-constexpr int KNN_CAP = 64;     // candidates kept per row for the selection
+constexpr int KNN_CAP = 48;     // candidates kept per row for the selection
 
 // d2 of row (frame-relative li) and source slot j of list entry e: the
-// table argument in FP32 with direct differences in the tile frame.  The
-// scan and the selection both call this, so a candidate's d2 is the same
-// number wherever it is evaluated.
+// table argument in FP32 with direct differences in the tile frame, the
+// same operations in every pass.
 __device__ __forceinline__ float knn_d2(const float4& li, const float4& o,
                                         const float4& pj) {
   const float xi = li.x + o.x, yi = li.y + o.y, zi = li.z + o.z;
@@ -70,27 +69,61 @@ __device__ __forceinline__ float knn_d2(const float4& li, const float4& o,
   return fmaf(dZ, dZ, fmaf(dY, dY, dX * dX));
 }
 
-// Visit every candidate neighbour of target `row` (block I): slots of the
-// block's listed source tiles below n, other than row itself, whose d2 is
-// inside the cutoff; visit(d2, e, jj) gets the list entry and the slot in
-// its tile.  The tile data is wave-uniform and read-only here (restrict:
-// scalar loads); tile slots exist up to npad, those past n are padding.
+// Visit every slot of the block's listed source tiles whose entry passes
+// the distance filter: visit(d2, j, ok) with ok = the slot is a candidate
+// neighbour of target `row` (below n, not row itself, d2 inside the cutoff,
+// row itself below n).  The visitor runs for every slot of a visited entry
+// so that it can be written without branches.  The tile data is
+// wave-uniform and read-only here (restrict: scalar loads).
+//
+// Filter: an entry's pairs are all at least lb = |o| - r_blk - r_tile
+// apart (o: block frame minus tile frame, r_blk: the largest |row| in the
+// block frame, r_tile: the largest |source| in the tile frame, all in the
+// scaled units where distance^2 = d2), less a margin that covers FP32
+// rounding; entries with lb in [lo, hi) are visited.  With hi = r, every
+// pair with computed d2 < r^2 lies in a visited entry; with lo = r, none
+// of the visited pairs has d2 < r^2.
 template <class F>
 __device__ __forceinline__ void knn_scan(const KnnArgs& a,
                                          const float4* __restrict__ loc,
                                          const float4* __restrict__ toff,
                                          int I, int row, const float4& li,
+                                         float r_blk, float lo, float hi,
                                          F&& visit) {
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]);
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  const bool live = row < a.n;
+  const bool all = lo == -__builtin_inff() && hi == __builtin_inff();
   for (int e = kb; e < ke; ++e) {
     const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
     const float4 o = toff[e];
+    // the tile's 16 records in one batch of scalar loads
+    float4 pj[DS_TS];
 #pragma unroll
-    for (int jj = 0; jj < DS_TS; ++jj) {
-      const int j = j0 + jj;
-      const float d2 = knn_d2(li, o, loc[j]);
-      if (j < a.n && j != row && d2 < a.cutoff) visit(d2, e, jj);
+    for (int jj = 0; jj < DS_TS; ++jj) pj[jj] = loc[j0 + jj];
+    if (!all) {
+      float rt2 = pj[0].w;
+#pragma unroll
+      for (int jj = 1; jj < DS_TS; ++jj) rt2 = fmaxf(rt2, pj[jj].w);
+      const float on = sqrtf(fmaf(o.z, o.z, fmaf(o.y, o.y, o.x * o.x)));
+      const float rt = sqrtf(rt2);
+      const float lb = on - r_blk - rt - (1e-3f + 1e-4f * (on + r_blk + rt));
+      if (!(lb >= lo && lb < hi)) continue;
+    }
+    if (j0 + DS_TS <= a.n) {   // no padding slot in this tile
+#pragma unroll
+      for (int jj = 0; jj < DS_TS; ++jj) {
+        const int j = j0 + jj;
+        const float d2 = knn_d2(li, o, pj[jj]);
+        visit(d2, j, live && j != row && d2 < a.cutoff);
+      }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < DS_TS; ++jj) {
+        const int j = j0 + jj;
+        const float d2 = knn_d2(li, o, pj[jj]);
+        visit(d2, j, live && j < a.n && j != row && d2 < a.cutoff);
+      }
     }
   }
 }
@@ -124,19 +157,25 @@ __device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
 // split into KNN_BINS bins at the next level while the candidates up to it
 // would exceed KNN_CAP.  The accepted set {bin path <= crossing path} is
 // closed downward in d2 and holds at least K candidates (or all of them),
-// ties at the threshold included.  An append pass keeps it in LDS (the
-// list entry and tile slot of each candidate, so d2 is recomputed rather
-// than stored), then each lane selects its K nearest (replace-worst, then
-// an odd-even transposition sort).  More than KNN_CAP survive only when
-// more than about KNN_CAP - K candidates share one FP32 distance after
-// three refinements; then the first KNN_CAP in scan order are kept.
+// ties at the threshold included.  An append pass keeps it in LDS as
+// {d2, j}; each kept candidate's rank in (d2, j) order among the row's kept
+// ones is its position in the neighbour list (ranks >= K are dropped).  The
+// first histogram pass and, when every lane settled at level 0, the append
+// pass run without branches (lane-private LDS columns; rejected slots add 0
+// to the histogram, resp. store into a spare row).  More than KNN_CAP
+// survive only when more than about KNN_CAP - K candidates share one FP32
+// distance after three refinements; then the first KNN_CAP in scan order
+// are kept.
 template <int K>
 __global__ __launch_bounds__(64) void k_knn(
     const float4* __restrict__ loc, const float4* __restrict__ toff,
     KnnArgs a) {
-  // [bin][lane] histograms, then [slot][lane] candidates (e * 16 + jj)
-  __shared__ unsigned buf[KNN_BINS * 64];   // 16 KB
-  static_assert(KNN_CAP <= KNN_BINS, "candidates reuse the histogram");
+  // [bin][lane] histograms, then [slot][lane] candidates {d2, j} (the row
+  // KNN_CAP takes the rejected stores of the branch-free append)
+  constexpr int ROWS = KNN_BINS > 2 * (KNN_CAP + 1) ? KNN_BINS
+                                                    : 2 * (KNN_CAP + 1);
+  __shared__ unsigned buf[ROWS * 64];
+  uint2* cand = reinterpret_cast<uint2*>(buf);
   const int lane = threadIdx.x;
   const int I = blockIdx.x;
   const int row = I * DS_BS + lane;
@@ -170,13 +209,39 @@ __global__ __launch_bounds__(64) void k_knn(
     return c;
   };
 
+  const float kInf = __builtin_inff();
+  // the largest |row| of the block (block frame, scaled units)
+  float rb2 = fmaf(li.z, li.z, fmaf(li.y, li.y, li.x * li.x));
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+    rb2 = fmaxf(rb2, __shfl_xor(rb2, off, 64));
+  const float r_blk = sqrtf(rb2);
+
   for (int p = 0; p < KNN_LEVELS; ++p) {
     if (!__any(!settled)) break;
     for (int b = 0; b < KNN_BINS; ++b) buf[b * 64 + lane] = 0u;
-    knn_scan(a, loc, toff, I, row, li, [&](float d2, int, int) {
-      if (!settled && classify(d2, p) == 0)   // lane-private column
-        atomicAdd(&buf[bin_at(d2, p) * 64 + lane], 1u);
-    });
+    if (p == 0) {
+      // Level 0 in two parts: first the entries that can hold d2 < R0^2;
+      // a lane that already finds K candidates there has its crossing bin
+      // below kNearBins, where the counts are complete.  The other entries
+      // are scanned only when some lane did not.
+      constexpr float R0 = 8.0f;        // d2 < 64: bins below kNearBins
+      constexpr int kNearBins = 48;
+      auto add = [&](float d2, int, bool ok) {
+        atomicAdd(&buf[knn_bin0(d2) * 64 + lane], ok ? 1u : 0u);
+      };
+      knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, R0, add);
+      int near = 0;
+      for (int b = 0; b < kNearBins; ++b) near += (int)buf[b * 64 + lane];
+      if (__any(!settled && near < K))
+        knn_scan(a, loc, toff, I, row, li, r_blk, R0, kInf, add);
+    } else {
+      knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, kInf,
+               [&](float d2, int, bool ok) {
+                 if (ok && !settled && classify(d2, p) == 0)
+                   atomicAdd(&buf[bin_at(d2, p) * 64 + lane], 1u);
+               });
+    }
     if (settled) continue;
     int cum = below, cross = -1, before = below;
     for (int b = 0; b < KNN_BINS; ++b) {
@@ -216,62 +281,61 @@ __global__ __launch_bounds__(64) void k_knn(
   // append: below the crossing path, or inside the final crossing bin (or
   // the whole final domain when it holds fewer than K)
   int cnt = 0;
-  knn_scan(a, loc, toff, I, row, li, [&](float d2, int e, int jj) {
-    if (row < a.n && classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
-      buf[cnt * 64 + lane] = ((unsigned)e << 4) | (unsigned)jj;
-      ++cnt;
-    }
-  });
+  if (__all(nlev <= 1)) {
+    const int tb = nlev == 0 ? KNN_BINS - 1 : cb[0];
+    // kept candidates have d2 below the upper edge of bin tb
+    float rk = row >= a.n ? 0.0f
+                          : (nlev == 0 ? kInf : sqrtf(knn_bin0_lower(tb + 1)));
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+      rk = fmaxf(rk, __shfl_xor(rk, off, 64));
+    knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, rk,
+             [&](float d2, int j, bool ok) {
+      const bool keep = ok && knn_bin0(d2) <= tb && cnt < KNN_CAP;
+      cand[(keep ? cnt : KNN_CAP) * 64 + lane] =
+          make_uint2(__float_as_uint(d2), (unsigned)j);
+      cnt += keep ? 1 : 0;
+    });
+  } else {
+    knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, kInf,
+             [&](float d2, int j, bool ok) {
+      if (ok && classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
+        cand[cnt * 64 + lane] = make_uint2(__float_as_uint(d2),
+                                           (unsigned)j);
+        ++cnt;
+      }
+    });
+  }
 
-  float bd[K];
-  int bi[K];
+  // rank of each kept candidate among the row's kept ones, four at a time
+  int m = cnt;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    bd[k] = __builtin_inff();
-    bi[k] = -1;
-  }
-  float wd = __builtin_inff();
-  int wj = -1;
-  for (int s = 0; s < cnt; ++s) {
-    const unsigned c = buf[s * 64 + lane];
-    const int e = (int)(c >> 4);
-    const int j = a.tidx[e] * DS_TS + (int)(c & 15u);
-    const float d2 = knn_d2(li, toff[e], loc[j]);
-    if (!knn_less(d2, j, wd, wj)) continue;
-    bool done = false;
-    float nd = -1.0f;
-    int nj = -1;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const bool hit = !done && bd[k] == wd && bi[k] == wj;
-      bd[k] = hit ? d2 : bd[k];
-      bi[k] = hit ? j : bi[k];
-      done = done || hit;
-      const bool gt = knn_less(nd, nj, bd[k], bi[k]);
-      nd = gt ? bd[k] : nd;
-      nj = gt ? bi[k] : nj;
-    }
-    wd = nd;
-    wj = nj;
-  }
-#pragma unroll
-  for (int pass = 0; pass < K; ++pass) {
-#pragma unroll
-    for (int k = pass & 1; k + 1 < K; k += 2) {
-      const bool sw = knn_less(bd[k + 1], bi[k + 1], bd[k], bi[k]);
-      const float td = sw ? bd[k + 1] : bd[k];
-      const float tu = sw ? bd[k] : bd[k + 1];
-      const int ti = sw ? bi[k + 1] : bi[k];
-      const int tj = sw ? bi[k] : bi[k + 1];
-      bd[k] = td;
-      bd[k + 1] = tu;
-      bi[k] = ti;
-      bi[k + 1] = tj;
-    }
-  }
+  for (int off = 1; off < 64; off <<= 1) m = max(m, __shfl_xor(m, off, 64));
   int* o = a.nbr + (size_t)row * K;
+  for (int s0 = 0; s0 < m; s0 += 4) {
+    float ds[4];
+    int js[4], rk[4];
 #pragma unroll
-  for (int k = 0; k < K; ++k) o[k] = bi[k];
+    for (int u = 0; u < 4; ++u) {
+      const uint2 c = cand[min(s0 + u, KNN_CAP) * 64 + lane];
+      ds[u] = __uint_as_float(c.x);
+      js[u] = (int)c.y;
+      rk[u] = 0;
+    }
+    for (int t = 0; t < m; ++t) {
+      const uint2 c = cand[t * 64 + lane];
+      const float dt = __uint_as_float(c.x);
+      const int jt = (int)c.y;
+      const bool in = t < cnt;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        rk[u] += (in && knn_less(dt, jt, ds[u], js[u])) ? 1 : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u < cnt && rk[u] < K) o[rk[u]] = js[u];
+  }
+  for (int k = cnt; k < K; ++k) o[k] = -1;
 }
 
 // The WS_FEAT features of one row from its neighbour list and the current
