@@ -124,6 +124,9 @@ SIGNATURES = {
     "ds_problem_smp_grid": (C.c_int, [_vp, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_int,
                                       _dp]),
+    "ds_problem_smp_grid_noalloc": (C.c_int, [_vp, C.c_double, C.c_double,
+                                              C.c_double, C.c_int, C.c_int,
+                                              C.c_int, _dp]),
     "ds_smp_product": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _dp,
                                  C.POINTER(DsCovar), _dp, _dp]),
     "ds_smp_product_grid": (C.c_int, [_vp, C.c_int, _dp, C.c_double,

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -30,12 +31,10 @@ int fail(ds_ctx* ctx, int code, const char* fmt, ...)
 
 int fail(ds_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
     va_end(ap);
-    ctx->err = buf;
   }
   return code;
 }
@@ -419,6 +418,69 @@ int check_pcg_params(ds_problem* p, const ds_pcg_params* prm) {
 
 }  // namespace
 
+// ---- static scratch of the no-alloc entry point -------------------------
+// Every GPU the library runs on holds one copy of this array until the
+// process ends.  In host code its name is only a stand-in, never a device
+// address: noalloc_setup() looks the address up once per device (with that
+// device current, at context creation) and host code passes that pointer
+// to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[DS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+hipError_t noalloc_setup(int device, unsigned char** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_noalloc_ptr[device];
+  return hipSuccess;
+}
+
+// Exclusive offsets of the per-map-tile list counts: ptr[T] = sum of
+// cnt[t < T], ptr[nt] = the total, or -1 when the total exceeds cap
+// entries (then no offset is written).  One workgroup of 1024 threads, each
+// summing a contiguous run of counts.
+__global__ __launch_bounds__(1024) void k_list_offsets(const int* cnt, int nt,
+                                                       long long cap,
+                                                       int* ptr) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (nt + 1023) / 1024;
+  const int b = min(nt, t * per), e = min(nt, b + per);
+  long long sum = 0;
+  for (int i = b; i < e; ++i) sum += cnt[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan
+    const long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const long long total = part[1023];
+  const bool ok = total <= cap;
+  long long run = t > 0 ? part[t - 1] : 0;
+  if (ok)
+    for (int i = b; i < e; ++i) {
+      ptr[i] = (int)run;
+      run += cnt[i];
+    }
+  if (t == 0) ptr[nt] = ok ? (int)total : -1;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ds_plan_tiles(int n, const double* pts, int stride,
@@ -459,6 +521,7 @@ int ds_ctx_create(int device, ds_ctx** out) {
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount,
                             device) != hipSuccess)
     return DS_E_HIP;
+  if (noalloc_setup(device, &c->noalloc) != hipSuccess) return DS_E_HIP;
   *out = c.release();
   return DS_OK;
 }
@@ -472,7 +535,7 @@ int ds_ctx_destroy(ds_ctx* ctx) {
 }
 
 const char* ds_ctx_last_error(const ds_ctx* ctx) {
-  return ctx ? ctx->err.c_str() : "NULL context";
+  return ctx ? ctx->err : "NULL context";
 }
 
 int ds_ctx_synchronize(ds_ctx* ctx) {
@@ -1095,6 +1158,77 @@ int ds_problem_smp_grid(ds_problem* p, double lx, double ly, double lz,
     return fail(p->ctx, DS_E_STATE, "map projection needs a single volume");
   return map_grid_run(p->ctx, ds::MM_SMP, g, p->blk, p->pos64, p->x,
                       p->table64, p->cv, m);
+}
+
+// ds_problem_smp_grid without device allocations: the source-tile boxes,
+// the map tiles' list counts, offsets and entries and the map itself live
+// in the static scratch array (layout below, 256-byte aligned parts); the
+// offsets are formed on the device, so the host allocates nothing either.
+// The lists and the projection are ds_problem_smp_grid's (same kernels,
+// same lists), so the maps are equal bit for bit.
+int ds_problem_smp_grid_noalloc(ds_problem* p, double lx, double ly,
+                                double lz, int nx, int ny, int nz,
+                                double* m) {
+  if (!p || !m) return DS_E_ARG;
+  ds_ctx* ctx = p->ctx;
+  ds::MapGrid g;
+  if (int rc = check_grid(ctx, lx, ly, lz, nx, ny, nz, g)) return rc;
+  if (p->n_sub != 1)
+    return fail(ctx, DS_E_STATE, "map projection needs a single volume");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device));
+  if (!ctx->noalloc)
+    return fail(ctx, DS_E_STATE, "no-alloc scratch not set up");
+  const Blocking& blk = p->blk;
+  const size_t nt = (size_t)g.tx * g.ty * g.tz;
+  const size_t n_map = (size_t)g.nx * g.ny * g.nz;
+  const size_t nb = 3 * (size_t)blk.ntile;
+  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_lo = 0;
+  const size_t o_hi = align(o_lo + sizeof(double) * nb);
+  const size_t o_cnt = align(o_hi + sizeof(double) * nb);
+  const size_t o_ptr = align(o_cnt + sizeof(int) * nt);
+  const size_t o_map = align(o_ptr + sizeof(int) * (nt + 1));
+  const size_t o_idx = align(o_map + sizeof(double) * n_map);
+  if (o_idx >= (size_t)DS_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, DS_E_ARG,
+                "map grid of %zu cells needs %zu bytes before its lists, "
+                "more than the %u-byte no-alloc scratch",
+                n_map, o_idx, (unsigned)DS_NOALLOC_SCRATCH_BYTES);
+  const long long cap = std::min<long long>(
+      ((size_t)DS_NOALLOC_SCRATCH_BYTES - o_idx) / sizeof(int), INT_MAX);
+  unsigned char* base = ctx->noalloc;
+  double* dlo = reinterpret_cast<double*>(base + o_lo);
+  double* dhi = reinterpret_cast<double*>(base + o_hi);
+  int* dcnt = reinterpret_cast<int*>(base + o_cnt);
+  int* dptr = reinterpret_cast<int*>(base + o_ptr);
+  double* dmap = reinterpret_cast<double*>(base + o_map);
+  int* didx = reinterpret_cast<int*>(base + o_idx);
+  hipStream_t st = ctx->stream;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(dlo, blk.tlo.data(), sizeof(double) * nb,
+                                  hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(dhi, blk.thi.data(), sizeof(double) * nb,
+                                  hipMemcpyHostToDevice, st));
+  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, blk.ntile, p->cv, dcnt,
+                                         nullptr, nullptr, st));
+  hipLaunchKernelGGL(k_list_offsets, dim3(1), dim3(1024), 0, st, dcnt,
+                     (int)nt, cap, dptr);
+  HIP_OR_FAIL(ctx, hipGetLastError());
+  int total = 0;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(&total, dptr + nt, sizeof(int),
+                                  hipMemcpyDeviceToHost, st));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+  if (total < 0)
+    return fail(ctx, DS_E_ARG,
+                "map grid lists need more than the %lld entries the no-alloc "
+                "scratch holds beside this %zu-cell map", cap, n_map);
+  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, blk.ntile, p->cv,
+                                         nullptr, dptr, didx, st));
+  HIP_OR_FAIL(ctx, ds::launch_map64(ds::MM_SMP, g, p->pos64, p->x, dptr, didx,
+                                    p->table64, p->cv, dmap, st));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(m, dmap, sizeof(double) * n_map,
+                                  hipMemcpyDeviceToHost, st));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+  return DS_OK;
 }
 
 int ds_smp_product(ds_ctx* ctx, int n_pix, const double* pix_pts, int n_map,

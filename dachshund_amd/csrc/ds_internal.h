@@ -123,10 +123,14 @@ struct ds_cg_scalars {
 struct ds_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::string err;
+  char err[512] = {0};         // last error (fixed storage: setting it
+                               // allocates nothing)
   int num_cu = 0;
   double* scratch = nullptr;   // reduction partials
   size_t scratch_n = 0;
+  // the device address of the no-alloc entry points' static scratch array
+  // on this context's device (looked up once per device at creation)
+  unsigned char* noalloc = nullptr;
 };
 
 struct ds_problem {

@@ -34,7 +34,7 @@ struct ds_warmstart {
 namespace {
 
 int ws_fail(ds_ctx* ctx, int code, const char* msg) {
-  if (ctx) ctx->err = msg;
+  if (ctx) snprintf(ctx->err, sizeof(ctx->err), "%s", msg);
   return code;
 }
 

@@ -359,6 +359,17 @@ class _WFVector:
         return m
 
 
+    def smp_grid_noalloc(self, lx, ly, lz, nx, ny, nz):
+        """smp_grid with every per-call device buffer in the library's
+        static scratch array (ds_problem_smp_grid_noalloc): no device
+        allocation; the same map bit for bit."""
+        m = np.empty(nx * ny * nz)
+        check(lib().ds_problem_smp_grid_noalloc(self.h, lx, ly, lz, nx, ny,
+                                                nz, _dp(m)), self.ctx.h,
+              "smp_grid_noalloc")
+        return m
+
+
 class BatchWF(_WFVector):
     """Many independent subvolumes solved together (ds_batch_*): every pair
     and CG kernel processes all of them in one launch; each has its own

@@ -261,6 +261,23 @@ int ds_problem_get_norms(ds_problem* p, double* norms, int cap, int* count);
 int ds_problem_smp_grid(ds_problem* p, double lx, double ly, double lz,
                         int nx, int ny, int nz, double* m);
 
+/* ds_problem_smp_grid without device allocations.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * per-call device buffer -- the source tiles' boxes, the map tiles' list
+ * counts, offsets and entries, and the map before its copy to m -- lives in
+ * one static device array of DS_NOALLOC_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends; the call makes no device
+ * or host allocation.  Returns DS_E_ARG, having written nothing to m, when
+ * the grid and its lists do not fit (8 bytes per map cell plus 4 per list
+ * entry plus the boxes and offsets; the 2.53M-cell 60 x 48 x 880 grid of
+ * a 76,800-pixel problem fits).  Calls on one GPU share the array: calls
+ * that overlap on one GPU race and must be serialized by the caller.  The
+ * map equals ds_problem_smp_grid's bit for bit. */
+#define DS_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int ds_problem_smp_grid_noalloc(ds_problem* p, double lx, double ly,
+                                double lz, int nx, int ny, int nz, double* m);
+
 /* smp_product (lib/map.cc:20-43) with arbitrary pixel and map points. */
 int ds_smp_product(ds_ctx* ctx, int n_pix, const double* pix_pts, int n_map,
                    const double* map_pts, const ds_covar* covar,
