@@ -100,6 +100,7 @@ SIGNATURES = {
                                           C.c_double,
                                           C.POINTER(DsPcgResult)]),
     "ds_problem_get_norms": (C.c_int, [_vp, _dp, C.c_int, _ip]),
+    "ds_problem_get_deltas": (C.c_int, [_vp, _dp, C.c_int, _ip]),
     "ds_problem_fill_x": (C.c_int, [_vp, C.c_double]),
     "ds_problem_time_matvec": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "ds_solve_cf": (C.c_int, [_vp, C.c_int, _dp]),

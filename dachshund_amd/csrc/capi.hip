@@ -1126,6 +1126,18 @@ int ds_problem_get_norms(ds_problem* p, double* norms, int cap, int* count) {
   return DS_OK;
 }
 
+int ds_problem_get_deltas(ds_problem* p, double* deltas, int cap,
+                          int* count) {
+  if (!p || (!deltas && cap > 0)) return DS_E_ARG;
+  const int nn = p->sc_host ? p->sc_host[0].n_norms : 0;
+  const int m = std::min(nn, std::max(cap, 0));
+  if (m > 0 && p->deltas)
+    HIP_OR_FAIL(p->ctx, hipMemcpy(deltas, p->deltas, sizeof(double) * m,
+                                  hipMemcpyDeviceToHost));
+  if (count) *count = nn;
+  return DS_OK;
+}
+
 static int solve_common(ds_problem* p, const ds_pcg_params* prm, int mixed,
                         double drop, double* x, double* norms, int cap,
                         ds_pcg_result* res) {

@@ -192,6 +192,7 @@ struct FinArgs {
   const int* sub_blk0;
   ds_cg_scalars* sc;
   double* norms;
+  double* deltas;
   int* done_count;
   double drop;
   int nblk, norms_cap;
@@ -215,9 +216,9 @@ __global__ __launch_bounds__(1024) void k_finalize(FinArgs a) {
   if (NV == 1)
     finalize_alpha(sc, v[0]);
   else
-    finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, v[0],
-                  v[NV - 1], a.init, a.mixed, a.drop, a.is_true,
-                  a.done_count);
+    finalize_step(sc, a.norms + (size_t)sv * a.norms_cap,
+                  a.deltas + (size_t)sv * a.norms_cap, v[0], v[NV - 1],
+                  a.init, a.mixed, a.drop, a.is_true, a.done_count);
 }
 
 // Shared-geometry batches with block-Jacobi: the subvolumes share the
@@ -273,6 +274,7 @@ struct DirArgs {
   const double* partials;
   const int* sub_blk0;
   double* norms;
+  double* deltas;
   int* done_count;
   double drop;
   int norms_cap;
@@ -316,7 +318,8 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_dir(DirArgs a) {
     rr = wave_sum(rr);
     beta = rs / sc->saved_delta_old;
     if (I == b0 && lane == 0)
-      finalize_step(sc, a.norms + (size_t)sv * a.norms_cap, rs, rr, 0, 1,
+      finalize_step(sc, a.norms + (size_t)sv * a.norms_cap,
+                    a.deltas + (size_t)sv * a.norms_cap, rs, rr, 0, 1,
                     a.drop, 1, a.done_count);
   }
   const double di = live ? si + beta * d0 : d0;
@@ -413,6 +416,7 @@ hipError_t launch_finalize(ds_problem* p, const CgEpilogue& e,
   a.sub_blk0 = p->sub_blk0;
   a.sc = p->sc;
   a.norms = p->norms;
+  a.deltas = p->deltas;
   a.done_count = p->done_count;
   a.drop = e.drop;
   a.nblk = p->nblk;
@@ -508,6 +512,7 @@ hipError_t launch_dir(ds_problem* p, int mixed, hipStream_t st,
   a.partials = p->partials;
   a.sub_blk0 = p->sub_blk0;
   a.norms = p->norms;
+  a.deltas = p->deltas;
   a.done_count = p->done_count;
   a.drop = drop;
   a.norms_cap = p->norms_cap;
@@ -602,8 +607,10 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
   if (cap > p->norms_cap) {
     if (p->norms) DS_TRY(hipFree(p->norms));
     p->norms = nullptr;
-    DS_TRY(hipMalloc(&p->norms, sizeof(double) * (size_t)cap * ns));
+    // residual norms, then deltas, cap per subvolume each
+    DS_TRY(hipMalloc(&p->norms, 2 * sizeof(double) * (size_t)cap * ns));
     p->norms_cap = cap;
+    p->deltas = p->norms + (size_t)cap * ns;
   }
   hipEvent_t e0, e1;
   DS_TRY(hipEventCreate(&e0));

@@ -255,13 +255,15 @@ __device__ __forceinline__ void mark_done(ds_cg_scalars* sc, int success,
 }
 
 // lib/pcg.h:54-61 and :123-135 (or, with init, :100-110): new delta, beta,
-// |r|, residual history and the stop rule.  rs = r . M^-1 r, rr = r . r.
+// |r|, residual and delta histories (what the verbose trace of
+// lib/pcg.h:112-139 prints) and the stop rule.  rs = r . M^-1 r, rr = r . r.
 // Mixed solver (mixed != 0): the recurrence residual never stops the solve
 // itself; the next true-residual kernel runs (want_true) when it meets the
 // tolerance, has dropped by `drop` since the last FP64 true residual, or
 // refresh_every iterations have passed since that one (a recurrence that
 // stagnates at the inner operator's precision is refreshed all the same).
 __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
+                                              double* deltas,
                                               double rs, double rr, int init,
                                               int mixed, double drop,
                                               int is_true, int* done_count) {
@@ -272,6 +274,7 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
     sc->iter = 0;
     sc->n_norms = 1;
     norms[0] = norm_r;
+    deltas[0] = rs;
     if (mixed) {
       sc->ref_norm = norm_r;
       sc->true_norm = norm_r;
@@ -289,6 +292,7 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
   const int iter = is_true ? sc->iter : sc->iter + 1;
   sc->iter = iter;
   norms[iter] = norm_r;
+  deltas[iter] = rs;
   sc->n_norms = iter + 1;
   if (!mixed) {
     if (norm_r <= sc->tol_norm_b) {

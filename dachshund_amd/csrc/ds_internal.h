@@ -226,6 +226,7 @@ struct ds_problem {
   double* partials = nullptr;          // 4 * nblk
   ds_cg_scalars* sc = nullptr;         // n_sub entries
   double* norms = nullptr;             // residual histories, n_sub x cap
+  double* deltas = nullptr;            // delta histories (after norms)
   int norms_cap = 0;
   ds_cg_scalars* sc_host = nullptr;    // pinned, n_sub entries (final)
   int* done_host = nullptr;            // pinned poll slots (2)

@@ -113,16 +113,22 @@ void write_doubles(const std::string& path, const double* v, size_t n) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  bool mixed = false;
+  bool mixed = false, single_conversion = false;
   std::vector<std::string> args;
   for (int i = 1; i < argc; ++i) {
     if (std::strcmp(argv[i], "--mixed") == 0) mixed = true;
+    else if (std::strcmp(argv[i], "--single-conversion") == 0)
+      single_conversion = true;
     else args.push_back(argv[i]);
   }
   if (args.size() != 1) {
     fputs("Bad number of arguments. Expected 1.\n", stderr);
-    puts("Usage: dachshund_gpu.exe input [--mixed]");
+    puts("Usage: dachshund_gpu.exe input [--mixed] [--single-conversion]");
     puts("    input     path to config file.");
+    puts("    --mixed   mixed FP32/FP64 solve (same FP64 stopping rule).");
+    puts("    --single-conversion  with option_noise_covar and");
+    puts("              option_map_covar both set, convert the pixel buffer");
+    puts("              to NPixel once (the reference converts it twice).");
     exit(1);
   }
   Config c;
@@ -173,7 +179,12 @@ int main(int argc, char** argv) {
   if (c.option_map_covar) {
     puts("Computing covariance diag.");
     NPixel* npixels = (NPixel*)pixels.data();
-    if (!c.option_noise_covar) {
+    // app/dachshund.cc:276-282 converts the buffer again even when the
+    // noise-dominated block above already holds NPixel records in it, so
+    // MapCovarDiag then sees N' = 1/n^4 and d' = d/n^2; that is kept (the
+    // drop-in reproduces the reference's output), --single-conversion
+    // skips the second conversion.
+    if (!(c.option_noise_covar && single_conversion)) {
       wpixel_to_pixel(n, wpixels, pixels.data());
       pixel_to_npixel(n, pixels.data(), npixels);
     }

@@ -70,16 +70,23 @@ PCGResult run_solve(ds_problem* p, const PCGParams& prm, double* x,
     return res;
   }
   if (prm.verbose) {
-    printf("[PCG] Solving %i x %i problem on the GPU%s.\n", prm.n, prm.n,
-           mixed ? " (FP32 pairs, FP64 refinement)" : "");
-    printf("    Goal |r| < (tol) |b| = %e\n", prm.tol * r.norm_b);
+    // lib/pcg.h:112-139's trace, printed once the device solve returns
+    std::vector<double> deltas(norms.size());
+    int nd = 0;
+    check(ds_problem_get_deltas(p, deltas.data(), (int)deltas.size(), &nd),
+          "ds_problem_get_deltas");
+    const double tol_norm_b = prm.tol * r.norm_b;
+    printf("[PCG] Solving %i x %i problem.\n", prm.n, prm.n);
+    printf("    Goal |r| < (tol) |b| = %e,  delta = %e\n", tol_norm_b,
+           deltas[0]);
     const int last = (int)res.residual_norms.size() - 1;
-    for (int i = 1; i <= last; ++i) {
+    for (int i = 1; i <= last && i < nd; ++i) {
       if (i == last && res.success)
         printf("    iter %i reached |r| = %e < tol |b| = %e\n", i,
-               res.residual_norms[i], prm.tol * r.norm_b);
+               res.residual_norms[i], tol_norm_b);
       else
-        printf("    iter %i, |r| %e\n", i, res.residual_norms[i]);
+        printf("    iter %i, |r| %e, delta %e\n", i, res.residual_norms[i],
+               deltas[i]);
     }
   }
   if (!res.success)

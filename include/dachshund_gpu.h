@@ -253,6 +253,10 @@ int ds_problem_time_matvec(ds_problem* p, int prec, int reps,
                            double* ms_per_launch);
 /* Residual history of the last solve on this problem. */
 int ds_problem_get_norms(ds_problem* p, double* norms, int cap, int* count);
+/* The delta = r . M^-1 r after each iteration of the last solve (entry 0:
+ * the initial one), the value lib/pcg.h:112-139's verbose trace prints
+ * beside |r| (pcg_solve keeps it in a local).  Same count as the norms. */
+int ds_problem_get_deltas(ds_problem* p, double* deltas, int cap, int* count);
 
 /* smp_product (lib/map.cc:20-43) onto the uniform grid of
  * init_uniform_map_coords (lib/util.cc:25-46), with the problem's pixels as

@@ -131,3 +131,37 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+REFERENCE = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "app")),
+                    reason="needs the reference sources (build container)")
+def test_reference_callers_compile_against_dropin(tmp_path):
+    """The reference's callers -- app/dachshund.cc and its gtest suite
+    tests/*.cc -- compile unchanged against the drop-in headers
+    (dachshund_amd/cxx/include); the app also links against
+    libdachshund.so + libdachshund_gpu.so.  Compile only: nothing built
+    here is kept."""
+    import subprocess
+    inc = os.path.join(ROOT, "dachshund_amd", "cxx", "include")
+    libdir = os.path.join(ROOT, "dachshund_amd")
+    if not os.path.exists(os.path.join(libdir, "libdachshund.so")):
+        pytest.skip("libdachshund.so not built")
+    app = subprocess.run(
+        ["g++", "-std=c++11", "-w", f"-I{inc}",
+         os.path.join(REFERENCE, "app", "dachshund.cc"), "-o",
+         str(tmp_path / "app.exe"), f"-L{libdir}", "-ldachshund",
+         "-ldachshund_gpu", "-lpthread"], capture_output=True, text=True)
+    assert app.returncode == 0, app.stderr[-3000:]
+    gt = [f"-I{REFERENCE}/ext", f"-I{REFERENCE}/ext/gtest",
+          f"-I{REFERENCE}/ext/gtest/include", f"-I{REFERENCE}/tests"]
+    for name in ("wfx_tests.cc", "pixel_tests.cc", "signal_covar_tests.cc",
+                 "map_covar_tests.cc", "performance_tests.cc",
+                 "test_utils.cc"):
+        r = subprocess.run(["g++", "-std=c++11", "-w", "-fsyntax-only",
+                            f"-I{inc}", *gt,
+                            os.path.join(REFERENCE, "tests", name)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (name, r.stderr[-3000:])
