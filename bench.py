@@ -308,7 +308,8 @@ def bench_chunk(args):
         log(f"{name}: {np.mean(ts) * 1e3:.1f} ms, {r.num_iters} iters")
     ms64 = kernel_ms(wf, D.DS_PREC_F64)
     ms32 = kernel_ms(wf, D.DS_PREC_F32)
-    log(f"matvec: FP64 {ms64:.3f} ms ({st.pairs_per_matvec / ms64 * 1e3:.3e} "
+    log(f"matvec: FP64 {ms64:.3f} ms "
+        f"({st.f64_pairs_per_matvec / ms64 * 1e3:.3e} "
         f"pairs/s), FP32 {ms32:.3f} ms "
         f"({st.sub_pairs_per_matvec / ms32 * 1e3:.3e} pairs/s)")
     box, shape = w.box, w.map_shape
@@ -337,7 +338,7 @@ def bench_chunk(args):
         "time_to_tol_s": {k: v["s"] for k, v in out.items()},
         "iterations": {k: v["iters"] for k, v in out.items()},
         "matvec_ms": {"f64": ms64, "f32": ms32},
-        "pairs_per_matvec": {"f64": st.pairs_per_matvec,
+        "pairs_per_matvec": {"f64": st.f64_pairs_per_matvec,
                              "f32": st.sub_pairs_per_matvec},
         "map_s": t_map, "map_norm": float(np.linalg.norm(m)),
     }), flush=True)
@@ -374,7 +375,11 @@ def bench_kernels(args):
                        ("k_matvec32m", D.DS_F32_MFMA)):
         wf.set_f32_kernel(kind)
         out[name] = wf.time_matvec(D.DS_PREC_F32, reps=args.steps)
-    out["k_matvec64"] = wf.time_matvec(D.DS_PREC_F64, reps=args.steps)
+    for rows in (16, 32, 64):   # FP64 tile shapes; 64 last (the default)
+        wf.set_tile(rows)
+        out[f"k_matvec64_r{rows}"] = wf.time_matvec(D.DS_PREC_F64,
+                                                   reps=args.steps)
+    out["k_matvec64"] = out["k_matvec64_r64"]
     for k, ms in out.items():
         log(f"{k}: {ms * 1e3:.1f} us per launch")
     print(json.dumps({
@@ -388,7 +393,8 @@ def bench_kernels(args):
         "us_per_launch": {k: v * 1e3 for k, v in out.items()},
         "pairs_per_launch": {"k_matvec32": st.pairs_per_matvec,
                              "k_matvec32m": st.sub_pairs_per_matvec,
-                             "k_matvec64": st.pairs_per_matvec},
+                             "k_matvec64": st.pairs_per_matvec,
+                             "k_matvec64_r16": st.sub_pairs_per_matvec},
     }), flush=True)
 
 
@@ -660,7 +666,7 @@ def main():
     # dominant kernel: the FP32 operator (symmetric pair kernel + gather)
     ms32 = kernel_ms(wf, D.DS_PREC_F32)
     ms64 = kernel_ms(wf, D.DS_PREC_F64)
-    pairs = st.pairs_per_matvec          # FP64 kernel (64 x 16 culling)
+    pairs = st.f64_pairs_per_matvec      # FP64 kernel at its tile shape
     prod32 = st.sub_pairs_per_matvec     # FP32 pair products (16 x 16)
     eval32 = st.sub_pairs_evaluated      # FP32 evaluations (each once)
     rate32 = eval32 / (ms32 * 1e-3)
@@ -727,6 +733,7 @@ def main():
                                      "plain_fp64": rel_64},
         "map_rel_l2_headline_vs_plain_fp64": rel_hp,
         "pairs_per_matvec": {"fp64_evaluated": pairs,
+                             "fp64_64x16_tiles": st.pairs_per_matvec,
                              "fp32_evaluated": eval32,
                              "fp32_pair_products": prod32,
                              "inside_cutoff": inside,
@@ -754,7 +761,8 @@ def main():
             "pairs_evaluated_per_launch": eval32,
             "pair_products_per_launch": prod32,
         },
-        "fp64_kernel": {"kernel": "k_matvec64", "achieved": rate64,
+        "fp64_kernel": {"kernel": f"k_matvec64 ({wf.launch()[2]} x 16 "
+                                  f"tiles)", "achieved": rate64,
                         "peak": PAIR_PEAK, "unit": "pair-evals/s",
                         "frac": rate64 / PAIR_PEAK, "ms_per_launch": ms64,
                         "pairs_evaluated_per_launch": pairs,

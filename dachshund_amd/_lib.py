@@ -48,7 +48,8 @@ class DsProblemStats(C.Structure):
                 ("pairs_per_matvec", C.c_longlong),
                 ("sub_pairs_per_matvec", C.c_longlong),
                 ("setup_seconds", C.c_double),
-                ("sub_pairs_evaluated", C.c_longlong)]
+                ("sub_pairs_evaluated", C.c_longlong),
+                ("f64_pairs_per_matvec", C.c_longlong)]
 
 
 # name -> (restype, argtypes); every symbol of include/dachshund_gpu.h
@@ -81,6 +82,8 @@ SIGNATURES = {
     "ds_problem_set_dense": (C.c_int, [_vp, C.c_int]),
     "ds_problem_dense_product": (C.c_int, [_vp, _dp, _dp]),
     "ds_problem_time_dense": (C.c_int, [_vp, C.c_int, _dp]),
+    "ds_problem_set_tile": (C.c_int, [_vp, C.c_int]),
+    "ds_problem_get_launch": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "ds_problem_autotune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip,
                                       _dp]),
     "ds_A_product": (C.c_int, [_vp, C.c_int, _dp, _dp]),

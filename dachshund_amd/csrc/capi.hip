@@ -322,6 +322,30 @@ int upload_table64(ds_ctx* ctx, const ds_covar* c, DevBuf<double2>& t) {
   return DS_OK;
 }
 
+// Launch order of `blocks` (ascending block indices, spatially ordered):
+// workgroups go to the 8 XCDs round-robin by launch slot (slot i to XCD
+// i mod 8), so slot 8k + x gets the k-th block of XCD x's contiguous share
+// of the list, each share taken longest (cost) first.  Neighbouring
+// blocks, which read the same source tiles, then run on one XCD and share
+// its L2 instead of each XCD fetching every tile.
+template <class Cost>
+std::vector<int> xcd_order(const std::vector<int>& blocks, Cost cost) {
+  constexpr int kXcd = 8;
+  const int n = (int)blocks.size();
+  std::vector<int> out(n);
+  int start = 0;
+  for (int x = 0; x < kXcd; ++x) {
+    const int cnt = n / kXcd + (x < n % kXcd ? 1 : 0);
+    std::vector<int> share(blocks.begin() + start,
+                           blocks.begin() + start + cnt);
+    std::stable_sort(share.begin(), share.end(),
+                     [&](int a, int b) { return cost(a) > cost(b); });
+    for (int k = 0; k < cnt; ++k) out[(size_t)k * kXcd + x] = share[k];
+    start += cnt;
+  }
+  return out;
+}
+
 void problem_free(ds_problem* p) {
   dev_free(p->knn_nbr);
   dev_free(p->pos64);
@@ -417,6 +441,19 @@ int check_pcg_params(ds_problem* p, const ds_pcg_params* prm) {
 }
 
 }  // namespace
+
+namespace ds {
+
+void ctx_retain(ds_ctx* c) { c->refs.fetch_add(1); }
+
+void ctx_release(ds_ctx* c) {
+  if (c->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace ds
 
 // ---- static scratch of the no-alloc entry point -------------------------
 // Every GPU the library runs on holds one copy of this array until the
@@ -528,9 +565,7 @@ int ds_ctx_create(int device, ds_ctx** out) {
 
 int ds_ctx_destroy(ds_ctx* ctx) {
   if (!ctx) return DS_E_ARG;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  ds::ctx_release(ctx);
   return DS_OK;
 }
 
@@ -674,13 +709,21 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   }
   p->ntile_pairs = (long long)idx.size();
   p->nsub_pairs = 0;
-  for (unsigned char m : msk) p->nsub_pairs += __builtin_popcount(m);
+  p->nhalf_pairs = 0;
+  for (unsigned char m : msk) {
+    p->nsub_pairs += __builtin_popcount(m);
+    p->nhalf_pairs += ((m & 3u) != 0) + ((m & 12u) != 0);
+  }
   p->sub_pairs64.assign(n_sub, 0);
   p->sub_pairs32.assign(n_sub, 0);
+  p->sub_half64.assign(n_sub, 0);
   for (int k = 0; k < n_sub; ++k) {
     const int B0 = p->sub_blk0_h[k], B1 = p->sub_blk0_h[k + 1];
-    for (int e = ptr[B0]; e < ptr[B1]; ++e)
+    for (int e = ptr[B0]; e < ptr[B1]; ++e) {
       p->sub_pairs32[k] += __builtin_popcount(msk[e]);
+      p->sub_half64[k] += (long long)(((msk[e] & 3u) != 0) +
+                                      ((msk[e] & 12u) != 0)) * (32 * DS_TS);
+    }
     p->sub_pairs64[k] = (long long)(ptr[B1] - ptr[B0]) * (DS_BS * DS_TS);
     p->sub_pairs32[k] *= DS_TS * DS_TS;
   }
@@ -829,12 +872,11 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   if (nslot > (size_t)UINT32_MAX / (16 * sizeof(float)))
     return fail(ctx, DS_E_ARG, "%zu column-sum slots exceed the 32-bit "
                 "offsets of the update kernel", nslot);
-  // launch order: longest tile list first (for the autotuner)
+  // launch order: longest tile list first (for the autotuner), grouped by
+  // XCD (xcd_order)
   std::vector<int> order(nblk);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return p->blk_len_h[a] > p->blk_len_h[b];
-  });
+  order = xcd_order(order, [&](int I) { return p->blk_len_h[I]; });
   // FP32 launch lists: longest list first and block order, each split into
   // the blocks of subvolumes without far points (lengths of their half
   // lists, which the MFMA kernel runs), then the others
@@ -844,23 +886,22 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     int nf = 0;
     for (int I : order) nf += !direct(I);
     p->n32_fast = nf;
-    // the MFMA kernel runs the half lists: longest half list first
-    std::vector<int> horder(nblk);
-    std::iota(horder.begin(), horder.end(), 0);
-    std::stable_sort(horder.begin(), horder.end(), [&](int a, int b) {
-      return hptr[a + 1] - hptr[a] > hptr[b + 1] - hptr[b];
+    // the MFMA kernel runs the half lists: longest half list first; each
+    // part is launched on its own, so each is grouped by XCD on its own
+    std::vector<int> fast, slow;
+    for (int I = 0; I < nblk; ++I) (direct(I) ? slow : fast).push_back(I);
+    const std::vector<int> f0 = xcd_order(fast, [&](int I) {
+      return hptr[I + 1] - hptr[I];
     });
-    for (int part = 0; part < 2; ++part) {
-      int* dst = ord32.data() + (size_t)part * nblk;
-      int f = 0, d = nf;
-      for (int k = 0; k < nblk; ++k) {
-        if (part == 0) {
-          if (!direct(horder[k])) dst[f++] = horder[k];
-          if (direct(order[k])) dst[d++] = order[k];
-        } else {
-          if (direct(k)) dst[d++] = k; else dst[f++] = k;
-        }
-      }
+    const std::vector<int> s0 = xcd_order(slow, [&](int I) {
+      return p->blk_len_h[I];
+    });
+    std::copy(f0.begin(), f0.end(), ord32.begin());
+    std::copy(s0.begin(), s0.end(), ord32.begin() + nf);
+    int f = 0, d = nf;
+    for (int k = 0; k < nblk; ++k) {
+      int* dst = ord32.data() + nblk;
+      if (direct(k)) dst[d++] = k; else dst[f++] = k;
     }
   }
   std::vector<double2> t64;
@@ -969,6 +1010,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, up(p->sc, p->sc_host, sizeof(ds_cg_scalars) * n_sub));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
   p->setup_seconds = now_s() - t0;
+  ds::ctx_retain(ctx);
   *out = P.release();
   return DS_OK;
 }
@@ -996,8 +1038,10 @@ int ds_problem_destroy(ds_problem* p) {
   if (!p) return DS_E_ARG;
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
+  ds_ctx* ctx = p->ctx;
   problem_free(p);
   delete p;
+  ds::ctx_release(ctx);
   return DS_OK;
 }
 
@@ -1011,6 +1055,10 @@ int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out) {
   out->sub_pairs_per_matvec = p->nsub_pairs * (long long)(DS_TS * DS_TS);
   out->setup_seconds = p->setup_seconds;
   out->sub_pairs_evaluated = p->nsub_eval * (long long)(DS_TS * DS_TS);
+  out->f64_pairs_per_matvec =
+      p->f64_rows == 16 ? out->sub_pairs_per_matvec
+      : p->f64_rows == 32 ? p->nhalf_pairs * (long long)(32 * DS_TS)
+                          : out->pairs_per_matvec;
   return DS_OK;
 }
 
@@ -1407,7 +1455,9 @@ int ds_batch_sub_pairs(const ds_problem* p, long long* pairs64,
                        long long* pairs32) {
   if (!p || !pairs64 || !pairs32) return DS_E_ARG;
   for (int k = 0; k < p->n_sub; ++k) {
-    pairs64[k] = p->sub_pairs64[k];
+    pairs64[k] = p->f64_rows == 16   ? p->sub_pairs32[k]
+                 : p->f64_rows == 32 ? p->sub_half64[k]
+                                     : p->sub_pairs64[k];
     pairs32[k] = p->sub_pairs32[k];
   }
   return DS_OK;
@@ -1458,30 +1508,67 @@ int ds_problem_set_f32_kernel(ds_problem* p, int kind) {
   return DS_OK;
 }
 
+int ds_problem_set_tile(ds_problem* p, int f64_rows) {
+  if (!p) return DS_E_ARG;
+  if (f64_rows != 16 && f64_rows != 32 && f64_rows != 64)
+    return fail(p->ctx, DS_E_ARG, "FP64 tile rows must be 16, 32 or 64");
+  p->f64_rows = f64_rows;
+  return DS_OK;
+}
+
+int ds_problem_get_launch(const ds_problem* p, int* waves, int* order,
+                          int* f64_rows, int* f32_kind) {
+  if (!p) return DS_E_ARG;
+  if (waves) *waves = p->waves_per_block;
+  if (order) *order = p->order_by_cost;
+  if (f64_rows) *f64_rows = p->f64_rows;
+  if (f32_kind) *f32_kind = p->f32_kind;
+  return DS_OK;
+}
+
 int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
                         int* order_out, double* ms_out) {
   if (!p || reps < 1) return DS_E_ARG;
+  if (prec != DS_PREC_F64 && prec != DS_PREC_F32)
+    return fail(p->ctx, DS_E_ARG, "unknown precision %d", prec);
+  // the candidates: FP32 kernel form (VALU 64 x 16 or MFMA 16 x 16) or FP64
+  // tile rows (64, 32, 16 x 16 slots), x waves per block x launch order
+  const int w0 = p->waves_per_block, o0 = p->order_by_cost;
+  const int k0 = p->f32_kind, r0 = p->f64_rows;
   double best = 1e300;
-  int bw = 4, bo = 0, bk = p->f32_kind;
-  const int nkind = prec == DS_PREC_F32 ? 2 : 1;
-  for (int kind = 0; kind < nkind; ++kind)
+  int bw = w0, bo = o0, bk = k0, br = r0;
+  const bool f32 = prec == DS_PREC_F32;
+  const int kinds[2] = {DS_F32_VALU, DS_F32_MFMA};
+  const int rows[3] = {64, 32, 16};
+  const int nshape = f32 ? 2 : 3;
+  for (int sh = 0; sh < nshape; ++sh)
     for (int order = 0; order < 2; ++order)
       for (int waves : {1, 2, 4}) {
         p->waves_per_block = waves;
         p->order_by_cost = order;
-        if (nkind == 2) p->f32_kind = kind;
+        if (f32) p->f32_kind = kinds[sh];
+        else p->f64_rows = rows[sh];
         double ms = 0.0;
-        if (int rc = ds_problem_time_matvec(p, prec, reps, &ms)) return rc;
+        if (int rc = ds_problem_time_matvec(p, prec, reps, &ms)) {
+          // leave the launch settings as they were before the call
+          p->waves_per_block = w0;
+          p->order_by_cost = o0;
+          p->f32_kind = k0;
+          p->f64_rows = r0;
+          return rc;
+        }
         if (ms < best) {
           best = ms;
           bw = waves;
           bo = order;
           bk = p->f32_kind;
+          br = p->f64_rows;
         }
       }
   p->waves_per_block = bw;
   p->order_by_cost = bo;
   p->f32_kind = bk;
+  p->f64_rows = br;
   if (waves_out) *waves_out = bw;
   if (order_out) *order_out = bo;
   if (ms_out) *ms_out = best;

@@ -22,6 +22,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -131,7 +132,16 @@ struct ds_ctx {
   // the device address of the no-alloc entry points' static scratch array
   // on this context's device (looked up once per device at creation)
   unsigned char* noalloc = nullptr;
+  // owner + every live problem and warm start created on this context: the
+  // context is freed when the last of them lets go (so destroying it before
+  // its problems, as a garbage collector may, frees nothing they still use)
+  std::atomic<int> refs{1};
 };
+
+namespace ds {
+void ctx_retain(ds_ctx* c);
+void ctx_release(ds_ctx* c);
+}  // namespace ds
 
 struct ds_problem {
   ds_ctx* ctx = nullptr;
@@ -143,7 +153,9 @@ struct ds_problem {
   std::vector<int> perm;               // slot -> caller index, -1 for padding
   long long ntile_pairs = 0;           // (block, tile) pairs after culling
   long long nsub_pairs = 0;            // (16-row sub-block, tile) pairs
+  long long nhalf_pairs = 0;           // (32-row half block, tile) pairs
   std::vector<long long> sub_pairs64, sub_pairs32;   // per subvolume
+  std::vector<long long> sub_half64;   // per subvolume, 32-row FP64 tiles
   double norm_b = 0.0;                 // |b| of subvolume 0
   // subvolumes
   int n_sub = 1;
@@ -178,6 +190,9 @@ struct ds_problem {
   int dense_ld = 0, dense_mpad = 0, dense_split = 4;
   double setup_seconds = 0.0;
   int waves_per_block = 4;             // pair-kernel launch shape
+  int f64_rows = 16;                   // FP64 kernel tile: rows per group
+                                       // (64, or 16 / 32 with sub-block
+                                       // culling) x 16 source slots
   int order_by_cost = 1;               // launch blocks longest list first
   // device buffers
   double4* pos64 = nullptr;

@@ -50,13 +50,64 @@ __device__ __forceinline__ double accumulate_tiles64(
       double xx = (cp * x_perp_2 + cl * x_para_2) * dxi;
       xx = fmin(xx, tlast);
       const int ix = (int)xx;
-      const double t = xx - (double)ix;
+      const double t = __builtin_amdgcn_fract(xx);   // == xx - ix, exactly
       const double2 f = tab[ix];
       const double gp = fma(f.y, t, f.x);
       acc = SQUARE ? fma(gp * gp, uj, acc) : fma(gp, uj, acc);
     }
   }
   return acc;
+}
+
+// The same sums with the target block cut into row groups of R rows
+// (R = 16 or 32): lane (r, q) = (lane mod R, lane / R) holds rows g R + r of
+// every group g and sources q, q + 64/R, ... of each listed tile, so a
+// wave evaluates only the groups an entry's sub-block mask marks live
+// (16-row sub-blocks that pass the cutoff box test; a 32-row group is live
+// when either of its two is).  acc[g] holds the lane's partial sum of row
+// g R + r.  Every pair's value is the R = 64 form's (same operations); the
+// sums run in another order.
+template <int R>
+__device__ __forceinline__ void accumulate_groups64(
+    const double (&px)[DS_BS / R], const double (&py)[DS_BS / R],
+    const double (&pz)[DS_BS / R], double (&acc)[DS_BS / R],
+    const int* __restrict__ tidx, const unsigned char* __restrict__ tmask,
+    int k, const int ke, const int step, const double4* __restrict__ pos,
+    const double* __restrict__ u, const double2* tab, const double cp,
+    const double cl, const double dxi, const double tlast) {
+  constexpr int G = DS_BS / R, NQ = 64 / R, NJ = DS_TS / NQ;
+  const int q = (threadIdx.x & 63) / R;
+  for (; k < ke; k += step) {
+    const int j0 = __builtin_amdgcn_readfirstlane(tidx[k]) * DS_TS;
+    const unsigned m = __builtin_amdgcn_readfirstlane(tmask[k]);
+    double4 pj[NJ];
+    double uj[NJ];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      pj[t] = pos[j0 + q + t * NQ];
+      uj[t] = u[j0 + q + t * NQ];
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const unsigned bits = R == 16 ? (m >> g) & 1u : (m >> (2 * g)) & 3u;
+      if (!bits) continue;   // wave-uniform
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) {
+        const double dx = px[g] - pj[t].x;
+        const double dy = py[g] - pj[t].y;
+        const double dz = pz[g] - pj[t].z;
+        const double x_perp_2 = dx * dx + dy * dy;
+        const double x_para_2 = dz * dz;
+        double xx = (cp * x_perp_2 + cl * x_para_2) * dxi;
+        xx = fmin(xx, tlast);
+        const int ix = (int)xx;
+        const double tt = __builtin_amdgcn_fract(xx);   // == xx - ix
+        const double2 f = tab[ix];
+        const double gp = fma(f.y, tt, f.x);
+        acc[g] = fma(gp, uj[t], acc[g]);
+      }
+    }
+  }
 }
 
 struct MatvecArgs {
@@ -69,6 +120,7 @@ struct MatvecArgs {
   int minv_mod;
   const int* tptr;
   const int* tidx;
+  const unsigned char* tmask;   // per entry: live 16-row sub-blocks
   const double2* table;
   const int* blk_sub;     // subvolume of each block
   const int* blk_nreal;   // real rows of each block
@@ -85,7 +137,7 @@ struct MatvecArgs {
   int need_want;          // skip unless the subvolume wants a true residual
 };
 
-template <int FORM, int EPI>
+template <int FORM, int EPI, int R>
 __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   extern __shared__ double2 tab[];
   __shared__ double red[4 * 64];
@@ -107,10 +159,37 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   const bool real = lane < a.blk_nreal[I];
   const double4 pi = a.pos[row];
   const int kb = a.tptr[I], ke = a.tptr[I + 1];
-  const double acc = accumulate_tiles64<false>(
-      pi.x, pi.y, pi.z, a.tidx, kb + wave, ke, nw, a.pos, a.u, tab, cvs.y,
-      cvs.z, a.dx_inv, a.tlast);
-  red[wave * 64 + lane] = acc;
+  if (R == DS_BS) {
+    const double acc = accumulate_tiles64<false>(
+        pi.x, pi.y, pi.z, a.tidx, kb + wave, ke, nw, a.pos, a.u, tab, cvs.y,
+        cvs.z, a.dx_inv, a.tlast);
+    red[wave * 64 + lane] = acc;
+  } else {
+    constexpr int G = DS_BS / (R < DS_BS ? R : DS_BS);
+    constexpr int RR = DS_BS / G;
+    const int r = lane % RR;
+    double px[G], py[G], pz[G], acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const double4 pg = a.pos[I * DS_BS + g * RR + r];
+      px[g] = pg.x;
+      py[g] = pg.y;
+      pz[g] = pg.z;
+      acc[g] = 0.0;
+    }
+    accumulate_groups64<RR>(px, py, pz, acc, a.tidx, a.tmask, kb + wave, ke,
+                            nw, a.pos, a.u, tab, cvs.y, cvs.z, a.dx_inv,
+                            a.tlast);
+    // sum the 64/R source columns of each row in a fixed order
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      for (int off = RR; off < 64; off <<= 1)
+        acc[g] += __shfl_xor(acc[g], off, 64);
+    if (lane < RR) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) red[wave * 64 + g * RR + r] = acc[g];
+    }
+  }
   __syncthreads();
   double v[2] = {0.0, 0.0};
   if (wave == 0) {
@@ -216,7 +295,7 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
       double xx = (a.c_perp * x_perp_2 + a.c_para * x_para_2) * a.dx_inv;
       xx = fmin(xx, a.tlast);
       const int ix = (int)xx;
-      const double t = xx - (double)ix;
+      const double t = __builtin_amdgcn_fract(xx);   // == xx - ix, exactly
       const double2 f = tab[ix];
       const double gp = fma(f.y, t, f.x);
 #pragma unroll
@@ -447,6 +526,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.minv_mod = p->minv_mod;
   a.tptr = p->tptr;
   a.tidx = p->tidx;
+  a.tmask = p->tmask;
   a.table = p->table64;
   a.blk_sub = p->blk_sub;
   a.blk_nreal = p->blk_nreal;
@@ -466,8 +546,15 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   const dim3 grid(p->nblk), block(64 * p->waves_per_block);
   const size_t shm = sizeof(double2) * p->cv.table_n;
   const bool w = p->form == DS_FORM_W;
-#define DS_L(F, E) \
-  hipLaunchKernelGGL((k_matvec64<F, E>), grid, block, shm, st, a)
+#define DS_L(F, E)                                                         \
+  do {                                                                     \
+    if (p->f64_rows == 16)                                                 \
+      hipLaunchKernelGGL((k_matvec64<F, E, 16>), grid, block, shm, st, a); \
+    else if (p->f64_rows == 32)                                            \
+      hipLaunchKernelGGL((k_matvec64<F, E, 32>), grid, block, shm, st, a); \
+    else                                                                   \
+      hipLaunchKernelGGL((k_matvec64<F, E, 64>), grid, block, shm, st, a); \
+  } while (0)
   if (e.kind == EPI_NONE) {
     if (w) DS_L(DS_FORM_W, EPI_NONE); else DS_L(DS_FORM_SN, EPI_NONE);
   } else if (e.kind == EPI_CG_DQ) {

@@ -141,6 +141,7 @@ int ds_ws_create(ds_ctx* ctx, unsigned long long seed, ds_warmstart** out) {
   WS_HIP(ws, hipMemcpyAsync(T.iota, iota.data(), sizeof(int) * WS_MAX_BATCH,
                             hipMemcpyHostToDevice, st));
   WS_HIP(ws, hipStreamSynchronize(st));
+  ds::ctx_retain(ctx);
   *out = ws.release();
   return DS_OK;
 }
@@ -156,7 +157,9 @@ int ds_ws_destroy(ds_warmstart* ws) {
   if (ws->fscratch) (void)hipFree(ws->fscratch);
   if (ws->tscratch) (void)hipFree(ws->tscratch);
   if (ws->iscratch) (void)hipFree(ws->iscratch);
+  ds_ctx* ctx = ws->ctx;
   delete ws;
+  ds::ctx_release(ctx);
   return DS_OK;
 }
 

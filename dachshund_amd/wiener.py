@@ -314,6 +314,19 @@ class _WFVector:
         """DS_F32_MFMA (default) or DS_F32_VALU form of the FP32 kernel."""
         check(lib().ds_problem_set_f32_kernel(self.h, kind), self.ctx.h)
 
+    def set_tile(self, f64_rows):
+        """FP64 pair-kernel tile: rows per group (64, 32 or 16) x 16
+        source slots; 32 and 16 skip dead 16-row sub-blocks."""
+        check(lib().ds_problem_set_tile(self.h, f64_rows), self.ctx.h)
+
+    def launch(self):
+        """(waves per block, longest-first order, FP64 tile rows, FP32
+        kernel form) in use."""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().ds_problem_get_launch(self.h, *[C.byref(x) for x in v]),
+              self.ctx.h)
+        return tuple(x.value for x in v)
+
     def set_precond(self, kind):
         """DS_PRECOND_JACOBI (the reference's; default) or DS_PRECOND_BLOCK
         (exact inverses of the 64-row target blocks) for this problem's

@@ -102,6 +102,8 @@ typedef struct {
   long long sub_pairs_evaluated;  /* of those, evaluated by the MFMA FP32
                                      kernel (each tile pair once, used for
                                      both products) */
+  long long f64_pairs_per_matvec; /* pairs the FP64 kernel evaluates per
+                                     matvec at its current tile shape */
 } ds_problem_stats;
 
 /* ---- host-only planning (no device needed) ---------------------------- */
@@ -119,6 +121,10 @@ int ds_plan_tiles(int n, const double* pts, int stride, const ds_covar* covar,
 
 /* ---- context ---------------------------------------------------------- */
 int ds_ctx_create(int device, ds_ctx** out);
+/* Releases the caller's hold on ctx; the context (its stream) is freed once
+ * every problem and warm start created on it has been destroyed too, in
+ * whatever order the caller (or a garbage collector) destroys them.  The
+ * caller must not use ctx after this call. */
 int ds_ctx_destroy(ds_ctx* ctx);
 const char* ds_ctx_last_error(const ds_ctx* ctx);
 int ds_ctx_synchronize(ds_ctx* ctx);
@@ -147,8 +153,9 @@ int ds_batch_create(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
                     const double* pixels, const ds_covar* covars,
                     ds_problem** out);
 int ds_batch_num_sub(const ds_problem* p);
-/* Evaluated pairs per matvec of each subvolume: FP64 kernel (64-row
- * blocks) and MFMA FP32 kernel (16-row sub-block culling). */
+/* Evaluated pairs per matvec of each subvolume: FP64 kernel (at its
+ * current tile shape, ds_problem_set_tile) and MFMA FP32 kernel (16-row
+ * sub-block culling). */
 int ds_batch_sub_pairs(const ds_problem* p, long long* pairs64,
                        long long* pairs32);
 /* Solve every subvolume from the resident x (W space for the W form); res
@@ -170,8 +177,22 @@ int ds_problem_set_launch(ds_problem* p, int waves_per_block,
 int ds_problem_set_precond(ds_problem* p, int kind);
 /* FP32 pair kernel variant (DS_F32_VALU or DS_F32_MFMA). */
 int ds_problem_set_f32_kernel(ds_problem* p, int kind);
-/* Time every launch shape (and, for FP32, both kernel variants) on this
- * problem's pair kernel (prec) and keep the fastest. */
+/* FP64 pair kernel tile shape (no reference counterpart; the loop it
+ * tiles is lib/wf_vector.cc:199-219): rows per group x 16 source slots.
+ * 64: one group per 64-row target block, every listed (block, tile) pair
+ * evaluated; 32 or 16: the block's 16-row sub-blocks that fail the cutoff
+ * box test against a tile are skipped (lanes hold 16 or 32 rows x 4 or 2
+ * source columns).  Changes only the order of a row's sums. */
+int ds_problem_set_tile(ds_problem* p, int f64_rows);
+/* The current launch settings (any pointer may be NULL). */
+int ds_problem_get_launch(const ds_problem* p, int* waves, int* order,
+                          int* f64_rows, int* f32_kind);
+/* Time every launch shape on this problem's pair kernel of precision prec
+ * and keep the fastest: waves per block (1, 2, 4) x launch order x, for
+ * FP32, the kernel form (VALU 64 x 16 tiles or MFMA 16 x 16 sub-block
+ * tiles) or, for FP64, the tile rows (64, 32, 16).  On an error the
+ * previous settings are restored.  ds_problem_get_launch reports the
+ * choice. */
 int ds_problem_autotune(ds_problem* p, int prec, int reps, int* waves_out,
                         int* order_out, double* ms_out);
 

@@ -170,15 +170,68 @@ def test_dense_bf16_block_solve(ora, mode):
 def test_multi_rhs_residual_equals_per_subvolume(waves):
     """The one-pass FP64 residual of a shared-geometry batch (the mixed
     solver's true residual when a multi-RHS operator is selected) equals the
-    per-subvolume parity kernel bit for bit: same pair values, same source
-    order, same wave split."""
+    per-subvolume parity kernel at its 64-row tile bit for bit: same pair
+    values, same source order, same wave split."""
     import dachshund_amd as D
     w, sets = _realizations(7, seed=31)
     sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
     batch = D.BatchWF([D.pixel_to_wpixel(p) for p in sets], [sp] * len(sets))
     batch.set_launch(waves, True)
+    batch.set_tile(64)
     x = np.random.default_rng(2).standard_normal(batch.n)
     r_sub = batch.A_residual(x)
     batch.set_fused(True)
     r_one = batch.A_residual(x)
     assert np.array_equal(r_one, r_sub)
+
+
+@pytest.mark.parametrize("rows", [16, 32])
+def test_batch_tile_shapes_equal_single(rows):
+    """At the sub-block-culled FP64 tile shapes a batch's plain FP64 solve
+    of each subvolume still equals that subvolume solved alone, bit for
+    bit, and agrees with the 64-row shape to the solve tolerance."""
+    import dachshund_amd as D
+    subs, pix = _subvolumes(count=4)
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    wpix = [D.pixel_to_wpixel(p) for p in pix]
+    batch = D.BatchWF(wpix, sps)
+    batch.set_launch(4, False)
+    batch.set_tile(rows)
+    prm = D.PCGParams(batch.n, 1000, 50, 1e-8)
+    batch.fill_x(0.0)
+    res = batch.solve(prm, mixed=False, use_graph=False)
+    xb = batch.split(batch.get_x())
+    batch.set_tile(64)
+    batch.fill_x(0.0)
+    batch.solve(prm, mixed=False, use_graph=False)
+    x64 = batch.split(batch.get_x())
+    for k, (p, sp) in enumerate(zip(wpix, sps)):
+        wf = D.WFVectorW(p, sp)
+        wf.set_launch(4, False)
+        wf.set_tile(rows)
+        x = np.zeros(len(p))
+        r = wf.solve_pcg(D.PCGParams(len(p), 1000, 50, 1e-8), x)
+        assert res[k].success and r.success
+        assert np.array_equal(xb[k], x), k
+        assert np.linalg.norm(xb[k] - x64[k]) <= 1e-6 * np.linalg.norm(x64[k])
+
+
+def test_autotune_picks_a_shape_and_keeps_results():
+    """The autotuner times every FP64 tile shape x waves x order and keeps
+    the fastest; the solution at the tuned shape agrees with the default
+    shape's to the solve tolerance (only the order of sums changes)."""
+    import dachshund_amd as D
+    subs, pix = _subvolumes(count=3)
+    sps = [D.SignalCovarParams(w.var_s, w.l_perp, w.l_para) for w in subs]
+    batch = D.BatchWF([D.pixel_to_wpixel(p) for p in pix], sps)
+    prm = D.PCGParams(batch.n, 1000, 50, 1e-8)
+    batch.fill_x(0.0)
+    batch.solve(prm, mixed=False, use_graph=False)
+    x0 = batch.get_x()
+    waves, order, ms = batch.autotune(prec=D.DS_PREC_F64, reps=3)
+    w, o, rows, _ = batch.launch()
+    assert (w, o) == (waves, order) and rows in (16, 32, 64) and ms > 0
+    batch.fill_x(0.0)
+    batch.solve(prm, mixed=False, use_graph=False)
+    x1 = batch.get_x()
+    assert np.linalg.norm(x1 - x0) <= 1e-6 * np.linalg.norm(x0)

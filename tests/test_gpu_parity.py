@@ -513,3 +513,70 @@ def test_matvec64_cutoff_decision(D, ora, form, lp, ll):
         assert np.array_equal(got == 0.0, ref == 0.0)
         assert np.all(np.abs(got - ref) <= 8 * np.spacing(np.abs(ref)))
     assert 0 < np.sum(xs >= 199.0) < m
+
+
+@pytest.mark.parametrize("rows", [16, 32, 64])
+@pytest.mark.parametrize("form", ["w", "sn"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_matvec64_tile_shapes(D, ora, case, form, rows):
+    """The FP64 kernel's tile shapes (16, 32 or 64 rows per group x 16
+    slots; the smaller ones skip the sub-blocks the cutoff box test
+    culls): A_product and A_residual meet the 1e-12 row bound against the
+    oracle at every shape, and the launch settings report the shape."""
+    pix = CASES[case]
+    f = FORMS[form]
+    params = (0.05, 2.5, 2.1)
+    fp = ora.to_form(pix, f)
+    c = ora.covar(*params)
+    wf, _ = _problem(D, form, pix, params)
+    wf.set_tile(rows)
+    assert wf.launch()[2] == rows
+    a = np.random.default_rng(len(pix) + 3).standard_normal(len(pix))
+    bound = ora.A_product(f, fp, c, np.abs(a))
+    assert np.all(np.abs(wf.A_product(a) - ora.A_product(f, fp, c, a))
+                  <= 1e-12 * bound)
+    rb = bound + np.abs(fp[:, 4])
+    assert np.all(np.abs(wf.A_residual(a) - ora.A_residual(f, fp, c, a))
+                  <= 1e-12 * rb)
+
+
+@pytest.mark.parametrize("rows", [16, 32])
+def test_matvec64_cutoff_decision_tiles(D, ora, rows):
+    """The cutoff-straddling pairs through the sub-block-culled shapes:
+    the oracle's zero pattern exactly, values to 8 ulps."""
+    pairs = _cutoff_pairs(2.5, 2.1)
+    m = len(pairs)
+    pix = np.zeros((2 * m, 5))
+    for p, (dy, dz, _) in enumerate(pairs):
+        pix[2 * p, 0] = pix[2 * p + 1, 0] = 25.0 * p
+        pix[2 * p + 1, 1] = dy
+        pix[2 * p + 1, 2] = dz
+    pix[:, 3] = 0.2
+    fp = ora.to_form(pix, 0)
+    c = ora.covar(0.05, 2.5, 2.1)
+    wf, _ = _problem(D, "sn", pix, (0.05, 2.5, 2.1))
+    wf.set_tile(rows)
+    a = np.zeros(2 * m)
+    a[0::2] = 1.0
+    ref = ora.A_product(0, fp, c, a)[1::2]
+    got = wf.A_product(a)[1::2]
+    assert np.array_equal(got == 0.0, ref == 0.0)
+    assert np.all(np.abs(got - ref) <= 8 * np.spacing(np.abs(ref)))
+
+
+def test_context_released_before_its_problem(D):
+    """A context destroyed while a problem made on it is alive (the order a
+    garbage collector may choose at interpreter exit) stays usable by that
+    problem until the problem is destroyed: the library counts the
+    context's holders and frees it with the last one."""
+    from dachshund_amd import synth
+    ctx = D.Context()
+    w = synth.workload("small")
+    pix = synth.make_pixels(w)
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp, ctx=ctx)
+    a = np.ones(len(pix))
+    before = wf.A_product(a)
+    ctx.close()
+    assert np.array_equal(wf.A_product(a), before)
+    wf.close()
