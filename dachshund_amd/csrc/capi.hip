@@ -449,6 +449,7 @@ void ctx_retain(ds_ctx* c) { c->refs.fetch_add(1); }
 void ctx_release(ds_ctx* c) {
   if (c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -555,6 +556,8 @@ int ds_ctx_create(int device, ds_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return DS_E_HIP;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return DS_E_HIP;
+  if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess)
+    return DS_E_HIP;
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount,
                             device) != hipSuccess)
     return DS_E_HIP;
@@ -576,6 +579,7 @@ const char* ds_ctx_last_error(const ds_ctx* ctx) {
 int ds_ctx_synchronize(ds_ctx* ctx) {
   if (!ctx) return DS_E_ARG;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->aux));
   return DS_OK;
 }
 
@@ -1039,6 +1043,11 @@ int ds_problem_destroy(ds_problem* p) {
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
   ds_ctx* ctx = p->ctx;
+  if (p->minv_ev) {
+    (void)hipEventSynchronize(p->minv_ev);   // an inverse build in flight
+    (void)hipEventDestroy(p->minv_ev);
+    p->minv_ev = nullptr;
+  }
   problem_free(p);
   delete p;
   ds::ctx_release(ctx);
@@ -1488,13 +1497,22 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
   if (kind != DS_PRECOND_JACOBI && kind != DS_PRECOND_BLOCK)
     return fail(p->ctx, DS_E_ARG, "unknown preconditioner %d", kind);
   if (kind == DS_PRECOND_BLOCK && !p->minv) {
-    hipStream_t st = p->ctx->stream;
+    // built on the context's second stream, after the work already on the
+    // main stream, so that it overlaps what the caller enqueues next (the
+    // warm start's neighbour search); the first reader of the inverses on
+    // the main stream waits for it (ds::minv_join)
+    ds_ctx* ctx = p->ctx;
     // subvolumes sharing one geometry share one set of block inverses
     p->minv_mod = p->shared_geometry ? p->nblk / p->n_sub : p->nblk;
-    HIP_OR_FAIL(p->ctx, dev_alloc(&p->minv,
-                                  (size_t)p->minv_mod * DS_BJ_SIZE));
-    HIP_OR_FAIL(p->ctx, ds::bj_build(p, st));
-    HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+    HIP_OR_FAIL(ctx, dev_alloc(&p->minv, (size_t)p->minv_mod * DS_BJ_SIZE));
+    if (!p->minv_ev)
+      HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&p->minv_ev,
+                                               hipEventDisableTiming));
+    HIP_OR_FAIL(ctx, hipEventRecord(p->minv_ev, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamWaitEvent(ctx->aux, p->minv_ev, 0));
+    HIP_OR_FAIL(ctx, ds::bj_build(p, ctx->aux));
+    HIP_OR_FAIL(ctx, hipEventRecord(p->minv_ev, ctx->aux));
+    p->minv_pending = true;
   }
   p->precond_kind = kind;
   return DS_OK;

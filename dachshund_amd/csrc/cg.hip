@@ -601,6 +601,7 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
 hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
                       double drop, ds_pcg_result* res, int use_graph) {
   hipStream_t st = p->ctx->stream;
+  DS_TRY(minv_join(p, st));   // before any capture below
   const int max_iter = prm->max_iter;
   const int ns = p->n_sub;
   const int cap = std::max(max_iter, 1) + 1;

@@ -124,6 +124,8 @@ struct ds_cg_scalars {
 struct ds_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;   // second stream (default priority): work
+                               // that may overlap the main stream's
   char err[512] = {0};         // last error (fixed storage: setting it
                                // allocates nothing)
   int num_cu = 0;
@@ -173,6 +175,10 @@ struct ds_problem {
                                        // (shared geometry: one set)
   int precond_kind = DS_PRECOND_JACOBI;
   bool use_minv() const { return precond_kind == DS_PRECOND_BLOCK && minv; }
+  // the inverses are built on the context's second stream; the first
+  // kernel on the main stream that reads them waits for this event
+  hipEvent_t minv_ev = nullptr;
+  mutable bool minv_pending = false;
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
   int* sub_blk0 = nullptr;             // first block per subvolume
@@ -250,6 +256,12 @@ struct ds_problem {
 };
 
 namespace ds {
+// Order `st` after the block-Jacobi build (once per build).
+inline hipError_t minv_join(const ds_problem* p, hipStream_t st) {
+  if (!p->minv_pending) return hipSuccess;
+  p->minv_pending = false;
+  return hipStreamWaitEvent(st, p->minv_ev, 0);
+}
 hipError_t launch_scale(const double* v, const double* w, double* u,
                         float* u32, int npad, hipStream_t st);
 // Block-Jacobi: build the inverses (minv allocated); apply the problem's

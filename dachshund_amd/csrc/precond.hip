@@ -141,6 +141,7 @@ hipError_t bj_build(ds_problem* p, hipStream_t st) {
 hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
                             hipStream_t st) {
   if (!p->use_minv()) return launch_precond(b, p->diag, x, p->npad, st);
+  if (hipError_t e = minv_join(p, st)) return e;
   hipLaunchKernelGGL(k_bj_apply, dim3((p->nblk + 3) / 4), dim3(256), 0, st, b,
                      p->minv, x, p->nblk, p->minv_mod);
   return hipGetLastError();

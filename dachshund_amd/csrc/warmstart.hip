@@ -50,6 +50,7 @@ struct KnnArgs {
   const int* tptr;
   const int* tidx;
   int* nbr;               // npad x K neighbour rows, -1 = none
+  const float4* tbox;     // per tile: min, max of its records (tile frame)
   unsigned short* feat;   // npad x WS_FEAT
   int n, form;
   float inv_lp, inv_ll, inv_sqrt_vs, cutoff;
@@ -76,53 +77,98 @@ __device__ __forceinline__ float knn_d2(const float4& li, const float4& o,
 // so that it can be written without branches.  The tile data is
 // wave-uniform and read-only here (restrict: scalar loads).
 //
-// Filter: an entry's pairs are all at least lb = |o| - r_blk - r_tile
-// apart (o: block frame minus tile frame, r_blk: the largest |row| in the
-// block frame, r_tile: the largest |source| in the tile frame, all in the
-// scaled units where distance^2 = d2), less a margin that covers FP32
-// rounding; entries with lb in [lo, hi) are visited.  With hi = r, every
-// pair with computed d2 < r^2 lies in a visited entry; with lo = r, none
-// of the visited pairs has d2 < r^2.
+// Filter: an entry's pairs are all at least lb apart, the gap between the
+// block's box (bmin, bmax: the rows in the block frame, shifted into the
+// tile frame by o = block frame minus tile frame) and the tile's box (its
+// 16 records in the tile frame), in the scaled units where distance^2 =
+// d2, less a margin that covers FP32 rounding; entries with lb in [lo, hi)
+// are visited.  With hi = r, every pair with computed d2 < r^2 lies in a
+// visited entry; with lo = r, none of the visited pairs has d2 < r^2.
 template <class F>
 __device__ __forceinline__ void knn_scan(const KnnArgs& a,
                                          const float4* __restrict__ loc,
                                          const float4* __restrict__ toff,
                                          int I, int row, const float4& li,
-                                         float r_blk, float lo, float hi,
-                                         F&& visit) {
+                                         const float4& bmin,
+                                         const float4& bmax, float lo,
+                                         float hi, F&& visit) {
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]);
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
   const bool live = row < a.n;
   const bool all = lo == -__builtin_inff() && hi == __builtin_inff();
-  for (int e = kb; e < ke; ++e) {
-    const int j0 = __builtin_amdgcn_readfirstlane(a.tidx[e]) * DS_TS;
-    const float4 o = toff[e];
-    // the tile's 16 records in one batch of scalar loads
-    float4 pj[DS_TS];
-#pragma unroll
-    for (int jj = 0; jj < DS_TS; ++jj) pj[jj] = loc[j0 + jj];
-    if (!all) {
-      float rt2 = pj[0].w;
-#pragma unroll
-      for (int jj = 1; jj < DS_TS; ++jj) rt2 = fmaxf(rt2, pj[jj].w);
-      const float on = sqrtf(fmaf(o.z, o.z, fmaf(o.y, o.y, o.x * o.x)));
-      const float rt = sqrtf(rt2);
-      const float lb = on - r_blk - rt - (1e-3f + 1e-4f * (on + r_blk + rt));
-      if (!(lb >= lo && lb < hi)) continue;
-    }
-    if (j0 + DS_TS <= a.n) {   // no padding slot in this tile
-#pragma unroll
-      for (int jj = 0; jj < DS_TS; ++jj) {
-        const int j = j0 + jj;
-        const float d2 = knn_d2(li, o, pj[jj]);
-        visit(d2, j, live && j != row && d2 < a.cutoff);
+  const int lane = threadIdx.x & 63;
+  // 64 entries at a time: lane k reads entry base + k and applies the
+  // filter, so the entries' loads overlap instead of forming a chain; the
+  // passing entries then run in ascending order
+  for (int base = kb; base < ke; base += 64) {
+    const int el = base + lane;
+    int Tl = 0;
+    float4 ol = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool pass = false;
+    if (el < ke) {
+      Tl = a.tidx[el];
+      ol = toff[el];
+      pass = true;
+      if (!all) {
+        const float4 tmin = a.tbox[2 * Tl], tmax = a.tbox[2 * Tl + 1];
+        const float gx = fmaxf(0.f, fmaxf(tmin.x - (bmax.x + ol.x),
+                                          (bmin.x + ol.x) - tmax.x));
+        const float gy = fmaxf(0.f, fmaxf(tmin.y - (bmax.y + ol.y),
+                                          (bmin.y + ol.y) - tmax.y));
+        const float gz = fmaxf(0.f, fmaxf(tmin.z - (bmax.z + ol.z),
+                                          (bmin.z + ol.z) - tmax.z));
+        const float scale = fabsf(ol.x) + fabsf(ol.y) + fabsf(ol.z) +
+                            (bmax.x - bmin.x) + (bmax.y - bmin.y) +
+                            (bmax.z - bmin.z);
+        const float lb = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx))) -
+                         (1e-3f + 1e-5f * scale);
+        pass = lb >= lo && lb < hi;
       }
-    } else {
+    }
+    unsigned long long m = __ballot(pass);
+    // the passing entries' records come in one per lane (lanes l and
+    // l + 16k hold record l), the next entry's loading while the current
+    // one computes, and are broadcast with readlane
+    const int sl = lane & (DS_TS - 1);
+    int k = m ? __builtin_ctzll(m) : 0;
+    float4 nxt = m ? loc[__builtin_amdgcn_readlane(Tl, k) * DS_TS + sl]
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+    while (m) {
+      m &= m - 1;
+      const int T = __builtin_amdgcn_readlane(Tl, k);
+      float4 o;
+      o.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ol.x), k));
+      o.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ol.y), k));
+      o.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ol.z), k));
+      o.w = 0.f;
+      const float4 cur = nxt;
+      if (m) {
+        k = __builtin_ctzll(m);
+        nxt = loc[__builtin_amdgcn_readlane(Tl, k) * DS_TS + sl];
+      }
+      const int j0 = T * DS_TS;
+      auto rec = [&](int jj) {
+        float4 r;
+        r.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.x), jj));
+        r.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), jj));
+        r.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), jj));
+        r.w = 0.f;
+        return r;
+      };
+      if (j0 + DS_TS <= a.n) {   // no padding slot in this tile
 #pragma unroll
-      for (int jj = 0; jj < DS_TS; ++jj) {
-        const int j = j0 + jj;
-        const float d2 = knn_d2(li, o, pj[jj]);
-        visit(d2, j, live && j < a.n && j != row && d2 < a.cutoff);
+        for (int jj = 0; jj < DS_TS; ++jj) {
+          const int j = j0 + jj;
+          const float d2 = knn_d2(li, o, rec(jj));
+          visit(d2, j, live && j != row && d2 < a.cutoff);
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < DS_TS; ++jj) {
+          const int j = j0 + jj;
+          const float d2 = knn_d2(li, o, rec(jj));
+          visit(d2, j, live && j < a.n && j != row && d2 < a.cutoff);
+        }
       }
     }
   }
@@ -147,6 +193,26 @@ __device__ __forceinline__ int knn_bin(float d2, float lo, float scale) {
 // (d, j) < (e, k) in (distance, index) order.
 __device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
   return d < e || (d == e && (unsigned)j < (unsigned)k);
+}
+
+// The box of each tile's 16 records in its frame (padding slots included:
+// the box only grows).
+__global__ __launch_bounds__(256) void k_tile_boxes(
+    const float4* __restrict__ loc, int ntile, float4* tbox) {
+  const int T = blockIdx.x * 256 + threadIdx.x;
+  if (T >= ntile) return;
+  float4 lo = loc[T * DS_TS], hi = lo;
+  for (int jj = 1; jj < DS_TS; ++jj) {
+    const float4 v = loc[T * DS_TS + jj];
+    lo.x = fminf(lo.x, v.x);
+    lo.y = fminf(lo.y, v.y);
+    lo.z = fminf(lo.z, v.z);
+    hi.x = fmaxf(hi.x, v.x);
+    hi.y = fmaxf(hi.y, v.y);
+    hi.z = fmaxf(hi.z, v.z);
+  }
+  tbox[2 * T] = lo;
+  tbox[2 * T + 1] = hi;
 }
 
 // The K nearest neighbours of each row in (d2, index) order, one wave per
@@ -210,12 +276,17 @@ __global__ __launch_bounds__(64) void k_knn(
   };
 
   const float kInf = __builtin_inff();
-  // the largest |row| of the block (block frame, scaled units)
-  float rb2 = fmaf(li.z, li.z, fmaf(li.y, li.y, li.x * li.x));
+  // the block's box (its rows in the block frame, scaled units)
+  float4 bmin = li, bmax = li;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1)
-    rb2 = fmaxf(rb2, __shfl_xor(rb2, off, 64));
-  const float r_blk = sqrtf(rb2);
+  for (int off = 1; off < 64; off <<= 1) {
+    bmin.x = fminf(bmin.x, __shfl_xor(bmin.x, off, 64));
+    bmin.y = fminf(bmin.y, __shfl_xor(bmin.y, off, 64));
+    bmin.z = fminf(bmin.z, __shfl_xor(bmin.z, off, 64));
+    bmax.x = fmaxf(bmax.x, __shfl_xor(bmax.x, off, 64));
+    bmax.y = fmaxf(bmax.y, __shfl_xor(bmax.y, off, 64));
+    bmax.z = fmaxf(bmax.z, __shfl_xor(bmax.z, off, 64));
+  }
 
   for (int p = 0; p < KNN_LEVELS; ++p) {
     if (!__any(!settled)) break;
@@ -230,13 +301,13 @@ __global__ __launch_bounds__(64) void k_knn(
       auto add = [&](float d2, int, bool ok) {
         atomicAdd(&buf[knn_bin0(d2) * 64 + lane], ok ? 1u : 0u);
       };
-      knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, R0, add);
+      knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, R0, add);
       int near = 0;
       for (int b = 0; b < kNearBins; ++b) near += (int)buf[b * 64 + lane];
       if (__any(!settled && near < K))
-        knn_scan(a, loc, toff, I, row, li, r_blk, R0, kInf, add);
+        knn_scan(a, loc, toff, I, row, li, bmin, bmax, R0, kInf, add);
     } else {
-      knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, kInf,
+      knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, kInf,
                [&](float d2, int, bool ok) {
                  if (ok && !settled && classify(d2, p) == 0)
                    atomicAdd(&buf[bin_at(d2, p) * 64 + lane], 1u);
@@ -289,7 +360,7 @@ __global__ __launch_bounds__(64) void k_knn(
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1)
       rk = fmaxf(rk, __shfl_xor(rk, off, 64));
-    knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, rk,
+    knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, rk,
              [&](float d2, int j, bool ok) {
       const bool keep = ok && knn_bin0(d2) <= tb && cnt < KNN_CAP;
       cand[(keep ? cnt : KNN_CAP) * 64 + lane] =
@@ -297,7 +368,7 @@ __global__ __launch_bounds__(64) void k_knn(
       cnt += keep ? 1 : 0;
     });
   } else {
-    knn_scan(a, loc, toff, I, row, li, r_blk, -kInf, kInf,
+    knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, kInf,
              [&](float d2, int j, bool ok) {
       if (ok && classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
         cand[cnt * 64 + lane] = make_uint2(__float_as_uint(d2),
@@ -947,10 +1018,16 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     // problem, like its tile lists.  Candidates are kept as list entry * 16
     // + tile slot in 32 bits.
     if (p->ntile_pairs >= (1LL << 28)) return hipErrorInvalidValue;
+    // one allocation: the lists, then the tiles' boxes (kept with them)
     int* nbr = nullptr;
-    hipError_t e = hipMalloc(&nbr, sizeof(int) * (size_t)p->npad * WS_K);
+    const size_t nb = sizeof(int) * (size_t)p->npad * WS_K;
+    hipError_t e = hipMalloc(&nbr, nb + 2 * sizeof(float4) * p->ntile);
     if (e != hipSuccess) return e;
     a.nbr = nbr;
+    float4* tbox = reinterpret_cast<float4*>(reinterpret_cast<char*>(nbr) + nb);
+    a.tbox = tbox;
+    hipLaunchKernelGGL(k_tile_boxes, dim3((p->ntile + 255) / 256), dim3(256),
+                       0, st, p->loc32, p->ntile, tbox);
     hipLaunchKernelGGL(k_knn<WS_K>, dim3(p->nblk), dim3(64), 0, st,
                        p->loc32, p->toff32, a);
     e = hipGetLastError();

@@ -127,6 +127,8 @@ int ds_ctx_create(int device, ds_ctx** out);
  * caller must not use ctx after this call. */
 int ds_ctx_destroy(ds_ctx* ctx);
 const char* ds_ctx_last_error(const ds_ctx* ctx);
+/* Waits for all work the context's calls have enqueued (its main stream
+ * and the second stream that block-Jacobi builds run on). */
 int ds_ctx_synchronize(ds_ctx* ctx);
 int ds_ctx_num_cu(const ds_ctx* ctx);
 
@@ -173,7 +175,9 @@ int ds_problem_set_launch(ds_problem* p, int waves_per_block,
  * counterpart for DS_PRECOND_BLOCK: the reference's preconditioner is
  * Jacobi, lib/wf_vector.cc:42-49, :185-194).  The block inverses are built
  * on the device on first selection (8.4 KB of FP32 per 64 rows: the
- * symmetric inverse stored by diagonals). */
+ * symmetric inverse stored by diagonals), on the context's second stream:
+ * the call returns at once and the build overlaps the work enqueued next
+ * until the first solve or ds_precond, which waits for it. */
 int ds_problem_set_precond(ds_problem* p, int kind);
 /* FP32 pair kernel variant (DS_F32_VALU or DS_F32_MFMA). */
 int ds_problem_set_f32_kernel(ds_problem* p, int kind);
