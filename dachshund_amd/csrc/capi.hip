@@ -1395,52 +1395,94 @@ int ds_map_covar_diag(ds_ctx* ctx, int n_pix, const double* npix, int n_map,
                       int* num_iters, int* success) {
   if (!ctx || !prm || (n_map > 0 && (!sigma || !map_pts))) return DS_E_ARG;
   if (int rc = check_points(ctx, map_pts, 3, n_map, "map points")) return rc;
+  if (int rc = check_covar(ctx, covar)) return rc;
+  if (n_pix <= 0) return fail(ctx, DS_E_ARG, "n must be positive");
+  if (int rc = check_points(ctx, npix, 5, n_pix, "pixels")) return rc;
+  if (n_map == 0) return DS_OK;
+  // The cells' solves share one operator (S + N): they run kMR at a time
+  // as the subvolumes of a shared-geometry batch (R copies of the pixels,
+  // one cell's column as each copy's data), so every kernel of a step
+  // serves all of them; each subvolume's PCG is the single-volume solve bit
+  // for bit (same blocking, same reduction order).
+  constexpr int kMR = DS_DENSE_R;
+  const int R = std::min(kMR, n_map);
+  const size_t n = (size_t)n_pix;
+  std::vector<double> copies(5 * n * R);
+  for (int k = 0; k < R; ++k)
+    for (size_t i = 0; i < n; ++i) {
+      for (int c = 0; c < 4; ++c)
+        copies[5 * (n * k + i) + c] = npix[5 * i + c];
+      copies[5 * (n * k + i) + 4] = 0.0;   // replaced by the columns
+    }
+  std::vector<int> sub_n(R, n_pix);
+  std::vector<ds_covar> covs(R, *covar);
   ds_problem* p = nullptr;
-  if (int rc = ds_problem_create(ctx, DS_FORM_SN, n_pix, npix, covar, &p))
+  if (int rc = ds_batch_create(ctx, DS_FORM_SN, R, sub_n.data(),
+                               copies.data(), covs.data(), &p))
     return rc;
   std::unique_ptr<ds_problem, int (*)(ds_problem*)> guard(p,
                                                          ds_problem_destroy);
-  if (int rc = check_pcg_params(p, prm)) return rc;
+  if (prm->n != n_pix)
+    return fail(ctx, DS_E_ARG, "params.n = %d but the problem has %d pixels",
+                prm->n, n_pix);
+  ds_pcg_params bprm = *prm;   // the batch holds R copies of the pixels
+  bprm.n = p->n;
+  if (int rc = check_pcg_params(p, &bprm)) return rc;
+  if (!p->shared_geometry)
+    return fail(ctx, DS_E_STATE, "map covariance: copies not shared");
+  const int npad1 = p->npad / R;
+  // slot of pixel j in subvolume 0 (every copy has the same blocking)
+  std::vector<int> slot0(n);
+  for (int s = 0; s < npad1; ++s)
+    if (p->perm[s] >= 0) slot0[p->perm[s]] = s;
   hipStream_t st = ctx->stream;
-  DevBuf<double> col, dot;
-  HIP_OR_FAIL(ctx, col.alloc(p->npad));
-  HIP_OR_FAIL(ctx, dot.alloc(1));
-  std::vector<int> slot(p->n);
-  for (int k = 0; k < p->n; ++k) slot[p->blk.perm[k]] = k;
-  double* rhs_saved = p->rhs64;
-  std::vector<double> hcol(p->npad);
-  for (int i = 0; i < n_map; ++i) {
-    // lib/map_covar.cc:40-71: rhs = S(map_i, pix_k), x = 0, SN-form PCG
-    HIP_OR_FAIL(ctx, ds::launch_column(p, p->table_raw, map_pts + 3 * (size_t)i,
-                                       col.p, st));
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(hcol.data(), col.p,
+  DevBuf<double> dots;
+  HIP_OR_FAIL(ctx, dots.alloc(R));
+  std::vector<double> hcol((size_t)p->npad), hdot(R);
+  std::vector<ds_pcg_result> res(R);
+  for (int i0 = 0; i0 < n_map; i0 += R) {
+    const int r = std::min(R, n_map - i0);
+    // lib/map_covar.cc:40-71: rhs = S(map_i, pix_k), x = 0, SN-form PCG;
+    // the copies past the last cell get rhs 0 (|b| = 0: no solve)
+    for (int k = 0; k < R; ++k) {
+      double* col = p->rhs64 + (size_t)k * npad1;
+      if (k < r)
+        HIP_OR_FAIL(ctx, ds::launch_column_sub(
+                             p, k, p->table_raw,
+                             map_pts + 3 * (size_t)(i0 + k), col, st));
+      else
+        HIP_OR_FAIL(ctx, ds::launch_fill(col, 0.0, npad1, st));
+    }
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(hcol.data(), p->rhs64,
                                     sizeof(double) * p->npad,
                                     hipMemcpyDeviceToHost, st));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
-    double sum_b2 = 0.0;   // |b| in the reference's order
-    for (int j = 0; j < p->n; ++j) {
-      const double bj = hcol[slot[j]];
-      sum_b2 += bj * bj;
+    for (int k = 0; k < R; ++k) {
+      double sum_b2 = 0.0;   // |b| in the reference's order
+      const double* col = hcol.data() + (size_t)k * npad1;
+      for (size_t j = 0; j < n; ++j) {
+        const double bj = col[slot0[j]];
+        sum_b2 += bj * bj;
+      }
+      p->sub_norm_b[k] = std::sqrt(sum_b2);
     }
-    p->norm_b = std::sqrt(sum_b2);
-    p->sub_norm_b[0] = p->norm_b;
-    p->rhs64 = col.p;
+    p->norm_b = p->sub_norm_b[0];
     HIP_OR_FAIL(ctx, ds::launch_fill(p->x, 0.0, p->npad, st));
-    ds_pcg_result res;
-    const hipError_t e = ds::run_solver(p, prm, 0, 0.0, &res, 0);
-    p->rhs64 = rhs_saved;
-    HIP_OR_FAIL(ctx, e);
-    double s = 0.0;
-    if (res.success) {
-      // lib/map_covar.cc:95-109: sigma_i = sum_j S_ij x_j
-      HIP_OR_FAIL(ctx, ds::launch_dot1(col.p, p->x, p->npad, dot.p, st));
-      HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, dot.p, sizeof(double),
-                                      hipMemcpyDeviceToHost, st));
-      HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+    HIP_OR_FAIL(ctx, ds::run_solver(p, &bprm, 0, 0.0, res.data(), 0));
+    for (int k = 0; k < r; ++k)
+      if (res[k].success)
+        // lib/map_covar.cc:95-109: sigma_i = sum_j S_ij x_j
+        HIP_OR_FAIL(ctx, ds::launch_dot1(p->rhs64 + (size_t)k * npad1,
+                                         p->x + (size_t)k * npad1, npad1,
+                                         dots.p + k, st));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(hdot.data(), dots.p, sizeof(double) * R,
+                                    hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+    for (int k = 0; k < r; ++k) {
+      sigma[i0 + k] = res[k].success ? hdot[k] : 0.0;
+      if (num_iters) num_iters[i0 + k] = res[k].num_iters;
+      if (success) success[i0 + k] = res[k].success;
     }
-    sigma[i] = s;
-    if (num_iters) num_iters[i] = res.num_iters;
-    if (success) success[i] = res.success;
   }
   return DS_OK;
 }

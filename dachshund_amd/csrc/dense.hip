@@ -114,6 +114,18 @@ hipError_t launch_column(const ds_problem* p, const double* table_dev,
   return hipGetLastError();
 }
 
+hipError_t launch_column_sub(const ds_problem* p, int k,
+                             const double* table_dev, const double* map_pt,
+                             double* col, hipStream_t st) {
+  const int b0 = p->sub_blk0_h[k], b1 = p->sub_blk0_h[k + 1];
+  const int npad1 = (b1 - b0) * DS_BS;
+  hipLaunchKernelGGL(k_column, dim3((npad1 + 255) / 256), dim3(256), 0, st,
+                     p->pos64 + (size_t)b0 * DS_BS, p->sub_n[k], npad1,
+                     map_pt[0], map_pt[1], map_pt[2], elem_args(p, table_dev),
+                     col);
+  return hipGetLastError();
+}
+
 hipError_t launch_dot1(const double* a, const double* b, int n, double* out,
                        hipStream_t st) {
   hipLaunchKernelGGL(k_dot1, dim3(1), dim3(1024), 0, st, a, b, n, out);

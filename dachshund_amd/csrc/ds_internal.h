@@ -293,6 +293,10 @@ hipError_t dense_epilogue(ds_problem* p, hipStream_t st);
 // Fused multi-RHS operator of a shared-geometry batch: q and the d . q
 // partials of every subvolume from one pass over subvolume 0's tile lists.
 hipError_t fused_mrhs(ds_problem* p, hipStream_t st);
+// k_column over subvolume k of a batch (its slots, its pixel count).
+hipError_t launch_column_sub(const ds_problem* p, int k,
+                             const double* table_dev, const double* map_pt,
+                             double* col, hipStream_t st);
 hipError_t launch_column(const ds_problem* p, const double* table_dev,
                          const double* map_pt, double* col, hipStream_t st);
 hipError_t launch_dot1(const double* a, const double* b, int n, double* out,

@@ -428,6 +428,54 @@ def test_map_side_golden(D, ora):
                       + 1e-300)
 
 
+def test_map_covar_diag_golden(D, ora):
+    """MapCovarDiag on the reference fixture against the reference's own
+    sigma, flags and iteration counts (lib/map_covar.cc:73-113)."""
+    g = golden("map_fixture.npz")
+    pix = fixture_pixels()
+    mc = ora.map_coords((10.0, 10.0, 10.0), (1, 1, 5))
+    sp = D.SignalCovarParams(0.05, 2.0, 2.0)
+    s, it, ok = D.map_covar_diag(D.pixel_to_npixel(pix), mc, sp,
+                                 D.PCGParams(len(pix), 1000, 50, 1e-2))
+    np.testing.assert_array_equal(ok, g["mcov_status"][:, 0] == 1)
+    # the fixture's weights reach 1.15e4: iteration counts within 10%, and
+    # where the paths are equally long sigma agrees to rounding
+    g_it = g["mcov_status"][:, 1]
+    assert np.all(np.abs(it - g_it) <= np.maximum(1, 0.1 * g_it))
+    same = it == g_it
+    assert same.sum() >= 3
+    np.testing.assert_allclose(s[same], g["mcov_sigma"][same], rtol=1e-6)
+    np.testing.assert_allclose(s, g["mcov_sigma"], rtol=2e-2)
+
+
+def test_map_covar_diag_batches_equal_single_cells(D, ora):
+    """Cells are solved as shared-geometry batches of 32: 70 cells (two
+    full batches and a ragged one of 6, cells far from every pixel among
+    them) give the same sigma, flags and iteration counts bit for bit as
+    one call per cell, and agree with the oracle."""
+    pix = CASES["small"]
+    npx = D.pixel_to_npixel(pix)
+    rng = np.random.default_rng(21)
+    lo, hi = pix[:, :3].min(0), pix[:, :3].max(0)
+    mc = rng.uniform(lo, hi, (70, 3))
+    mc[[5, 40, 69]] = hi + 50.0          # |b| = 0: no solve, sigma 0
+    sp = D.SignalCovarParams(0.05, 2.5, 2.1)
+    prm = D.PCGParams(len(pix), 200, 50, 1e-6)
+    s, it, ok = D.map_covar_diag(npx, mc, sp, prm)
+    assert not ok[[5, 40, 69]].any() and np.all(s[[5, 40, 69]] == 0.0)
+    assert ok.sum() == 67
+    for i in (0, 5, 31, 32, 63, 64, 68, 69):
+        s1, it1, ok1 = D.map_covar_diag(npx, mc[i:i + 1], sp, prm)
+        assert s1[0] == s[i] and it1[0] == it[i] and ok1[0] == ok[i]
+    c = ora.covar(0.05, 2.5, 2.1)
+    sel = np.r_[0:4, 30:34, 62:70]
+    s_ref, it_ref, ok_ref = ora.map_covar_diag(
+        ora.to_form(pix, ora.FORM_SN), mc[sel], c, 200, 50, 1e-6)
+    np.testing.assert_array_equal(ok[sel], ok_ref)
+    assert np.all(np.abs(it[sel] - it_ref) <= 1)
+    np.testing.assert_allclose(s[sel], s_ref, rtol=1e-5)
+
+
 @pytest.mark.parametrize("case", ["tiny", "n1", "n65", "dups"])
 def test_map_points_edge_cases(D, ora, case):
     pix = CASES[case]
