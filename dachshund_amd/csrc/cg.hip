@@ -722,6 +722,12 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
       }
       ++k;
     }
+    // The loop leaves with the last chunk still in flight (the host has
+    // seen only the previous chunk's count): its graph launch, pinned
+    // copies and event records finish before the graph and the events are
+    // destroyed.  The copy and wait below would wait for the same work.
+    const hipError_t drained = hipStreamSynchronize(st);
+    if (err == hipSuccess) err = drained;
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     (void)hipEventDestroy(ev[0]);
