@@ -61,10 +61,13 @@ SIGNATURES = {
     "ds_ctx_last_error": (C.c_char_p, [_vp]),
     "ds_ctx_synchronize": (C.c_int, [_vp]),
     "ds_ctx_num_cu": (C.c_int, [_vp]),
+    "ds_ctx_set_host_geometry": (C.c_int, [_vp, C.c_int]),
     "ds_problem_create": (C.c_int, [_vp, C.c_int, C.c_int, _dp,
                                     C.POINTER(DsCovar), C.POINTER(_vp)]),
     "ds_problem_destroy": (C.c_int, [_vp]),
     "ds_problem_get_stats": (C.c_int, [_vp, C.POINTER(DsProblemStats)]),
+    "ds_problem_get_geometry": (C.c_int, [_vp, C.c_int, _vp, C.c_longlong,
+                                          C.POINTER(C.c_longlong)]),
     "ds_problem_set_waves": (C.c_int, [_vp, C.c_int]),
     "ds_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp,
                                   C.POINTER(DsCovar), C.POINTER(_vp)]),

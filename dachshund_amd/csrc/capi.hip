@@ -199,10 +199,13 @@ int check_grid(ds_ctx* ctx, double lx, double ly, double lz, int nx, int ny,
   return DS_OK;
 }
 
-// Map onto a grid from a blocked source set (u in sorted order on device).
-int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g,
-                 const Blocking& blk, const double4* pos, const double* u,
-                 const double2* table64, const CovarDev& cv, double* m_host) {
+// Map onto a grid from a blocked source set (u in sorted order on device)
+// whose ntile tile boxes are on the device (dlo / dhi, 3 per tile); hblk,
+// when given, is the host blocking of the same set.
+int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g, int ntile,
+                 const double* dlo, const double* dhi, const Blocking* hblk,
+                 const double4* pos, const double* u, const double2* table64,
+                 const CovarDev& cv, double* m_host) {
   const int nt = g.tx * g.ty * g.tz;
   std::vector<int> ptr, idx;
   DevBuf<int> dptr, didx;
@@ -211,7 +214,8 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g,
   hipStream_t st = ctx->stream;
   HIP_OR_FAIL(ctx, dptr.alloc((size_t)nt + 1));
   const char* host_lists = std::getenv("DS_HOST_GRID_LISTS");
-  if (host_lists && host_lists[0] == '1') {
+  if (hblk && host_lists && host_lists[0] == '1') {
+    const Blocking& blk = *hblk;
     // diagnostic: the host's binned builder (the lists are identical)
     std::vector<double> lo, hi;
     grid_tiles(g, lo, hi);
@@ -226,18 +230,10 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g,
                                       hipMemcpyHostToDevice, st));
   } else {
     // full scan on the device: count, scan on the host, fill
-    DevBuf<double> dlo, dhi;
     DevBuf<int> dcnt;
-    const size_t nb = 3 * (size_t)blk.ntile;
-    HIP_OR_FAIL(ctx, dlo.alloc(nb));
-    HIP_OR_FAIL(ctx, dhi.alloc(nb));
     HIP_OR_FAIL(ctx, dcnt.alloc(nt));
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(dlo.p, blk.tlo.data(), sizeof(double) * nb,
-                                    hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(dhi.p, blk.thi.data(), sizeof(double) * nb,
-                                    hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo.p, dhi.p, blk.ntile, cv,
-                                           dcnt.p, nullptr, nullptr, st));
+    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, ntile, cv, dcnt.p,
+                                           nullptr, nullptr, st));
     std::vector<int> cnt(nt);
     HIP_OR_FAIL(ctx, hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * nt,
                                     hipMemcpyDeviceToHost, st));
@@ -252,9 +248,9 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g,
                                     sizeof(int) * ptr.size(),
                                     hipMemcpyHostToDevice, st));
     HIP_OR_FAIL(ctx, didx.alloc(std::max(ptr[nt], 1)));
-    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo.p, dhi.p, blk.ntile, cv,
-                                           nullptr, dptr.p, didx.p, st));
-    // the boxes and counts are freed on return: finish with them first
+    HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, ntile, cv, nullptr,
+                                           dptr.p, didx.p, st));
+    // the counts are freed on return: finish with them first
     HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
   }
   HIP_OR_FAIL(ctx, dm.alloc(n_map));
@@ -348,6 +344,9 @@ std::vector<int> xcd_order(const std::vector<int>& blocks, Cost cost) {
 
 void problem_free(ds_problem* p) {
   dev_free(p->knn_nbr);
+  dev_free(p->perm_dev);
+  dev_free(p->tbox_lo);
+  dev_free(p->tbox_hi);
   dev_free(p->pos64);
   dev_free(p->rhs64);
   dev_free(p->diag);
@@ -585,6 +584,12 @@ int ds_ctx_synchronize(ds_ctx* ctx) {
 
 int ds_ctx_num_cu(const ds_ctx* ctx) { return ctx ? ctx->num_cu : 0; }
 
+int ds_ctx_set_host_geometry(ds_ctx* ctx, int on) {
+  if (!ctx) return DS_E_ARG;
+  ctx->host_geometry = on ? 1 : 0;
+  return DS_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -593,6 +598,261 @@ namespace {
 // records consecutive in `pixels`, covariance covars[k]).  Each subvolume is
 // ordered and culled on its own; its blocks are concatenated after the
 // previous subvolume's.
+// The host builder of a problem's geometry: the ordering, culled lists,
+// per-slot arrays and column-sum slots that build_geometry_device makes,
+// made here by the host threads and uploaded (the reference for the device
+// builder, selected with ds_ctx_set_host_geometry).  Fills bstat (5 counts
+// per block, as the device builder) and the per-subvolume far-point test.
+//
+// FP32 frames: every target block and every source tile gets the per-axis
+// median of its real pixels as origin (an outlier cannot move it).  Sources
+// are stored relative to their tile's frame, targets relative to their
+// block's frame, and each list entry carries the difference of the two
+// frames; for any pair within the cutoff both points are then near the
+// source frame, which keeps the FP32 expansion |Xi|^2 + |Xj|^2 - 2 Xi.Xj
+// well conditioned.
+int geometry_host(ds_problem* p, const double* pixels,
+                  std::vector<float>& max_src_out, std::vector<int>& bstat) {
+  ds_ctx* ctx = p->ctx;
+  const int n_sub = p->n_sub, form = p->form;
+  std::vector<Blocking> blks(n_sub);
+  std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
+  std::vector<std::vector<unsigned char>> msks(n_sub);
+  auto build = [&](int k) {
+    const ds::CovarDev& cv = p->sub_cv_h[k];
+    double sc[3];
+    axis_scale(cv, sc);
+    const double* pk = pixels + 5 * (size_t)p->sub_off[k];
+    ds::build_blocking(pk, 5, p->sub_n[k], sc, blks[k]);
+    // a block's 16-row sub-blocks are its own tiles (targets = sources)
+    ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
+                         blks[k], cv, ptrs[k], idxs[k], blks[k].tlo.data(),
+                         blks[k].thi.data(), &msks[k]);
+  };
+  if (n_sub == 1) {
+    build(0);
+  } else {
+    // subvolumes in parallel (each ordering is itself serial)
+    std::vector<std::thread> th;
+    const int nt = std::max(1, std::min(n_sub, 16));
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t]() {
+        for (int k = t; k < n_sub; k += nt) build(k);
+      });
+    for (auto& x : th) x.join();
+  }
+  const int nblk = p->nblk, npad = p->npad, ntile = p->ntile;
+  // concatenated tile lists
+  std::vector<int> ptr(nblk + 1, 0), idx;
+  for (int k = 0; k < n_sub; ++k) {
+    const int B0 = p->sub_blk0_h[k];
+    for (int I = 0; I < blks[k].nblk; ++I)
+      ptr[B0 + I + 1] = ptr[B0 + I] + (ptrs[k][I + 1] - ptrs[k][I]);
+  }
+  idx.resize(ptr[nblk]);
+  std::vector<unsigned char> msk(ptr[nblk]);
+  for (int k = 0; k < n_sub; ++k) {
+    const int B0 = p->sub_blk0_h[k];
+    for (int I = 0; I < blks[k].nblk; ++I)
+      for (int e = ptrs[k][I]; e < ptrs[k][I + 1]; ++e) {
+        idx[ptr[B0 + I] + (e - ptrs[k][I])] = idxs[k][e] + B0 * DS_TPB;
+        msk[ptr[B0 + I] + (e - ptrs[k][I])] = msks[k][e];
+      }
+  }
+  bstat.assign(5 * (size_t)nblk, 0);
+  for (int B = 0; B < nblk; ++B) {
+    int* c = &bstat[5 * (size_t)B];
+    c[0] = ptr[B + 1] - ptr[B];
+    for (int e = ptr[B]; e < ptr[B + 1]; ++e) {
+      c[2] += __builtin_popcount(msk[e]);
+      c[3] += ((msk[e] & 3u) != 0) + ((msk[e] & 12u) != 0);
+    }
+  }
+  // per-slot, per-block and per-tile host arrays
+  std::vector<double4> hpos(npad);
+  std::vector<double> hrhs(npad, 0.0), hdiag(npad, 1.0), hw(npad, 0.0);
+  std::vector<float4> hloc(npad), htgt(npad);
+  std::vector<int> hblk_sub(nblk), hnreal(nblk), htcount(ntile);
+  std::vector<double> tlo(3 * (size_t)ntile), thi(3 * (size_t)ntile);
+  p->perm.assign(npad, -1);
+  std::vector<double> cen(3 * (size_t)nblk), tcen(3 * (size_t)ntile);
+  std::vector<double> max_src(n_sub, 0.0);
+  for (int k = 0; k < n_sub; ++k) {
+    const Blocking& b = blks[k];
+    const ds::CovarDev& cv = p->sub_cv_h[k];
+    const int B0 = p->sub_blk0_h[k];
+    const int off = p->sub_off[k];
+    const double* pk = pixels + 5 * (size_t)off;
+    const double var_s = cv.var_s;
+    auto median_frame = [&](int k0, int k1, double* c) {
+      std::vector<double> v(k1 - k0);
+      for (int a = 0; a < 3; ++a) {
+        for (int q = k0; q < k1; ++q) v[q - k0] = pk[5 * (size_t)b.perm[q] + a];
+        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+        c[a] = v[v.size() / 2];
+      }
+    };
+    for (int B = 0; B < b.nblk; ++B) {
+      hblk_sub[B0 + B] = k;
+      hnreal[B0 + B] = std::min(DS_BS, b.n - B * DS_BS);
+      median_frame(B * DS_BS, std::min(b.n, (B + 1) * DS_BS),
+                   &cen[3 * (size_t)(B0 + B)]);
+    }
+    for (int T = 0; T < b.ntile; ++T) {
+      const int TG = B0 * DS_TPB + T;
+      htcount[TG] = b.tcount[T];
+      for (int a = 0; a < 3; ++a) {
+        tlo[3 * (size_t)TG + a] = b.tlo[3 * (size_t)T + a];
+        thi[3 * (size_t)TG + a] = b.thi[3 * (size_t)T + a];
+      }
+      if (b.tcount[T] > 0)
+        median_frame(T * DS_TS, T * DS_TS + b.tcount[T], &tcen[3 * (size_t)TG]);
+      else
+        for (int a = 0; a < 3; ++a)
+          tcen[3 * (size_t)TG + a] = cen[3 * (size_t)(B0 + T / DS_TPB) + a];
+    }
+    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
+    for (int q = 0; q < b.npad; ++q) {
+      const size_t g = (size_t)B0 * DS_BS + q;
+      const int j = b.perm[q];
+      const double* Pj = pk + 5 * (size_t)j;
+      const bool real = q < b.n;
+      if (real) p->perm[g] = off + j;
+      const double wN = real ? Pj[3] : (form == DS_FORM_W ? 0.0 : 1.0);
+      hpos[g] = make_double4(Pj[0], Pj[1], Pj[2], wN);
+      if (real) {
+        hrhs[g] = Pj[4];
+        // lib/wf_vector.cc:190-192 (W) and :44-47 (SN)
+        hdiag[g] = form == DS_FORM_W ? wN * wN * var_s + 1.0 : var_s + wN;
+        hw[g] = form == DS_FORM_W ? wN : 1.0;
+      }
+      const size_t B = (size_t)B0 + q / DS_BS, T = (size_t)B0 * DS_TPB + q / DS_TS;
+      float l[3], t[3];
+      for (int a = 0; a < 3; ++a) {
+        l[a] = (float)((Pj[a] - tcen[3 * T + a]) * s3[a]);
+        t[a] = (float)((Pj[a] - cen[3 * B + a]) * s3[a]);
+        if (real) max_src[k] = std::max(max_src[k], std::fabs((double)l[a]));
+      }
+      hloc[g] = make_float4(l[0], l[1], l[2],
+                            l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+      htgt[g] = make_float4(t[0], t[1], t[2], 0.0f);
+    }
+  }
+  // per list entry: target block frame minus source tile frame (pre-scaled)
+  std::vector<float4> hoff(idx.size());
+  for (int I = 0; I < nblk; ++I) {
+    const ds::CovarDev& cv = p->sub_cv_h[hblk_sub[I]];
+    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
+    for (int e = ptr[I]; e < ptr[I + 1]; ++e) {
+      const size_t T = (size_t)idx[e];
+      float o[3];
+      for (int a = 0; a < 3; ++a)
+        o[a] = (float)((cen[3 * (size_t)I + a] - tcen[3 * T + a]) * s3[a]);
+      // .w carries the bits of (live sub-block mask | tile << 4), so the
+      // MFMA kernel reads an entry in one 16-byte load
+      const unsigned bits = (unsigned)msk[e] | ((unsigned)T << 4);
+      float wbits;
+      std::memcpy(&wbits, &bits, sizeof(float));
+      hoff[e] = make_float4(o[0], o[1], o[2], wbits);
+    }
+  }
+  // Half lists of the symmetric MFMA kernel.  The 16-row sub-blocks of a
+  // block are its own tiles (tile 4 B + s), and the sub-block test is the
+  // tile-pair test, so the pair {P, Q} of tiles is live in P's block's
+  // entry for Q exactly when it is live in Q's block's entry for P: keep
+  // it once, in the block of the smaller tile (both products from one
+  // evaluation), and a tile with itself once (its own product only).
+  std::vector<int> hptr(nblk + 1, 0);
+  std::vector<float4> hrec;
+  std::vector<int> gcnt(ntile + 1, 0), gsrc;   // (entry, tile) pairs
+  hrec.reserve(idx.size() / 2 + nblk);
+  for (int B = 0; B < nblk; ++B) {
+    for (int e = ptr[B]; e < ptr[B + 1]; ++e) {
+      const int T = idx[e], dT = T - DS_TPB * B;
+      if (dT < 0) continue;
+      // sub-blocks s <= dT evaluate, s < dT also give the transposed part
+      const unsigned keep = dT >= DS_TPB - 1 ? 15u : (2u << dT) - 1u;
+      const unsigned m = msk[e] & keep;
+      if (!m) continue;
+      const unsigned tr = m & (dT >= DS_TPB ? 15u : (1u << dT) - 1u);
+      const unsigned bits = m | ((unsigned)T << 4);
+      float wbits;
+      std::memcpy(&wbits, &bits, sizeof(float));
+      if (tr) {
+        gsrc.push_back((int)hrec.size());
+        gsrc.push_back(T);
+        ++gcnt[T + 1];
+      }
+      hrec.push_back(make_float4(hoff[e].x, hoff[e].y, hoff[e].z, wbits));
+      bstat[5 * (size_t)B + 4] += __builtin_popcount(m);
+    }
+    hptr[B + 1] = (int)hrec.size();
+    bstat[5 * (size_t)B + 1] = hptr[B + 1] - hptr[B];
+  }
+  std::vector<int> gptr(ntile + 1, 0), gslot(hrec.size(), -1);
+  for (int T = 0; T < ntile; ++T) gptr[T + 1] = gptr[T] + gcnt[T + 1];
+  {
+    // slots in block order within each tile (gsrc is in block order)
+    std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+    for (size_t k = 0; k < gsrc.size(); k += 2)
+      gslot[gsrc[k]] = fill[gsrc[k + 1]]++;
+  }
+  p->nslot = (long long)(gsrc.size() / 2);
+  max_src_out.resize(n_sub);
+  for (int k = 0; k < n_sub; ++k) max_src_out[k] = (float)max_src[k];
+  hipStream_t st = ctx->stream;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
+                 : hipSuccess;
+  };
+  HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->rhs64, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->wvec, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->loc32, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tgt32, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->perm_dev, npad));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tptr, ptr.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tidx, idx.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tmask, msk.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->hptr, nblk + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->hrec, hrec.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gslot, gslot.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, ntile + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, ntile));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tbox_lo, tlo.size()));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->tbox_hi, thi.size()));
+  HIP_OR_FAIL(ctx, up(p->pos64, hpos.data(), sizeof(double4) * npad));
+  HIP_OR_FAIL(ctx, up(p->rhs64, hrhs.data(), sizeof(double) * npad));
+  HIP_OR_FAIL(ctx, up(p->diag, hdiag.data(), sizeof(double) * npad));
+  HIP_OR_FAIL(ctx, up(p->wvec, hw.data(), sizeof(double) * npad));
+  HIP_OR_FAIL(ctx, up(p->loc32, hloc.data(), sizeof(float4) * npad));
+  HIP_OR_FAIL(ctx, up(p->tgt32, htgt.data(), sizeof(float4) * npad));
+  HIP_OR_FAIL(ctx, up(p->perm_dev, p->perm.data(), sizeof(int) * npad));
+  HIP_OR_FAIL(ctx, up(p->tptr, ptr.data(), sizeof(int) * ptr.size()));
+  HIP_OR_FAIL(ctx, up(p->tidx, idx.data(), sizeof(int) * idx.size()));
+  HIP_OR_FAIL(ctx, up(p->tmask, msk.data(), msk.size()));
+  HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
+  HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
+  HIP_OR_FAIL(ctx, up(p->hptr, hptr.data(), sizeof(int) * (nblk + 1)));
+  HIP_OR_FAIL(ctx, up(p->hrec, hrec.data(), sizeof(float4) * hrec.size()));
+  HIP_OR_FAIL(ctx, up(p->gslot, gslot.data(), sizeof(int) * gslot.size()));
+  HIP_OR_FAIL(ctx, up(p->gptr, gptr.data(), sizeof(int) * (ntile + 1)));
+  HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
+                      sizeof(int) * (n_sub + 1)));
+  HIP_OR_FAIL(ctx, up(p->tcount, htcount.data(), sizeof(int) * ntile));
+  HIP_OR_FAIL(ctx, up(p->tbox_lo, tlo.data(), sizeof(double) * tlo.size()));
+  HIP_OR_FAIL(ctx, up(p->tbox_hi, thi.data(), sizeof(double) * thi.size()));
+  // the host arrays are freed on return
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+  return DS_OK;
+}
+
 int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
                    const double* pixels, const ds_covar* covars,
                    ds_problem** out) {
@@ -634,21 +894,16 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   for (int k = 0; k < n_sub; ++k) p->sub_off[k + 1] = p->sub_off[k] + sub_n[k];
   p->sub_cv_h.resize(n_sub);
   p->sub_norm_b.assign(n_sub, 0.0);
-  std::vector<Blocking> blks(n_sub);
-  std::vector<std::vector<int>> ptrs(n_sub), idxs(n_sub);
-  std::vector<std::vector<unsigned char>> msks(n_sub);
-  auto build = [&](int k) {
+  // per subvolume: covariance constants, the ordering's quantization and
+  // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
+  std::vector<ds::KdQuant> quant(n_sub);
+  auto prep = [&](int k) {
     const ds::CovarDev cv = ds::make_covar(&covars[k]);
     p->sub_cv_h[k] = cv;
     double sc[3];
     axis_scale(cv, sc);
     const double* pk = pixels + 5 * (size_t)p->sub_off[k];
-    ds::build_blocking(pk, 5, sub_n[k], sc, blks[k]);
-    // a block's 16-row sub-blocks are its own tiles (targets = sources)
-    ds::build_tile_lists(blks[k].nblk, blks[k].blo.data(), blks[k].bhi.data(),
-                         blks[k], cv, ptrs[k], idxs[k], blks[k].tlo.data(),
-                         blks[k].thi.data(), &msks[k]);
-    // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
+    quant[k] = ds::kd_quant(pk, 5, sub_n[k], sc);
     double sum_b2 = 0.0;
     for (int i = 0; i < sub_n[k]; ++i) {
       const double bi = pk[5 * (size_t)i + 4];
@@ -657,14 +912,13 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     p->sub_norm_b[k] = std::sqrt(sum_b2);
   };
   if (n_sub == 1) {
-    build(0);
+    prep(0);
   } else {
-    // subvolumes in parallel (each ordering is itself serial)
     std::vector<std::thread> th;
     const int nt = std::max(1, std::min(n_sub, 16));
     for (int t = 0; t < nt; ++t)
       th.emplace_back([&, t]() {
-        for (int k = t; k < n_sub; k += nt) build(k);
+        for (int k = t; k < n_sub; k += nt) prep(k);
       });
     for (auto& x : th) x.join();
   }
@@ -688,194 +942,59 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   }
   p->sub_blk0_h.assign(n_sub + 1, 0);
   for (int k = 0; k < n_sub; ++k)
-    p->sub_blk0_h[k + 1] = p->sub_blk0_h[k] + blks[k].nblk;
+    p->sub_blk0_h[k + 1] = p->sub_blk0_h[k] + (sub_n[k] + DS_BS - 1) / DS_BS;
   p->nblk = p->sub_blk0_h[n_sub];
   p->ntile = p->nblk * DS_TPB;
   p->npad = p->nblk * DS_BS;
-  if (n_sub == 1) p->blk = blks[0];
   const int nblk = p->nblk, npad = p->npad;
-  // concatenated tile lists
-  std::vector<int> ptr(nblk + 1, 0), idx;
-  for (int k = 0; k < n_sub; ++k) {
-    const int B0 = p->sub_blk0_h[k];
-    for (int I = 0; I < blks[k].nblk; ++I)
-      ptr[B0 + I + 1] = ptr[B0 + I] + (ptrs[k][I + 1] - ptrs[k][I]);
+  // ordering, culled lists, per-slot arrays and column-sum slots: on the
+  // device, or with the host builder (the same arrays)
+  std::vector<float> max_src;
+  std::vector<int> bstat;
+  if (ctx->host_geometry) {
+    if (int rc = geometry_host(p, pixels, max_src, bstat)) return rc;
+  } else {
+    int gerr = 0;
+    HIP_OR_FAIL(ctx, ds::build_geometry_device(p, pixels, quant, max_src,
+                                               bstat, &gerr));
+    if (gerr == 1)
+      return fail(ctx, DS_E_ARG, "more than INT_MAX tile-list entries");
+    if (gerr)
+      return fail(ctx, DS_E_STATE, "device geometry builder: inconsistent "
+                  "result (code %d)", gerr);
   }
-  idx.resize(ptr[nblk]);
-  std::vector<unsigned char> msk(ptr[nblk]);
-  for (int k = 0; k < n_sub; ++k) {
-    const int B0 = p->sub_blk0_h[k];
-    for (int I = 0; I < blks[k].nblk; ++I)
-      for (int e = ptrs[k][I]; e < ptrs[k][I + 1]; ++e) {
-        idx[ptr[B0 + I] + (e - ptrs[k][I])] = idxs[k][e] + B0 * DS_TPB;
-        msk[ptr[B0 + I] + (e - ptrs[k][I])] = msks[k][e];
-      }
-  }
-  p->ntile_pairs = (long long)idx.size();
-  p->nsub_pairs = 0;
-  p->nhalf_pairs = 0;
-  for (unsigned char m : msk) {
-    p->nsub_pairs += __builtin_popcount(m);
-    p->nhalf_pairs += ((m & 3u) != 0) + ((m & 12u) != 0);
-  }
+  // pair counts from the per-block counts {list, half list, live
+  // sub-block entries, live 32-row halves, evaluated sub-block entries}
+  std::vector<int> hblk_sub(nblk);
+  for (int k = 0; k < n_sub; ++k)
+    for (int B = p->sub_blk0_h[k]; B < p->sub_blk0_h[k + 1]; ++B)
+      hblk_sub[B] = k;
+  p->blk_len_h.resize(nblk);
+  p->ntile_pairs = p->nsub_pairs = p->nhalf_pairs = p->nsub_eval = 0;
   p->sub_pairs64.assign(n_sub, 0);
   p->sub_pairs32.assign(n_sub, 0);
   p->sub_half64.assign(n_sub, 0);
-  for (int k = 0; k < n_sub; ++k) {
-    const int B0 = p->sub_blk0_h[k], B1 = p->sub_blk0_h[k + 1];
-    for (int e = ptr[B0]; e < ptr[B1]; ++e) {
-      p->sub_pairs32[k] += __builtin_popcount(msk[e]);
-      p->sub_half64[k] += (long long)(((msk[e] & 3u) != 0) +
-                                      ((msk[e] & 12u) != 0)) * (32 * DS_TS);
-    }
-    p->sub_pairs64[k] = (long long)(ptr[B1] - ptr[B0]) * (DS_BS * DS_TS);
-    p->sub_pairs32[k] *= DS_TS * DS_TS;
-  }
-  p->blk_len_h.resize(nblk);
-  for (int I = 0; I < nblk; ++I) p->blk_len_h[I] = ptr[I + 1] - ptr[I];
-
-  // per-slot, per-block and per-tile host arrays
-  std::vector<double4> hpos(npad);
-  std::vector<double> hrhs(npad, 0.0), hdiag(npad, 1.0), hw(npad, 0.0);
-  std::vector<float4> hloc(npad), htgt(npad);
-  std::vector<int> hblk_sub(nblk), hnreal(nblk), htcount(p->ntile);
-  std::vector<double4> hsubcv(n_sub);
-  p->perm.assign(npad, -1);
-  // FP32 frames: every target block and every source tile gets the
-  // per-axis median of its real pixels as origin (an outlier cannot move
-  // it).  Sources are stored relative to their tile's frame, targets
-  // relative to their block's frame, and each list entry carries the
-  // difference of the two frames; for any pair within the cutoff both
-  // points are then near the source frame, which keeps the FP32 expansion
-  // |Xi|^2 + |Xj|^2 - 2 Xi.Xj well conditioned.
-  std::vector<double> cen(3 * (size_t)nblk), tcen(3 * (size_t)p->ntile);
-  std::vector<double> max_src(n_sub, 0.0);
-  for (int k = 0; k < n_sub; ++k) {
-    const Blocking& b = blks[k];
-    const ds::CovarDev& cv = p->sub_cv_h[k];
-    const int B0 = p->sub_blk0_h[k];
-    const int off = p->sub_off[k];
-    const double* pk = pixels + 5 * (size_t)off;
-    const double var_s = cv.var_s;
-    hsubcv[k] = make_double4(cv.var_s, cv.c_perp, cv.c_para, 0.0);
-    auto median_frame = [&](int k0, int k1, double* c) {
-      std::vector<double> v(k1 - k0);
-      for (int a = 0; a < 3; ++a) {
-        for (int q = k0; q < k1; ++q) v[q - k0] = pk[5 * (size_t)b.perm[q] + a];
-        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
-        c[a] = v[v.size() / 2];
-      }
-    };
-    for (int B = 0; B < b.nblk; ++B) {
-      hblk_sub[B0 + B] = k;
-      hnreal[B0 + B] = std::min(DS_BS, b.n - B * DS_BS);
-      median_frame(B * DS_BS, std::min(b.n, (B + 1) * DS_BS),
-                   &cen[3 * (size_t)(B0 + B)]);
-    }
-    for (int T = 0; T < b.ntile; ++T) {
-      const int TG = B0 * DS_TPB + T;
-      htcount[TG] = b.tcount[T];
-      if (b.tcount[T] > 0)
-        median_frame(T * DS_TS, T * DS_TS + b.tcount[T], &tcen[3 * (size_t)TG]);
-      else
-        for (int a = 0; a < 3; ++a)
-          tcen[3 * (size_t)TG + a] = cen[3 * (size_t)(B0 + T / DS_TPB) + a];
-    }
-    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
-    for (int q = 0; q < b.npad; ++q) {
-      const size_t g = (size_t)B0 * DS_BS + q;
-      const int j = b.perm[q];
-      const double* Pj = pk + 5 * (size_t)j;
-      const bool real = q < b.n;
-      if (real) p->perm[g] = off + j;
-      const double wN = real ? Pj[3] : (form == DS_FORM_W ? 0.0 : 1.0);
-      hpos[g] = make_double4(Pj[0], Pj[1], Pj[2], wN);
-      if (real) {
-        hrhs[g] = Pj[4];
-        // lib/wf_vector.cc:190-192 (W) and :44-47 (SN)
-        hdiag[g] = form == DS_FORM_W ? wN * wN * var_s + 1.0 : var_s + wN;
-        hw[g] = form == DS_FORM_W ? wN : 1.0;
-      }
-      const size_t B = (size_t)B0 + q / DS_BS, T = (size_t)B0 * DS_TPB + q / DS_TS;
-      float l[3], t[3];
-      for (int a = 0; a < 3; ++a) {
-        l[a] = (float)((Pj[a] - tcen[3 * T + a]) * s3[a]);
-        t[a] = (float)((Pj[a] - cen[3 * B + a]) * s3[a]);
-        if (real) max_src[k] = std::max(max_src[k], std::fabs((double)l[a]));
-      }
-      hloc[g] = make_float4(l[0], l[1], l[2],
-                            l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
-      htgt[g] = make_float4(t[0], t[1], t[2], 0.0f);
-    }
-  }
-  // A tile holding a point far from its median (isolated outliers) would
-  // make the expansion lose its accuracy: that subvolume's blocks use
-  // direct differences (decided per subvolume, so a batch runs each
-  // subvolume's blocks with the kernel it would get alone).
-  // per list entry: target block frame minus source tile frame (pre-scaled)
-  std::vector<float4> hoff(idx.size());
-  for (int I = 0; I < nblk; ++I) {
-    const ds::CovarDev& cv = p->sub_cv_h[hblk_sub[I]];
-    const double s3[3] = {cv.s_perp, cv.s_perp, cv.s_para};
-    for (int e = ptr[I]; e < ptr[I + 1]; ++e) {
-      const size_t T = (size_t)idx[e];
-      float o[3];
-      for (int a = 0; a < 3; ++a)
-        o[a] = (float)((cen[3 * (size_t)I + a] - tcen[3 * T + a]) * s3[a]);
-      // .w carries the bits of (live sub-block mask | tile << 4), so the
-      // MFMA kernel reads an entry in one 16-byte load
-      const unsigned bits = (unsigned)msk[e] | ((unsigned)T << 4);
-      float wbits;
-      std::memcpy(&wbits, &bits, sizeof(float));
-      hoff[e] = make_float4(o[0], o[1], o[2], wbits);
-    }
-  }
-  // Half lists of the symmetric MFMA kernel.  The 16-row sub-blocks of a
-  // block are its own tiles (tile 4 B + s), and the sub-block test is the
-  // tile-pair test, so the pair {P, Q} of tiles is live in P's block's
-  // entry for Q exactly when it is live in Q's block's entry for P: keep
-  // it once, in the block of the smaller tile (both products from one
-  // evaluation), and a tile with itself once (its own product only).
-  std::vector<int> hptr(nblk + 1, 0);
-  std::vector<float4> hrec;
-  std::vector<int> gcnt(p->ntile + 1, 0), gsrc;   // (entry, tile) pairs
-  hrec.reserve(idx.size() / 2 + nblk);
-  p->nsub_eval = 0;
   for (int B = 0; B < nblk; ++B) {
-    for (int e = ptr[B]; e < ptr[B + 1]; ++e) {
-      const int T = idx[e], dT = T - DS_TPB * B;
-      if (dT < 0) continue;
-      // sub-blocks s <= dT evaluate, s < dT also give the transposed part
-      const unsigned keep = dT >= DS_TPB - 1 ? 15u : (2u << dT) - 1u;
-      const unsigned m = msk[e] & keep;
-      if (!m) continue;
-      const unsigned tr = m & (dT >= DS_TPB ? 15u : (1u << dT) - 1u);
-      const unsigned bits = m | ((unsigned)T << 4);
-      float wbits;
-      std::memcpy(&wbits, &bits, sizeof(float));
-      if (tr) {
-        gsrc.push_back((int)hrec.size());
-        gsrc.push_back(T);
-        ++gcnt[T + 1];
-      }
-      hrec.push_back(make_float4(hoff[e].x, hoff[e].y, hoff[e].z, wbits));
-      p->nsub_eval += __builtin_popcount(m);
-    }
-    hptr[B + 1] = (int)hrec.size();
+    const int* c = &bstat[5 * (size_t)B];
+    const int k = hblk_sub[B];
+    p->blk_len_h[B] = c[0];
+    p->ntile_pairs += c[0];
+    p->nsub_pairs += c[2];
+    p->nhalf_pairs += c[3];
+    p->nsub_eval += c[4];
+    p->sub_pairs64[k] += c[0];
+    p->sub_pairs32[k] += c[2];
+    p->sub_half64[k] += c[3];
   }
-  std::vector<int> gptr(p->ntile + 1, 0), gslot(hrec.size(), -1);
-  for (int T = 0; T < p->ntile; ++T) gptr[T + 1] = gptr[T] + gcnt[T + 1];
-  {
-    // slots in block order within each tile (gsrc is in block order)
-    std::vector<int> fill(gptr.begin(), gptr.end() - 1);
-    for (size_t k = 0; k < gsrc.size(); k += 2)
-      gslot[gsrc[k]] = fill[gsrc[k + 1]]++;
+  for (int k = 0; k < n_sub; ++k) {
+    p->sub_pairs64[k] *= DS_BS * DS_TS;
+    p->sub_pairs32[k] *= DS_TS * DS_TS;
+    p->sub_half64[k] *= 32 * DS_TS;
   }
-  const size_t nslot = gsrc.size() / 2;
   // the CG update kernel addresses the slots by 32-bit byte offsets
-  if (nslot > (size_t)UINT32_MAX / (16 * sizeof(float)))
-    return fail(ctx, DS_E_ARG, "%zu column-sum slots exceed the 32-bit "
-                "offsets of the update kernel", nslot);
+  if ((size_t)p->nslot > (size_t)UINT32_MAX / (16 * sizeof(float)))
+    return fail(ctx, DS_E_ARG, "%lld column-sum slots exceed the 32-bit "
+                "offsets of the update kernel", (long long)p->nslot);
   // launch order: longest tile list first (for the autotuner), grouped by
   // XCD (xcd_order)
   std::vector<int> order(nblk);
@@ -883,10 +1002,14 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   order = xcd_order(order, [&](int I) { return p->blk_len_h[I]; });
   // FP32 launch lists: longest list first and block order, each split into
   // the blocks of subvolumes without far points (lengths of their half
-  // lists, which the MFMA kernel runs), then the others
+  // lists, which the MFMA kernel runs), then the others.  A tile holding a
+  // point far from its median (isolated outliers) would make the expansion
+  // lose its accuracy: that subvolume's blocks use direct differences
+  // (decided per subvolume, so a batch runs each subvolume's blocks with
+  // the kernel it would get alone).
   std::vector<int> ord32(2 * (size_t)nblk);
   {
-    auto direct = [&](int I) { return max_src[hblk_sub[I]] > 16.0; };
+    auto direct = [&](int I) { return max_src[hblk_sub[I]] > 16.0f; };
     int nf = 0;
     for (int I : order) nf += !direct(I);
     p->n32_fast = nf;
@@ -895,7 +1018,7 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     std::vector<int> fast, slow;
     for (int I = 0; I < nblk; ++I) (direct(I) ? slow : fast).push_back(I);
     const std::vector<int> f0 = xcd_order(fast, [&](int I) {
-      return hptr[I + 1] - hptr[I];
+      return bstat[5 * (size_t)I + 1];
     });
     const std::vector<int> s0 = xcd_order(slow, [&](int I) {
       return p->blk_len_h[I];
@@ -907,6 +1030,11 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
       int* dst = ord32.data() + nblk;
       if (direct(k)) dst[d++] = k; else dst[f++] = k;
     }
+  }
+  std::vector<double4> hsubcv(n_sub);
+  for (int k = 0; k < n_sub; ++k) {
+    const ds::CovarDev& cv = p->sub_cv_h[k];
+    hsubcv[k] = make_double4(cv.var_s, cv.c_perp, cv.c_para, 0.0);
   }
   std::vector<double2> t64;
   std::vector<float2> t32;
@@ -924,34 +1052,17 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->rhs64, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->wvec, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->loc32, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tgt32, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tptr, ptr.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tidx, idx.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tmask, std::max<size_t>(msk.size(), 1)));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table64, t64.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table32, t32.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table32g, t32g.size()));
   HIP_OR_FAIL(ctx, dev_alloc(&p->table_raw, p->table.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->blk_order, nblk));
   HIP_OR_FAIL(ctx, dev_alloc(&p->ord32, 2 * (size_t)nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->hptr, nblk + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->hrec, std::max<size_t>(hrec.size(), 1)));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gslot, std::max<size_t>(gslot.size(), 1)));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * nslot, 1)));
+  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * (size_t)p->nslot,
+                                                        1)));
   HIP_OR_FAIL(ctx, dev_alloc(&p->ydir, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, p->ntile + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_sym, n_sub));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
   HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, p->ntile));
   HIP_OR_FAIL(ctx, dev_alloc(&p->done_count, 1));
   double** vecs[] = {&p->x, &p->r, &p->d, &p->s, &p->q, &p->tmp};
   for (double** v : vecs) HIP_OR_FAIL(ctx, dev_alloc(v, npad));
@@ -975,42 +1086,20 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->host_tmp, sizeof(double) * npad,
                                  hipHostMallocDefault));
   std::memset(p->sc_host, 0, sizeof(ds_cg_scalars) * n_sub);
-  HIP_OR_FAIL(ctx, up(p->pos64, hpos.data(), sizeof(double4) * npad));
-  HIP_OR_FAIL(ctx, up(p->rhs64, hrhs.data(), sizeof(double) * npad));
-  HIP_OR_FAIL(ctx, up(p->diag, hdiag.data(), sizeof(double) * npad));
-  HIP_OR_FAIL(ctx, up(p->wvec, hw.data(), sizeof(double) * npad));
-  HIP_OR_FAIL(ctx, up(p->loc32, hloc.data(), sizeof(float4) * npad));
-  HIP_OR_FAIL(ctx, up(p->tgt32, htgt.data(), sizeof(float4) * npad));
-  HIP_OR_FAIL(ctx, up(p->tptr, ptr.data(), sizeof(int) * ptr.size()));
-  HIP_OR_FAIL(ctx, up(p->tidx, idx.data(), sizeof(int) * idx.size()));
-  HIP_OR_FAIL(ctx, up(p->tmask, msk.data(), msk.size()));
-  HIP_OR_FAIL(ctx, up(p->toff32, hoff.data(), sizeof(float4) * hoff.size()));
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));
   HIP_OR_FAIL(ctx, up(p->table32g, t32g.data(), sizeof(float2) * t32g.size()));
   HIP_OR_FAIL(ctx, up(p->table_raw, p->table.data(),
                       sizeof(double) * p->table.size()));
-  HIP_OR_FAIL(ctx, up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
-  HIP_OR_FAIL(ctx, up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->blk_order, order.data(), sizeof(int) * nblk));
   HIP_OR_FAIL(ctx, up(p->ord32, ord32.data(), sizeof(int) * 2 * (size_t)nblk));
-  HIP_OR_FAIL(ctx, up(p->hptr, hptr.data(), sizeof(int) * (nblk + 1)));
-  if (!hrec.empty())
-    HIP_OR_FAIL(ctx, up(p->hrec, hrec.data(), sizeof(float4) * hrec.size()));
-  HIP_OR_FAIL(ctx, up(p->gptr, gptr.data(), sizeof(int) * (p->ntile + 1)));
   {
     std::vector<int> sym(n_sub);
-    for (int k = 0; k < n_sub; ++k) sym[k] = max_src[k] > 16.0 ? 0 : 1;
+    for (int k = 0; k < n_sub; ++k) sym[k] = max_src[k] > 16.0f ? 0 : 1;
     HIP_OR_FAIL(ctx, up(p->sub_sym, sym.data(), sizeof(int) * n_sub));
   }
-  if (!gslot.empty())
-    HIP_OR_FAIL(ctx, up(p->gslot, gslot.data(), sizeof(int) * gslot.size()));
-  HIP_OR_FAIL(ctx, up(p->sub_blk0, p->sub_blk0_h.data(),
-                      sizeof(int) * (n_sub + 1)));
   HIP_OR_FAIL(ctx, up(p->sub_cv, hsubcv.data(), sizeof(double4) * n_sub));
-  HIP_OR_FAIL(ctx, up(p->tcount, htcount.data(), sizeof(int) * p->ntile));
-  std::vector<double> zero(npad, 0.0);
-  HIP_OR_FAIL(ctx, up(p->x, zero.data(), sizeof(double) * npad));
+  HIP_OR_FAIL(ctx, ds::launch_fill(p->x, 0.0, npad, st));
   HIP_OR_FAIL(ctx, up(p->sc, p->sc_host, sizeof(ds_cg_scalars) * n_sub));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
   p->setup_seconds = now_s() - t0;
@@ -1068,6 +1157,52 @@ int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out) {
       p->f64_rows == 16 ? out->sub_pairs_per_matvec
       : p->f64_rows == 32 ? p->nhalf_pairs * (long long)(32 * DS_TS)
                           : out->pairs_per_matvec;
+  return DS_OK;
+}
+
+int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
+                            long long cap_bytes, long long* bytes) {
+  if (!p || !bytes) return DS_E_ARG;
+  const size_t npad = p->npad, nblk = p->nblk, ntile = p->ntile;
+  int tot[2] = {0, 0};   // list entries, half-list records
+  HIP_OR_FAIL(p->ctx, hipMemcpy(&tot[0], p->tptr + nblk, sizeof(int),
+                                hipMemcpyDeviceToHost));
+  HIP_OR_FAIL(p->ctx, hipMemcpy(&tot[1], p->hptr + nblk, sizeof(int),
+                                hipMemcpyDeviceToHost));
+  const void* src = nullptr;
+  size_t nb = 0;
+  switch (which) {
+    case DS_GEOM_PERM: src = p->perm_dev; nb = 4 * npad; break;
+    case DS_GEOM_POS: src = p->pos64; nb = 32 * npad; break;
+    case DS_GEOM_RHS: src = p->rhs64; nb = 8 * npad; break;
+    case DS_GEOM_DIAG: src = p->diag; nb = 8 * npad; break;
+    case DS_GEOM_WVEC: src = p->wvec; nb = 8 * npad; break;
+    case DS_GEOM_LOC32: src = p->loc32; nb = 16 * npad; break;
+    case DS_GEOM_TGT32: src = p->tgt32; nb = 16 * npad; break;
+    case DS_GEOM_TCOUNT: src = p->tcount; nb = 4 * ntile; break;
+    case DS_GEOM_TBOX_LO: src = p->tbox_lo; nb = 24 * ntile; break;
+    case DS_GEOM_TBOX_HI: src = p->tbox_hi; nb = 24 * ntile; break;
+    case DS_GEOM_TPTR: src = p->tptr; nb = 4 * (nblk + 1); break;
+    case DS_GEOM_TIDX: src = p->tidx; nb = 4 * (size_t)tot[0]; break;
+    case DS_GEOM_TMASK: src = p->tmask; nb = (size_t)tot[0]; break;
+    case DS_GEOM_TOFF32: src = p->toff32; nb = 16 * (size_t)tot[0]; break;
+    case DS_GEOM_HPTR: src = p->hptr; nb = 4 * (nblk + 1); break;
+    case DS_GEOM_HREC: src = p->hrec; nb = 16 * (size_t)tot[1]; break;
+    case DS_GEOM_GPTR: src = p->gptr; nb = 4 * (ntile + 1); break;
+    case DS_GEOM_GSLOT: src = p->gslot; nb = 4 * (size_t)tot[1]; break;
+    case DS_GEOM_BLK_ORDER: src = p->blk_order; nb = 4 * nblk; break;
+    case DS_GEOM_ORD32: src = p->ord32; nb = 8 * nblk; break;
+    default:
+      return fail(p->ctx, DS_E_ARG, "unknown geometry array %d", which);
+  }
+  *bytes = (long long)nb;
+  if (!out) return DS_OK;
+  if (cap_bytes < (long long)nb)
+    return fail(p->ctx, DS_E_ARG, "geometry array %d needs %zu bytes", which,
+                nb);
+  HIP_OR_FAIL(p->ctx, hipStreamSynchronize(p->ctx->stream));
+  if (nb)
+    HIP_OR_FAIL(p->ctx, hipMemcpy(out, src, nb, hipMemcpyDeviceToHost));
   return DS_OK;
 }
 
@@ -1225,8 +1360,9 @@ int ds_problem_smp_grid(ds_problem* p, double lx, double ly, double lz,
   if (int rc = check_grid(p->ctx, lx, ly, lz, nx, ny, nz, g)) return rc;
   if (p->n_sub != 1)
     return fail(p->ctx, DS_E_STATE, "map projection needs a single volume");
-  return map_grid_run(p->ctx, ds::MM_SMP, g, p->blk, p->pos64, p->x,
-                      p->table64, p->cv, m);
+  return map_grid_run(p->ctx, ds::MM_SMP, g, p->ntile, p->tbox_lo,
+                      p->tbox_hi, nullptr, p->pos64, p->x, p->table64, p->cv,
+                      m);
 }
 
 // ds_problem_smp_grid without device allocations: the source-tile boxes,
@@ -1247,14 +1383,10 @@ int ds_problem_smp_grid_noalloc(ds_problem* p, double lx, double ly,
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device));
   if (!ctx->noalloc)
     return fail(ctx, DS_E_STATE, "no-alloc scratch not set up");
-  const Blocking& blk = p->blk;
   const size_t nt = (size_t)g.tx * g.ty * g.tz;
   const size_t n_map = (size_t)g.nx * g.ny * g.nz;
-  const size_t nb = 3 * (size_t)blk.ntile;
   auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t o_lo = 0;
-  const size_t o_hi = align(o_lo + sizeof(double) * nb);
-  const size_t o_cnt = align(o_hi + sizeof(double) * nb);
+  const size_t o_cnt = 0;
   const size_t o_ptr = align(o_cnt + sizeof(int) * nt);
   const size_t o_map = align(o_ptr + sizeof(int) * (nt + 1));
   const size_t o_idx = align(o_map + sizeof(double) * n_map);
@@ -1266,18 +1398,14 @@ int ds_problem_smp_grid_noalloc(ds_problem* p, double lx, double ly,
   const long long cap = std::min<long long>(
       ((size_t)DS_NOALLOC_SCRATCH_BYTES - o_idx) / sizeof(int), INT_MAX);
   unsigned char* base = ctx->noalloc;
-  double* dlo = reinterpret_cast<double*>(base + o_lo);
-  double* dhi = reinterpret_cast<double*>(base + o_hi);
+  const double* dlo = p->tbox_lo;
+  const double* dhi = p->tbox_hi;
   int* dcnt = reinterpret_cast<int*>(base + o_cnt);
   int* dptr = reinterpret_cast<int*>(base + o_ptr);
   double* dmap = reinterpret_cast<double*>(base + o_map);
   int* didx = reinterpret_cast<int*>(base + o_idx);
   hipStream_t st = ctx->stream;
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(dlo, blk.tlo.data(), sizeof(double) * nb,
-                                  hipMemcpyHostToDevice, st));
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(dhi, blk.thi.data(), sizeof(double) * nb,
-                                  hipMemcpyHostToDevice, st));
-  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, blk.ntile, p->cv, dcnt,
+  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, p->ntile, p->cv, dcnt,
                                          nullptr, nullptr, st));
   hipLaunchKernelGGL(k_list_offsets, dim3(1), dim3(1024), 0, st, dcnt,
                      (int)nt, cap, dptr);
@@ -1290,7 +1418,7 @@ int ds_problem_smp_grid_noalloc(ds_problem* p, double lx, double ly,
     return fail(ctx, DS_E_ARG,
                 "map grid lists need more than the %lld entries the no-alloc "
                 "scratch holds beside this %zu-cell map", cap, n_map);
-  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, blk.ntile, p->cv,
+  HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, p->ntile, p->cv,
                                          nullptr, dptr, didx, st));
   HIP_OR_FAIL(ctx, ds::launch_map64(ds::MM_SMP, g, p->pos64, p->x, dptr, didx,
                                     p->table64, p->cv, dmap, st));
@@ -1340,8 +1468,18 @@ int ds_smp_product_grid(ds_ctx* ctx, int n_pix, const double* pix_pts,
   if (int rc = make_srcset(ctx, pix_pts, 3, n_pix, x, 0, cv, src)) return rc;
   DevBuf<double2> t64;
   if (int rc = upload_table64(ctx, covar, t64)) return rc;
-  return map_grid_run(ctx, ds::MM_SMP, g, src.blk, src.pos.p, src.u.p, t64.p,
-                      cv, m);
+  DevBuf<double> dlo, dhi;
+  const size_t nb = 3 * (size_t)src.blk.ntile;
+  HIP_OR_FAIL(ctx, dlo.alloc(nb));
+  HIP_OR_FAIL(ctx, dhi.alloc(nb));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(dlo.p, src.blk.tlo.data(),
+                                  sizeof(double) * nb, hipMemcpyHostToDevice,
+                                  ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(dhi.p, src.blk.thi.data(),
+                                  sizeof(double) * nb, hipMemcpyHostToDevice,
+                                  ctx->stream));
+  return map_grid_run(ctx, ds::MM_SMP, g, src.blk.ntile, dlo.p, dhi.p,
+                      &src.blk, src.pos.p, src.u.p, t64.p, cv, m);
 }
 
 int ds_sigma_m_noise_dom(ds_ctx* ctx, int n_pix, const double* npix,

@@ -54,6 +54,41 @@ struct CovarDev {
   double s_para;   // sqrt(c_para * dx_inv)
 };
 
+// Quantization of a point set for the k-d ordering: per axis the set's
+// minimum, 2^32 - 1 over its extent (0 for a flat axis) and the weight that
+// turns a quantized extent back into a metric one (scale x extent / 2^32-1).
+struct KdQuant {
+  double lo[3], inv[3], w[3];
+};
+KdQuant kd_quant(const double* pts, int stride, int n, const double scale[3]);
+void kd_quant_axis(double lo, double hi, double scale, KdQuant& Q, int a);
+
+// Quantized coordinate (0 .. 2^32 - 1), monotone in x; host and device
+// evaluate the same two IEEE operations.
+__host__ __device__ inline unsigned kd_q(double x, double lo, double inv) {
+  const double t = (x - lo) * inv;
+  return t >= 4294967295.0 ? 0xFFFFFFFFu : (unsigned)t;
+}
+// Split key: quantized coordinate, then point index (unique per point).
+__host__ __device__ inline unsigned long long kd_key(unsigned q, int idx) {
+  return ((unsigned long long)q << 32) | (unsigned)idx;
+}
+// Split axis: the largest weighted quantized extent, the first on ties.
+__host__ __device__ inline int kd_axis(const unsigned lo[3],
+                                       const unsigned hi[3],
+                                       const double w[3]) {
+  int axis = 0;
+  double best = -1.0;
+  for (int a = 0; a < 3; ++a) {
+    const double ext = (double)(hi[a] - lo[a]) * w[a];
+    if (ext > best) {
+      best = ext;
+      axis = a;
+    }
+  }
+  return axis;
+}
+
 // Host-side result of spatial blocking.
 struct Blocking {
   int n = 0;
@@ -128,6 +163,7 @@ struct ds_ctx {
                                // that may overlap the main stream's
   char err[512] = {0};         // last error (fixed storage: setting it
                                // allocates nothing)
+  int host_geometry = 0;       // build problems' geometry on the host
   int num_cu = 0;
   double* scratch = nullptr;   // reduction partials
   size_t scratch_n = 0;
@@ -151,8 +187,11 @@ struct ds_problem {
   int n = 0, nblk = 0, ntile = 0, npad = 0;
   ds::CovarDev cv{};                   // subvolume 0 (table shared by all)
   std::vector<double> table;           // host copy, table_n entries
-  ds::Blocking blk;                    // one-subvolume problems: its blocking
   std::vector<int> perm;               // slot -> caller index, -1 for padding
+  int* perm_dev = nullptr;             // the same on the device
+  double* tbox_lo = nullptr;           // per tile: box of its real pixels
+  double* tbox_hi = nullptr;           // (3 each; empty tiles lo > hi)
+  long long nslot = 0;                 // column-sum slots (gptr[ntile])
   long long ntile_pairs = 0;           // (block, tile) pairs after culling
   long long nsub_pairs = 0;            // (16-row sub-block, tile) pairs
   long long nhalf_pairs = 0;           // (32-row half block, tile) pairs
@@ -256,6 +295,18 @@ struct ds_problem {
 };
 
 namespace ds {
+// A problem's ordering, culled lists, per-slot arrays and column-sum slots
+// built on the device from the caller's pixel records (geometry_dev.hip);
+// p's sizes, subvolume arrays and covariance constants are set.  Fills
+// p->perm, p->perm_dev, p->nslot and the device arrays, bstat (per block:
+// list length, half-list length, live sub-block entries, live 32-row
+// halves, evaluated sub-block entries) and max_src (per subvolume: largest
+// |source coordinate| in its tile frame).  *error: 1 = more than INT_MAX
+// entries, 2 = inconsistent symmetric lists.
+hipError_t build_geometry_device(ds_problem* p, const double* pixels,
+                                 const std::vector<KdQuant>& quant,
+                                 std::vector<float>& max_src,
+                                 std::vector<int>& bstat, int* error);
 // Order `st` after the block-Jacobi build (once per build).
 inline hipError_t minv_join(const ds_problem* p, hipStream_t st) {
   if (!p->minv_pending) return hipSuccess;

@@ -11,6 +11,7 @@
 // pair's computed index from below: the dropped terms are the reference's
 // exact +0.0.
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <limits>
 #include <numeric>
@@ -43,39 +44,46 @@ void parallel_for(int n, F f) {
 }
 
 // Recursively split perm[b, e) into leaves of `leaf` points; every left child
-// holds a multiple of `leaf` points so only the last leaf can be short.
-void kd_sort(const double* pts, int stride, const double scale[3], int* perm,
-             int b, int e, int leaf) {
+// holds a multiple of `leaf` points so only the last leaf can be short.  A
+// node splits along the axis of its largest quantized extent (weighted by
+// the covariance metric); its left child takes the `left` points of
+// smallest key (quantized coordinate on that axis, then point index) and
+// both children keep the node's order (a stable partition), so the
+// ordering is a function of the points alone: the device builder makes the
+// same splits with a radix select and a scan.
+void kd_order(const double* pts, int stride, const KdQuant& Q, int* perm,
+              int b, int e, int leaf) {
   struct Node { int b, e; };
   std::vector<Node> stack;
+  std::vector<uint64_t> keys;
   if (e > b) stack.push_back({b, e});
   while (!stack.empty()) {
     Node nd = stack.back();
     stack.pop_back();
     const int cnt = nd.e - nd.b;
     if (cnt <= leaf) continue;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0u, 0u, 0u};
     for (int k = nd.b; k < nd.e; ++k) {
       const double* p = pts + (size_t)stride * perm[k];
       for (int a = 0; a < 3; ++a) {
-        lo[a] = std::min(lo[a], p[a]);
-        hi[a] = std::max(hi[a], p[a]);
+        const unsigned q = kd_q(p[a], Q.lo[a], Q.inv[a]);
+        lo[a] = std::min(lo[a], q);
+        hi[a] = std::max(hi[a], q);
       }
     }
-    int axis = 0;
-    double best = -1.0;
-    for (int a = 0; a < 3; ++a) {
-      const double ext = (hi[a] - lo[a]) * scale[a];
-      if (ext > best) { best = ext; axis = a; }
-    }
+    const int axis = kd_axis(lo, hi, Q.w);
     const int nleaf = (cnt + leaf - 1) / leaf;
     const int left = leaf * (nleaf / 2);
-    std::nth_element(perm + nd.b, perm + nd.b + left, perm + nd.e,
-                     [&](int i, int j) {
-                       const double a = pts[(size_t)stride * i + axis];
-                       const double c = pts[(size_t)stride * j + axis];
-                       return a < c || (a == c && i < j);
-                     });
+    auto key = [&](int i) {
+      return kd_key(kd_q(pts[(size_t)stride * i + axis], Q.lo[axis],
+                         Q.inv[axis]), i);
+    };
+    keys.resize(cnt);
+    for (int k = 0; k < cnt; ++k) keys[k] = key(perm[nd.b + k]);
+    std::nth_element(keys.begin(), keys.begin() + left, keys.end());
+    const uint64_t split = keys[left];
+    std::stable_partition(perm + nd.b, perm + nd.e,
+                          [&](int i) { return key(i) < split; });
     stack.push_back({nd.b + left, nd.e});
     stack.push_back({nd.b, nd.b + left});
   }
@@ -95,6 +103,32 @@ void box_of(const double* pts, int stride, const int* perm, int b, int e,
 }
 
 }  // namespace
+
+KdQuant kd_quant(const double* pts, int stride, int n, const double scale[3]) {
+  KdQuant Q;
+  for (int a = 0; a < 3; ++a) {
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (int i = 0; i < n; ++i) {
+      const double x = pts[(size_t)stride * i + a];
+      lo = std::min(lo, x);
+      hi = std::max(hi, x);
+    }
+    kd_quant_axis(lo, hi, scale[a], Q, a);
+  }
+  return Q;
+}
+
+void kd_quant_axis(double lo, double hi, double scale, KdQuant& Q, int a) {
+  const double span = hi - lo;
+  Q.lo[a] = std::isfinite(lo) ? lo : 0.0;
+  if (span > 0.0 && std::isfinite(span)) {
+    Q.inv[a] = 4294967295.0 / span;
+    Q.w[a] = scale * (span / 4294967295.0);
+  } else {
+    Q.inv[a] = 0.0;
+    Q.w[a] = 0.0;
+  }
+}
 
 CovarDev make_covar(const ds_covar* c) {
   CovarDev cv;
@@ -116,11 +150,12 @@ void build_blocking(const double* pts, int stride, int n, const double scale[3],
   out.npad = out.nblk * DS_BS;
   std::vector<int> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
-  kd_sort(pts, stride, scale, perm.data(), 0, n, DS_BS);
+  const KdQuant Q = kd_quant(pts, stride, n, scale);
+  kd_order(pts, stride, Q, perm.data(), 0, n, DS_BS);
   // split each block into tiles
   parallel_for(out.nblk, [&](int B) {
     const int b = B * DS_BS, e = std::min(n, b + DS_BS);
-    kd_sort(pts, stride, scale, perm.data(), b, e, DS_TS);
+    kd_order(pts, stride, Q, perm.data(), b, e, DS_TS);
   });
   out.perm.assign(out.npad, n > 0 ? perm[n - 1] : 0);
   std::copy(perm.begin(), perm.end(), out.perm.begin());

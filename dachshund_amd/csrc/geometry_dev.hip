@@ -1,0 +1,1228 @@
+// geometry_dev.hip -- a problem's ordering, culled tile lists and per-slot
+// arrays, built on the device.
+//
+// The reference evaluates all n^2 pairs (lib/wf_vector.cc:54-73); the pair
+// kernels here visit only (64-row block, 16-slot tile) pairs whose boxes
+// pass the exact cutoff test.  This file builds everything those kernels
+// read from the caller's pixel records, with the host threads doing only
+// the O(n) checks, the tree schedule (a function of the pixel counts) and
+// the launch orders:
+//
+//   1. k-d ordering.  A node splits along its largest weighted quantized
+//      extent; its left child takes the `left` points of smallest key
+//      (quantized coordinate, point index) and both children keep the
+//      node's order.  Nodes above 1024 points are split level by level by
+//      all workgroups (radix select of the split key over 8-bit digits, a
+//      per-chunk count, a scan and a stable scatter); each node of at most
+//      1024 points is then finished, down to its 16-slot tiles, by one
+//      wavefront in LDS.  The splits are those of build_blocking.
+//   2. One wavefront per block: per-slot records (positions, right-hand
+//      side, diagonal, weights), the FP32 frames (per-axis medians of the
+//      block and of each tile), tile and block boxes, the far-point test.
+//   3. Culled lists: per target block, groups of 64 source tiles are tested
+//      first (one box per group), then the tiles of the groups that pass,
+//      with the 16-row sub-block masks: counts, a scan, then the entries
+//      (ascending tile order, the host builder's lists).
+//   4. The symmetric kernel's half lists and their column-sum slots: the
+//      blocks whose records feed tile T are read from T's own block's list
+//      (the sub-block test is symmetric), so no sort is needed.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ds_internal.h"
+
+namespace ds {
+
+namespace {
+
+constexpr int kChunk = 2048;      // big-node chunk: 256 threads x 8 points
+constexpr int kEpt = kChunk / 256;
+constexpr int kWaveMax = 1024;    // nodes up to this size: one wavefront
+constexpr int kStack = 64;
+
+struct BigNode {
+  int b, e, left, sub, c0, nc;    // slots [b, e), chunks [c0, c0 + nc)
+};
+struct Chunk {
+  int node, b, e;
+};
+
+__device__ inline unsigned qax(const uint4& r, int a) {
+  return a == 0 ? r.y : (a == 1 ? r.z : r.w);
+}
+
+__device__ inline unsigned wmin_u(unsigned v) {
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ inline unsigned wmax_u(unsigned v) {
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ inline int wsum_i(int v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline int wscan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+__device__ inline int mbcnt64(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+// min / max within aligned groups of `width` lanes
+template <int W>
+__device__ inline double gmin_d(double v) {
+  for (int o = W / 2; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+template <int W>
+__device__ inline double gmax_d(double v) {
+  for (int o = W / 2; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// The culling test of build_tile_lists: box_gap_xx(a, b) < limit with the
+// same operations (this file is compiled without contraction).
+__device__ inline bool box_pass(const double alo[3], const double ahi[3],
+                                const double blo[3], const double bhi[3],
+                                double c_perp, double c_para, double dx_inv,
+                                double limit) {
+  double g[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (blo[a] > ahi[a]) g[a] = blo[a] - ahi[a];
+    else if (alo[a] > bhi[a]) g[a] = alo[a] - bhi[a];
+    else g[a] = 0.0;
+  }
+  const double x_perp_2 = (g[0] * g[0] + g[1] * g[1]);
+  const double x_para_2 = g[2] * g[2];
+  const double ga_perp = c_perp * x_perp_2;
+  const double ga_para = c_para * x_para_2;
+  return (ga_perp + ga_para) * dx_inv < limit;
+}
+
+struct KdArgs {
+  const double* pix;        // n x 5, caller order
+  uint4* rec;               // npad records {index, q0, q1, q2}
+  uint4* tmp;
+  const int* blk_sub;
+  const int* sub_blk0;
+  const int* sub_n;
+  const int* sub_off;
+  const double* quant;      // per subvolume: lo[3], inv[3], w[3]
+  int npad;
+};
+
+__global__ __launch_bounds__(256) void k_kd_init(KdArgs a) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= a.npad) return;
+  const int k = a.blk_sub[g / DS_BS];
+  const int loc = g - a.sub_blk0[k] * DS_BS;
+  uint4 r = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+  if (loc < a.sub_n[k]) {
+    const int idx = a.sub_off[k] + loc;
+    const double* P = a.pix + 5 * (size_t)idx;
+    const double* Q = a.quant + 9 * (size_t)k;
+    r = make_uint4((unsigned)idx, kd_q(P[0], Q[0], Q[3]),
+                   kd_q(P[1], Q[1], Q[4]), kd_q(P[2], Q[2], Q[5]));
+  }
+  a.rec[g] = r;
+}
+
+// ---- nodes above kWaveMax points: level-synchronous, all workgroups -----
+
+struct BigArgs {
+  KdArgs kd;
+  const BigNode* nodes;
+  const Chunk* chunks;
+  unsigned* nq;               // per node: qlo[3], qhi[3]
+  unsigned long long* pre;    // per node: split key prefix
+  int* rank;                  // per node: rank left within the prefix
+  int* hist;                  // per node: 256 bins
+  int* ccount;                // per chunk: keys below the split
+  int* coff;                  // per chunk: left keys of earlier chunks
+  int* err;                   // bit 8: a split count differs from `left`
+  int nn, nc;
+};
+
+__device__ inline int node_axis(const BigArgs& a, int i) {
+  unsigned lo[3], hi[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    lo[t] = a.nq[6 * i + t];
+    hi[t] = a.nq[6 * i + 3 + t];
+  }
+  return kd_axis(lo, hi, a.kd.quant + 9 * (size_t)a.nodes[i].sub + 6);
+}
+
+__global__ __launch_bounds__(256) void k_big_init(BigArgs a) {
+  const int i = blockIdx.x, t = threadIdx.x;
+  a.hist[256 * (size_t)i + t] = 0;
+  if (t < 3) {
+    a.nq[6 * i + t] = ~0u;
+    a.nq[6 * i + 3 + t] = 0u;
+  }
+  if (t == 0) {
+    a.pre[i] = 0ull;
+    a.rank[i] = a.nodes[i].left;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_bbox(BigArgs a) {
+  const Chunk c = a.chunks[blockIdx.x];
+  unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0u, 0u, 0u};
+  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
+    const uint4 r = a.kd.rec[j];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      lo[t] = min(lo[t], qax(r, t));
+      hi[t] = max(hi[t], qax(r, t));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    lo[t] = wmin_u(lo[t]);
+    hi[t] = wmax_u(hi[t]);
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      atomicMin(&a.nq[6 * c.node + t], lo[t]);
+      atomicMax(&a.nq[6 * c.node + 3 + t], hi[t]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_hist(BigArgs a, int pass) {
+  __shared__ int lh[256];
+  lh[threadIdx.x] = 0;
+  __syncthreads();
+  const Chunk c = a.chunks[blockIdx.x];
+  const int axis = node_axis(a, c.node);
+  const int shift = 56 - 8 * pass;
+  const unsigned long long p = a.pre[c.node];
+  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
+    const uint4 r = a.kd.rec[j];
+    const unsigned long long key = kd_key(qax(r, axis), (int)r.x);
+    if (pass == 0 || (key >> (shift + 8)) == (p >> (shift + 8)))
+      atomicAdd(&lh[(key >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  const int v = lh[threadIdx.x];
+  if (v) atomicAdd(&a.hist[256 * (size_t)c.node + threadIdx.x], v);
+}
+
+// One wavefront per node: the digit holding rank `rank`, appended to the
+// prefix; the histogram is cleared for the next digit.
+__global__ __launch_bounds__(64) void k_big_select(BigArgs a, int pass) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  int* h = a.hist + 256 * (size_t)i;
+  int c[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) c[t] = h[4 * lane + t];
+  const int s = c[0] + c[1] + c[2] + c[3];
+  const int incl = wscan_incl(s), excl = incl - s;
+  const int rk = a.rank[i];
+  const bool mine = excl <= rk && rk < incl;
+  int d = 0, below = excl;
+  if (mine) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (rk < below + c[t]) {
+        d = 4 * lane + t;
+        break;
+      }
+      below += c[t];
+    }
+  }
+  const unsigned long long m = __ballot(mine);
+  const int src = m ? __builtin_ctzll(m) : 0;
+  d = __shfl(d, src);
+  below = __shfl(below, src);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) h[4 * lane + t] = 0;
+  if (lane == 0 && m) {
+    a.pre[i] |= (unsigned long long)d << (56 - 8 * pass);
+    a.rank[i] = rk - below;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_count(BigArgs a) {
+  __shared__ int part[4];
+  const Chunk c = a.chunks[blockIdx.x];
+  const int axis = node_axis(a, c.node);
+  const unsigned long long K = a.pre[c.node];
+  int n = 0;
+  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
+    const uint4 r = a.kd.rec[j];
+    n += kd_key(qax(r, axis), (int)r.x) < K;
+  }
+  n = wsum_i(n);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    a.ccount[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ __launch_bounds__(64) void k_big_scan(BigArgs a) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= a.nn) return;
+  const BigNode nd = a.nodes[i];
+  int run = 0;
+  for (int c = nd.c0; c < nd.c0 + nd.nc; ++c) {
+    a.coff[c] = run;
+    run += a.ccount[c];
+  }
+  if (run != nd.left) atomicOr(a.err, 8);
+}
+
+// Stable partition of each chunk into tmp: keys below the split to the
+// left child (after the earlier chunks' left keys), the rest to the right.
+__global__ __launch_bounds__(256) void k_big_scatter(BigArgs a) {
+  __shared__ int wsum[4];
+  const Chunk c = a.chunks[blockIdx.x];
+  const BigNode nd = a.nodes[c.node];
+  const int axis = node_axis(a, c.node);
+  const unsigned long long K = a.pre[c.node];
+  const int j0 = c.b + (int)threadIdx.x * kEpt;
+  uint4 v[kEpt];
+  unsigned fl = 0;
+  int nl = 0;
+#pragma unroll
+  for (int t = 0; t < kEpt; ++t) {
+    const int j = j0 + t;
+    v[t] = j < c.e ? a.kd.rec[j] : make_uint4(0u, 0u, 0u, 0u);
+    const bool f = j < c.e && kd_key(qax(v[t], axis), (int)v[t].x) < K;
+    fl |= (unsigned)f << t;
+    nl += f;
+  }
+  // exclusive scan of the left counts over the 256 threads
+  const int incl = wscan_incl(nl);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  int li = before + incl - nl;                 // left keys before me
+  int ri = (j0 - c.b) - li;                    // right keys before me
+  const int lbase = nd.b + a.coff[blockIdx.x];
+  const int rbase = nd.b + nd.left + (c.b - nd.b) - a.coff[blockIdx.x];
+#pragma unroll
+  for (int t = 0; t < kEpt; ++t) {
+    const int j = j0 + t;
+    const bool f = (fl >> t) & 1u;
+    const int dst = f ? lbase + li : rbase + ri;
+    if (j < c.e && dst >= nd.b && dst < nd.e) a.kd.tmp[dst] = v[t];
+    li += f;
+    ri += !f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_copy(BigArgs a) {
+  const Chunk c = a.chunks[blockIdx.x];
+  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256)
+    a.kd.rec[j] = a.kd.tmp[j];
+}
+
+// ---- nodes of at most kWaveMax points: one wavefront each, in LDS -------
+
+// Node (b, e) of subvolume `sub` in block phase (leaf 64, then the tiles of
+// each block with leaf 16): every descendant split made in place.
+__global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
+  __shared__ uint4 buf[kWaveMax], buf2[kWaveMax];
+  __shared__ int hist[256];
+  __shared__ int stk[3 * kStack];
+  const int lane = threadIdx.x;
+  const int4 node = wn[blockIdx.x];
+  const int b = node.x, cnt = node.y - node.x;
+  const double* W = a.quant + 9 * (size_t)node.z + 6;
+  for (int j = lane; j < cnt; j += 64) buf[j] = a.rec[b + j];
+  int sp = 0;
+  if (lane == 0) {
+    stk[0] = 0;
+    stk[1] = cnt;
+    stk[2] = cnt > DS_BS ? DS_BS : DS_TS;
+  }
+  sp = cnt > DS_TS ? 1 : 0;
+  __syncthreads();
+  while (sp > 0) {
+    --sp;
+    const int lb = stk[3 * sp], le = stk[3 * sp + 1], L = stk[3 * sp + 2];
+    const int m = le - lb;
+    unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0u, 0u, 0u};
+    for (int j = lb + lane; j < le; j += 64) {
+      const uint4 r = buf[j];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        lo[t] = min(lo[t], qax(r, t));
+        hi[t] = max(hi[t], qax(r, t));
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      lo[t] = wmin_u(lo[t]);
+      hi[t] = wmax_u(hi[t]);
+    }
+    const int axis = kd_axis(lo, hi, W);
+    const int nleaf = (m + L - 1) / L;
+    const int left = L * (nleaf / 2);
+    // radix select of the key of rank `left`
+    unsigned long long pre = 0ull;
+    int rk = left;
+    for (int pass = 0; pass < 8; ++pass) {
+      const int shift = 56 - 8 * pass;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) hist[4 * lane + t] = 0;
+      __syncthreads();
+      for (int j = lb + lane; j < le; j += 64) {
+        const uint4 r = buf[j];
+        const unsigned long long key = kd_key(qax(r, axis), (int)r.x);
+        if (pass == 0 || (key >> (shift + 8)) == (pre >> (shift + 8)))
+          atomicAdd(&hist[(key >> shift) & 255u], 1);
+      }
+      __syncthreads();
+      int c[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) c[t] = hist[4 * lane + t];
+      const int s = c[0] + c[1] + c[2] + c[3];
+      const int incl = wscan_incl(s), excl = incl - s;
+      const bool mine = excl <= rk && rk < incl;
+      int d = 0, below = excl;
+      if (mine) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (rk < below + c[t]) {
+            d = 4 * lane + t;
+            break;
+          }
+          below += c[t];
+        }
+      }
+      const unsigned long long mm = __ballot(mine);
+      const int src = mm ? __builtin_ctzll(mm) : 0;
+      d = __shfl(d, src);
+      below = __shfl(below, src);
+      pre |= (unsigned long long)d << shift;
+      rk -= below;
+      __syncthreads();
+    }
+    // stable partition into buf2, then back
+    int nl = 0, nr = 0;
+    for (int base = lb; base < le; base += 64) {
+      const int j = base + lane;
+      const bool valid = j < le;
+      uint4 r = make_uint4(0u, 0u, 0u, 0u);
+      bool f = false;
+      if (valid) {
+        r = buf[j];
+        f = kd_key(qax(r, axis), (int)r.x) < pre;
+      }
+      const unsigned long long bl = __ballot(f), bv = __ballot(valid);
+      const int rl = mbcnt64(bl), rv = mbcnt64(bv);
+      const int dst = f ? lb + nl + rl : lb + left + nr + (rv - rl);
+      if (valid && dst < le) buf2[dst] = r;
+      nl += __popcll(bl);
+      nr += __popcll(bv) - __popcll(bl);
+    }
+    __syncthreads();
+    for (int j = lb + lane; j < le; j += 64) buf[j] = buf2[j];
+    __syncthreads();
+    // children: leaf 64 until a child fits a block, then leaf 16
+    const int cb[2] = {lb, lb + left}, ce[2] = {lb + left, le};
+    int push = 0;
+#pragma unroll
+    for (int h = 1; h >= 0; --h) {
+      const int cn = ce[h] - cb[h];
+      int Lc = 0;
+      if (L == DS_BS && cn > DS_BS) Lc = DS_BS;
+      else if (cn > DS_TS) Lc = DS_TS;
+      if (Lc) {
+        if (lane == 0) {
+          stk[3 * (sp + push)] = cb[h];
+          stk[3 * (sp + push) + 1] = ce[h];
+          stk[3 * (sp + push) + 2] = Lc;
+        }
+        ++push;
+      }
+    }
+    sp += push;
+    __syncthreads();
+  }
+  for (int j = lane; j < cnt; j += 64) a.rec[b + j] = buf[j];
+}
+
+// ---- per block: slots, frames, boxes ------------------------------------
+
+struct BlkArgs {
+  const uint4* rec;
+  const double* pix;
+  const int* blk_sub;
+  const int* sub_blk0;
+  const int* sub_n;
+  const double4* subcv;     // {var_s, c_perp, c_para, 0}
+  const double* s3;         // per subvolume: s_perp, s_para
+  int form;
+  int* perm;
+  double4* pos;
+  double* rhs;
+  double* diag;
+  double* wvec;
+  float4* loc;
+  float4* tgt;
+  double* tlo;
+  double* thi;
+  int* tcount;
+  double* cen;
+  double* tcen;
+  unsigned* msrc;           // per subvolume: max |source coordinate| bits
+  int* err;                 // bit 4: a slot without a valid record
+  int n;
+};
+
+// The value of rank cnt / 2 among the real lanes of a group (median_frame
+// of the host builder: nth_element at size / 2).
+template <int W>
+__device__ inline double group_median(double v, bool real, int cnt) {
+  const int lane = threadIdx.x & 63, g0 = lane & ~(W - 1);
+  int rank = 0;
+  for (int j = 0; j < W; ++j) {
+    const double u = __shfl(v, g0 + j);
+    const bool rj = j < cnt;
+    rank += rj && (u < v || (u == v && g0 + j < lane));
+  }
+  const bool hit = real && rank == cnt / 2;
+  const unsigned long long m = __ballot(hit);
+  const unsigned long long gm = W == 64 ? m : (m >> g0) & ((1ull << W) - 1);
+  const int src = gm ? g0 + __builtin_ctzll(gm) : lane;
+  return __shfl(v, src);
+}
+
+__global__ __launch_bounds__(256) void k_kd_blocks(BlkArgs a, int nblk) {
+  const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (B >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int k = a.blk_sub[B];
+  const int nreal = min(DS_BS, a.sub_n[k] - (B - a.sub_blk0[k]) * DS_BS);
+  const bool real = lane < nreal;
+  const size_t g = (size_t)B * DS_BS + lane;
+  const uint4 r = a.rec[(size_t)B * DS_BS + (real ? lane : nreal - 1)];
+  int idx = (int)r.x;
+  if (idx < 0 || idx >= a.n) {
+    atomicOr(a.err, 4);
+    idx = 0;
+  }
+  const double* P = a.pix + 5 * (size_t)idx;
+  const double p[3] = {P[0], P[1], P[2]};
+  const double w0 = P[3], dval = P[4];
+  const double4 cv = a.subcv[k];
+  const double var_s = cv.x;
+  a.perm[g] = real ? idx : -1;
+  const double wN = real ? w0 : (a.form == DS_FORM_W ? 0.0 : 1.0);
+  a.pos[g] = make_double4(p[0], p[1], p[2], wN);
+  // lib/wf_vector.cc:190-192 (W) and :44-47 (SN)
+  a.rhs[g] = real ? dval : 0.0;
+  a.diag[g] = real ? (a.form == DS_FORM_W ? wN * wN * var_s + 1.0
+                                          : var_s + wN)
+                   : 1.0;
+  a.wvec[g] = real ? (a.form == DS_FORM_W ? wN : 1.0) : 0.0;
+  const int t = lane >> 4;
+  const int tcnt = max(0, min(DS_TS, nreal - DS_TS * t));
+  const int T = B * DS_TPB + t;
+  const double inf = __builtin_huge_val();
+  double c[3], tc[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const double tl = gmin_d<16>(real ? p[ax] : inf);
+    const double th = gmax_d<16>(real ? p[ax] : -inf);
+    if ((lane & 15) == 0) {
+      a.tlo[3 * (size_t)T + ax] = tl;
+      a.thi[3 * (size_t)T + ax] = th;
+    }
+    c[ax] = group_median<64>(p[ax], real, nreal);
+    const double tm = group_median<16>(p[ax], real, tcnt);
+    tc[ax] = tcnt > 0 ? tm : c[ax];   // an empty tile takes its block's
+    if (lane == 0) a.cen[3 * (size_t)B + ax] = c[ax];
+    if ((lane & 15) == 0) a.tcen[3 * (size_t)T + ax] = tc[ax];
+  }
+  if ((lane & 15) == 0) a.tcount[T] = tcnt;
+  const double s3[3] = {a.s3[2 * k], a.s3[2 * k], a.s3[2 * k + 1]};
+  float l[3], tg[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    l[ax] = (float)((p[ax] - tc[ax]) * s3[ax]);
+    tg[ax] = (float)((p[ax] - c[ax]) * s3[ax]);
+  }
+  a.loc[g] = make_float4(l[0], l[1], l[2],
+                         l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+  a.tgt[g] = make_float4(tg[0], tg[1], tg[2], 0.0f);
+  float ms = real ? fmaxf(fabsf(l[0]), fmaxf(fabsf(l[1]), fabsf(l[2]))) : 0.0f;
+  for (int o = 32; o >= 1; o >>= 1) ms = fmaxf(ms, __shfl_xor(ms, o));
+  if (lane == 0) atomicMax(&a.msrc[k], __float_as_uint(ms));
+}
+
+// Box of each group of 64 consecutive tiles (live tiles only).
+__global__ __launch_bounds__(256) void k_kd_groups(const double* tlo,
+                                                   const double* thi,
+                                                   const int* tcount,
+                                                   int ntile, int ngroup,
+                                                   double* glo, double* ghi) {
+  const int G = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (G >= ngroup) return;
+  const int T = G * 64 + (threadIdx.x & 63);
+  const bool live = T < ntile && tcount[T] > 0;
+  const double inf = __builtin_huge_val();
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const double lo = gmin_d<64>(live ? tlo[3 * (size_t)T + ax] : inf);
+    const double hi = gmax_d<64>(live ? thi[3 * (size_t)T + ax] : -inf);
+    if ((threadIdx.x & 63) == 0) {
+      glo[3 * (size_t)G + ax] = lo;
+      ghi[3 * (size_t)G + ax] = hi;
+    }
+  }
+}
+
+struct ListArgs {
+  const double* tlo;
+  const double* thi;
+  const int* tcount;
+  const double* glo;
+  const double* ghi;
+  const int* blk_sub;
+  const int* sub_blk0;
+  const double4* subcv;
+  const double* s3;
+  const double* cen;
+  const double* tcen;
+  double dx_inv, limit;
+  int nblk;
+  int* bstat;               // count pass: 5 per block
+  const int* tptr;          // fill pass
+  int* tidx;
+  unsigned char* tmask;
+  float4* toff;
+};
+
+// Per target block: the source tiles of its subvolume that pass the cutoff
+// test, with the mask of its 16-row sub-blocks (its own tiles) that do.
+// Count pass: list length, half-list length and the pair counts; fill
+// pass: the entries.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_kd_lists(ListArgs a) {
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (I >= a.nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int k = a.blk_sub[I];
+  const int T0 = a.sub_blk0[k] * DS_TPB, T1 = a.sub_blk0[k + 1] * DS_TPB;
+  const double c_perp = a.subcv[k].y, c_para = a.subcv[k].z;
+  const double inf = __builtin_huge_val();
+  double slo[DS_TPB][3], shi[DS_TPB][3], alo[3], ahi[3];
+  bool slive[DS_TPB];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    alo[ax] = inf;
+    ahi[ax] = -inf;
+  }
+#pragma unroll
+  for (int s = 0; s < DS_TPB; ++s) {
+    const size_t Ts = (size_t)I * DS_TPB + s;
+    slive[s] = a.tcount[Ts] > 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      slo[s][ax] = a.tlo[3 * Ts + ax];
+      shi[s][ax] = a.thi[3 * Ts + ax];
+      if (slive[s]) {
+        alo[ax] = fmin(alo[ax], slo[s][ax]);
+        ahi[ax] = fmax(ahi[ax], shi[s][ax]);
+      }
+    }
+  }
+  float o3[3] = {0.f, 0.f, 0.f};
+  double cI[3] = {0.0, 0.0, 0.0}, s3[3] = {0.0, 0.0, 0.0};
+  if (FILL) {
+    s3[0] = s3[1] = a.s3[2 * k];
+    s3[2] = a.s3[2 * k + 1];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) cI[ax] = a.cen[3 * (size_t)I + ax];
+  }
+  const int base = FILL ? a.tptr[I] : 0;
+  int cnt = 0, hc = 0, npop = 0, nhalf = 0, neval = 0;
+  const int G0 = T0 >> 6, G1 = (T1 - 1) >> 6;
+  for (int Gb = G0; Gb <= G1; Gb += 64) {
+    const int Gl = Gb + lane;
+    bool okg = false;
+    if (Gl <= G1) {
+      double gl[3], gh[3];
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        gl[ax] = a.glo[3 * (size_t)Gl + ax];
+        gh[ax] = a.ghi[3 * (size_t)Gl + ax];
+      }
+      okg = gl[0] <= gh[0] &&
+            box_pass(alo, ahi, gl, gh, c_perp, c_para, a.dx_inv, a.limit);
+    }
+    unsigned long long bm = __ballot(okg);
+    while (bm) {
+      const int G = Gb + __builtin_ctzll(bm);
+      bm &= bm - 1;
+      const int T = G * 64 + lane;
+      bool ok = T >= T0 && T < T1 && a.tcount[T] > 0;
+      unsigned msk = 0;
+      if (ok) {
+        double jl[3], jh[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          jl[ax] = a.tlo[3 * (size_t)T + ax];
+          jh[ax] = a.thi[3 * (size_t)T + ax];
+        }
+        ok = box_pass(alo, ahi, jl, jh, c_perp, c_para, a.dx_inv, a.limit);
+        if (ok) {
+#pragma unroll
+          for (int s = 0; s < DS_TPB; ++s)
+            if (slive[s] && box_pass(slo[s], shi[s], jl, jh, c_perp, c_para,
+                                     a.dx_inv, a.limit))
+              msk |= 1u << s;
+          ok = msk != 0;
+        }
+      }
+      const unsigned long long bb = __ballot(ok);
+      if (ok) {
+        const int e = base + cnt + mbcnt64(bb);
+        if (FILL && e < a.tptr[I + 1]) {
+          a.tidx[e] = T;
+          a.tmask[e] = (unsigned char)msk;
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            o3[ax] = (float)((cI[ax] - a.tcen[3 * (size_t)T + ax]) * s3[ax]);
+          const unsigned bits = msk | ((unsigned)T << 4);
+          a.toff[e] = make_float4(o3[0], o3[1], o3[2], __uint_as_float(bits));
+        } else if (!FILL) {
+          npop += __popc(msk);
+          nhalf += ((msk & 3u) != 0) + ((msk & 12u) != 0);
+          const int dT = T - DS_TPB * I;
+          if (dT >= 0) {
+            const unsigned keep = dT >= DS_TPB - 1 ? 15u : (2u << dT) - 1u;
+            const unsigned mm = msk & keep;
+            if (mm) {
+              ++hc;
+              neval += __popc(mm);
+            }
+          }
+        }
+      }
+      cnt += __popcll(bb);
+    }
+  }
+  if (!FILL) {
+    hc = wsum_i(hc);
+    npop = wsum_i(npop);
+    nhalf = wsum_i(nhalf);
+    neval = wsum_i(neval);
+    if (lane == 0) {
+      int* st = a.bstat + 5 * (size_t)I;
+      st[0] = cnt;
+      st[1] = hc;
+      st[2] = npop;
+      st[3] = nhalf;
+      st[4] = neval;
+    }
+  }
+}
+
+// Exclusive scan of in[i * stride] (n values) into out[0 .. n]; out[n] = -1
+// when the total exceeds INT_MAX.  One workgroup.
+__global__ __launch_bounds__(1024) void k_kd_exscan(const int* in, int stride,
+                                                    int n, int* out) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const long long per = ((long long)n + 1023) / 1024;
+  const long long b = per * t, e = std::min<long long>(n, b + per);
+  long long s = 0;
+  for (long long i = b; i < e; ++i) s += in[i * stride];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const long long v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;
+  const long long total = part[1023];
+  const bool ok = total <= 2147483647LL;
+  for (long long i = b; i < e; ++i) {
+    out[i] = ok ? (int)run : 0;
+    run += in[i * stride];
+  }
+  if (t == 1023) out[n] = ok ? (int)total : -1;
+}
+
+struct HalfArgs {
+  const int* tptr;
+  const int* tidx;
+  const unsigned char* tmask;
+  const float4* toff;
+  const int* hptr;
+  float4* hrec;
+  int* gslot;
+  int* gcnt;
+  const int* gptr;
+  int* err;
+  int nblk;
+};
+
+// Block B's half list: its entries for tiles T >= 4 B, live bits kept for
+// the sub-blocks s with 4 B + s <= T.
+__global__ __launch_bounds__(256) void k_kd_half(HalfArgs a) {
+  const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (B >= a.nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int e0 = a.tptr[B], e1 = a.tptr[B + 1], base = a.hptr[B];
+  int c = 0;
+  for (int eb = e0; eb < e1; eb += 64) {
+    const int e = eb + lane;
+    bool keep_rec = false;
+    unsigned mm = 0;
+    int T = 0;
+    if (e < e1) {
+      T = a.tidx[e];
+      const int dT = T - DS_TPB * B;
+      if (dT >= 0) {
+        const unsigned keep = dT >= DS_TPB - 1 ? 15u : (2u << dT) - 1u;
+        mm = a.tmask[e] & keep;
+        keep_rec = mm != 0;
+      }
+    }
+    const unsigned long long bb = __ballot(keep_rec);
+    const int r = base + c + mbcnt64(bb);
+    if (keep_rec && r < a.hptr[B + 1]) {
+      const float4 o = a.toff[e];
+      const unsigned bits = mm | ((unsigned)T << 4);
+      a.hrec[r] = make_float4(o.x, o.y, o.z, __uint_as_float(bits));
+      a.gslot[r] = -1;
+    }
+    c += __popcll(bb);
+  }
+}
+
+// Column-sum slots: tile T is fed by the records (B', T) of the blocks B'
+// with a sub-block s, 4 B' + s < T, live against T; by the symmetry of the
+// box test these are the tiles T' = 4 B' + s < T in the list of T's own
+// block with T's bit set, in ascending order.  Count pass: slots per tile;
+// fill pass: each record's slot (tile's first slot + the block's rank).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_kd_gather(HalfArgs a) {
+  const int Bt = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (Bt >= a.nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int e0 = a.tptr[Bt], e1 = a.tptr[Bt + 1];
+  int carry[DS_TPB], run[DS_TPB];
+#pragma unroll
+  for (int t = 0; t < DS_TPB; ++t) {
+    carry[t] = -1;
+    run[t] = 0;
+  }
+  for (int eb = e0; eb < e1; eb += 64) {
+    const int e = eb + lane;
+    const bool valid = e < e1;
+    const int Tp = valid ? a.tidx[e] : 0;
+    const unsigned m = valid ? a.tmask[e] : 0u;
+    const int Bp = Tp >> 2;
+#pragma unroll
+    for (int t = 0; t < DS_TPB; ++t) {
+      const int T = DS_TPB * Bt + t;
+      const bool q = valid && Tp < T && ((m >> t) & 1u);
+      const unsigned long long bq = __ballot(q);
+      const unsigned long long below =
+          lane ? bq & ((1ull << lane) - 1ull) : 0ull;
+      const int src = below ? 63 - __builtin_clzll(below) : lane;
+      const int prevB = __shfl(Bp, src);
+      const bool first = q && Bp != (below ? prevB : carry[t]);
+      const unsigned long long bf = __ballot(first);
+      if (FILL && first) {
+        const int rnk = run[t] + mbcnt64(bf);
+        int lo = a.hptr[Bp], hi = a.hptr[Bp + 1];
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          const int Tm = (int)(__float_as_uint(a.hrec[mid].w) >> 4);
+          if (Tm < T) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < a.hptr[Bp + 1] &&
+            (int)(__float_as_uint(a.hrec[lo].w) >> 4) == T)
+          a.gslot[lo] = a.gptr[T] + rnk;
+        else
+          atomicOr(a.err, 1);
+      }
+      run[t] += __popcll(bf);
+      const int last = bq ? 63 - __builtin_clzll(bq) : 0;
+      const int lastB = __shfl(Bp, last);
+      if (bq) carry[t] = lastB;
+    }
+  }
+  if (!FILL && lane < DS_TPB) {
+    int v = run[0];
+#pragma unroll
+    for (int t = 1; t < DS_TPB; ++t)
+      if (lane == t) v = run[t];
+    a.gcnt[DS_TPB * Bt + lane] = v;
+  }
+}
+
+template <class T>
+hipError_t galloc(T** p, size_t count) {
+  *p = nullptr;
+  return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+template <class T>
+struct Tmp {
+  T* p = nullptr;
+  hipError_t alloc(size_t n) { return galloc(&p, n); }
+  ~Tmp() {
+    if (p) (void)hipFree((void*)p);
+  }
+};
+
+#define GTRY(x)                                   \
+  do {                                            \
+    const hipError_t e_ = (x);                    \
+    if (e_ != hipSuccess) return e_;              \
+  } while (0)
+
+}  // namespace
+
+hipError_t build_geometry_device(ds_problem* p, const double* pixels,
+                                 const std::vector<KdQuant>& quant,
+                                 std::vector<float>& max_src,
+                                 std::vector<int>& bstat, int* error) {
+  *error = 0;
+  hipStream_t st = p->ctx->stream;
+  const int n = p->n, n_sub = p->n_sub, nblk = p->nblk, ntile = p->ntile,
+            npad = p->npad;
+  // ---- the tree schedule, from the pixel counts alone ----
+  std::vector<std::vector<BigNode>> lv_nodes;
+  std::vector<std::vector<Chunk>> lv_chunks;
+  std::vector<int4> wave_nodes;
+  struct Pend { int b, e, sub, depth; };
+  std::vector<Pend> todo;
+  for (int k = 0; k < n_sub; ++k) {
+    const int b = p->sub_blk0_h[k] * DS_BS;
+    todo.push_back({b, b + p->sub_n[k], k, 0});
+  }
+  while (!todo.empty()) {
+    const Pend d = todo.back();
+    todo.pop_back();
+    const int cnt = d.e - d.b;
+    if (cnt <= DS_TS) continue;
+    if (cnt <= kWaveMax) {
+      wave_nodes.push_back(make_int4(d.b, d.e, d.sub, 0));
+      continue;
+    }
+    if ((int)lv_nodes.size() <= d.depth) {
+      lv_nodes.resize(d.depth + 1);
+      lv_chunks.resize(d.depth + 1);
+    }
+    const int nleaf = (cnt + DS_BS - 1) / DS_BS;
+    const int left = DS_BS * (nleaf / 2);
+    std::vector<Chunk>& ch = lv_chunks[d.depth];
+    BigNode nd{d.b, d.e, left, d.sub, (int)ch.size(), 0};
+    const int id = (int)lv_nodes[d.depth].size();
+    for (int c = d.b; c < d.e; c += kChunk)
+      ch.push_back({id, c, std::min(d.e, c + kChunk)});
+    nd.nc = (int)ch.size() - nd.c0;
+    lv_nodes[d.depth].push_back(nd);
+    todo.push_back({d.b + left, d.e, d.sub, d.depth + 1});
+    todo.push_back({d.b, d.b + left, d.sub, d.depth + 1});
+  }
+  size_t max_nn = 1, max_nc = 1, tot_nn = 0, tot_nc = 0;
+  for (size_t l = 0; l < lv_nodes.size(); ++l) {
+    max_nn = std::max(max_nn, lv_nodes[l].size());
+    max_nc = std::max(max_nc, lv_chunks[l].size());
+    tot_nn += lv_nodes[l].size();
+    tot_nc += lv_chunks[l].size();
+  }
+  // ---- inputs ----
+  std::vector<double> hq(9 * (size_t)n_sub), hs3(2 * (size_t)n_sub);
+  std::vector<int> hoff(n_sub), hblk_sub(nblk), hnreal(nblk);
+  std::vector<double4> hcv(n_sub);
+  for (int k = 0; k < n_sub; ++k) {
+    for (int a = 0; a < 3; ++a) {
+      hq[9 * k + a] = quant[k].lo[a];
+      hq[9 * k + 3 + a] = quant[k].inv[a];
+      hq[9 * k + 6 + a] = quant[k].w[a];
+    }
+    const CovarDev& cv = p->sub_cv_h[k];
+    hs3[2 * k] = cv.s_perp;
+    hs3[2 * k + 1] = cv.s_para;
+    hcv[k] = make_double4(cv.var_s, cv.c_perp, cv.c_para, 0.0);
+    hoff[k] = p->sub_off[k];
+    for (int B = p->sub_blk0_h[k]; B < p->sub_blk0_h[k + 1]; ++B) {
+      hblk_sub[B] = k;
+      hnreal[B] = std::min(DS_BS, p->sub_n[k] - (B - p->sub_blk0_h[k]) * DS_BS);
+    }
+  }
+  std::vector<BigNode> all_nodes;
+  std::vector<Chunk> all_chunks;
+  all_nodes.reserve(tot_nn);
+  all_chunks.reserve(tot_nc);
+  for (size_t l = 0; l < lv_nodes.size(); ++l) {
+    all_nodes.insert(all_nodes.end(), lv_nodes[l].begin(), lv_nodes[l].end());
+    all_chunks.insert(all_chunks.end(), lv_chunks[l].begin(),
+                      lv_chunks[l].end());
+  }
+  Tmp<double> dpix, dq, ds3, dcen, dtcen, dglo, dghi;
+  Tmp<int> dsub_n, dsub_off, dcc, dco, drank, dhist, dgcnt, derr, dstat;
+  Tmp<double4> dcv;
+  Tmp<uint4> drec, dtmp;
+  Tmp<BigNode> dnodes;
+  Tmp<Chunk> dchunks;
+  Tmp<unsigned> dnq, dmsrc;
+  Tmp<unsigned long long> dpre;
+  Tmp<int4> dwave;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
+                 : hipSuccess;
+  };
+  GTRY(dpix.alloc(5 * (size_t)n));
+  GTRY(up(dpix.p, pixels, sizeof(double) * 5 * (size_t)n));
+  GTRY(dq.alloc(hq.size()));
+  GTRY(up(dq.p, hq.data(), sizeof(double) * hq.size()));
+  GTRY(ds3.alloc(hs3.size()));
+  GTRY(up(ds3.p, hs3.data(), sizeof(double) * hs3.size()));
+  GTRY(dcv.alloc(n_sub));
+  GTRY(up(dcv.p, hcv.data(), sizeof(double4) * n_sub));
+  GTRY(dsub_n.alloc(n_sub));
+  GTRY(up(dsub_n.p, p->sub_n.data(), sizeof(int) * n_sub));
+  GTRY(dsub_off.alloc(n_sub));
+  GTRY(up(dsub_off.p, hoff.data(), sizeof(int) * n_sub));
+  GTRY(galloc(&p->blk_sub, nblk));
+  GTRY(up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
+  GTRY(galloc(&p->blk_nreal, nblk));
+  GTRY(up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
+  GTRY(galloc(&p->sub_blk0, n_sub + 1));
+  GTRY(up(p->sub_blk0, p->sub_blk0_h.data(), sizeof(int) * (n_sub + 1)));
+  GTRY(drec.alloc(npad));
+  GTRY(derr.alloc(1));
+  GTRY(hipMemsetAsync(derr.p, 0, sizeof(int), st));
+  if (tot_nn) {
+    GTRY(dtmp.alloc(npad));
+    GTRY(dnodes.alloc(tot_nn));
+    GTRY(up(dnodes.p, all_nodes.data(), sizeof(BigNode) * tot_nn));
+    GTRY(dchunks.alloc(tot_nc));
+    GTRY(up(dchunks.p, all_chunks.data(), sizeof(Chunk) * tot_nc));
+    GTRY(dnq.alloc(6 * max_nn));
+    GTRY(dpre.alloc(max_nn));
+    GTRY(drank.alloc(max_nn));
+    GTRY(dhist.alloc(256 * max_nn));
+    GTRY(dcc.alloc(max_nc));
+    GTRY(dco.alloc(max_nc));
+  }
+  KdArgs kd{};
+  kd.pix = dpix.p;
+  kd.rec = drec.p;
+  kd.tmp = dtmp.p;
+  kd.blk_sub = p->blk_sub;
+  kd.sub_blk0 = p->sub_blk0;
+  kd.sub_n = dsub_n.p;
+  kd.sub_off = dsub_off.p;
+  kd.quant = dq.p;
+  kd.npad = npad;
+  hipLaunchKernelGGL(k_kd_init, dim3((npad + 255) / 256), dim3(256), 0, st,
+                     kd);
+  GTRY(hipGetLastError());
+  // ---- big nodes, level by level ----
+  size_t no = 0, co = 0;
+  for (size_t l = 0; l < lv_nodes.size(); ++l) {
+    BigArgs b{};
+    b.kd = kd;
+    b.nodes = dnodes.p + no;
+    b.chunks = dchunks.p + co;
+    b.nq = dnq.p;
+    b.pre = dpre.p;
+    b.rank = drank.p;
+    b.hist = dhist.p;
+    b.ccount = dcc.p;
+    b.coff = dco.p;
+    b.err = derr.p;
+    b.nn = (int)lv_nodes[l].size();
+    b.nc = (int)lv_chunks[l].size();
+    const dim3 gn(b.nn), gc(b.nc);
+    hipLaunchKernelGGL(k_big_init, gn, dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_big_bbox, gc, dim3(256), 0, st, b);
+    for (int pass = 0; pass < 8; ++pass) {
+      hipLaunchKernelGGL(k_big_hist, gc, dim3(256), 0, st, b, pass);
+      hipLaunchKernelGGL(k_big_select, gn, dim3(64), 0, st, b, pass);
+    }
+    hipLaunchKernelGGL(k_big_count, gc, dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_big_scan, dim3((b.nn + 63) / 64), dim3(64), 0, st,
+                       b);
+    hipLaunchKernelGGL(k_big_scatter, gc, dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_big_copy, gc, dim3(256), 0, st, b);
+    GTRY(hipGetLastError());
+    no += lv_nodes[l].size();
+    co += lv_chunks[l].size();
+  }
+  // ---- the rest of each tree, one wavefront per node ----
+  if (!wave_nodes.empty()) {
+    GTRY(dwave.alloc(wave_nodes.size()));
+    GTRY(up(dwave.p, wave_nodes.data(), sizeof(int4) * wave_nodes.size()));
+    hipLaunchKernelGGL(k_kd_wave, dim3((unsigned)wave_nodes.size()), dim3(64),
+                       0, st, kd, (const int4*)dwave.p);
+    GTRY(hipGetLastError());
+  }
+  // ---- per block ----
+  GTRY(galloc(&p->pos64, npad));
+  GTRY(galloc(&p->rhs64, npad));
+  GTRY(galloc(&p->diag, npad));
+  GTRY(galloc(&p->wvec, npad));
+  GTRY(galloc(&p->loc32, npad));
+  GTRY(galloc(&p->tgt32, npad));
+  GTRY(galloc(&p->perm_dev, npad));
+  GTRY(galloc(&p->tbox_lo, 3 * (size_t)ntile));
+  GTRY(galloc(&p->tbox_hi, 3 * (size_t)ntile));
+  GTRY(galloc(&p->tcount, ntile));
+  GTRY(dcen.alloc(3 * (size_t)nblk));
+  GTRY(dtcen.alloc(3 * (size_t)ntile));
+  GTRY(dmsrc.alloc(n_sub));
+  GTRY(hipMemsetAsync(dmsrc.p, 0, sizeof(unsigned) * n_sub, st));
+  BlkArgs ba{};
+  ba.rec = drec.p;
+  ba.pix = dpix.p;
+  ba.blk_sub = p->blk_sub;
+  ba.sub_blk0 = p->sub_blk0;
+  ba.sub_n = dsub_n.p;
+  ba.subcv = dcv.p;
+  ba.s3 = ds3.p;
+  ba.form = p->form;
+  ba.perm = p->perm_dev;
+  ba.pos = p->pos64;
+  ba.rhs = p->rhs64;
+  ba.diag = p->diag;
+  ba.wvec = p->wvec;
+  ba.loc = p->loc32;
+  ba.tgt = p->tgt32;
+  ba.tlo = p->tbox_lo;
+  ba.thi = p->tbox_hi;
+  ba.tcount = p->tcount;
+  ba.cen = dcen.p;
+  ba.tcen = dtcen.p;
+  ba.msrc = dmsrc.p;
+  ba.err = derr.p;
+  ba.n = n;
+  const dim3 gb((nblk + 3) / 4);
+  hipLaunchKernelGGL(k_kd_blocks, gb, dim3(256), 0, st, ba, nblk);
+  GTRY(hipGetLastError());
+  const int ngroup = (ntile + 63) / 64;
+  GTRY(dglo.alloc(3 * (size_t)ngroup));
+  GTRY(dghi.alloc(3 * (size_t)ngroup));
+  hipLaunchKernelGGL(k_kd_groups, dim3((ngroup + 3) / 4), dim3(256), 0, st,
+                     p->tbox_lo, p->tbox_hi, p->tcount, ntile, ngroup, dglo.p,
+                     dghi.p);
+  GTRY(hipGetLastError());
+  // ---- culled lists: count, scan, fill ----
+  GTRY(dstat.alloc(5 * (size_t)nblk));
+  ListArgs la{};
+  la.tlo = p->tbox_lo;
+  la.thi = p->tbox_hi;
+  la.tcount = p->tcount;
+  la.glo = dglo.p;
+  la.ghi = dghi.p;
+  la.blk_sub = p->blk_sub;
+  la.sub_blk0 = p->sub_blk0;
+  la.subcv = dcv.p;
+  la.s3 = ds3.p;
+  la.cen = dcen.p;
+  la.tcen = dtcen.p;
+  la.dx_inv = p->cv.dx_inv;
+  la.limit = cull_limit(p->cv);
+  la.nblk = nblk;
+  la.bstat = dstat.p;
+  hipLaunchKernelGGL(k_kd_lists<false>, gb, dim3(256), 0, st, la);
+  GTRY(hipGetLastError());
+  GTRY(galloc(&p->tptr, nblk + 1));
+  GTRY(galloc(&p->hptr, nblk + 1));
+  hipLaunchKernelGGL(k_kd_exscan, dim3(1), dim3(1024), 0, st, dstat.p, 5,
+                     nblk, p->tptr);
+  hipLaunchKernelGGL(k_kd_exscan, dim3(1), dim3(1024), 0, st, dstat.p + 1, 5,
+                     nblk, p->hptr);
+  GTRY(hipGetLastError());
+  int tot[2] = {0, 0};
+  GTRY(hipMemcpyAsync(&tot[0], p->tptr + nblk, sizeof(int),
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipMemcpyAsync(&tot[1], p->hptr + nblk, sizeof(int),
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipStreamSynchronize(st));
+  if (tot[0] < 0 || tot[1] < 0) {
+    *error = 1;   // more than INT_MAX list entries
+    return hipSuccess;
+  }
+  GTRY(galloc(&p->tidx, tot[0]));
+  GTRY(galloc(&p->tmask, tot[0]));
+  GTRY(galloc(&p->toff32, tot[0]));
+  la.tptr = p->tptr;
+  la.tidx = p->tidx;
+  la.tmask = p->tmask;
+  la.toff = p->toff32;
+  hipLaunchKernelGGL(k_kd_lists<true>, gb, dim3(256), 0, st, la);
+  GTRY(hipGetLastError());
+  // ---- half lists and column-sum slots ----
+  GTRY(galloc(&p->hrec, tot[1]));
+  GTRY(galloc(&p->gslot, tot[1]));
+  GTRY(galloc(&p->gptr, ntile + 1));
+  GTRY(dgcnt.alloc(ntile));
+  HalfArgs ha{};
+  ha.tptr = p->tptr;
+  ha.tidx = p->tidx;
+  ha.tmask = p->tmask;
+  ha.toff = p->toff32;
+  ha.hptr = p->hptr;
+  ha.hrec = p->hrec;
+  ha.gslot = p->gslot;
+  ha.gcnt = dgcnt.p;
+  ha.gptr = p->gptr;
+  ha.err = derr.p;
+  ha.nblk = nblk;
+  hipLaunchKernelGGL(k_kd_half, gb, dim3(256), 0, st, ha);
+  hipLaunchKernelGGL(k_kd_gather<false>, gb, dim3(256), 0, st, ha);
+  hipLaunchKernelGGL(k_kd_exscan, dim3(1), dim3(1024), 0, st, dgcnt.p, 1,
+                     ntile, p->gptr);
+  hipLaunchKernelGGL(k_kd_gather<true>, gb, dim3(256), 0, st, ha);
+  GTRY(hipGetLastError());
+  // ---- what the host needs: per-block counts, far points, slot order ----
+  bstat.resize(5 * (size_t)nblk);
+  std::vector<unsigned> ms(n_sub);
+  int nslot = 0, gerr = 0;
+  p->perm.resize(npad);
+  GTRY(hipMemcpyAsync(bstat.data(), dstat.p, sizeof(int) * bstat.size(),
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipMemcpyAsync(ms.data(), dmsrc.p, sizeof(unsigned) * n_sub,
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipMemcpyAsync(&nslot, p->gptr + ntile, sizeof(int),
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipMemcpyAsync(&gerr, derr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  GTRY(hipMemcpyAsync(p->perm.data(), p->perm_dev, sizeof(int) * npad,
+                      hipMemcpyDeviceToHost, st));
+  GTRY(hipStreamSynchronize(st));
+  if (nslot < 0 || gerr) {
+    *error = nslot < 0 ? 1 : 2 + gerr;
+    return hipSuccess;
+  }
+  max_src.resize(n_sub);
+  for (int k = 0; k < n_sub; ++k) {
+    float f;
+    std::memcpy(&f, &ms[k], sizeof(float));
+    max_src[k] = f;
+  }
+  p->nslot = nslot;
+  return hipSuccess;
+}
+
+}  // namespace ds

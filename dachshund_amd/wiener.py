@@ -173,6 +173,12 @@ class Context:
         """Wait for the work enqueued on the context's stream."""
         check(lib().ds_ctx_synchronize(self.h), self.h, "synchronize")
 
+    def set_host_geometry(self, on: bool):
+        """Build later problems' ordering and lists with the host threads
+        (the device builder's reference; identical arrays) instead of on
+        the device."""
+        check(lib().ds_ctx_set_host_geometry(self.h, int(bool(on))), self.h)
+
     def close(self):
         if self.h:
             lib().ds_ctx_destroy(self.h)
@@ -233,6 +239,32 @@ class _WFVector:
 
     def set_waves(self, waves: int):
         check(lib().ds_problem_set_waves(self.h, waves), self.ctx.h)
+
+    # ds_problem_get_geometry: array id -> element dtype (and width)
+    GEOMETRY = {
+        "perm": (0, np.int32, 1), "pos": (1, np.float64, 4),
+        "rhs": (2, np.float64, 1), "diag": (3, np.float64, 1),
+        "wvec": (4, np.float64, 1), "loc32": (5, np.float32, 4),
+        "tgt32": (6, np.float32, 4), "tcount": (7, np.int32, 1),
+        "tbox_lo": (8, np.float64, 3), "tbox_hi": (9, np.float64, 3),
+        "tptr": (10, np.int32, 1), "tidx": (11, np.int32, 1),
+        "tmask": (12, np.uint8, 1), "toff32": (13, np.float32, 4),
+        "hptr": (14, np.int32, 1), "hrec": (15, np.float32, 4),
+        "gptr": (16, np.int32, 1), "gslot": (17, np.int32, 1),
+        "blk_order": (18, np.int32, 1), "ord32": (19, np.int32, 1),
+    }
+
+    def geometry(self, name: str) -> np.ndarray:
+        """One of the problem's geometry arrays as the kernels read it."""
+        which, dt, width = self.GEOMETRY[name]
+        nb = C.c_longlong()
+        check(lib().ds_problem_get_geometry(self.h, which, None, 0,
+                                            C.byref(nb)), self.ctx.h)
+        out = np.empty(nb.value // np.dtype(dt).itemsize, dtype=dt)
+        check(lib().ds_problem_get_geometry(
+            self.h, which, out.ctypes.data_as(C.c_void_p), nb.value,
+            C.byref(nb)), self.ctx.h)
+        return out.reshape(-1, width) if width > 1 else out
 
     def _vec(self, a):
         a = _f64(a)

@@ -131,6 +131,13 @@ const char* ds_ctx_last_error(const ds_ctx* ctx);
  * and the second stream that block-Jacobi builds run on). */
 int ds_ctx_synchronize(ds_ctx* ctx);
 int ds_ctx_num_cu(const ds_ctx* ctx);
+/* Where the problems created on ctx get their ordering, culled tile lists,
+ * per-slot arrays and column-sum slots: on the device (0, the default) or
+ * from the host threads (1), the reference implementation of the device
+ * builder, which gives identical arrays (ds_problem_get_geometry).  No
+ * reference counterpart: both replace the all-pairs loops of
+ * lib/wf_vector.cc:54-73. */
+int ds_ctx_set_host_geometry(ds_ctx* ctx, int on);
 
 /* ---- pixel problem: replaces the construction of WFVectorSN / WFVectorW
  *      (lib/wf_vector.h:33-96).  Pixels and covariance are copied to HBM,
@@ -139,6 +146,34 @@ int ds_problem_create(ds_ctx* ctx, int form, int n, const double* pixels,
                       const ds_covar* covar, ds_problem** out);
 int ds_problem_destroy(ds_problem* p);
 int ds_problem_get_stats(const ds_problem* p, ds_problem_stats* out);
+/* A problem's geometry arrays, as the pair and CG kernels read them (no
+ * reference counterpart; for inspection and for comparing the device and
+ * host builders).  Copies array `which` (DS_GEOM_*) to out when cap_bytes
+ * holds it; *bytes receives its size (out may be NULL to ask for it). */
+enum {
+  DS_GEOM_PERM = 0,      /* int per slot: caller index, -1 for padding */
+  DS_GEOM_POS = 1,       /* double4 per slot: x, y, z, w (W) or N (SN) */
+  DS_GEOM_RHS = 2,       /* double per slot */
+  DS_GEOM_DIAG = 3,      /* double per slot: diagonal of A */
+  DS_GEOM_WVEC = 4,      /* double per slot */
+  DS_GEOM_LOC32 = 5,     /* float4 per slot: source, tile frame */
+  DS_GEOM_TGT32 = 6,     /* float4 per slot: target, block frame */
+  DS_GEOM_TCOUNT = 7,    /* int per tile */
+  DS_GEOM_TBOX_LO = 8,   /* 3 doubles per tile */
+  DS_GEOM_TBOX_HI = 9,
+  DS_GEOM_TPTR = 10,     /* int per block + 1: list offsets */
+  DS_GEOM_TIDX = 11,     /* int per entry: source tile */
+  DS_GEOM_TMASK = 12,    /* byte per entry: live 16-row sub-blocks */
+  DS_GEOM_TOFF32 = 13,   /* float4 per entry: frame offset, mask | tile<<4 */
+  DS_GEOM_HPTR = 14,     /* int per block + 1: half-list offsets */
+  DS_GEOM_HREC = 15,     /* float4 per half-list record */
+  DS_GEOM_GPTR = 16,     /* int per tile + 1: column-sum slot offsets */
+  DS_GEOM_GSLOT = 17,    /* int per half-list record (-1: none) */
+  DS_GEOM_BLK_ORDER = 18,/* int per block: FP64 launch order */
+  DS_GEOM_ORD32 = 19     /* int per 2 blocks: FP32 launch lists */
+};
+int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
+                            long long cap_bytes, long long* bytes);
 /* Pair-kernel launch shape: waves (of 64 lanes) per target block, 1, 2 or 4.
  * Changes only the order of floating-point sums. */
 int ds_problem_set_waves(ds_problem* p, int waves_per_block);
