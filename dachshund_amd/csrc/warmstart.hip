@@ -70,25 +70,33 @@ __device__ __forceinline__ float knn_d2(const float4& li, const float4& o,
   return fmaf(dZ, dZ, fmaf(dY, dY, dX * dX));
 }
 
-// Visit every slot of the block's listed source tiles whose entry passes
-// the distance filter: visit(d2, j, ok) with ok = the slot is a candidate
-// neighbour of target `row` (below n, not row itself, d2 inside the cutoff,
-// row itself below n).  The visitor runs for every slot of a visited entry
-// so that it can be written without branches.  The tile data is
-// wave-uniform and read-only here (restrict: scalar loads).
+// Rows per wavefront of the neighbour search: the 16 rows of one tile, each
+// row's candidates split over 4 lane groups (group g takes slots 4 g .. 4 g
+// + 3 of every visited source tile).
+constexpr int KNN_RW = DS_TS;
+constexpr int KNN_G = 64 / KNN_RW;
+
+// Visit every slot of the listed source tiles of the rows' block whose
+// entry passes the distance filter: visit(d2, j, ok) with ok = the slot is
+// a candidate neighbour of target `row` (below n, not row itself, d2 inside
+// the cutoff, row itself below n).  The visitor runs for every slot of a
+// visited entry so that it can be written without branches.
 //
 // Filter: an entry's pairs are all at least lb apart, the gap between the
-// block's box (bmin, bmax: the rows in the block frame, shifted into the
+// rows' box (bmin, bmax: the 16 rows in the block frame, shifted into the
 // tile frame by o = block frame minus tile frame) and the tile's box (its
 // 16 records in the tile frame), in the scaled units where distance^2 =
 // d2, less a margin that covers FP32 rounding; entries with lb in [lo, hi)
 // are visited.  With hi = r, every pair with computed d2 < r^2 lies in a
 // visited entry; with lo = r, none of the visited pairs has d2 < r^2.
+// A visited tile's 16 records are staged in LDS (trec, two buffers) by
+// lanes 0-15 while the previous tile is computed.
 template <class F>
 __device__ __forceinline__ void knn_scan(const KnnArgs& a,
                                          const float4* __restrict__ loc,
                                          const float4* __restrict__ toff,
-                                         int I, int row, const float4& li,
+                                         float4* trec, int I, int row,
+                                         const float4& li,
                                          const float4& bmin,
                                          const float4& bmax, float lo,
                                          float hi, F&& visit) {
@@ -97,6 +105,8 @@ __device__ __forceinline__ void knn_scan(const KnnArgs& a,
   const bool live = row < a.n;
   const bool all = lo == -__builtin_inff() && hi == __builtin_inff();
   const int lane = threadIdx.x & 63;
+  const int g = lane / KNN_RW;
+  int buf = 0;
   // 64 entries at a time: lane k reads entry base + k and applies the
   // filter, so the entries' loads overlap instead of forming a chain; the
   // passing entries then run in ascending order
@@ -126,9 +136,6 @@ __device__ __forceinline__ void knn_scan(const KnnArgs& a,
       }
     }
     unsigned long long m = __ballot(pass);
-    // the passing entries' records come in one per lane (lanes l and
-    // l + 16k hold record l), the next entry's loading while the current
-    // one computes, and are broadcast with readlane
     const int sl = lane & (DS_TS - 1);
     int k = m ? __builtin_ctzll(m) : 0;
     float4 nxt = m ? loc[__builtin_amdgcn_readlane(Tl, k) * DS_TS + sl]
@@ -141,35 +148,23 @@ __device__ __forceinline__ void knn_scan(const KnnArgs& a,
       o.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ol.y), k));
       o.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ol.z), k));
       o.w = 0.f;
-      const float4 cur = nxt;
+      if (lane < DS_TS) trec[buf * DS_TS + lane] = nxt;
       if (m) {
         k = __builtin_ctzll(m);
         nxt = loc[__builtin_amdgcn_readlane(Tl, k) * DS_TS + sl];
       }
-      const int j0 = T * DS_TS;
-      auto rec = [&](int jj) {
-        float4 r;
-        r.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.x), jj));
-        r.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), jj));
-        r.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), jj));
-        r.w = 0.f;
-        return r;
-      };
-      if (j0 + DS_TS <= a.n) {   // no padding slot in this tile
+      __syncthreads();
+      const int j0 = T * DS_TS + KNN_G * g;
+      const float4* rb = trec + buf * DS_TS + KNN_G * g;
+      const bool full = T * DS_TS + DS_TS <= a.n;   // no padding slot
 #pragma unroll
-        for (int jj = 0; jj < DS_TS; ++jj) {
-          const int j = j0 + jj;
-          const float d2 = knn_d2(li, o, rec(jj));
-          visit(d2, j, live && j != row && d2 < a.cutoff);
-        }
-      } else {
-#pragma unroll
-        for (int jj = 0; jj < DS_TS; ++jj) {
-          const int j = j0 + jj;
-          const float d2 = knn_d2(li, o, rec(jj));
-          visit(d2, j, live && j < a.n && j != row && d2 < a.cutoff);
-        }
+      for (int it = 0; it < KNN_G; ++it) {
+        const int j = j0 + it;
+        const float d2 = knn_d2(li, o, rb[it]);
+        visit(d2, j, live && (full || j < a.n) && j != row &&
+                         d2 < a.cutoff);
       }
+      buf ^= 1;
     }
   }
 }
@@ -196,27 +191,31 @@ __device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
 }
 
 // The box of each tile's 16 records in its frame (padding slots included:
-// the box only grows).
+// the box only grows).  One lane per record (coalesced loads), reduced over
+// the tile's 16 lanes.
 __global__ __launch_bounds__(256) void k_tile_boxes(
     const float4* __restrict__ loc, int ntile, float4* tbox) {
-  const int T = blockIdx.x * 256 + threadIdx.x;
-  if (T >= ntile) return;
-  float4 lo = loc[T * DS_TS], hi = lo;
-  for (int jj = 1; jj < DS_TS; ++jj) {
-    const float4 v = loc[T * DS_TS + jj];
-    lo.x = fminf(lo.x, v.x);
-    lo.y = fminf(lo.y, v.y);
-    lo.z = fminf(lo.z, v.z);
-    hi.x = fmaxf(hi.x, v.x);
-    hi.y = fmaxf(hi.y, v.y);
-    hi.z = fmaxf(hi.z, v.z);
+  const int g = blockIdx.x * 256 + threadIdx.x;   // record
+  const int T = g / DS_TS;
+  const float4 v = loc[min(g, ntile * DS_TS - 1)];
+  float lx = v.x, ly = v.y, lz = v.z, hx = v.x, hy = v.y, hz = v.z;
+#pragma unroll
+  for (int o = DS_TS / 2; o >= 1; o >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, o));
+    ly = fminf(ly, __shfl_xor(ly, o));
+    lz = fminf(lz, __shfl_xor(lz, o));
+    hx = fmaxf(hx, __shfl_xor(hx, o));
+    hy = fmaxf(hy, __shfl_xor(hy, o));
+    hz = fmaxf(hz, __shfl_xor(hz, o));
   }
-  tbox[2 * T] = lo;
-  tbox[2 * T + 1] = hi;
+  if ((g & (DS_TS - 1)) == 0 && T < ntile) {
+    tbox[2 * T] = make_float4(lx, ly, lz, v.w);
+    tbox[2 * T + 1] = make_float4(hx, hy, hz, v.w);
+  }
 }
 
 // The K nearest neighbours of each row in (d2, index) order, one wave per
-// target block, one lane per row.  A lane never inserts into a top-K list
+// 16-row tile (4 lanes per row).  A row never inserts into a top-K list
 // while scanning (the whole wave would run the insertion whenever any lane
 // accepts a candidate).  Histogram passes find a threshold instead: level 0
 // bins d2 logarithmically; the bin where the cumulative count reaches K is
@@ -226,25 +225,23 @@ __global__ __launch_bounds__(256) void k_tile_boxes(
 // ties at the threshold included.  An append pass keeps it in LDS as
 // {d2, j}; each kept candidate's rank in (d2, j) order among the row's kept
 // ones is its position in the neighbour list (ranks >= K are dropped).  The
-// first histogram pass and, when every lane settled at level 0, the append
-// pass run without branches (lane-private LDS columns; rejected slots add 0
-// to the histogram, resp. store into a spare row).  More than KNN_CAP
-// survive only when more than about KNN_CAP - K candidates share one FP32
-// distance after three refinements; then the first KNN_CAP in scan order
-// are kept.
+// row's state (bins, crossing path) is computed alike by its 4 lanes.  More
+// than KNN_CAP survive only when more than about KNN_CAP - K candidates
+// share one FP32 distance after three refinements; then the first KNN_CAP
+// appended are kept.
 template <int K>
 __global__ __launch_bounds__(64) void k_knn(
     const float4* __restrict__ loc, const float4* __restrict__ toff,
     KnnArgs a) {
-  // [bin][lane] histograms, then [slot][lane] candidates {d2, j} (the row
-  // KNN_CAP takes the rejected stores of the branch-free append)
-  constexpr int ROWS = KNN_BINS > 2 * (KNN_CAP + 1) ? KNN_BINS
-                                                    : 2 * (KNN_CAP + 1);
-  __shared__ unsigned buf[ROWS * 64];
-  uint2* cand = reinterpret_cast<uint2*>(buf);
+  __shared__ unsigned hist[KNN_BINS * KNN_RW];   // [bin][row]
+  __shared__ uint2 cand[KNN_CAP * KNN_RW];       // [slot][row]
+  __shared__ int ccnt[KNN_RW];
+  __shared__ float4 trec[2 * DS_TS];
   const int lane = threadIdx.x;
-  const int I = blockIdx.x;
-  const int row = I * DS_BS + lane;
+  const int r = lane & (KNN_RW - 1), g = lane / KNN_RW;
+  const int Tt = blockIdx.x;              // the rows' tile
+  const int I = Tt / DS_TPB;              // and its block
+  const int row = Tt * DS_TS + r;
   const float4 li = a.tgt[row];
 
   float lo[KNN_LEVELS], sc[KNN_LEVELS];
@@ -274,12 +271,14 @@ __global__ __launch_bounds__(64) void k_knn(
     }
     return c;
   };
+  auto hbin = [&](int b) -> unsigned& { return hist[b * KNN_RW + r]; };
 
   const float kInf = __builtin_inff();
-  // the block's box (its rows in the block frame, scaled units)
+  // the rows' box (block frame, scaled units); the 4 lane groups hold the
+  // same 16 rows
   float4 bmin = li, bmax = li;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
+  for (int off = 1; off < KNN_RW; off <<= 1) {
     bmin.x = fminf(bmin.x, __shfl_xor(bmin.x, off, 64));
     bmin.y = fminf(bmin.y, __shfl_xor(bmin.y, off, 64));
     bmin.z = fminf(bmin.z, __shfl_xor(bmin.z, off, 64));
@@ -290,33 +289,36 @@ __global__ __launch_bounds__(64) void k_knn(
 
   for (int p = 0; p < KNN_LEVELS; ++p) {
     if (!__any(!settled)) break;
-    for (int b = 0; b < KNN_BINS; ++b) buf[b * 64 + lane] = 0u;
+    for (int b = g; b < KNN_BINS; b += KNN_G) hbin(b) = 0u;
+    __syncthreads();
     if (p == 0) {
       // Level 0 in two parts: first the entries that can hold d2 < R0^2;
-      // a lane that already finds K candidates there has its crossing bin
+      // a row that already finds K candidates there has its crossing bin
       // below kNearBins, where the counts are complete.  The other entries
-      // are scanned only when some lane did not.
+      // are scanned only when some row did not.
       constexpr float R0 = 8.0f;        // d2 < 64: bins below kNearBins
       constexpr int kNearBins = 48;
       auto add = [&](float d2, int, bool ok) {
-        atomicAdd(&buf[knn_bin0(d2) * 64 + lane], ok ? 1u : 0u);
+        atomicAdd(&hbin(knn_bin0(d2)), ok ? 1u : 0u);
       };
-      knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, R0, add);
+      knn_scan(a, loc, toff, trec, I, row, li, bmin, bmax, -kInf, R0, add);
+      __syncthreads();
       int near = 0;
-      for (int b = 0; b < kNearBins; ++b) near += (int)buf[b * 64 + lane];
+      for (int b = 0; b < kNearBins; ++b) near += (int)hbin(b);
       if (__any(!settled && near < K))
-        knn_scan(a, loc, toff, I, row, li, bmin, bmax, R0, kInf, add);
+        knn_scan(a, loc, toff, trec, I, row, li, bmin, bmax, R0, kInf, add);
     } else {
-      knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, kInf,
+      knn_scan(a, loc, toff, trec, I, row, li, bmin, bmax, -kInf, kInf,
                [&](float d2, int, bool ok) {
                  if (ok && !settled && classify(d2, p) == 0)
-                   atomicAdd(&buf[bin_at(d2, p) * 64 + lane], 1u);
+                   atomicAdd(&hbin(bin_at(d2, p)), 1u);
                });
     }
+    __syncthreads();
     if (settled) continue;
     int cum = below, cross = -1, before = below;
     for (int b = 0; b < KNN_BINS; ++b) {
-      const int h = (int)buf[b * 64 + lane];
+      const int h = (int)hbin(b);
       if (cross < 0 && cum + h >= K) {
         cross = b;
         before = cum;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(64) void k_knn(
     }
     cb[p] = cross;
     nlev = p + 1;
-    const int take = before + (int)buf[cross * 64 + lane];
+    const int take = before + (int)hbin(cross);
     if (take <= KNN_CAP || p + 1 == KNN_LEVELS) {
       settled = true;
       continue;
@@ -351,7 +353,13 @@ __global__ __launch_bounds__(64) void k_knn(
 
   // append: below the crossing path, or inside the final crossing bin (or
   // the whole final domain when it holds fewer than K)
-  int cnt = 0;
+  if (lane < KNN_RW) ccnt[lane] = 0;
+  __syncthreads();
+  auto append = [&](float d2, int j) {
+    const int s = atomicAdd(&ccnt[r], 1);
+    if (s < KNN_CAP)
+      cand[s * KNN_RW + r] = make_uint2(__float_as_uint(d2), (unsigned)j);
+  };
   if (__all(nlev <= 1)) {
     const int tb = nlev == 0 ? KNN_BINS - 1 : cb[0];
     // kept candidates have d2 below the upper edge of bin tb
@@ -360,53 +368,34 @@ __global__ __launch_bounds__(64) void k_knn(
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1)
       rk = fmaxf(rk, __shfl_xor(rk, off, 64));
-    knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, rk,
+    knn_scan(a, loc, toff, trec, I, row, li, bmin, bmax, -kInf, rk,
              [&](float d2, int j, bool ok) {
-      const bool keep = ok && knn_bin0(d2) <= tb && cnt < KNN_CAP;
-      cand[(keep ? cnt : KNN_CAP) * 64 + lane] =
-          make_uint2(__float_as_uint(d2), (unsigned)j);
-      cnt += keep ? 1 : 0;
+      if (ok && knn_bin0(d2) <= tb) append(d2, j);
     });
   } else {
-    knn_scan(a, loc, toff, I, row, li, bmin, bmax, -kInf, kInf,
+    knn_scan(a, loc, toff, trec, I, row, li, bmin, bmax, -kInf, kInf,
              [&](float d2, int j, bool ok) {
-      if (ok && classify(d2, nlev) <= 0 && cnt < KNN_CAP) {
-        cand[cnt * 64 + lane] = make_uint2(__float_as_uint(d2),
-                                           (unsigned)j);
-        ++cnt;
-      }
+      if (ok && classify(d2, nlev) <= 0) append(d2, j);
     });
   }
+  __syncthreads();
 
-  // rank of each kept candidate among the row's kept ones, four at a time
-  int m = cnt;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) m = max(m, __shfl_xor(m, off, 64));
+  // rank of each kept candidate among the row's kept ones; the row's 4
+  // lanes take candidates g, g + 4, ...
+  const int cnt = min(ccnt[r], KNN_CAP);
   int* o = a.nbr + (size_t)row * K;
-  for (int s0 = 0; s0 < m; s0 += 4) {
-    float ds[4];
-    int js[4], rk[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint2 c = cand[min(s0 + u, KNN_CAP) * 64 + lane];
-      ds[u] = __uint_as_float(c.x);
-      js[u] = (int)c.y;
-      rk[u] = 0;
+  for (int u = g; u < cnt; u += KNN_G) {
+    const uint2 cu = cand[u * KNN_RW + r];
+    const float du = __uint_as_float(cu.x);
+    const int ju = (int)cu.y;
+    int rk = 0;
+    for (int t = 0; t < cnt; ++t) {
+      const uint2 c = cand[t * KNN_RW + r];
+      rk += knn_less(__uint_as_float(c.x), (int)c.y, du, ju) ? 1 : 0;
     }
-    for (int t = 0; t < m; ++t) {
-      const uint2 c = cand[t * 64 + lane];
-      const float dt = __uint_as_float(c.x);
-      const int jt = (int)c.y;
-      const bool in = t < cnt;
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        rk[u] += (in && knn_less(dt, jt, ds[u], js[u])) ? 1 : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (s0 + u < cnt && rk[u] < K) o[rk[u]] = js[u];
+    if (rk < K) o[rk] = ju;
   }
-  for (int k = cnt; k < K; ++k) o[k] = -1;
+  for (int k = cnt + g; k < K; k += KNN_G) o[k] = -1;
 }
 
 // The WS_FEAT features of one row from its neighbour list and the current
@@ -1026,9 +1015,10 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     a.nbr = nbr;
     float4* tbox = reinterpret_cast<float4*>(reinterpret_cast<char*>(nbr) + nb);
     a.tbox = tbox;
-    hipLaunchKernelGGL(k_tile_boxes, dim3((p->ntile + 255) / 256), dim3(256),
-                       0, st, p->loc32, p->ntile, tbox);
-    hipLaunchKernelGGL(k_knn<WS_K>, dim3(p->nblk), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_tile_boxes,
+                       dim3((unsigned)(((size_t)p->ntile * DS_TS + 255) / 256)),
+                       dim3(256), 0, st, p->loc32, p->ntile, tbox);
+    hipLaunchKernelGGL(k_knn<WS_K>, dim3(p->ntile), dim3(64), 0, st,
                        p->loc32, p->toff32, a);
     e = hipGetLastError();
     if (e != hipSuccess) {
