@@ -16,14 +16,17 @@
 // (columns 32..63), built as A_ij = w_i S_ij w_j + delta_ij (W form) or
 // S_ij + N_i delta_ij (SN form) with the FP64 table interpolant of
 // lib/signal_covar.h:53-77 (padding rows: identity).  The block is inverted
-// by the in-place Gauss-Jordan sweep (no pivoting: the block is SPD).  Step
-// k: the two threads of row k publish it in LDS (entry k replaced by 1, the
-// pivot kept aside), and every row i != k is updated with the factor a_ik,
-// which the sweep's structure gives from the published row (a_ik = a_ki
-// while both are unswept, a_ik = -a_ki once i < k is swept), so no thread
-// indexes its registers by k; row k becomes the scaled pivot row.  The
-// inverse goes through LDS once as FP32 to be symmetrised, rounded to FP32
-// and stored by diagonals (bj_apply's layout):
+// by the in-place Gauss-Jordan sweep (no pivoting: the block is SPD), with
+// each swept row kept unscaled: row i holds its true values divided by its
+// pivot's reciprocal p_i (times 1 while unswept) and is scaled once at the
+// end.  Step k: the two threads of row k publish it in LDS (its true values:
+// it is unswept) and every row applies m_c -= g raw_c with one factor
+// g = a_ik d_i p_k (d_i = 1 / p_i, or 1), a_ik taken from the published row
+// by the sweep's structure (a_ik = a_ki while both are unswept, -a_ki once
+// i < k is swept) so that no thread indexes its registers by k; column k
+// becomes -g, row k keeps its values (g = 0) with 1 in column k.  One FMA
+// per entry and step.  The inverse goes through LDS once as FP32 to be
+// symmetrised, rounded to FP32 and stored by diagonals (bj_apply's layout):
 // minv[I][k][i] = M_{i,(i+k) mod 64}, k = 0..32.
 // COMPILE WITH -ffp-contract=off.
 #include "cg_device.h"
@@ -46,10 +49,11 @@ struct BjArgs {
 
 __global__ __launch_bounds__(128) void k_bj_build(BjArgs a) {
   constexpr int HC = DS_BS / 2;        // columns per thread
-  __shared__ double raw[DS_BS + 1];    // step k: row k (entry k = 1), pivot
+  __shared__ double raw[2][DS_BS];     // step k (buffer k & 1): row k
   __shared__ float mt[DS_BS][DS_BS + 1];
   const int I = blockIdx.x, t = threadIdx.x;
-  const int i = t & (DS_BS - 1), h = t >> 6, c0 = h * HC;
+  const int i = t & (DS_BS - 1);
+  const int h = __builtin_amdgcn_readfirstlane(t >> 6), c0 = h * HC;
   const double4 cv = a.sub_cv[a.blk_sub[I]];
   const int nreal = a.blk_nreal[I];
   const double4 pi = a.pos[I * DS_BS + i];
@@ -75,32 +79,34 @@ __global__ __launch_bounds__(128) void k_bj_build(BjArgs a) {
     }
     m[c] = v;
   }
+  double d_own = 1.0, p_own = 1.0;   // this row's pivot and its reciprocal
   for (int k = 0; k < DS_BS; ++k) {
+    double* rw = raw[k & 1];
     if (i == k) {
 #pragma unroll
-      for (int c = 0; c < HC; ++c) raw[c0 + c] = m[c];
-      if ((k >> 5) == h) {   // this thread holds the pivot
-        raw[DS_BS] = raw[k];
-        raw[k] = 1.0;
-      }
+      for (int c = 0; c < HC; ++c) rw[c0 + c] = m[c];
     }
     __syncthreads();
-    const double p = 1.0 / raw[DS_BS];
-    const double f = i < k ? -raw[i] : raw[i];
-#pragma unroll
-    for (int c = 0; c < HC; ++c) {
-      const double pr = raw[c0 + c] * p;
-      if (c0 + c == k) m[c] = -f * pr;   // wave-uniform
-      else m[c] = fma(-f, pr, m[c]);
-    }
+    const double piv = rw[k];
+    const double p = 1.0 / piv;
+    const double fr = rw[i];
+    const double g = i == k ? 0.0 : ((i < k ? -fr : fr) * d_own) * p;
     if (i == k) {
-#pragma unroll
-      for (int c = 0; c < HC; ++c) m[c] = raw[c0 + c] * p;
+      d_own = piv;
+      p_own = p;
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < HC; ++c) m[c] = fma(-g, rw[c0 + c], m[c]);
+    const int kc = k - c0;   // wave-uniform: column k in this half?
+    if (kc >= 0 && kc < HC) {
+      const double mk = i == k ? 1.0 : -g;
+#pragma unroll
+      for (int c = 0; c < HC; ++c)
+        if (c == kc) m[c] = mk;
+    }
   }
 #pragma unroll
-  for (int c = 0; c < HC; ++c) mt[i][c0 + c] = (float)m[c];
+  for (int c = 0; c < HC; ++c) mt[i][c0 + c] = (float)(m[c] * p_own);
   __syncthreads();
   float* out = a.minv + (size_t)I * DS_BJ_SIZE;
   for (int k = h; k < DS_BJ_ROWS; k += 2) {
