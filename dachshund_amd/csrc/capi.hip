@@ -923,6 +923,19 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     for (auto& x : th) x.join();
   }
   p->norm_b = p->sub_norm_b[0];
+  // The neighbour search's first ring radius (square root of table-argument
+  // units): the ball that holds about 64 points at subvolume 0's mean
+  // density over its bounding box (a flat axis counts as one unit wide).
+  {
+    const double sd = std::sqrt(p->cv.dx_inv);
+    double vol = 1.0;
+    for (int ax = 0; ax < 3; ++ax)
+      vol *= std::max(1.0, quant[0].w[ax] * 4294967295.0 * sd);
+    const double rho = sub_n[0] / vol;
+    const double r0 = std::cbrt(64.0 / (rho * (4.0 / 3.0) * M_PI));
+    p->knn_r0 = (float)std::min(std::max(r0, 1.0),
+                                std::sqrt((double)(p->cv.table_n - 1)));
+  }
   // do all subvolumes share one geometry (coordinates and weights)?
   p->shared_geometry = true;
   for (int k = 1; k < n_sub && p->shared_geometry; ++k) {
@@ -1192,6 +1205,13 @@ int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
     case DS_GEOM_GSLOT: src = p->gslot; nb = 4 * (size_t)tot[1]; break;
     case DS_GEOM_BLK_ORDER: src = p->blk_order; nb = 4 * nblk; break;
     case DS_GEOM_ORD32: src = p->ord32; nb = 8 * nblk; break;
+    case DS_GEOM_KNN:
+      if (!p->knn_nbr)
+        return fail(p->ctx, DS_E_STATE, "no neighbour lists yet (they are "
+                    "built by the first warm-start prediction)");
+      src = p->knn_nbr;
+      nb = 4 * 32 * npad;
+      break;
     default:
       return fail(p->ctx, DS_E_ARG, "unknown geometry array %d", which);
   }

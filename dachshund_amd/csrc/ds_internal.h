@@ -220,6 +220,7 @@ struct ds_problem {
   mutable bool minv_pending = false;
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
+  float knn_r0 = 1.0f;                 // its first ring radius
   int* sub_blk0 = nullptr;             // first block per subvolume
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}
   int* tcount = nullptr;               // real slots per tile

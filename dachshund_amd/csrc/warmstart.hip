@@ -16,8 +16,9 @@
 // (layout [in][out]), activations rounded to BF16 (optional), output
 // t = x_i sqrt(var_s).
 //
-// Kernels: k_knn (neighbour search over the problem's culled tile lists,
-// once per problem), k_ws_features, k_fwd (inference), k_fwd_bwd (training forward + per-pixel
+// Kernels: k_knn_ring and k_knn (neighbour search over the problem's
+// culled tile lists, once per problem: the ring search, then the general
+// search on the tiles it leaves), k_ws_features, k_fwd (inference), k_fwd_bwd (training forward + per-pixel
 // back-propagated deltas), k_wgrad (deterministic chunked weight-gradient
 // sums), k_reduce_chunks, k_adam, k_gather_pred.
 #include <cmath>
@@ -52,8 +53,13 @@ struct KnnArgs {
   int* nbr;               // npad x K neighbour rows, -1 = none
   const float4* tbox;     // per tile: min, max of its records (tile frame)
   unsigned short* feat;   // npad x WS_FEAT
+  // tiles the ring search could not finish: fb[0] = count, fb[1..] = tiles
+  // (k_knn takes tile fb[1 + blockIdx.x] when fb is set)
+  int* fb;
   int n, form;
   float inv_lp, inv_ll, inv_sqrt_vs, cutoff;
+  float r0;               // first ring radius (sqrt of d2 units)
+  int ntile;
 };
 
 constexpr int KNN_BINS = 64;    // histogram bins per refinement level
@@ -193,9 +199,11 @@ __device__ __forceinline__ bool knn_less(float d, int j, float e, int k) {
 // The box of each tile's 16 records in its frame (padding slots included:
 // the box only grows).  One lane per record (coalesced loads), reduced over
 // the tile's 16 lanes.
+// (Also clears the ring search's unfinished-tile count.)
 __global__ __launch_bounds__(256) void k_tile_boxes(
-    const float4* __restrict__ loc, int ntile, float4* tbox) {
+    const float4* __restrict__ loc, int ntile, float4* tbox, int* fb) {
   const int g = blockIdx.x * 256 + threadIdx.x;   // record
+  if (g == 0) fb[0] = 0;
   const int T = g / DS_TS;
   const float4 v = loc[min(g, ntile * DS_TS - 1)];
   float lx = v.x, ly = v.y, lz = v.z, hx = v.x, hy = v.y, hz = v.z;
@@ -239,7 +247,12 @@ __global__ __launch_bounds__(64) void k_knn(
   __shared__ float4 trec[2 * DS_TS];
   const int lane = threadIdx.x;
   const int r = lane & (KNN_RW - 1), g = lane / KNN_RW;
-  const int Tt = blockIdx.x;              // the rows' tile
+  // the rows' tile (with a.fb: the ring search's unfinished tiles only)
+  int Tt = blockIdx.x;
+  if (a.fb) {
+    if ((int)blockIdx.x >= a.fb[0]) return;
+    Tt = a.fb[1 + blockIdx.x];
+  }
   const int I = Tt / DS_TPB;              // and its block
   const int row = Tt * DS_TS + r;
   const float4 li = a.tgt[row];
@@ -396,6 +409,327 @@ __global__ __launch_bounds__(64) void k_knn(
     if (rk < K) o[rk] = ju;
   }
   for (int k = cnt + g; k < K; k += KNN_G) o[k] = -1;
+}
+
+// ---- ring search: the common case in one pass over the near entries ----
+//
+// The same K nearest neighbours in (d2, index) order as k_knn, found by
+// collecting every candidate of a row inside a radius and selecting from
+// LDS.  Ring m covers d2 in [R_{m-1}^2, R_m^2): R_0 = a.r0 (the radius that
+// holds about 64 points at the problem's mean density), R_{m+1} = 1.26 R_m
+// (twice the volume), the last ring ending at the cutoff.  A row takes part
+// in ring m only while it has fewer than K candidates, so a row's
+// candidates are all its points with d2 below the end of its last ring:
+// every point nearer than its K-th (or all of its neighbours when it has
+// fewer than K).  A ring visits the entries whose box gap is below its
+// outer radius (knn_scan's filter), 4 entries per step: lane (s, g) loads
+// slot s of entry 4 m + g (the next two steps' records in flight), the 64
+// records go through LDS, and lane (r, g) computes row r against the 16
+// slots of entry g.  Candidates are appended per row (positions from the
+// lanes' accept masks); a tile with a row beyond KNN_NCAP candidates is
+// left to k_knn (fb).
+// Selection: a KNN_NBIN-bin histogram of the row's d2 finds the bin where
+// the count reaches K, the candidates up to that bin are compacted to the
+// front of the row's list (16 per round, so no slot is overwritten before
+// it is read), and each kept candidate's rank in (d2, j) order among the
+// kept ones is its position.
+constexpr int KNN_NCAP = 128;
+constexpr int KNN_NBIN = 32;
+constexpr float KNN_RING = 1.26f;
+
+// A candidate as one 64-bit key: d2's bits (d2 >= 0, so they order like
+// the values) above the index, so (d2, j) order is unsigned key order.
+__device__ __forceinline__ unsigned long long knn_key(float d2, int j) {
+  return ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned)j;
+}
+
+struct KnnRingSmem {
+  unsigned long long cand[KNN_NCAP * KNN_RW];   // [slot][row]: knn_key
+  int cross[KNN_RW];
+  union {
+    struct {
+      float4 rec[2][4 * 17];       // a step's 4 entries x 16 records
+                                   // (stride 17: the 4 entries' records
+                                   // start in different LDS banks)
+      float4 po[64];               // passing entries: frame offset
+      int pT[64];                  // and tile
+    } s;
+    unsigned hist[KNN_NBIN * KNN_RW];   // [bin][row]
+  } u;
+};
+
+// One ring: append the candidates of rows with `want` whose d2 is in
+// [lo2, hi2), from the block's entries whose box gap is below `hi`.  A
+// row's count lives in `cnt` (the same in its 4 lanes): each step's
+// accepted slots form a 16-bit mask per lane, and the positions come from
+// the masks' counts, summed over the row's 4 lanes (one per 16-lane row of
+// the wave) with two permlane swaps, so no LDS atomic is waited for.  The
+// next chunk's entry indices and offsets are loaded during this chunk's
+// steps, and its tile boxes at this chunk's end.
+template <bool FIRST>
+__device__ __forceinline__ void knn_ring(
+    const KnnArgs& a, const float4* __restrict__ loc,
+    const float4* __restrict__ toff, KnnRingSmem& sm, int I, int row,
+    const float4& li, const float4& bmin, const float4& bmax, float hi,
+    float lo2, float hi2, bool want, int& cnt) {
+  const int lane = threadIdx.x;
+  const int r = lane & (KNN_RW - 1), g = lane / KNN_RW;
+  const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]);
+  const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
+  // chunk metadata, one entry per lane, one chunk ahead
+  int Tn = 0;
+  float4 on = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 bn0 = on, bn1 = on;
+  if (kb + lane < ke) {
+    Tn = a.tidx[kb + lane];
+    on = toff[kb + lane];
+    bn0 = a.tbox[2 * Tn];
+    bn1 = a.tbox[2 * Tn + 1];
+  }
+  for (int base = kb; base < ke; base += 64) {
+    const bool inb = base + lane < ke;
+    const int Tl = Tn;
+    float4 ol = on;
+    ol.w = 0.f;
+    const float4 tmin = bn0, tmax = bn1;
+    const bool more = base + 64 + lane < ke;
+    if (more) {
+      Tn = a.tidx[base + 64 + lane];
+      on = toff[base + 64 + lane];
+    }
+    bool pass = false;
+    if (inb) {
+      const float gx = fmaxf(0.f, fmaxf(tmin.x - (bmax.x + ol.x),
+                                        (bmin.x + ol.x) - tmax.x));
+      const float gy = fmaxf(0.f, fmaxf(tmin.y - (bmax.y + ol.y),
+                                        (bmin.y + ol.y) - tmax.y));
+      const float gz = fmaxf(0.f, fmaxf(tmin.z - (bmax.z + ol.z),
+                                        (bmin.z + ol.z) - tmax.z));
+      const float scale = fabsf(ol.x) + fabsf(ol.y) + fabsf(ol.z) +
+                          (bmax.x - bmin.x) + (bmax.y - bmin.y) +
+                          (bmax.z - bmin.z);
+      const float lb = sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx))) -
+                       (1e-3f + 1e-5f * scale);
+      pass = lb < hi;
+    }
+    const unsigned long long m = __ballot(pass);
+    const int np = __popcll(m);
+    if (np > 0) {
+      const int pr = __builtin_amdgcn_mbcnt_hi(
+          (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      if (pass) {
+        sm.u.s.pT[pr] = Tl;
+        sm.u.s.po[pr] = ol;
+      }
+      __syncthreads();
+      const int steps = (np + 3) >> 2;
+      const int slot = lane & (DS_TS - 1);
+      auto load = [&](int b) {
+        const int e = min(4 * b + g, np - 1);
+        return loc[sm.u.s.pT[e] * DS_TS + slot];
+      };
+      float4 n0 = load(0);
+      float4 n1 = steps > 1 ? load(1) : n0;
+      for (int b = 0; b < steps; ++b) {
+        const float4 cur = n0;
+        n0 = n1;
+        if (b + 2 < steps) n1 = load(b + 2);
+        sm.u.s.rec[b & 1][17 * g + slot] = cur;
+        __syncthreads();
+        const int e = 4 * b + g;
+        const int ec = min(e, np - 1);
+        const int T = sm.u.s.pT[ec];
+        const float4 o = sm.u.s.po[ec];
+        const int j0 = T * DS_TS;
+        const bool on_e = want && e < np;
+        const bool full = j0 + DS_TS <= a.n;   // no padding slot
+        const float4* rb = sm.u.s.rec[b & 1] + 17 * g;
+        // the entry's slots that can hold a candidate of this row: below
+        // n, not the row itself
+        unsigned valid = full ? 0xFFFFu : (a.n > j0 ? (1u << (a.n - j0)) - 1u
+                                                    : 0u);
+        const unsigned self = (unsigned)(row - j0);
+        if (self < (unsigned)DS_TS) valid &= ~(1u << self);
+        if (!on_e) valid = 0u;
+        float d2v[DS_TS];
+        unsigned acc = 0u;
+#pragma unroll
+        for (int t = 0; t < DS_TS; ++t) {
+          const float d2 = knn_d2(li, o, rb[t]);
+          d2v[t] = d2;
+          const bool in = FIRST ? d2 < hi2 : (d2 >= lo2 && d2 < hi2);
+          acc |= in ? 1u << t : 0u;
+        }
+        acc &= valid;
+        // positions: this lane's accepted slots follow those of the row's
+        // lanes in lower 16-lane rows (g), after the row's count so far
+        const unsigned c = (unsigned)__popc(acc);
+        const auto p16 = __builtin_amdgcn_permlane16_swap(c, c, false, false);
+        const unsigned x1 = (g & 1) ? p16[0] : p16[1];
+        const unsigned s2 = p16[0] + p16[1];
+        const auto p32 =
+            __builtin_amdgcn_permlane32_swap(s2, s2, false, false);
+        const unsigned x2 = (g & 2) ? p32[0] : p32[1];
+        int pos = cnt + (int)((g & 1) ? x1 : 0u) + (int)((g & 2) ? x2 : 0u);
+        cnt += (int)(s2 + x2);
+        if (acc) {
+#pragma unroll
+          for (int t = 0; t < DS_TS; ++t) {
+            if (acc & (1u << t)) {
+              if (pos < KNN_NCAP)
+                sm.cand[pos * KNN_RW + r] =
+                    knn_key(d2v[t], j0 + t);
+              ++pos;
+            }
+          }
+        }
+      }
+    }
+    // the next chunk's tile boxes (its indices have arrived by now)
+    if (more) {
+      bn0 = a.tbox[2 * Tn];
+      bn1 = a.tbox[2 * Tn + 1];
+    }
+    __syncthreads();
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_knn_ring(
+    const float4* __restrict__ loc, const float4* __restrict__ toff,
+    KnnArgs a) {
+  __shared__ KnnRingSmem sm;
+  const int lane = threadIdx.x;
+  const int r = lane & (KNN_RW - 1), g = lane / KNN_RW;
+  // Workgroups go to the 8 XCDs round-robin by index: XCD x = i mod 8
+  // takes the contiguous tile range x * span .. (x + 1) * span - 1, so
+  // spatially adjacent tiles (which read the same source records) share an
+  // L2.  The grid is 8 * span workgroups.
+  const int span = (a.ntile + 7) / 8;
+  const int Tt = (int)(blockIdx.x % 8u) * span + (int)(blockIdx.x / 8u);
+  if (Tt >= a.ntile) return;
+  const int I = Tt / DS_TPB;
+  const int row = Tt * DS_TS + r;
+  const bool live = row < a.n;
+  const float4 li = a.tgt[row];
+  float4 bmin = li, bmax = li;
+#pragma unroll
+  for (int off = 1; off < KNN_RW; off <<= 1) {
+    bmin.x = fminf(bmin.x, __shfl_xor(bmin.x, off, 64));
+    bmin.y = fminf(bmin.y, __shfl_xor(bmin.y, off, 64));
+    bmin.z = fminf(bmin.z, __shfl_xor(bmin.z, off, 64));
+    bmax.x = fmaxf(bmax.x, __shfl_xor(bmax.x, off, 64));
+    bmax.y = fmaxf(bmax.y, __shfl_xor(bmax.y, off, 64));
+    bmax.z = fmaxf(bmax.z, __shfl_xor(bmax.z, off, 64));
+  }
+  int cnt = 0;   // the row's candidates so far (same in its 4 lanes)
+  const float rmax2 = a.cutoff;
+  float rlo = 0.0f, rhi = a.r0;
+  float row_hi2 = 0.0f;   // end of the last ring this row took part in
+  bool want = live;
+  for (;;) {
+    const bool last = rhi * rhi >= rmax2;
+    const float hi2 = last ? rmax2 : rhi * rhi;
+    if (rlo == 0.0f)
+      knn_ring<true>(a, loc, toff, sm, I, row, li, bmin, bmax,
+                     last ? __builtin_inff() : rhi, 0.0f, hi2, want, cnt);
+    else
+      knn_ring<false>(a, loc, toff, sm, I, row, li, bmin, bmax,
+                      last ? __builtin_inff() : rhi, rlo * rlo, hi2, want,
+                      cnt);
+    if (want) row_hi2 = hi2;
+    const int c = cnt;
+    if (__any(c > KNN_NCAP)) {
+      if (lane == 0) {
+        const int k = atomicAdd(&a.fb[0], 1);
+        a.fb[1 + k] = Tt;
+      }
+      return;
+    }
+    want = live && c < K;
+    if (last || !__any(want)) break;
+    rlo = rhi;
+    rhi *= KNN_RING;
+  }
+
+  // selection: the bin where the row's count reaches K
+  const float inv_w = row_hi2 > 0.0f ? (float)KNN_NBIN / row_hi2 : 0.0f;
+  auto bin_of = [&](unsigned long long key) {
+    return min((int)(__uint_as_float((unsigned)(key >> 32)) * inv_w),
+               KNN_NBIN - 1);
+  };
+  for (int b = g; b < KNN_NBIN; b += KNN_G) sm.u.hist[b * KNN_RW + r] = 0u;
+  __syncthreads();
+  if (cnt > K)
+    for (int u = g; u < cnt; u += KNN_G)
+      atomicAdd(&sm.u.hist[bin_of(sm.cand[u * KNN_RW + r]) * KNN_RW + r],
+                1u);
+  __syncthreads();
+  constexpr int BPL = KNN_NBIN / KNN_G;   // bins per lane
+  int hs = 0;
+#pragma unroll
+  for (int b = 0; b < BPL; ++b) hs += (int)sm.u.hist[(BPL * g + b) * KNN_RW + r];
+  const int h0 = __shfl(hs, r, 64), h1 = __shfl(hs, r + 16, 64);
+  const int h2 = __shfl(hs, r + 32, 64);
+  int cum = (g > 0 ? h0 : 0) + (g > 1 ? h1 : 0) + (g > 2 ? h2 : 0);
+  if (cnt > K) {
+#pragma unroll
+    for (int b = 0; b < BPL; ++b) {
+      const int h = (int)sm.u.hist[(BPL * g + b) * KNN_RW + r];
+      if (cum < K && cum + h >= K) sm.cross[r] = BPL * g + b;
+      cum += h;
+    }
+  } else if (g == 0) {
+    sm.cross[r] = KNN_NBIN - 1;   // fewer than K + 1: keep them all
+  }
+  __syncthreads();
+  const int cross = sm.cross[r];
+  // compact the kept candidates (bin <= cross) to the front, 16 per round
+  int rounds = (cnt + 15) >> 4;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+    rounds = max(rounds, __shfl_xor(rounds, off, 64));
+  int kept = 0;
+  for (int q = 0; q < rounds; ++q) {
+    unsigned long long c4[4];
+    bool kp[4];
+    int kc = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int u = 16 * q + 4 * g + i;
+      kp[i] = u < cnt;
+      c4[i] = kp[i] ? sm.cand[u * KNN_RW + r] : 0ull;
+      kp[i] = kp[i] && bin_of(c4[i]) <= cross;
+      kc += kp[i] ? 1 : 0;
+    }
+    const int k0 = __shfl(kc, r, 64), k1 = __shfl(kc, r + 16, 64);
+    const int k2 = __shfl(kc, r + 32, 64), k3 = __shfl(kc, r + 48, 64);
+    int pos = kept + (g > 0 ? k0 : 0) + (g > 1 ? k1 : 0) + (g > 2 ? k2 : 0);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (kp[i]) sm.cand[(pos++) * KNN_RW + r] = c4[i];
+    kept += k0 + k1 + k2 + k3;
+    __syncthreads();
+  }
+  // rank of each kept candidate among the kept ones (the others read
+  // eight at a time, so their LDS latencies overlap)
+  int* o = a.nbr + (size_t)row * K;
+  for (int u = g; u < kept; u += KNN_G) {
+    const unsigned long long cu = sm.cand[u * KNN_RW + r];
+    int rk = 0;
+    int t0 = 0;
+    for (; t0 + 8 <= kept; t0 += 8) {
+      unsigned long long c[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) c[i] = sm.cand[(t0 + i) * KNN_RW + r];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) rk += c[i] < cu ? 1 : 0;
+    }
+    for (; t0 < kept; ++t0) rk += sm.cand[t0 * KNN_RW + r] < cu ? 1 : 0;
+    if (rk < K) o[rk] = (int)(unsigned)cu;
+  }
+  for (int k = kept + g; k < K; k += KNN_G) o[k] = -1;
 }
 
 // The WS_FEAT features of one row from its neighbour list and the current
@@ -1007,19 +1341,28 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     // problem, like its tile lists.  Candidates are kept as list entry * 16
     // + tile slot in 32 bits.
     if (p->ntile_pairs >= (1LL << 28)) return hipErrorInvalidValue;
-    // one allocation: the lists, then the tiles' boxes (kept with them)
+    // one allocation: the lists, the tiles' boxes, then the unfinished-tile
+    // list of the ring search (count + tiles)
     int* nbr = nullptr;
     const size_t nb = sizeof(int) * (size_t)p->npad * WS_K;
-    hipError_t e = hipMalloc(&nbr, nb + 2 * sizeof(float4) * p->ntile);
+    const size_t nbox = 2 * sizeof(float4) * p->ntile;
+    hipError_t e = hipMalloc(&nbr, nb + nbox + sizeof(int) * (p->ntile + 1));
     if (e != hipSuccess) return e;
     a.nbr = nbr;
     float4* tbox = reinterpret_cast<float4*>(reinterpret_cast<char*>(nbr) + nb);
     a.tbox = tbox;
+    a.fb = reinterpret_cast<int*>(reinterpret_cast<char*>(nbr) + nb + nbox);
+    a.r0 = p->knn_r0;
     hipLaunchKernelGGL(k_tile_boxes,
                        dim3((unsigned)(((size_t)p->ntile * DS_TS + 255) / 256)),
-                       dim3(256), 0, st, p->loc32, p->ntile, tbox);
+                       dim3(256), 0, st, p->loc32, p->ntile, tbox, a.fb);
+    // the ring search, then the general search on the tiles it left
+    a.ntile = p->ntile;
+    hipLaunchKernelGGL(k_knn_ring<WS_K>, dim3(8 * ((p->ntile + 7) / 8)),
+                       dim3(64), 0, st, p->loc32, p->toff32, a);
     hipLaunchKernelGGL(k_knn<WS_K>, dim3(p->ntile), dim3(64), 0, st,
                        p->loc32, p->toff32, a);
+    a.fb = nullptr;
     e = hipGetLastError();
     if (e != hipSuccess) {
       (void)hipFree(nbr);

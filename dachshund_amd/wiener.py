@@ -252,6 +252,7 @@ class _WFVector:
         "hptr": (14, np.int32, 1), "hrec": (15, np.float32, 4),
         "gptr": (16, np.int32, 1), "gslot": (17, np.int32, 1),
         "blk_order": (18, np.int32, 1), "ord32": (19, np.int32, 1),
+        "knn": (20, np.int32, 32),
     }
 
     def geometry(self, name: str) -> np.ndarray:

@@ -170,7 +170,10 @@ enum {
   DS_GEOM_GPTR = 16,     /* int per tile + 1: column-sum slot offsets */
   DS_GEOM_GSLOT = 17,    /* int per half-list record (-1: none) */
   DS_GEOM_BLK_ORDER = 18,/* int per block: FP64 launch order */
-  DS_GEOM_ORD32 = 19     /* int per 2 blocks: FP32 launch lists */
+  DS_GEOM_ORD32 = 19,    /* int per 2 blocks: FP32 launch lists */
+  DS_GEOM_KNN = 20       /* int per slot x 32: the warm start's nearest
+                            neighbours (slots, nearest first, -1 = none);
+                            built by the first prediction */
 };
 int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
                             long long cap_bytes, long long* bytes);
