@@ -11,23 +11,24 @@
 // FP32), so PCG's theory is unchanged; solves stop on the same FP64 true
 // residual.
 //
-// k_bj_build: one 128-thread workgroup per target block; row i of the
-// block is held in registers by threads i (columns 0..31) and 64 + i
-// (columns 32..63), built as A_ij = w_i S_ij w_j + delta_ij (W form) or
-// S_ij + N_i delta_ij (SN form) with the FP64 table interpolant of
-// lib/signal_covar.h:53-77 (padding rows: identity).  The block is inverted
-// by the in-place Gauss-Jordan sweep (no pivoting: the block is SPD), with
-// each swept row kept unscaled: row i holds its true values divided by its
-// pivot's reciprocal p_i (times 1 while unswept) and is scaled once at the
-// end.  Step k: the two threads of row k publish it in LDS (its true values:
-// it is unswept) and every row applies m_c -= g raw_c with one factor
-// g = a_ik d_i p_k (d_i = 1 / p_i, or 1), a_ik taken from the published row
-// by the sweep's structure (a_ik = a_ki while both are unswept, -a_ki once
-// i < k is swept) so that no thread indexes its registers by k; column k
-// becomes -g, row k keeps its values (g = 0) with 1 in column k.  One FMA
-// per entry and step.  The inverse goes through LDS once as FP32 to be
-// symmetrised, rounded to FP32 and stored by diagonals (bj_apply's layout):
-// minv[I][k][i] = M_{i,(i+k) mod 64}, k = 0..32.
+// k_bj_build: one wave per target block; lane (bi, bj) = (lane / 8,
+// lane mod 8) holds in registers the 64 entries of rows bi + 8 u and
+// columns bj + 8 v (u, v = 0..7), built as A_ij = w_i S_ij w_j + delta_ij
+// (W form) or S_ij + N_i delta_ij (SN form) with the FP64 table interpolant
+// of lib/signal_covar.h:53-77 (padding rows: identity).  The block is
+// inverted in place by Gauss-Jordan elimination without pivoting (the
+// block is SPD): step k divides row k by its pivot (its pivot entry
+// becoming 1 / pivot) and subtracts a_ik times it from every other row i
+// (column k taken as 0 there first), so that after the last step the
+// registers hold the inverse.  Each step the 8 lanes of row k and the 8
+// lanes of column k publish them in LDS (two buffers, so one barrier per
+// step), and every lane reads the 8 row values of its columns and the 8
+// column values of its rows: one fma per entry and step.  The inverse is
+// rounded to FP32, each entry averaged with its transpose (fetched from
+// lane (bj, bi) by ds_bpermute), staged in LDS and stored by diagonals in
+// coalesced rows (bj_apply's layout): minv[I][k][i] = M_{i,(i+k) mod 64},
+// k = 0..32.  (Interleaved rows and columns make both the staging writes
+// and the diagonal reads free of LDS bank conflicts.)
 // COMPILE WITH -ffp-contract=off.
 #include "cg_device.h"
 #include "ds_internal.h"
@@ -47,73 +48,118 @@ struct BjArgs {
   int table_n, form;
 };
 
-__global__ __launch_bounds__(128) void k_bj_build(BjArgs a) {
-  constexpr int HC = DS_BS / 2;        // columns per thread
-  __shared__ double raw[2][DS_BS];     // step k (buffer k & 1): row k
-  __shared__ float mt[DS_BS][DS_BS + 1];
+__global__ __launch_bounds__(64) void k_bj_build(BjArgs a) {
+  constexpr int TB = 8;                  // entries per lane and axis
+  constexpr int RS = TB + 2;             // padded LDS row (16-byte aligned)
+  constexpr int MS = 72;                 // staging row stride (floats)
+  struct Elim {
+    double rowb[2][TB][RS];              // row k, by owning column lane
+    double colb[2][TB][RS];              // column k, by owning row lane
+    double spare[DS_BS][2 * TB + 2];     // unowned values (see below;
+                                         // padded: lanes start on
+                                         // different LDS banks)
+    double4 bpos[DS_BS];                 // the block's pixels
+  };
+  __shared__ __align__(16) union {
+    Elim e;
+    float mt[DS_BS * MS];                // symmetrized FP32 inverse
+  } sm;
   const int I = blockIdx.x, t = threadIdx.x;
-  const int i = t & (DS_BS - 1);
-  const int h = __builtin_amdgcn_readfirstlane(t >> 6), c0 = h * HC;
+  const int bi = t / TB, bj = t % TB;
   const double4 cv = a.sub_cv[a.blk_sub[I]];
   const int nreal = a.blk_nreal[I];
-  const double4 pi = a.pos[I * DS_BS + i];
-  double m[HC];
+  sm.e.bpos[t] = a.pos[I * DS_BS + t];
+  const double tlast = (double)(a.table_n - 1);
+  __syncthreads();
+  double m[TB][TB];   // m[u][v] = A_{bi + 8u, bj + 8v}
 #pragma unroll
-  for (int c = 0; c < HC; ++c) {
-    const int j = c0 + c;
-    double v = (i == j) ? 1.0 : 0.0;
-    if (i < nreal && j < nreal) {
-      const double4 pj = a.pos[I * DS_BS + j];
+  for (int u = 0; u < TB; ++u) {
+    const int i = bi + TB * u;
+    const double4 pi = sm.e.bpos[i];
+#pragma unroll
+    for (int v = 0; v < TB; ++v) {
+      const int j = bj + TB * v;
+      // branch-free: every entry is evaluated (the index clamped into the
+      // table), then the cutoff and the padding rows select
+      const double4 pj = sm.e.bpos[j];
       const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const double x_perp_2 = (dx * dx + dy * dy);
       const double x_para_2 = dz * dz;
       const double xx = (cv.y * x_perp_2 + cv.z * x_para_2) * a.dx_inv;
-      double S = 0.0;
-      if (xx < (double)(a.table_n - 1)) {
-        const int ix = (int)xx;
-        S = cv.x * (a.table[ix] * ((double)(ix + 1) - xx) +
-                    a.table[ix + 1] * (xx - (double)ix));
-      }
-      if (a.form == DS_FORM_W) v = pi.w * S * pj.w + v;
-      else v = S + (i == j ? pi.w : 0.0);
+      const int ix = min((int)fmin(xx, tlast), a.table_n - 2);
+      double S = cv.x * (a.table[ix] * ((double)(ix + 1) - xx) +
+                         a.table[ix + 1] * (xx - (double)ix));
+      S = xx < tlast ? S : 0.0;
+      const double id = (i == j) ? 1.0 : 0.0;
+      const double val = a.form == DS_FORM_W ? pi.w * S * pj.w + id
+                                             : S + (i == j ? pi.w : 0.0);
+      m[u][v] = (i < nreal && j < nreal) ? val : id;
     }
-    m[c] = v;
+    // one row's table loads in flight at a time (register budget)
+    __builtin_amdgcn_sched_barrier(0);
   }
-  double d_own = 1.0, p_own = 1.0;   // this row's pivot and its reciprocal
-  for (int k = 0; k < DS_BS; ++k) {
-    double* rw = raw[k & 1];
-    if (i == k) {
+  // Steps k = 8 u + o: row k is held by the lanes bi = o (entries m[u][.]),
+  // column k by the lanes bj = o (entries m[.][u]).  Branch-free (a branch
+  // here costs the register allocator dearly): every lane stores its
+  // candidate row and column values, the lanes that do not own them into a
+  // spare slot of their own, and the special row and column are handled by
+  // exact factors of 0 and 1.
 #pragma unroll
-      for (int c = 0; c < HC; ++c) rw[c0 + c] = m[c];
-    }
-    __syncthreads();
-    const double piv = rw[k];
-    const double p = 1.0 / piv;
-    const double fr = rw[i];
-    const double g = i == k ? 0.0 : ((i < k ? -fr : fr) * d_own) * p;
-    if (i == k) {
-      d_own = piv;
-      p_own = p;
-    }
+  for (int u = 0; u < TB; ++u) {
+    for (int o = 0; o < TB; ++o) {
+      const int buf = o & 1;
+      const bool own_row = bi == o, own_col = bj == o;
+      const double zr = own_row ? 0.0 : 1.0, zc = own_col ? 0.0 : 1.0;
+      double* rdst = own_row ? &sm.e.rowb[buf][bj][0] : &sm.e.spare[t][0];
+      double* cdst = own_col ? &sm.e.colb[buf][bi][0] : &sm.e.spare[t][TB];
 #pragma unroll
-    for (int c = 0; c < HC; ++c) m[c] = fma(-g, rw[c0 + c], m[c]);
-    const int kc = k - c0;   // wave-uniform: column k in this half?
-    if (kc >= 0 && kc < HC) {
-      const double mk = i == k ? 1.0 : -g;
+      for (int v = 0; v < TB; ++v) rdst[v] = m[u][v];
 #pragma unroll
-      for (int c = 0; c < HC; ++c)
-        if (c == kc) m[c] = mk;
+      for (int w = 0; w < TB; ++w) cdst[w] = m[w][u];
+      __syncthreads();
+      const double p = 1.0 / sm.e.rowb[buf][o][u];   // 1 / pivot
+      double rv[TB], cl[TB];
+#pragma unroll
+      for (int v = 0; v < TB; ++v) rv[v] = sm.e.rowb[buf][bj][v] * p;
+      rv[u] = own_col ? p : rv[u];
+#pragma unroll
+      for (int w = 0; w < TB; ++w) cl[w] = sm.e.colb[buf][bi][w];
+      // column k taken as 0 (it becomes -a_ik / pivot); row k as 0 with
+      // factor -1 (it becomes the scaled row)
+      cl[u] = own_row ? -1.0 : cl[u];
+#pragma unroll
+      for (int w = 0; w < TB; ++w) m[w][u] *= zc;
+#pragma unroll
+      for (int v = 0; v < TB; ++v) m[u][v] *= zr;
+#pragma unroll
+      for (int w = 0; w < TB; ++w)
+#pragma unroll
+        for (int v = 0; v < TB; ++v) m[w][v] = fma(-cl[w], rv[v], m[w][v]);
     }
   }
+  // FP32, averaged with the transpose: lane (bj, bi) holds M_{j,i} of this
+  // lane's (i, j) = (bi + 8 u, bj + 8 v) as its m[v][u]
+  const int tp = bj * TB + bi;
+  float f[TB][TB];
 #pragma unroll
-  for (int c = 0; c < HC; ++c) mt[i][c0 + c] = (float)(m[c] * p_own);
+  for (int u = 0; u < TB; ++u)
+#pragma unroll
+    for (int v = 0; v < TB; ++v) f[u][v] = (float)m[u][v];
+  __syncthreads();   // the elimination's LDS is reused below
+#pragma unroll
+  for (int u = 0; u < TB; ++u) {
+#pragma unroll
+    for (int v = 0; v < TB; ++v) {
+      const float tr = __int_as_float(
+          __builtin_amdgcn_ds_bpermute(tp * 4, __float_as_int(f[v][u])));
+      sm.mt[(bi + TB * u) * MS + bj + TB * v] =
+          (float)(0.5 * ((double)f[u][v] + (double)tr));
+    }
+  }
   __syncthreads();
   float* out = a.minv + (size_t)I * DS_BJ_SIZE;
-  for (int k = h; k < DS_BJ_ROWS; k += 2) {
-    const int j = (i + k) & (DS_BS - 1);
-    out[k * DS_BS + i] =
-        (float)(0.5 * ((double)mt[i][j] + (double)mt[j][i]));
-  }
+  for (int k = 0; k < DS_BJ_ROWS; ++k)
+    out[k * DS_BS + t] = sm.mt[t * MS + ((t + k) & (DS_BS - 1))];
 }
 
 // x = M^-1 b for every target block (one wave per block).
@@ -139,8 +185,7 @@ hipError_t bj_build(ds_problem* p, hipStream_t st) {
   a.dx_inv = p->cv.dx_inv;
   a.table_n = p->cv.table_n;
   a.form = p->form;
-  hipLaunchKernelGGL(k_bj_build, dim3(p->minv_mod), dim3(2 * DS_BS), 0, st,
-                     a);
+  hipLaunchKernelGGL(k_bj_build, dim3(p->minv_mod), dim3(64), 0, st, a);
   return hipGetLastError();
 }
 
