@@ -284,7 +284,7 @@ struct ds_problem {
   double* wvec = nullptr;              // w (W form), 0 on pad slots
   double *du = nullptr, *xu = nullptr; // w .* d, w .* x (W form)
   float* du32 = nullptr;               // FP32 copy of du
-  double* partials = nullptr;          // 4 * nblk
+  double* partials = nullptr;          // 2 * nblk (two partial vectors)
   ds_cg_scalars* sc = nullptr;         // n_sub entries
   double* norms = nullptr;             // residual histories, n_sub x cap
   double* deltas = nullptr;            // delta histories (after norms)
