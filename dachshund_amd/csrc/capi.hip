@@ -343,7 +343,12 @@ std::vector<int> xcd_order(const std::vector<int>& blocks, Cost cost) {
 }
 
 void problem_free(ds_problem* p) {
-  dev_free(p->knn_nbr);
+  // the stream work that used them has completed (ds_problem_destroy and
+  // the failed-creation path synchronize first)
+  ds::ctx_buf_put(p->ctx, p->knn_nbr, p->knn_bytes);
+  ds::ctx_buf_put(p->ctx, p->minv, p->minv_bytes);
+  p->knn_nbr = nullptr;
+  p->minv = nullptr;
   dev_free(p->perm_dev);
   dev_free(p->tbox_lo);
   dev_free(p->tbox_hi);
@@ -356,7 +361,6 @@ void problem_free(ds_problem* p) {
   dev_free(p->tptr);
   dev_free(p->tidx);
   dev_free(p->tmask);
-  dev_free(p->minv);
   dev_free(p->toff32);
   dev_free(p->table64);
   dev_free(p->table32);
@@ -448,9 +452,43 @@ void ctx_retain(ds_ctx* c) { c->refs.fetch_add(1); }
 void ctx_release(ds_ctx* c) {
   if (c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
+  for (auto& e : c->buf_cache) (void)hipFree(e.second);
+  c->buf_cache.clear();
   if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(c->buf_mu);
+    for (size_t k = 0; k < c->buf_cache.size(); ++k) {
+      if (c->buf_cache[k].first == bytes) {
+        *p = c->buf_cache[k].second;
+        c->buf_cache.erase(c->buf_cache.begin() + (long)k);
+        c->buf_cache_bytes -= bytes;
+        return hipSuccess;
+      }
+    }
+  }
+  return hipMalloc(p, bytes);
+}
+
+void ctx_buf_put(ds_ctx* c, void* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(c->buf_mu);
+  if (bytes > kBufCacheBytes) {
+    (void)hipFree(p);
+    return;
+  }
+  // oldest out first
+  while (!c->buf_cache.empty() && c->buf_cache_bytes + bytes > kBufCacheBytes) {
+    (void)hipFree(c->buf_cache.front().second);
+    c->buf_cache_bytes -= c->buf_cache.front().first;
+    c->buf_cache.erase(c->buf_cache.begin());
+  }
+  c->buf_cache.emplace_back(bytes, p);
+  c->buf_cache_bytes += bytes;
 }
 
 }  // namespace ds
@@ -1704,7 +1742,8 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
     ds_ctx* ctx = p->ctx;
     // subvolumes sharing one geometry share one set of block inverses
     p->minv_mod = p->shared_geometry ? p->nblk / p->n_sub : p->nblk;
-    HIP_OR_FAIL(ctx, dev_alloc(&p->minv, (size_t)p->minv_mod * DS_BJ_SIZE));
+    p->minv_bytes = sizeof(float) * (size_t)p->minv_mod * DS_BJ_SIZE;
+    HIP_OR_FAIL(ctx, ds::ctx_buf_get(ctx, (void**)&p->minv, p->minv_bytes));
     if (!p->minv_ev)
       HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&p->minv_ev,
                                                hipEventDisableTiming));

@@ -25,6 +25,7 @@
 #include <atomic>
 #include <cstdint>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dachshund_gpu.h"
@@ -174,11 +175,23 @@ struct ds_ctx {
   // context is freed when the last of them lets go (so destroying it before
   // its problems, as a garbage collector may, frees nothing they still use)
   std::atomic<int> refs{1};
+  // Per-geometry buffers of destroyed problems (block inverses, neighbour
+  // lists: tens of MB, ~0.1 ms of hipMalloc each), kept for the next
+  // problem that asks for the same size; at most kBufCacheBytes in all.
+  std::mutex buf_mu;
+  std::vector<std::pair<size_t, void*>> buf_cache;
+  size_t buf_cache_bytes = 0;
 };
 
 namespace ds {
 void ctx_retain(ds_ctx* c);
 void ctx_release(ds_ctx* c);
+constexpr size_t kBufCacheBytes = size_t(1) << 30;
+// A device buffer of exactly `bytes`: a cached one, else hipMalloc.
+hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes);
+// Return a buffer (the stream work that used it has completed) to the
+// cache, or free it when the cache is full.
+void ctx_buf_put(ds_ctx* c, void* p, size_t bytes);
 }  // namespace ds
 
 struct ds_problem {
@@ -220,6 +233,7 @@ struct ds_problem {
   mutable bool minv_pending = false;
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
+  size_t knn_bytes = 0, minv_bytes = 0;   // their buffers' sizes
   float knn_r0 = 1.0f;                 // its first ring radius
   int* sub_blk0 = nullptr;             // first block per subvolume
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}

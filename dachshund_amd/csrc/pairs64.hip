@@ -135,12 +135,18 @@ struct MatvecArgs {
   int nblk, table_n;
   int residual;           // out = rhs - A a
   int need_want;          // skip unless the subvolume wants a true residual
+  int single;             // one subvolume
 };
 
 template <int FORM, int EPI, int R>
 __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
   extern __shared__ double2 tab[];
   __shared__ double red[4 * 64];
+  // a single volume's true-residual gate returns after one load when no
+  // true residual is due (the usual case in a mixed solve)
+  if (EPI != EPI_NONE && a.need_want && a.single &&
+      (a.sc->done || !a.sc->want_true))
+    return;
   const int I = a.blk_order ? a.blk_order[blockIdx.x] : blockIdx.x;
   const int sv = a.blk_sub[I];
   const ds_cg_scalars* sc = a.sc + sv;
@@ -543,6 +549,7 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
   a.table_n = p->cv.table_n;
   a.residual = residual;
   a.need_want = e.need_want;
+  a.single = p->n_sub == 1;
   const dim3 grid(p->nblk), block(64 * p->waves_per_block);
   const size_t shm = sizeof(double2) * p->cv.table_n;
   const bool w = p->form == DS_FORM_W;

@@ -1346,7 +1346,8 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     int* nbr = nullptr;
     const size_t nb = sizeof(int) * (size_t)p->npad * WS_K;
     const size_t nbox = 2 * sizeof(float4) * p->ntile;
-    hipError_t e = hipMalloc(&nbr, nb + nbox + sizeof(int) * (p->ntile + 1));
+    const size_t bytes = nb + nbox + sizeof(int) * (p->ntile + 1);
+    hipError_t e = ds::ctx_buf_get(p->ctx, (void**)&nbr, bytes);
     if (e != hipSuccess) return e;
     a.nbr = nbr;
     float4* tbox = reinterpret_cast<float4*>(reinterpret_cast<char*>(nbr) + nb);
@@ -1369,6 +1370,7 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
       return e;
     }
     p->knn_nbr = nbr;
+    p->knn_bytes = bytes;
   }
   a.nbr = p->knn_nbr;
   return hipSuccess;
