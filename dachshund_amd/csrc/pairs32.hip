@@ -67,6 +67,7 @@ struct Matvec32Args {
   float tlast;
   int nblk, table_n;
   int lane_zero;          // 0; lane & lane_zero makes an index lane-varying
+  unsigned npad;          // slots (buffer extents of loc and u32)
   // symmetric MFMA kernel
   const int* hptr;
   float* pbuf;
@@ -162,6 +163,27 @@ __device__ __forceinline__ void butterfly_step(float* v, int upper) {
   }
 }
 
+// Raw buffer loads: the base and the per-entry offset stay in scalar
+// registers (no per-lane 64-bit address arithmetic); loads past
+// `bytes` return 0.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+__device__ __forceinline__ BufRsrc buf_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                           (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float4 buf_f4(BufRsrc r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_float4(__int_as_float(v[0]), __int_as_float(v[1]),
+                     __int_as_float(v[2]), __int_as_float(v[3]));
+}
+__device__ __forceinline__ float buf_f1(BufRsrc r, int voff, int soff) {
+  return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff,
+                                                             0));
+}
+__device__ __forceinline__ int buf_i1(BufRsrc r, int voff, int soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
 // Symmetric form: block I runs its half list (tiles T >= 4 I).  Every
 // live (entry, sub-block s) 16 x 16 tile of values S_ij is evaluated once
 // and used twice: y_i += S_ij u_j for the block's rows (summed in
@@ -225,12 +247,18 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   // vector loads, waited for where they are used)
   const unsigned cu = (unsigned)c;
   const unsigned lz = (unsigned)(lane & a.lane_zero);
+  const BufRsrc rrec = buf_rsrc(toff, 0x7FFFFFFFu);
+  const BufRsrc rslot = buf_rsrc(a.gslot, 0x7FFFFFFFu);
+  const BufRsrc rloc = buf_rsrc(loc, a.npad * 16u);
+  const BufRsrc ru = buf_rsrc(u32, a.npad * 4u);
+  const int vrec = (int)(lz * 16u), vslot = (int)(lz * 4u);
+  const int vloc = (int)(cu * 16u), vu = (int)(cu * 4u);
   // One entry e: its record o (frame offset; .w holds the bits of live
   // mask | tile << 4), the tile's source record X and vector value uj.
   // (The MFMA results live across entries: a sub-block an entry skips
   // keeps the last value, never read.)
   v4f d[4];
-  double dqt = 0.0;   // CG: sum of u_j times this lane's column sums
+  float dqt = 0.0f;   // CG: sum of u_j times this lane's column sums
   auto entry = [&](const float4& o, const float4& X, float uj, int slot)
       __attribute__((always_inline)) {
     const unsigned bits = __builtin_amdgcn_readfirstlane(__float_as_int(o.w));
@@ -278,7 +306,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
       at = __uint_as_float(h16[0]) + __uint_as_float(h16[1]);
       if (g == 0) {
         a.pbuf[16 * (size_t)slot + c] = at;
-        if (CG) dqt = fma((double)at, (double)uj, dqt);
+        if (CG) dqt = fmaf(at, uj, dqt);
       }
     }
   };
@@ -293,29 +321,29 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   };
   if (kb < ke) {
     // records and column-sum slots two entries ahead
-    float4 R0 = (toff + kb)[lz];
-    int G0 = (a.gslot + kb)[lz];
-    float4 R1 = (toff + min(kb + nw, kl))[lz];
-    int G1 = (a.gslot + min(kb + nw, kl))[lz];
+    float4 R0 = buf_f4(rrec, vrec, 16 * kb);
+    int G0 = buf_i1(rslot, vslot, 4 * kb);
+    float4 R1 = buf_f4(rrec, vrec, 16 * min(kb + nw, kl));
+    int G1 = buf_i1(rslot, vslot, 4 * min(kb + nw, kl));
     int j = tile_of(R0);
-    float4 X0 = (loc + j)[cu];
-    float u0 = (u32 + j)[cu];
+    float4 X0 = buf_f4(rloc, vloc, 16 * j);
+    float u0 = buf_f1(ru, vu, 4 * j);
     for (int k = kb;; k += 2 * nw) {
       // entry k; prefetch the sources of k + nw, the record of k + 2 nw
       j = tile_of(R1);
-      const float4 X1 = (loc + j)[cu];
-      const float u1 = (u32 + j)[cu];
-      const float4 R2 = (toff + min(k + 2 * nw, kl))[lz];
-      const int G2 = (a.gslot + min(k + 2 * nw, kl))[lz];
+      const float4 X1 = buf_f4(rloc, vloc, 16 * j);
+      const float u1 = buf_f1(ru, vu, 4 * j);
+      const float4 R2 = buf_f4(rrec, vrec, 16 * min(k + 2 * nw, kl));
+      const int G2 = buf_i1(rslot, vslot, 4 * min(k + 2 * nw, kl));
       entry(R0, X0, u0, G0);
       if (k + nw >= ke) break;
       // entry k + nw; prefetch the sources of k + 2 nw, the record of
       // k + 3 nw
       j = tile_of(R2);
-      X0 = (loc + j)[cu];
-      u0 = (u32 + j)[cu];
-      const float4 R3 = (toff + min(k + 3 * nw, kl))[lz];
-      const int G3 = (a.gslot + min(k + 3 * nw, kl))[lz];
+      X0 = buf_f4(rloc, vloc, 16 * j);
+      u0 = buf_f1(ru, vu, 4 * j);
+      const float4 R3 = buf_f4(rrec, vrec, 16 * min(k + 3 * nw, kl));
+      const int G3 = buf_i1(rslot, vslot, 4 * min(k + 3 * nw, kl));
       entry(R1, X1, u1, G1);
       if (k + 2 * nw >= ke) break;
       R0 = R2;
@@ -343,7 +371,7 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   butterfly_step<1, 0xB1>(v, c & 1);
   red[wave * 64 + 16 * (c >> 2) + 4 * g + (c & 3)] = v[0];
   if (CG) {
-    const double t = wave_sum(dqt);
+    const double t = wave_sum((double)dqt);
     if (lane == 0) dq_red[wave] = t;
   }
   __syncthreads();
@@ -414,6 +442,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.sc = p->sc;
   a.tlast = (float)(p->cv.table_n - 1);
   a.nblk = p->nblk;
+  a.npad = (unsigned)p->npad;
   a.table_n = p->cv.table_n;
   const dim3 block(64 * p->waves_per_block);
   const size_t shm = sizeof(float2) * p->cv.table_n;
