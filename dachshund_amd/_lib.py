@@ -92,6 +92,7 @@ SIGNATURES = {
     "ds_A_product": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "ds_A_residual": (C.c_int, [_vp, _dp, _dp]),
     "ds_precond": (C.c_int, [_vp, _dp, _dp]),
+    "ds_precond_r32": (C.c_int, [_vp, _dp, _dp]),
     "ds_norm_b": (C.c_int, [_vp, _dp]),
     "ds_solve_pcg": (C.c_int, [_vp, C.POINTER(DsPcgParams), _dp, _dp,
                                C.c_int, C.POINTER(DsPcgResult)]),

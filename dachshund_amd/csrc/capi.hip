@@ -1243,6 +1243,14 @@ int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
     case DS_GEOM_GSLOT: src = p->gslot; nb = 4 * (size_t)tot[1]; break;
     case DS_GEOM_BLK_ORDER: src = p->blk_order; nb = 4 * nblk; break;
     case DS_GEOM_ORD32: src = p->ord32; nb = 8 * nblk; break;
+    case DS_GEOM_MINV:
+      if (!p->use_minv())
+        return fail(p->ctx, DS_E_STATE, "block-Jacobi preconditioner not "
+                    "selected");
+      if (p->minv_ev) HIP_OR_FAIL(p->ctx, hipEventSynchronize(p->minv_ev));
+      src = p->minv;
+      nb = p->minv_bytes;
+      break;
     case DS_GEOM_KNN:
       if (!p->knn_nbr)
         return fail(p->ctx, DS_E_STATE, "no neighbour lists yet (they are "
@@ -1320,6 +1328,17 @@ int ds_precond(ds_problem* p, const double* b, double* x) {
   if (!p || !b || !x) return DS_E_ARG;
   if (int rc = upload_sorted(p, b, p->tmp)) return rc;
   HIP_OR_FAIL(p->ctx, ds::launch_precond_p(p, p->tmp, p->q, p->ctx->stream));
+  return download_sorted(p, p->q, x);
+}
+
+int ds_precond_r32(ds_problem* p, const double* b, double* x) {
+  if (!p || !b || !x) return DS_E_ARG;
+  if (!p->use_minv())
+    return fail(p->ctx, DS_E_STATE, "block-Jacobi preconditioner not "
+                "selected");
+  if (int rc = upload_sorted(p, b, p->tmp)) return rc;
+  HIP_OR_FAIL(p->ctx, ds::launch_precond_r32(p, p->tmp, p->q,
+                                             p->ctx->stream));
   return download_sorted(p, p->q, x);
 }
 

@@ -333,6 +333,10 @@ hipError_t launch_scale(const double* v, const double* w, double* u,
 // Block-Jacobi: build the inverses (minv allocated); apply the problem's
 // preconditioner (Jacobi or block) to a device vector.
 hipError_t bj_build(ds_problem* p, hipStream_t st);
+// The block-Jacobi product as the mixed solver's update applies it (b
+// rounded to FP32, bj_apply_regs_r32).
+hipError_t launch_precond_r32(const ds_problem* p, const double* b, double* x,
+                              hipStream_t st);
 hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
                             hipStream_t st);
 // Shared-geometry batches: the true residual of every subvolume that wants

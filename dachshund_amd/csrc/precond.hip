@@ -172,7 +172,30 @@ __global__ __launch_bounds__(256) void k_bj_apply(const double* b,
   x[i] = bj_apply(minv + (size_t)(I % mod) * DS_BJ_SIZE, b[i]);
 }
 
+// The same with b rounded to FP32 and the inverse's diagonals held in
+// registers: the mixed solver's update kernel's application
+// (bj_apply_regs_r32), one wave per block.
+__global__ __launch_bounds__(256) void k_bj_apply_r32(const double* b,
+                                                      const float* minv,
+                                                      double* x, int nblk,
+                                                      int mod) {
+  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (I >= nblk) return;
+  const int i = I * DS_BS + (threadIdx.x & 63);
+  float m[DS_BJ_ROWS];
+  load_bj_diagonals(minv + (size_t)(I % mod) * DS_BJ_SIZE, m);
+  x[i] = bj_apply_regs_r32(m, b[i]);
+}
+
 }  // namespace
+
+hipError_t launch_precond_r32(const ds_problem* p, const double* b, double* x,
+                              hipStream_t st) {
+  if (hipError_t e = minv_join(p, st)) return e;
+  hipLaunchKernelGGL(k_bj_apply_r32, dim3((p->nblk + 3) / 4), dim3(256), 0,
+                     st, b, p->minv, x, p->nblk, p->minv_mod);
+  return hipGetLastError();
+}
 
 hipError_t bj_build(ds_problem* p, hipStream_t st) {
   BjArgs a{};

@@ -252,7 +252,7 @@ class _WFVector:
         "hptr": (14, np.int32, 1), "hrec": (15, np.float32, 4),
         "gptr": (16, np.int32, 1), "gslot": (17, np.int32, 1),
         "blk_order": (18, np.int32, 1), "ord32": (19, np.int32, 1),
-        "knn": (20, np.int32, 32),
+        "knn": (20, np.int32, 32), "minv": (21, np.float32, 33 * 64),
     }
 
     def geometry(self, name: str) -> np.ndarray:
@@ -289,6 +289,14 @@ class _WFVector:
         b = self._vec(b)
         x = np.empty(self.n)
         check(lib().ds_precond(self.h, _dp(b), _dp(x)), self.ctx.h)
+        return x
+
+    def preconditioner_solve_r32(self, b):
+        """M^-1 b as the mixed solver applies the block-Jacobi inverse (b
+        rounded to FP32); DS_PRECOND_BLOCK must be selected."""
+        b = self._vec(b)
+        x = np.empty(self.n)
+        check(lib().ds_precond_r32(self.h, _dp(b), _dp(x)), self.ctx.h)
         return x
 
     def norm_b(self):

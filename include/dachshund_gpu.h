@@ -171,9 +171,12 @@ enum {
   DS_GEOM_GSLOT = 17,    /* int per half-list record (-1: none) */
   DS_GEOM_BLK_ORDER = 18,/* int per block: FP64 launch order */
   DS_GEOM_ORD32 = 19,    /* int per 2 blocks: FP32 launch lists */
-  DS_GEOM_KNN = 20       /* int per slot x 32: the warm start's nearest
+  DS_GEOM_KNN = 20,      /* int per slot x 32: the warm start's nearest
                             neighbours (slots, nearest first, -1 = none);
                             built by the first prediction */
+  DS_GEOM_MINV = 21      /* float x 33 x 64 per block: the block-Jacobi
+                            inverse by diagonals, row k holding
+                            M_{i,(i+k) mod 64} (DS_PRECOND_BLOCK) */
 };
 int ds_problem_get_geometry(const ds_problem* p, int which, void* out,
                             long long cap_bytes, long long* bytes);
@@ -270,6 +273,12 @@ int ds_A_product(ds_problem* p, int prec, const double* a, double* b);
 int ds_A_residual(ds_problem* p, const double* x, double* r);
 /* WFVector*::preconditioner_solve (lib/wf_vector.cc:42-49, :185-194). */
 int ds_precond(ds_problem* p, const double* b, double* x);
+/* x = M^-1 b with the block-Jacobi preconditioner as the mixed solver
+ * applies it (b rounded to FP32, products and sums in FP64, its own
+ * cross-lane order).  No reference counterpart: exposed so that tests can
+ * pin the mixed solver's application.  DS_E_STATE unless DS_PRECOND_BLOCK
+ * is selected. */
+int ds_precond_r32(ds_problem* p, const double* b, double* x);
 /* WFVector*::norm_b (lib/wf_vector.cc:105-112, :253-260), summed in the
  * reference's order on the host. */
 int ds_norm_b(ds_problem* p, double* out);

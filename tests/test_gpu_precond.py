@@ -77,6 +77,48 @@ def test_block_precond_apply(ora, case, form):
     assert np.array_equal(wf2.preconditioner_solve(r), ora.precond(f, fp, c, r))
 
 
+@pytest.mark.parametrize("case", ["fixture", "n200"])
+def test_block_precond_r32_apply_is_the_sequential_sum(case):
+    """The mixed solver's block-Jacobi application (its cross-lane moves by
+    whole-wave DPP rotations, the lower diagonals Horner-fashion) equals,
+    bit for bit, the sequential form: for row i of a block, the FP64 sum over
+    k = 0..63 in order of M_{i,(i+k) mod 64} r_j, j = (i+k) mod 64, with the
+    stored FP32 inverse and r rounded to FP32.  (The product of two floats
+    is exact in FP64, so each step rounds once, as an fma does.)"""
+    import dachshund_amd as D
+    pix = _cases()[case]
+    n = len(pix)
+    sp = D.SignalCovarParams(0.05, 2.5, 2.1)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    wf.set_precond(D.DS_PRECOND_BLOCK)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(n) * np.exp(rng.uniform(-6, 6, n))
+    s = wf.preconditioner_solve_r32(r)
+    minv = wf.geometry("minv").reshape(-1, 33, 64).astype(np.float64)
+    perm = wf.geometry("perm")
+    real = perm >= 0
+    rs = np.zeros(len(perm))
+    rs[real] = r[perm[real]]
+    rf = rs.astype(np.float32).astype(np.float64)
+    i = np.arange(64)
+    out = np.empty(len(perm))
+    for b in range(len(perm) // 64):
+        mt, rb = minv[b], rf[64 * b:64 * b + 64]
+        acc = np.zeros(64)
+        for k in range(64):
+            j = (i + k) % 64
+            m = mt[k, i] if k <= 32 else mt[64 - k, j]
+            acc = acc + m * rb[j]
+        out[64 * b:64 * b + 64] = acc
+    ref = np.empty(n)
+    ref[perm[real]] = out[real]
+    assert np.array_equal(s, ref)
+    # and it is the FP64-broadcast application to within the FP32 rounding
+    # of r
+    s64 = wf.preconditioner_solve(r)
+    assert np.allclose(s, s64, rtol=1e-6, atol=1e-6 * np.abs(s64).max())
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_block_precond_solves(ora, mixed):
     import dachshund_amd as D
