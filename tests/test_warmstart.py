@@ -228,3 +228,65 @@ def test_trained_warm_start_cuts_iterations():
     # and costs no iterations on average
     assert max(ratios) < 0.9, ratios
     assert np.mean(gains) >= 0, gains
+
+
+@pytest.mark.gpu
+def test_committed_network_gain_on_held_out_300k():
+    """The committed network (dachshund_amd/warmstart_c5.npz, trained on
+    seeds 1000-1199 of the 300,000-pixel workload) on held-out seeds 2001-
+    2003, each solve to the FP64 tolerance 1e-6 from x0 = 0 and from the
+    prediction, on a fresh problem (so the prediction's time includes the
+    neighbour search):
+
+      * plain FP64 PCG with Jacobi (the reference's solver): at least 7 %
+        fewer iterations, and less time including the neighbour search and
+        the prediction;
+      * the mixed solver with block-Jacobi (the benchmark's solver): at
+        least 2 fewer iterations on average (about 6 %).  Here the saved
+        iterations
+        save about as much time as a fresh geometry's neighbour search
+        and prediction cost; the gain in time needs the geometry's
+        neighbour lists to be reused (several data vectors over the same
+        pixels)."""
+    import os
+    import time
+    import dachshund_amd as D
+    from dachshund_amd import synth
+    from dachshund_amd.warmstart import WarmStart
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ws = WarmStart()
+    ws.load(os.path.join(root, "dachshund_amd", "warmstart_c5.npz"))
+    w = synth.workload("c5")
+    sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
+    ctx = D.default_context()
+    it0, it1, t0s, t1s, mit0, mit1 = [], [], [], [], [], []
+    for seed in (2001, 2002, 2003):
+        pw = D.pixel_to_wpixel(synth.make_pixels(w, seed=seed))
+        prm = D.PCGParams(len(pw), w.max_iter, w.step_r, 1e-6)
+        for learned in (False, True):
+            wf = D.WFVectorW(pw, sp)
+            ctx.synchronize()
+            t = time.perf_counter()
+            if learned:
+                ws.predict_into(wf)
+            else:
+                wf.fill_x(0.0)
+            r = wf.solve_resident(prm)
+            dt = time.perf_counter() - t
+            assert r.success
+            (it1 if learned else it0).append(r.num_iters)
+            (t1s if learned else t0s).append(dt)
+            wf.set_precond(D.DS_PRECOND_BLOCK)
+            if learned:
+                ws.predict_into(wf)
+            else:
+                wf.fill_x(0.0)
+            rm = wf.solve_resident(prm, mixed=True)
+            assert rm.success and rm.true_norm <= 1e-6 * wf.norm_b() * 1.0001
+            (mit1 if learned else mit0).append(rm.num_iters)
+            wf.close()
+    print(f"FP64 Jacobi iterations {it0} -> {it1}, s {t0s} -> {t1s}; "
+          f"mixed block-Jacobi {mit0} -> {mit1}")
+    assert np.sum(it1) <= 0.93 * np.sum(it0), (it0, it1)
+    assert np.sum(t1s) < np.sum(t0s), (t0s, t1s)
+    assert np.sum(mit1) <= np.sum(mit0) - 2 * len(mit0), (mit0, mit1)
