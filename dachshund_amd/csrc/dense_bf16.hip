@@ -18,19 +18,21 @@
 // its column of U (the MFMA's k-slots of a half are fed from that run; A
 // and B use the same assignment, so every product S[row][k] U[k][col] of
 // the chunk is formed exactly once).  Each row's 128 bytes per chunk are
-// read by two lanes.  Output tile layout of v_mfma_f32_32x32x8bf16_1k:
-// register j of lane l holds row 8*(j/4) + 4*(l/32) + j%4, column l%32.
+// read by two lanes, and fed to v_mfma_f32_32x32x16_bf16 (gfx950's BF16
+// MFMA, 16 k-slots: 8 per lane) in 4 steps.  Output tile layout: register
+// j of lane l holds row 8*(j/4) + 4*(l/32) + j%4, column l%32.
 //
 // k_fused_mrhs: the same multi-RHS product without materializing S.  A
 // batch of subvolumes sharing one geometry has one set of tile lists; per
 // (64-row target block, 16-source tile) entry the VALU evaluates the 64 x 16
 // interpolant values g_ij in FP32 (the expanded table argument of the FP32
-// pair kernel: lane = target row), rounds them to BF16, and two K-steps of
-// v_mfma_f32_32x32x8bf16_1k per 32-row half multiply them into all R right-
-// hand sides at once (A = g tile, B = the BF16 directions of the 16 sources
-// for the 32 RHS).  The MFMA A layout wants, in lane l, 4 consecutive
-// sources of row 32 rb + l%32; half of those values live in lane l ^ 32,
-// which sends them with one cross-half shuffle per K-step.  The epilogue
+// pair kernel: lane = target row), rounds them to BF16, and one K = 16
+// step of v_mfma_f32_32x32x16_bf16 per 32-row half multiplies them into all
+// R right-hand sides at once (A = g tile, B = the BF16 directions of the 16
+// sources for the 32 RHS).  The MFMA A layout wants, in lane l, the 8
+// sources 8 (l/32) .. 8 (l/32) + 7 of row 32 rb + l%32; half of those
+// values live in lane l ^ 32, which sends them with cross-half shuffles
+// (4 per tile).  The epilogue
 // scales by var_s in FP64, adds the identity or noise term per RHS and
 // stores the d . q partial of each subvolume's block.
 // COMPILE WITH -ffp-contract=off (the S build is FP64 parity arithmetic).
@@ -41,7 +43,7 @@ namespace ds {
 
 namespace {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kR = DS_DENSE_R;     // right-hand sides per GEMM
@@ -103,23 +105,16 @@ __global__ __launch_bounds__(256) void k_build_s(const double4* pos, int n,
 constexpr int kW = 8;                 // waves per workgroup
 constexpr int kUPitch = 64 * 2 + 16;  // LDS bytes per U column chunk
 
+// The lane's 32 BF16 values of the chunk (its run of S, of U) in 4 steps of
+// v_mfma_f32_32x32x16_bf16: step s feeds run elements 8 s .. 8 s + 7 as the
+// lane's 8 k-slots (A and B alike).
 __device__ __forceinline__ void mfma_chunk(const uint4 (&a)[4],
                                            const uint4 (&b)[4], v16f& acc) {
-  const unsigned* au = reinterpret_cast<const unsigned*>(a);
-  const unsigned* bu = reinterpret_cast<const unsigned*>(b);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    v4s av, bv;
-    av[0] = (short)(au[2 * s] & 0xFFFFu);
-    av[1] = (short)(au[2 * s] >> 16);
-    av[2] = (short)(au[2 * s + 1] & 0xFFFFu);
-    av[3] = (short)(au[2 * s + 1] >> 16);
-    bv[0] = (short)(bu[2 * s] & 0xFFFFu);
-    bv[1] = (short)(bu[2 * s] >> 16);
-    bv[2] = (short)(bu[2 * s + 1] & 0xFFFFu);
-    bv[3] = (short)(bu[2 * s + 1] >> 16);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
-  }
+  for (int s = 0; s < 4; ++s)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        __builtin_bit_cast(v8bf, a[s]), __builtin_bit_cast(v8bf, b[s]), acc,
+        0, 0, 0);
 }
 
 __global__ __launch_bounds__(64 * kW) void k_gemm_bf16(
@@ -191,14 +186,6 @@ struct FusedArgs {
   int nblk1, ld, n_sub, nblk, table_n, form;
 };
 
-__device__ __forceinline__ v4s as_v4s(unsigned lo, unsigned hi) {
-  v4s v;
-  v[0] = (short)(lo & 0xFFFFu);
-  v[1] = (short)(lo >> 16);
-  v[2] = (short)(hi & 0xFFFFu);
-  v[3] = (short)(hi >> 16);
-  return v;
-}
 
 __global__ __launch_bounds__(256) void k_fused_mrhs(
     const float4* __restrict__ loc, const int* __restrict__ tidx,
@@ -217,7 +204,7 @@ __global__ __launch_bounds__(256) void k_fused_mrhs(
   const float4 li = a.tgt[I * DS_BS + lane];
   const float tlast = a.tlast;
   v16f acc0 = {}, acc1 = {};
-  const unsigned short* ucol = a.U + (size_t)r * a.ld + 4 * h;
+  const unsigned short* ucol = a.U + (size_t)r * a.ld + 8 * h;
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
   for (int k = kb; k < ke; k += nw) {
@@ -242,22 +229,24 @@ __global__ __launch_bounds__(256) void k_fused_mrhs(
       }
       P[m] = gb[0] | ((unsigned)gb[1] << 16);
     }
+    // one K = 16 step per 32-row half: lane l of half h supplies the
+    // sources 8 h .. 8 h + 7 of row 32 rb + l % 32.  A lane keeps the half
+    // of its own row that its k-slots take and sends the other half to
+    // lane l ^ 32, which takes it for the same row.
+    unsigned own[4], rcv[4];
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const unsigned own0 = h ? P[4 * ks + 2] : P[4 * ks];
-      const unsigned own1 = h ? P[4 * ks + 3] : P[4 * ks + 1];
-      const unsigned snd0 = h ? P[4 * ks] : P[4 * ks + 2];
-      const unsigned snd1 = h ? P[4 * ks + 1] : P[4 * ks + 3];
-      const unsigned rcv0 = (unsigned)__shfl_xor((int)snd0, 32, 64);
-      const unsigned rcv1 = (unsigned)__shfl_xor((int)snd1, 32, 64);
-      const uint2 bu =
-          *reinterpret_cast<const uint2*>(ucol + j0 + 8 * ks);
-      const v4s bv = as_v4s(bu.x, bu.y);
-      const v4s a0 = h ? as_v4s(rcv0, rcv1) : as_v4s(own0, own1);
-      const v4s a1 = h ? as_v4s(own0, own1) : as_v4s(rcv0, rcv1);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0, bv, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, bv, acc1, 0, 0, 0);
+    for (int q = 0; q < 4; ++q) {
+      own[q] = h ? P[4 + q] : P[q];
+      rcv[q] = (unsigned)__shfl_xor((int)(h ? P[q] : P[4 + q]), 32, 64);
     }
+    const uint4 ow = make_uint4(own[0], own[1], own[2], own[3]);
+    const uint4 rv = make_uint4(rcv[0], rcv[1], rcv[2], rcv[3]);
+    const uint4 bu = *reinterpret_cast<const uint4*>(ucol + j0);
+    const v8bf bv = __builtin_bit_cast(v8bf, bu);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        __builtin_bit_cast(v8bf, h ? rv : ow), bv, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        __builtin_bit_cast(v8bf, h ? ow : rv), bv, acc1, 0, 0, 0);
   }
   // combine the waves' partial tiles
   if (wave > 0) {

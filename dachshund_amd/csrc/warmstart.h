@@ -39,7 +39,7 @@ struct WsTrainer {
 
 // Padded input widths of layers 1-3 in the MFMA forward and the size of
 // the split-weight buffer (3 BF16 parts x 64 units x padded width).
-#define WS_K1P 168
+#define WS_K1P 176
 #define WS_SPLIT_ELEMS (3 * WS_HIDDEN * (WS_K1P + 2 * WS_HIDDEN))
 
 hipError_t ws_features(ds_problem* p, unsigned short* feat,

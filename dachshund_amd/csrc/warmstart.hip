@@ -929,15 +929,15 @@ __global__ __launch_bounds__(64) void k_fwd(const float* __restrict__ P,
 // Each FP32 weight is split into three BF16 parts, w = hi + mid + lo (8
 // mantissa bits each, so the sum carries the FP32 value); activations are
 // BF16 already (features, and every hidden layer with quant), so the
-// products x * part are exact in FP32 and v_mfma_f32_32x32x8bf16_1k sums
-// them with FP32 accumulation: the same arithmetic class as the VALU
-// forward, in another summation order.  One wave runs 64 pixels: A = the
-// activation rows (LDS), B = 4 consecutive inputs of one unit of one part
-// (global, L2-resident), D = 32 pixels x 32 units per MFMA; the epilogue
+// products x * part are exact in FP32 and v_mfma_f32_32x32x16_bf16 (gfx950's
+// BF16 MFMA) sums them with FP32 accumulation: the same arithmetic class as
+// the VALU forward, in another summation order.  One wave runs 64 pixels:
+// A = the activation rows (LDS), B = 8 consecutive inputs of one unit of one
+// part (global, L2-resident), D = 32 pixels x 32 units per MFMA; the epilogue
 // adds the bias, applies ReLU and rounds to BF16 into LDS for the next
 // layer.  The 64 -> 1 output layer is a VALU dot product.
 
-typedef short v4s_ws __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf_ws __attribute__((ext_vector_type(8)));
 typedef float v16f_ws __attribute__((ext_vector_type(16)));
 
 // Wsplit layout per layer L (inputs K_L padded to Kp_L): [part][unit][Kp_L].
@@ -969,17 +969,13 @@ __global__ __launch_bounds__(256) void k_split_weights(const float* P,
   ws[base + 2 * plane + rem] = lo;
 }
 
-__device__ __forceinline__ v4s_ws ld4(const unsigned short* p) {
-  const uint2 u = *reinterpret_cast<const uint2*>(p);
-  v4s_ws v;
-  v[0] = (short)(u.x & 0xFFFFu);
-  v[1] = (short)(u.x >> 16);
-  v[2] = (short)(u.y & 0xFFFFu);
-  v[3] = (short)(u.y >> 16);
-  return v;
+// 8 BF16 values (16 bytes, 16-byte aligned).
+__device__ __forceinline__ v8bf_ws ld8(const unsigned short* p) {
+  return __builtin_bit_cast(v8bf_ws, *reinterpret_cast<const uint4*>(p));
 }
 
-// acc[mh][nh] += act[32 mh + r][k] * W[k][32 nh + r] over NKS K-steps of 8.
+// acc[mh][nh] += act[32 mh + r][k] * W[k][32 nh + r] over NKS K-steps of 16
+// (lane half h takes k-slots 8 h .. 8 h + 7 of each step, in A and B).
 template <int NKS>
 __device__ __forceinline__ void mfma_layer(const unsigned short* act,
                                            int pitch,
@@ -990,10 +986,10 @@ __device__ __forceinline__ void mfma_layer(const unsigned short* act,
   // The weights (the same for every wave, cache-resident) arrive in
   // batches of up to 8 k-steps, every load of a batch issued before its
   // first MFMA; the MFMAs run in k-step order as before.
-  constexpr int CH = NKS <= 8 ? NKS : (NKS + 2) / 3;
+  constexpr int CH = NKS <= 4 ? NKS : (NKS + 2) / 3;
 #pragma unroll
   for (int c0 = 0; c0 < NKS; c0 += CH) {
-    v4s_ws b[CH][3][2];
+    v8bf_ws b[CH][3][2];
 #pragma unroll
     for (int u = 0; u < CH; ++u)
       if (c0 + u < NKS)
@@ -1001,8 +997,8 @@ __device__ __forceinline__ void mfma_layer(const unsigned short* act,
         for (int part = 0; part < 3; ++part)
 #pragma unroll
           for (int nh = 0; nh < 2; ++nh)
-            b[u][part][nh] = ld4(w + part * plane + (size_t)(32 * nh + r) * kp +
-                                 8 * (c0 + u) + 4 * h);
+            b[u][part][nh] = ld8(w + part * plane + (size_t)(32 * nh + r) * kp +
+                                 16 * (c0 + u) + 8 * h);
     // keep the batch together (the scheduler would otherwise move each
     // load next to its MFMA and expose its latency)
     __builtin_amdgcn_sched_barrier(0);
@@ -1010,15 +1006,15 @@ __device__ __forceinline__ void mfma_layer(const unsigned short* act,
     for (int u = 0; u < CH; ++u) {
       if (c0 + u >= NKS) break;
       const int ks = c0 + u;
-      const v4s_ws a0 = ld4(act + r * pitch + 8 * ks + 4 * h);
-      const v4s_ws a1 = ld4(act + (32 + r) * pitch + 8 * ks + 4 * h);
+      const v8bf_ws a0 = ld8(act + r * pitch + 16 * ks + 8 * h);
+      const v8bf_ws a1 = ld8(act + (32 + r) * pitch + 16 * ks + 8 * h);
 #pragma unroll
       for (int part = 0; part < 3; ++part)
 #pragma unroll
         for (int nh = 0; nh < 2; ++nh) {
-          acc[0][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          acc[0][nh] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
               a0, b[u][part][nh], acc[0][nh], 0, 0, 0);
-          acc[1][nh] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          acc[1][nh] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
               a1, b[u][part][nh], acc[1][nh], 0, 0, 0);
         }
     }
@@ -1046,8 +1042,13 @@ __device__ __forceinline__ void mfma_epilogue(const v16f_ws (&acc)[2][2],
     }
 }
 
-constexpr int kFP = 172;   // LDS pitch (BF16) of the feature rows
-constexpr int kAP = 68;    // LDS pitch (BF16) of the hidden activations
+// LDS pitches (BF16) of the feature rows and the hidden activations: whole
+// 16-byte units (the MFMA operands are 16-byte loads), at least the padded
+// input widths, rows spread over the LDS banks
+constexpr int kFP = 184;
+constexpr int kAP = 72;
+static_assert(kFP >= WS_K1P && kFP % 8 == 0 && kAP >= WS_HIDDEN &&
+              kAP % 8 == 0, "MFMA operand rows");
 
 // The network on a block's 64 feature rows staged in sf (pitch kFP, pad
 // columns zero): layers on the matrix cores, activations through LDS, the
@@ -1065,15 +1066,15 @@ __device__ __forceinline__ void fwd_mfma_rows(
       for (int nh = 0; nh < 2; ++nh) acc[mh][nh] = (v16f_ws){};
   };
   zero();
-  mfma_layer<WS_K1P / 8>(sf, kFP, wsplit, WS_K1P, acc);
+  mfma_layer<WS_K1P / 16>(sf, kFP, wsplit, WS_K1P, acc);
   mfma_epilogue(acc, P + WS_OFF_B1, sa, kAP);
   __syncthreads();
   zero();
-  mfma_layer<H / 8>(sa, kAP, wsplit + 3 * n1, H, acc);
+  mfma_layer<H / 16>(sa, kAP, wsplit + 3 * n1, H, acc);
   mfma_epilogue(acc, P + WS_OFF_B2, sf, kAP);
   __syncthreads();
   zero();
-  mfma_layer<H / 8>(sf, kAP, wsplit + 3 * n1 + 3 * n2, H, acc);
+  mfma_layer<H / 16>(sf, kAP, wsplit + 3 * n1 + 3 * n2, H, acc);
   mfma_epilogue(acc, P + WS_OFF_B3, sa, kAP);
   __syncthreads();
   const int i = r0 + lane;
@@ -1106,7 +1107,7 @@ __global__ __launch_bounds__(64) void k_fwd_mfma(
   // buffers hold whole 64-row blocks, rows past npad only feed outputs
   // that are not stored) are one contiguous, 16-byte aligned run: every
   // 16-byte load is issued before the first is used, then the dwords go
-  // to pitch-172 rows whose pad columns are zero.
+  // to pitch-kFP rows whose pad columns are zero.
   constexpr int kRowD = WS_FEAT / 2, kPitchD = kFP / 2;
   constexpr int kQ = 64 * kRowD / 4, kIt = (kQ + 63) / 64;
   static_assert(64 * kRowD % 4 == 0, "feature block is whole uint4s");
