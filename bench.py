@@ -349,7 +349,10 @@ def bench_kernels(args):
     runs: first a 1 GiB device-to-device copy (twice; the runtime's copy
     kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the byte counters), then
     `--steps` launches each of the VALU FP32, the MFMA FP32 and the FP64
-    pair kernel (4 waves per block, longest list first)."""
+    pair kernel at its three tile shapes (4 waves per block, longest list
+    first; the 16-row shape, the default, last), then once each the
+    per-geometry kernels of the headline step: the block-Jacobi build, the
+    neighbour search and the learned prediction."""
     import ctypes
     import dachshund_amd as D
     from dachshund_amd import synth
@@ -375,13 +378,29 @@ def bench_kernels(args):
                        ("k_matvec32m", D.DS_F32_MFMA)):
         wf.set_f32_kernel(kind)
         out[name] = wf.time_matvec(D.DS_PREC_F32, reps=args.steps)
-    for rows in (16, 32, 64):   # FP64 tile shapes; 64 last (the default)
+    for rows in (64, 32, 16):   # FP64 tile shapes; 16 last (the default)
         wf.set_tile(rows)
         out[f"k_matvec64_r{rows}"] = wf.time_matvec(D.DS_PREC_F64,
                                                    reps=args.steps)
-    out["k_matvec64"] = out["k_matvec64_r64"]
+    out["k_matvec64"] = out["k_matvec64_r16"]
     for k, ms in out.items():
         log(f"{k}: {ms * 1e3:.1f} us per launch")
+    # the headline step's per-geometry kernels, once each
+    from dachshund_amd.warmstart import WarmStart
+    ctx = D.default_context()
+    ctx.synchronize()
+    t = time.perf_counter()
+    wf.set_precond(D.DS_PRECOND_BLOCK)
+    ctx.synchronize()
+    t_bj = time.perf_counter() - t
+    ws = WarmStart()
+    ws.load(args.warmstart)
+    t = time.perf_counter()
+    ws.predict_into(wf)
+    ctx.synchronize()
+    t_pred = time.perf_counter() - t
+    log(f"block-Jacobi build {t_bj * 1e3:.2f} ms, neighbour search + "
+        f"prediction {t_pred * 1e3:.2f} ms")
     print(json.dumps({
         "metric": "pair kernel launch time (300,000 pixels)",
         "value": out["k_matvec32m"] * 1e3, "unit": "us per launch",
@@ -393,8 +412,10 @@ def bench_kernels(args):
         "us_per_launch": {k: v * 1e3 for k, v in out.items()},
         "pairs_per_launch": {"k_matvec32": st.pairs_per_matvec,
                              "k_matvec32m": st.sub_pairs_per_matvec,
-                             "k_matvec64": st.pairs_per_matvec,
-                             "k_matvec64_r16": st.sub_pairs_per_matvec},
+                             "k_matvec64": st.f64_pairs_per_matvec,
+                             "k_matvec64_r64": st.pairs_per_matvec},
+        "setup_s": {"block_jacobi_build": t_bj,
+                    "neighbour_search_and_prediction": t_pred},
     }), flush=True)
 
 
