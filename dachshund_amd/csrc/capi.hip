@@ -345,58 +345,56 @@ std::vector<int> xcd_order(const std::vector<int>& blocks, Cost cost) {
 void problem_free(ds_problem* p) {
   // the stream work that used them has completed (ds_problem_destroy and
   // the failed-creation path synchronize first)
-  ds::ctx_buf_put(p->ctx, p->knn_nbr, p->knn_bytes);
-  ds::ctx_buf_put(p->ctx, p->minv, p->minv_bytes);
-  p->knn_nbr = nullptr;
-  p->minv = nullptr;
-  dev_free(p->perm_dev);
-  dev_free(p->tbox_lo);
-  dev_free(p->tbox_hi);
-  dev_free(p->pos64);
-  dev_free(p->rhs64);
-  dev_free(p->diag);
-  dev_free(p->wvec);
-  dev_free(p->loc32);
-  dev_free(p->tgt32);
-  dev_free(p->tptr);
-  dev_free(p->tidx);
-  dev_free(p->tmask);
-  dev_free(p->toff32);
-  dev_free(p->table64);
-  dev_free(p->table32);
-  dev_free(p->table32g);
-  dev_free(p->table_raw);
-  dev_free(p->blk_sub);
-  dev_free(p->blk_nreal);
-  dev_free(p->blk_order);
-  dev_free(p->ord32);
-  dev_free(p->hptr);
-  dev_free(p->hrec);
-  dev_free(p->pbuf);
-  dev_free(p->ydir);
-  dev_free(p->gptr);
-  dev_free(p->sub_sym);
-  dev_free(p->gslot);
-  dev_free(p->sub_blk0);
-  dev_free(p->sub_cv);
-  dev_free(p->tcount);
-  dev_free(p->done_count);
-  dev_free(p->dense_s);
-  dev_free(p->dense_y);
-  dev_free(p->du16);
-  if (p->xu != p->x) dev_free(p->xu);
-  if (p->du != p->d) dev_free(p->du);
+  ds::pfree(p, p->knn_nbr);
+  ds::pfree(p, p->minv);
+  ds::pfree(p, p->perm_dev);
+  ds::pfree(p, p->tbox_lo);
+  ds::pfree(p, p->tbox_hi);
+  ds::pfree(p, p->pos64);
+  ds::pfree(p, p->rhs64);
+  ds::pfree(p, p->diag);
+  ds::pfree(p, p->wvec);
+  ds::pfree(p, p->loc32);
+  ds::pfree(p, p->tgt32);
+  ds::pfree(p, p->tptr);
+  ds::pfree(p, p->tidx);
+  ds::pfree(p, p->tmask);
+  ds::pfree(p, p->toff32);
+  ds::pfree(p, p->table64);
+  ds::pfree(p, p->table32);
+  ds::pfree(p, p->table32g);
+  ds::pfree(p, p->table_raw);
+  ds::pfree(p, p->blk_sub);
+  ds::pfree(p, p->blk_nreal);
+  ds::pfree(p, p->blk_order);
+  ds::pfree(p, p->ord32);
+  ds::pfree(p, p->hptr);
+  ds::pfree(p, p->hrec);
+  ds::pfree(p, p->pbuf);
+  ds::pfree(p, p->ydir);
+  ds::pfree(p, p->gptr);
+  ds::pfree(p, p->sub_sym);
+  ds::pfree(p, p->gslot);
+  ds::pfree(p, p->sub_blk0);
+  ds::pfree(p, p->sub_cv);
+  ds::pfree(p, p->tcount);
+  ds::pfree(p, p->done_count);
+  ds::pfree(p, p->dense_s);
+  ds::pfree(p, p->dense_y);
+  ds::pfree(p, p->du16);
+  if (p->xu != p->x) ds::pfree(p, p->xu);
+  if (p->du != p->d) ds::pfree(p, p->du);
   p->xu = p->du = nullptr;
-  dev_free(p->x);
-  dev_free(p->r);
-  dev_free(p->d);
-  dev_free(p->s);
-  dev_free(p->q);
-  dev_free(p->tmp);
-  dev_free(p->du32);
-  dev_free(p->partials);
-  dev_free(p->sc);
-  dev_free(p->norms);
+  ds::pfree(p, p->x);
+  ds::pfree(p, p->r);
+  ds::pfree(p, p->d);
+  ds::pfree(p, p->s);
+  ds::pfree(p, p->q);
+  ds::pfree(p, p->tmp);
+  ds::pfree(p, p->du32);
+  ds::pfree(p, p->partials);
+  ds::pfree(p, p->sc);
+  ds::pfree(p, p->norms);
   if (p->sc_host) (void)hipHostFree(p->sc_host);
   if (p->done_host) (void)hipHostFree(p->done_host);
   p->done_host = nullptr;
@@ -405,6 +403,9 @@ void problem_free(ds_problem* p) {
   if (p->host_tmp) (void)hipHostFree(p->host_tmp);
   p->sc_host = nullptr;
   p->host_tmp = nullptr;
+  // anything else taken from the pool goes back too
+  for (const auto& e : p->pooled) ds::ctx_buf_put(p->ctx, e.first, e.second);
+  p->pooled.clear();
 }
 
 // Host-order vector -> sorted device vector (pads 0).
@@ -459,36 +460,68 @@ void ctx_release(ds_ctx* c) {
   delete c;
 }
 
-hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes) {
+hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes, size_t* cap) {
   {
     std::lock_guard<std::mutex> lk(c->buf_mu);
+    const size_t hi = bytes + bytes / 8 + (size_t(64) << 10);
+    size_t best = c->buf_cache.size();
     for (size_t k = 0; k < c->buf_cache.size(); ++k) {
-      if (c->buf_cache[k].first == bytes) {
-        *p = c->buf_cache[k].second;
-        c->buf_cache.erase(c->buf_cache.begin() + (long)k);
-        c->buf_cache_bytes -= bytes;
-        return hipSuccess;
-      }
+      const size_t b = c->buf_cache[k].first;
+      if (b >= bytes && b <= hi &&
+          (best == c->buf_cache.size() || b < c->buf_cache[best].first))
+        best = k;
+    }
+    if (best < c->buf_cache.size()) {
+      *p = c->buf_cache[best].second;
+      *cap = c->buf_cache[best].first;
+      c->buf_cache.erase(c->buf_cache.begin() + (long)best);
+      c->buf_cache_bytes -= *cap;
+      return hipSuccess;
     }
   }
+  *cap = bytes;
   return hipMalloc(p, bytes);
 }
 
-void ctx_buf_put(ds_ctx* c, void* p, size_t bytes) {
+void ctx_buf_put(ds_ctx* c, void* p, size_t cap) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(c->buf_mu);
-  if (bytes > kBufCacheBytes) {
+  if (cap > kBufCacheBytes) {
     (void)hipFree(p);
     return;
   }
+  if (c->pool_poison) (void)hipMemset(p, 0xFF, cap);
   // oldest out first
-  while (!c->buf_cache.empty() && c->buf_cache_bytes + bytes > kBufCacheBytes) {
+  while (!c->buf_cache.empty() && c->buf_cache_bytes + cap > kBufCacheBytes) {
     (void)hipFree(c->buf_cache.front().second);
     c->buf_cache_bytes -= c->buf_cache.front().first;
     c->buf_cache.erase(c->buf_cache.begin());
   }
-  c->buf_cache.emplace_back(bytes, p);
-  c->buf_cache_bytes += bytes;
+  c->buf_cache.emplace_back(cap, p);
+  c->buf_cache_bytes += cap;
+}
+
+hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes) {
+  size_t cap = 0;
+  const hipError_t e = ctx_buf_get(p->ctx, ptr, bytes, &cap);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return e;
+  }
+  p->pooled.emplace_back(*ptr, cap);
+  return hipSuccess;
+}
+
+void pool_free(ds_problem* p, void* ptr) {
+  for (size_t k = 0; k < p->pooled.size(); ++k) {
+    if (p->pooled[k].first == ptr) {
+      const size_t cap = p->pooled[k].second;
+      p->pooled.erase(p->pooled.begin() + (long)k);
+      ctx_buf_put(p->ctx, ptr, cap);
+      return;
+    }
+  }
+  (void)hipFree(ptr);
 }
 
 }  // namespace ds
@@ -599,6 +632,8 @@ int ds_ctx_create(int device, ds_ctx** out) {
                             device) != hipSuccess)
     return DS_E_HIP;
   if (noalloc_setup(device, &c->noalloc) != hipSuccess) return DS_E_HIP;
+  const char* poison = std::getenv("DS_POOL_POISON");
+  c->pool_poison = poison && poison[0] == '1';
   *out = c.release();
   return DS_OK;
 }
@@ -843,27 +878,27 @@ int geometry_host(ds_problem* p, const double* pixels,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  HIP_OR_FAIL(ctx, dev_alloc(&p->pos64, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->rhs64, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->diag, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->wvec, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->loc32, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tgt32, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->perm_dev, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tptr, ptr.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tidx, idx.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tmask, msk.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->toff32, hoff.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_sub, nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_nreal, nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->hptr, nblk + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->hrec, hrec.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gslot, gslot.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->gptr, ntile + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_blk0, n_sub + 1));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tcount, ntile));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tbox_lo, tlo.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->tbox_hi, thi.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->pos64, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->rhs64, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->diag, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->wvec, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->loc32, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tgt32, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->perm_dev, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tptr, ptr.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tidx, idx.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tmask, msk.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->toff32, hoff.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->blk_sub, nblk));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->blk_nreal, nblk));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->hptr, nblk + 1));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->hrec, hrec.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->gslot, gslot.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->gptr, ntile + 1));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->sub_blk0, n_sub + 1));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tcount, ntile));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tbox_lo, tlo.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->tbox_hi, thi.size()));
   HIP_OR_FAIL(ctx, up(p->pos64, hpos.data(), sizeof(double4) * npad));
   HIP_OR_FAIL(ctx, up(p->rhs64, hrhs.data(), sizeof(double) * npad));
   HIP_OR_FAIL(ctx, up(p->diag, hdiag.data(), sizeof(double) * npad));
@@ -919,7 +954,13 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device));
   const double t0 = now_s();
   std::unique_ptr<ds_problem, void (*)(ds_problem*)> P(
-      new ds_problem, [](ds_problem* q) { problem_free(q); delete q; });
+      new ds_problem, [](ds_problem* q) {
+        // a failed creation: its enqueued work ends before the buffers go
+        // back to the context's cache
+        if (q->ctx) (void)hipStreamSynchronize(q->ctx->stream);
+        problem_free(q);
+        delete q;
+      });
   ds_problem* p = P.get();
   p->ctx = ctx;
   p->form = form;
@@ -1103,30 +1144,30 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  HIP_OR_FAIL(ctx, dev_alloc(&p->table64, t64.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->table32, t32.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->table32g, t32g.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->table_raw, p->table.size()));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->blk_order, nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->ord32, 2 * (size_t)nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->pbuf, std::max<size_t>(16 * (size_t)p->nslot,
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->table64, t64.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->table32, t32.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->table32g, t32g.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->table_raw, p->table.size()));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->blk_order, nblk));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->ord32, 2 * (size_t)nblk));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->pbuf, std::max<size_t>(16 * (size_t)p->nslot,
                                                         1)));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->ydir, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_sym, n_sub));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sub_cv, n_sub));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->done_count, 1));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->ydir, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->sub_sym, n_sub));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->sub_cv, n_sub));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->done_count, 1));
   double** vecs[] = {&p->x, &p->r, &p->d, &p->s, &p->q, &p->tmp};
-  for (double** v : vecs) HIP_OR_FAIL(ctx, dev_alloc(v, npad));
+  for (double** v : vecs) HIP_OR_FAIL(ctx, ds::palloc(p, v, npad));
   if (form == DS_FORM_W) {
-    HIP_OR_FAIL(ctx, dev_alloc(&p->xu, npad));
-    HIP_OR_FAIL(ctx, dev_alloc(&p->du, npad));
+    HIP_OR_FAIL(ctx, ds::palloc(p, &p->xu, npad));
+    HIP_OR_FAIL(ctx, ds::palloc(p, &p->du, npad));
   } else {
     p->xu = p->x;
     p->du = p->d;
   }
-  HIP_OR_FAIL(ctx, dev_alloc(&p->du32, npad));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->partials, 2 * (size_t)nblk));
-  HIP_OR_FAIL(ctx, dev_alloc(&p->sc, n_sub));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->du32, npad));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->partials, 2 * (size_t)nblk));
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->sc, n_sub));
   HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->sc_host,
                                  sizeof(ds_cg_scalars) * n_sub,
                                  hipHostMallocDefault));
@@ -1762,7 +1803,7 @@ int ds_problem_set_precond(ds_problem* p, int kind) {
     // subvolumes sharing one geometry share one set of block inverses
     p->minv_mod = p->shared_geometry ? p->nblk / p->n_sub : p->nblk;
     p->minv_bytes = sizeof(float) * (size_t)p->minv_mod * DS_BJ_SIZE;
-    HIP_OR_FAIL(ctx, ds::ctx_buf_get(ctx, (void**)&p->minv, p->minv_bytes));
+    HIP_OR_FAIL(ctx, ds::palloc(p, &p->minv, p->minv_bytes / sizeof(float)));
     if (!p->minv_ev)
       HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&p->minv_ev,
                                                hipEventDisableTiming));
@@ -1896,15 +1937,18 @@ int ds_problem_build_dense_bf16(ds_problem* p) {
     return fail(p->ctx, DS_E_NOMEM, "explicit S would need %.1f GB",
                 bytes * 1e-9);
   hipStream_t st = p->ctx->stream;
-  dev_free(p->dense_s);
-  dev_free(p->dense_y);
-  dev_free(p->du16);
+  // a rebuild: the work that read the old block has completed before its
+  // buffers go back to the context's cache
+  HIP_OR_FAIL(p->ctx, hipStreamSynchronize(st));
+  ds::pfree(p, p->dense_s);
+  ds::pfree(p, p->dense_y);
+  ds::pfree(p, p->du16);
   p->dense_ld = ld;
   p->dense_mpad = mpad;
-  HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_s, (size_t)mpad * ld));
-  HIP_OR_FAIL(p->ctx, dev_alloc(&p->dense_y,
+  HIP_OR_FAIL(p->ctx, ds::palloc(p, &p->dense_s, (size_t)mpad * ld));
+  HIP_OR_FAIL(p->ctx, ds::palloc(p, &p->dense_y,
                                 (size_t)DS_DENSE_SPLIT_MAX * DS_DENSE_R * mpad));
-  HIP_OR_FAIL(p->ctx, dev_alloc(&p->du16, (size_t)DS_DENSE_R * ld));
+  HIP_OR_FAIL(p->ctx, ds::palloc(p, &p->du16, (size_t)DS_DENSE_R * ld));
   // unused right-hand-side columns stay zero
   HIP_OR_FAIL(p->ctx, hipMemsetAsync(p->du16, 0, sizeof(unsigned short) *
                                      (size_t)DS_DENSE_R * ld, st));
@@ -1963,7 +2007,7 @@ int ds_problem_set_fused(ds_problem* p, int on) {
                   "at most %d subvolumes sharing one geometry", DS_DENSE_R);
     if (!p->du16) {
       const size_t ld = (size_t)p->npad / p->n_sub;
-      HIP_OR_FAIL(p->ctx, dev_alloc(&p->du16, (size_t)DS_DENSE_R * ld));
+      HIP_OR_FAIL(p->ctx, ds::palloc(p, &p->du16, (size_t)DS_DENSE_R * ld));
       // unused right-hand-side columns stay zero
       HIP_OR_FAIL(p->ctx, hipMemsetAsync(p->du16, 0, sizeof(unsigned short) *
                                          DS_DENSE_R * ld, p->ctx->stream));

@@ -606,10 +606,10 @@ hipError_t run_solver(ds_problem* p, const ds_pcg_params* prm, int mixed,
   const int ns = p->n_sub;
   const int cap = std::max(max_iter, 1) + 1;
   if (cap > p->norms_cap) {
-    if (p->norms) DS_TRY(hipFree(p->norms));
-    p->norms = nullptr;
+    // the previous solve has completed (run_solver drains its stream)
+    ds::pfree(p, p->norms);
     // residual norms, then deltas, cap per subvolume each
-    DS_TRY(hipMalloc(&p->norms, 2 * sizeof(double) * (size_t)cap * ns));
+    DS_TRY(ds::palloc(p, &p->norms, 2 * (size_t)cap * ns));
     p->norms_cap = cap;
     p->deltas = p->norms + (size_t)cap * ns;
   }

@@ -175,27 +175,53 @@ struct ds_ctx {
   // context is freed when the last of them lets go (so destroying it before
   // its problems, as a garbage collector may, frees nothing they still use)
   std::atomic<int> refs{1};
-  // Per-geometry buffers of destroyed problems (block inverses, neighbour
-  // lists: tens of MB, ~0.1 ms of hipMalloc each), kept for the next
-  // problem that asks for the same size; at most kBufCacheBytes in all.
+  // Device buffers of destroyed problems (records, lists, vectors, block
+  // inverses, neighbour lists), kept for the next problems created on this
+  // context; at most kBufCacheBytes in all.  Reuse saves the hipMalloc
+  // calls, and a problem of the same size gets the same memory back: the
+  // pair kernels run up to ~7 % slower on memory newly placed by hipMalloc
+  // later in a process than on the first problem's.
   std::mutex buf_mu;
   std::vector<std::pair<size_t, void*>> buf_cache;
   size_t buf_cache_bytes = 0;
+  // DS_POOL_POISON=1 at creation: buffers returned to the cache are filled
+  // with 0xFF bytes, so that code reading a buffer it did not write first
+  // sees NaNs and wild indices instead of a previous problem's values
+  int pool_poison = 0;
 };
+
+struct ds_problem;
 
 namespace ds {
 void ctx_retain(ds_ctx* c);
 void ctx_release(ds_ctx* c);
 constexpr size_t kBufCacheBytes = size_t(1) << 30;
-// A device buffer of exactly `bytes`: a cached one, else hipMalloc.
-hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes);
-// Return a buffer (the stream work that used it has completed) to the
-// cache, or free it when the cache is full.
-void ctx_buf_put(ds_ctx* c, void* p, size_t bytes);
+// A device buffer of at least `bytes`: the smallest cached one of at most
+// bytes + bytes / 8 + 64 KiB, else a new hipMalloc; *cap is its size.
+hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes, size_t* cap);
+// Return a buffer of size `cap` (the stream work that used it has
+// completed) to the cache, or free it when it does not fit.
+void ctx_buf_put(ds_ctx* c, void* p, size_t cap);
+// A problem's own device buffers: taken from its context's cache and
+// registered with the problem, which returns them when it is freed.
+hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes);
+void pool_free(ds_problem* p, void* ptr);
+template <class T>
+hipError_t palloc(ds_problem* p, T** ptr, size_t count) {
+  *ptr = nullptr;
+  return pool_alloc(p, (void**)ptr, sizeof(T) * (count ? count : 1));
+}
+template <class T>
+void pfree(ds_problem* p, T*& ptr) {
+  if (ptr) pool_free(p, (void*)ptr);
+  ptr = nullptr;
+}
 }  // namespace ds
 
 struct ds_problem {
   ds_ctx* ctx = nullptr;
+  // device buffers taken with ds::pool_alloc: pointer and size
+  std::vector<std::pair<void*, size_t>> pooled;
   int form = DS_FORM_W;
   int n = 0, nblk = 0, ntile = 0, npad = 0;
   ds::CovarDev cv{};                   // subvolume 0 (table shared by all)
@@ -233,7 +259,7 @@ struct ds_problem {
   mutable bool minv_pending = false;
   int* knn_nbr = nullptr;              // learned warm start: K nearest
                                        // rows per row, built on first use
-  size_t knn_bytes = 0, minv_bytes = 0;   // their buffers' sizes
+  size_t minv_bytes = 0;               // the inverses' bytes
   float knn_r0 = 1.0f;                 // its first ring radius
   int* sub_blk0 = nullptr;             // first block per subvolume
   double4* sub_cv = nullptr;           // {var_s, c_perp, c_para, 0}

@@ -1003,11 +1003,11 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   GTRY(up(dsub_n.p, p->sub_n.data(), sizeof(int) * n_sub));
   GTRY(dsub_off.alloc(n_sub));
   GTRY(up(dsub_off.p, hoff.data(), sizeof(int) * n_sub));
-  GTRY(galloc(&p->blk_sub, nblk));
+  GTRY(ds::palloc(p, &p->blk_sub, nblk));
   GTRY(up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
-  GTRY(galloc(&p->blk_nreal, nblk));
+  GTRY(ds::palloc(p, &p->blk_nreal, nblk));
   GTRY(up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
-  GTRY(galloc(&p->sub_blk0, n_sub + 1));
+  GTRY(ds::palloc(p, &p->sub_blk0, n_sub + 1));
   GTRY(up(p->sub_blk0, p->sub_blk0_h.data(), sizeof(int) * (n_sub + 1)));
   GTRY(drec.alloc(npad));
   GTRY(derr.alloc(1));
@@ -1079,16 +1079,16 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     GTRY(hipGetLastError());
   }
   // ---- per block ----
-  GTRY(galloc(&p->pos64, npad));
-  GTRY(galloc(&p->rhs64, npad));
-  GTRY(galloc(&p->diag, npad));
-  GTRY(galloc(&p->wvec, npad));
-  GTRY(galloc(&p->loc32, npad));
-  GTRY(galloc(&p->tgt32, npad));
-  GTRY(galloc(&p->perm_dev, npad));
-  GTRY(galloc(&p->tbox_lo, 3 * (size_t)ntile));
-  GTRY(galloc(&p->tbox_hi, 3 * (size_t)ntile));
-  GTRY(galloc(&p->tcount, ntile));
+  GTRY(ds::palloc(p, &p->pos64, npad));
+  GTRY(ds::palloc(p, &p->rhs64, npad));
+  GTRY(ds::palloc(p, &p->diag, npad));
+  GTRY(ds::palloc(p, &p->wvec, npad));
+  GTRY(ds::palloc(p, &p->loc32, npad));
+  GTRY(ds::palloc(p, &p->tgt32, npad));
+  GTRY(ds::palloc(p, &p->perm_dev, npad));
+  GTRY(ds::palloc(p, &p->tbox_lo, 3 * (size_t)ntile));
+  GTRY(ds::palloc(p, &p->tbox_hi, 3 * (size_t)ntile));
+  GTRY(ds::palloc(p, &p->tcount, ntile));
   GTRY(dcen.alloc(3 * (size_t)nblk));
   GTRY(dtcen.alloc(3 * (size_t)ntile));
   GTRY(dmsrc.alloc(n_sub));
@@ -1147,8 +1147,8 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   la.bstat = dstat.p;
   hipLaunchKernelGGL(k_kd_lists<false>, gb, dim3(256), 0, st, la);
   GTRY(hipGetLastError());
-  GTRY(galloc(&p->tptr, nblk + 1));
-  GTRY(galloc(&p->hptr, nblk + 1));
+  GTRY(ds::palloc(p, &p->tptr, nblk + 1));
+  GTRY(ds::palloc(p, &p->hptr, nblk + 1));
   hipLaunchKernelGGL(k_kd_exscan, dim3(1), dim3(1024), 0, st, dstat.p, 5,
                      nblk, p->tptr);
   hipLaunchKernelGGL(k_kd_exscan, dim3(1), dim3(1024), 0, st, dstat.p + 1, 5,
@@ -1164,9 +1164,9 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     *error = 1;   // more than INT_MAX list entries
     return hipSuccess;
   }
-  GTRY(galloc(&p->tidx, tot[0]));
-  GTRY(galloc(&p->tmask, tot[0]));
-  GTRY(galloc(&p->toff32, tot[0]));
+  GTRY(ds::palloc(p, &p->tidx, tot[0]));
+  GTRY(ds::palloc(p, &p->tmask, tot[0]));
+  GTRY(ds::palloc(p, &p->toff32, tot[0]));
   la.tptr = p->tptr;
   la.tidx = p->tidx;
   la.tmask = p->tmask;
@@ -1174,9 +1174,9 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   hipLaunchKernelGGL(k_kd_lists<true>, gb, dim3(256), 0, st, la);
   GTRY(hipGetLastError());
   // ---- half lists and column-sum slots ----
-  GTRY(galloc(&p->hrec, tot[1]));
-  GTRY(galloc(&p->gslot, tot[1]));
-  GTRY(galloc(&p->gptr, ntile + 1));
+  GTRY(ds::palloc(p, &p->hrec, tot[1]));
+  GTRY(ds::palloc(p, &p->gslot, tot[1]));
+  GTRY(ds::palloc(p, &p->gptr, ntile + 1));
   GTRY(dgcnt.alloc(ntile));
   HalfArgs ha{};
   ha.tptr = p->tptr;

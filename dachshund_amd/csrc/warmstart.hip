@@ -1348,7 +1348,7 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     const size_t nb = sizeof(int) * (size_t)p->npad * WS_K;
     const size_t nbox = 2 * sizeof(float4) * p->ntile;
     const size_t bytes = nb + nbox + sizeof(int) * (p->ntile + 1);
-    hipError_t e = ds::ctx_buf_get(p->ctx, (void**)&nbr, bytes);
+    hipError_t e = ds::pool_alloc(p, (void**)&nbr, bytes);
     if (e != hipSuccess) return e;
     a.nbr = nbr;
     float4* tbox = reinterpret_cast<float4*>(reinterpret_cast<char*>(nbr) + nb);
@@ -1367,11 +1367,10 @@ hipError_t knn_args(ds_problem* p, unsigned short* feat, hipStream_t st,
     a.fb = nullptr;
     e = hipGetLastError();
     if (e != hipSuccess) {
-      (void)hipFree(nbr);
+      ds::pfree(p, nbr);
       return e;
     }
     p->knn_nbr = nbr;
-    p->knn_bytes = bytes;
   }
   a.nbr = p->knn_nbr;
   return hipSuccess;
