@@ -395,17 +395,12 @@ void problem_free(ds_problem* p) {
   ds::pfree(p, p->partials);
   ds::pfree(p, p->sc);
   ds::pfree(p, p->norms);
-  if (p->sc_host) (void)hipHostFree(p->sc_host);
-  if (p->done_host) (void)hipHostFree(p->done_host);
-  p->done_host = nullptr;
-  if (p->norm_host) (void)hipHostFree(p->norm_host);
-  p->norm_host = nullptr;
-  if (p->host_tmp) (void)hipHostFree(p->host_tmp);
-  p->sc_host = nullptr;
-  p->host_tmp = nullptr;
+  ds::pfree(p, p->sc_host);
+  ds::pfree(p, p->done_host);
+  ds::pfree(p, p->norm_host);
+  ds::pfree(p, p->host_tmp);
   // anything else taken from the pool goes back too
-  for (const auto& e : p->pooled) ds::ctx_buf_put(p->ctx, e.first, e.second);
-  p->pooled.clear();
+  while (!p->pooled.empty()) ds::pool_free(p, p->pooled.back().ptr);
 }
 
 // Host-order vector -> sorted device vector (pads 0).
@@ -455,29 +450,41 @@ void ctx_release(ds_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto& e : c->buf_cache) (void)hipFree(e.second);
   c->buf_cache.clear();
+  for (auto& e : c->host_cache) (void)hipHostFree(e.second);
+  c->host_cache.clear();
   if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
+namespace {
+
+// Smallest cached buffer of at least `bytes` and at most bytes + bytes / 8
+// + 64 KiB, taken out of the cache; false when there is none.
+bool cache_take(std::vector<std::pair<size_t, void*>>& cache, size_t& held,
+                size_t bytes, void** p, size_t* cap) {
+  const size_t hi = bytes + bytes / 8 + (size_t(64) << 10);
+  size_t best = cache.size();
+  for (size_t k = 0; k < cache.size(); ++k) {
+    const size_t b = cache[k].first;
+    if (b >= bytes && b <= hi && (best == cache.size() || b < cache[best].first))
+      best = k;
+  }
+  if (best == cache.size()) return false;
+  *p = cache[best].second;
+  *cap = cache[best].first;
+  cache.erase(cache.begin() + (long)best);
+  held -= *cap;
+  return true;
+}
+
+}  // namespace
+
 hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes, size_t* cap) {
   {
     std::lock_guard<std::mutex> lk(c->buf_mu);
-    const size_t hi = bytes + bytes / 8 + (size_t(64) << 10);
-    size_t best = c->buf_cache.size();
-    for (size_t k = 0; k < c->buf_cache.size(); ++k) {
-      const size_t b = c->buf_cache[k].first;
-      if (b >= bytes && b <= hi &&
-          (best == c->buf_cache.size() || b < c->buf_cache[best].first))
-        best = k;
-    }
-    if (best < c->buf_cache.size()) {
-      *p = c->buf_cache[best].second;
-      *cap = c->buf_cache[best].first;
-      c->buf_cache.erase(c->buf_cache.begin() + (long)best);
-      c->buf_cache_bytes -= *cap;
+    if (cache_take(c->buf_cache, c->buf_cache_bytes, bytes, p, cap))
       return hipSuccess;
-    }
   }
   *cap = bytes;
   return hipMalloc(p, bytes);
@@ -501,23 +508,56 @@ void ctx_buf_put(ds_ctx* c, void* p, size_t cap) {
   c->buf_cache_bytes += cap;
 }
 
-hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes) {
+namespace {
+
+hipError_t host_get(ds_ctx* c, void** p, size_t bytes, size_t* cap) {
+  {
+    std::lock_guard<std::mutex> lk(c->buf_mu);
+    if (cache_take(c->host_cache, c->host_cache_bytes, bytes, p, cap))
+      return hipSuccess;
+  }
+  *cap = bytes;
+  return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+
+void host_put(ds_ctx* c, void* p, size_t cap) {
+  std::lock_guard<std::mutex> lk(c->buf_mu);
+  if (cap > kHostCacheBytes) {
+    (void)hipHostFree(p);
+    return;
+  }
+  if (c->pool_poison) std::memset(p, 0xFF, cap);
+  while (!c->host_cache.empty() &&
+         c->host_cache_bytes + cap > kHostCacheBytes) {
+    (void)hipHostFree(c->host_cache.front().second);
+    c->host_cache_bytes -= c->host_cache.front().first;
+    c->host_cache.erase(c->host_cache.begin());
+  }
+  c->host_cache.emplace_back(cap, p);
+  c->host_cache_bytes += cap;
+}
+
+}  // namespace
+
+hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes, bool host) {
   size_t cap = 0;
-  const hipError_t e = ctx_buf_get(p->ctx, ptr, bytes, &cap);
+  const hipError_t e = host ? host_get(p->ctx, ptr, bytes, &cap)
+                            : ctx_buf_get(p->ctx, ptr, bytes, &cap);
   if (e != hipSuccess) {
     *ptr = nullptr;
     return e;
   }
-  p->pooled.emplace_back(*ptr, cap);
+  p->pooled.push_back({*ptr, cap, host});
   return hipSuccess;
 }
 
 void pool_free(ds_problem* p, void* ptr) {
   for (size_t k = 0; k < p->pooled.size(); ++k) {
-    if (p->pooled[k].first == ptr) {
-      const size_t cap = p->pooled[k].second;
+    if (p->pooled[k].ptr == ptr) {
+      const ds_problem::Pooled e = p->pooled[k];
       p->pooled.erase(p->pooled.begin() + (long)k);
-      ctx_buf_put(p->ctx, ptr, cap);
+      if (e.host) host_put(p->ctx, ptr, e.cap);
+      else ctx_buf_put(p->ctx, ptr, e.cap);
       return;
     }
   }
@@ -1168,15 +1208,14 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   HIP_OR_FAIL(ctx, ds::palloc(p, &p->du32, npad));
   HIP_OR_FAIL(ctx, ds::palloc(p, &p->partials, 2 * (size_t)nblk));
   HIP_OR_FAIL(ctx, ds::palloc(p, &p->sc, n_sub));
-  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->sc_host,
-                                 sizeof(ds_cg_scalars) * n_sub,
-                                 hipHostMallocDefault));
-  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->done_host, 2 * sizeof(int),
-                                 hipHostMallocDefault));
-  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->norm_host, 2 * sizeof(double),
-                                 hipHostMallocDefault));
-  HIP_OR_FAIL(ctx, hipHostMalloc((void**)&p->host_tmp, sizeof(double) * npad,
-                                 hipHostMallocDefault));
+  HIP_OR_FAIL(ctx, ds::pool_alloc(p, (void**)&p->sc_host,
+                                  sizeof(ds_cg_scalars) * n_sub, true));
+  HIP_OR_FAIL(ctx, ds::pool_alloc(p, (void**)&p->done_host, 2 * sizeof(int),
+                                  true));
+  HIP_OR_FAIL(ctx, ds::pool_alloc(p, (void**)&p->norm_host,
+                                  2 * sizeof(double), true));
+  HIP_OR_FAIL(ctx, ds::pool_alloc(p, (void**)&p->host_tmp,
+                                  sizeof(double) * npad, true));
   std::memset(p->sc_host, 0, sizeof(ds_cg_scalars) * n_sub);
   HIP_OR_FAIL(ctx, up(p->table64, t64.data(), sizeof(double2) * t64.size()));
   HIP_OR_FAIL(ctx, up(p->table32, t32.data(), sizeof(float2) * t32.size()));

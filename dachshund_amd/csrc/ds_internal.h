@@ -184,6 +184,10 @@ struct ds_ctx {
   std::mutex buf_mu;
   std::vector<std::pair<size_t, void*>> buf_cache;
   size_t buf_cache_bytes = 0;
+  // the same for pinned host buffers (hipHostMalloc pins pages: ~1 ms for
+  // the staging buffer of a 300k-pixel problem); at most kHostCacheBytes
+  std::vector<std::pair<size_t, void*>> host_cache;
+  size_t host_cache_bytes = 0;
   // DS_POOL_POISON=1 at creation: buffers returned to the cache are filled
   // with 0xFF bytes, so that code reading a buffer it did not write first
   // sees NaNs and wild indices instead of a previous problem's values
@@ -202,9 +206,12 @@ hipError_t ctx_buf_get(ds_ctx* c, void** p, size_t bytes, size_t* cap);
 // Return a buffer of size `cap` (the stream work that used it has
 // completed) to the cache, or free it when it does not fit.
 void ctx_buf_put(ds_ctx* c, void* p, size_t cap);
-// A problem's own device buffers: taken from its context's cache and
-// registered with the problem, which returns them when it is freed.
-hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes);
+constexpr size_t kHostCacheBytes = size_t(256) << 20;
+// A problem's own device buffers (host: pinned host buffers): taken from
+// its context's cache and registered with the problem, which returns them
+// when it is freed.
+hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes,
+                      bool host = false);
 void pool_free(ds_problem* p, void* ptr);
 template <class T>
 hipError_t palloc(ds_problem* p, T** ptr, size_t count) {
@@ -220,8 +227,13 @@ void pfree(ds_problem* p, T*& ptr) {
 
 struct ds_problem {
   ds_ctx* ctx = nullptr;
-  // device buffers taken with ds::pool_alloc: pointer and size
-  std::vector<std::pair<void*, size_t>> pooled;
+  // buffers taken with ds::pool_alloc: pointer, size, pinned host memory
+  struct Pooled {
+    void* ptr;
+    size_t cap;
+    bool host;
+  };
+  std::vector<Pooled> pooled;
   int form = DS_FORM_W;
   int n = 0, nblk = 0, ntile = 0, npad = 0;
   ds::CovarDev cv{};                   // subvolume 0 (table shared by all)

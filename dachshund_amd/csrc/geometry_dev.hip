@@ -876,19 +876,28 @@ __global__ __launch_bounds__(256) void k_kd_gather(HalfArgs a) {
   }
 }
 
-template <class T>
-hipError_t galloc(T** p, size_t count) {
-  *p = nullptr;
-  return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
-}
-
+// A temporary of the builder, taken from the context's buffer cache and
+// returned to it (the stream has been synchronized by then: see SyncGuard).
 template <class T>
 struct Tmp {
   T* p = nullptr;
-  hipError_t alloc(size_t n) { return galloc(&p, n); }
-  ~Tmp() {
-    if (p) (void)hipFree((void*)p);
+  ds_ctx* c = nullptr;
+  size_t cap = 0;
+  hipError_t alloc(ds_ctx* ctx, size_t n) {
+    c = ctx;
+    return ds::ctx_buf_get(c, (void**)&p, sizeof(T) * std::max<size_t>(n, 1),
+                           &cap);
   }
+  ~Tmp() {
+    if (p) ds::ctx_buf_put(c, (void*)p, cap);
+  }
+};
+
+// Waits for the stream when the builder returns, before the temporaries
+// (declared earlier, so destroyed later) go back to the cache.
+struct SyncGuard {
+  hipStream_t st;
+  ~SyncGuard() { (void)hipStreamSynchronize(st); }
 };
 
 #define GTRY(x)                                   \
@@ -987,21 +996,22 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   Tmp<unsigned> dnq, dmsrc;
   Tmp<unsigned long long> dpre;
   Tmp<int4> dwave;
+  SyncGuard guard{st};
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  GTRY(dpix.alloc(5 * (size_t)n));
+  GTRY(dpix.alloc(p->ctx, 5 * (size_t)n));
   GTRY(up(dpix.p, pixels, sizeof(double) * 5 * (size_t)n));
-  GTRY(dq.alloc(hq.size()));
+  GTRY(dq.alloc(p->ctx, hq.size()));
   GTRY(up(dq.p, hq.data(), sizeof(double) * hq.size()));
-  GTRY(ds3.alloc(hs3.size()));
+  GTRY(ds3.alloc(p->ctx, hs3.size()));
   GTRY(up(ds3.p, hs3.data(), sizeof(double) * hs3.size()));
-  GTRY(dcv.alloc(n_sub));
+  GTRY(dcv.alloc(p->ctx, n_sub));
   GTRY(up(dcv.p, hcv.data(), sizeof(double4) * n_sub));
-  GTRY(dsub_n.alloc(n_sub));
+  GTRY(dsub_n.alloc(p->ctx, n_sub));
   GTRY(up(dsub_n.p, p->sub_n.data(), sizeof(int) * n_sub));
-  GTRY(dsub_off.alloc(n_sub));
+  GTRY(dsub_off.alloc(p->ctx, n_sub));
   GTRY(up(dsub_off.p, hoff.data(), sizeof(int) * n_sub));
   GTRY(ds::palloc(p, &p->blk_sub, nblk));
   GTRY(up(p->blk_sub, hblk_sub.data(), sizeof(int) * nblk));
@@ -1009,21 +1019,21 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   GTRY(up(p->blk_nreal, hnreal.data(), sizeof(int) * nblk));
   GTRY(ds::palloc(p, &p->sub_blk0, n_sub + 1));
   GTRY(up(p->sub_blk0, p->sub_blk0_h.data(), sizeof(int) * (n_sub + 1)));
-  GTRY(drec.alloc(npad));
-  GTRY(derr.alloc(1));
+  GTRY(drec.alloc(p->ctx, npad));
+  GTRY(derr.alloc(p->ctx, 1));
   GTRY(hipMemsetAsync(derr.p, 0, sizeof(int), st));
   if (tot_nn) {
-    GTRY(dtmp.alloc(npad));
-    GTRY(dnodes.alloc(tot_nn));
+    GTRY(dtmp.alloc(p->ctx, npad));
+    GTRY(dnodes.alloc(p->ctx, tot_nn));
     GTRY(up(dnodes.p, all_nodes.data(), sizeof(BigNode) * tot_nn));
-    GTRY(dchunks.alloc(tot_nc));
+    GTRY(dchunks.alloc(p->ctx, tot_nc));
     GTRY(up(dchunks.p, all_chunks.data(), sizeof(Chunk) * tot_nc));
-    GTRY(dnq.alloc(6 * max_nn));
-    GTRY(dpre.alloc(max_nn));
-    GTRY(drank.alloc(max_nn));
-    GTRY(dhist.alloc(256 * max_nn));
-    GTRY(dcc.alloc(max_nc));
-    GTRY(dco.alloc(max_nc));
+    GTRY(dnq.alloc(p->ctx, 6 * max_nn));
+    GTRY(dpre.alloc(p->ctx, max_nn));
+    GTRY(drank.alloc(p->ctx, max_nn));
+    GTRY(dhist.alloc(p->ctx, 256 * max_nn));
+    GTRY(dcc.alloc(p->ctx, max_nc));
+    GTRY(dco.alloc(p->ctx, max_nc));
   }
   KdArgs kd{};
   kd.pix = dpix.p;
@@ -1072,7 +1082,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   }
   // ---- the rest of each tree, one wavefront per node ----
   if (!wave_nodes.empty()) {
-    GTRY(dwave.alloc(wave_nodes.size()));
+    GTRY(dwave.alloc(p->ctx, wave_nodes.size()));
     GTRY(up(dwave.p, wave_nodes.data(), sizeof(int4) * wave_nodes.size()));
     hipLaunchKernelGGL(k_kd_wave, dim3((unsigned)wave_nodes.size()), dim3(64),
                        0, st, kd, (const int4*)dwave.p);
@@ -1089,9 +1099,9 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   GTRY(ds::palloc(p, &p->tbox_lo, 3 * (size_t)ntile));
   GTRY(ds::palloc(p, &p->tbox_hi, 3 * (size_t)ntile));
   GTRY(ds::palloc(p, &p->tcount, ntile));
-  GTRY(dcen.alloc(3 * (size_t)nblk));
-  GTRY(dtcen.alloc(3 * (size_t)ntile));
-  GTRY(dmsrc.alloc(n_sub));
+  GTRY(dcen.alloc(p->ctx, 3 * (size_t)nblk));
+  GTRY(dtcen.alloc(p->ctx, 3 * (size_t)ntile));
+  GTRY(dmsrc.alloc(p->ctx, n_sub));
   GTRY(hipMemsetAsync(dmsrc.p, 0, sizeof(unsigned) * n_sub, st));
   BlkArgs ba{};
   ba.rec = drec.p;
@@ -1121,14 +1131,14 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   hipLaunchKernelGGL(k_kd_blocks, gb, dim3(256), 0, st, ba, nblk);
   GTRY(hipGetLastError());
   const int ngroup = (ntile + 63) / 64;
-  GTRY(dglo.alloc(3 * (size_t)ngroup));
-  GTRY(dghi.alloc(3 * (size_t)ngroup));
+  GTRY(dglo.alloc(p->ctx, 3 * (size_t)ngroup));
+  GTRY(dghi.alloc(p->ctx, 3 * (size_t)ngroup));
   hipLaunchKernelGGL(k_kd_groups, dim3((ngroup + 3) / 4), dim3(256), 0, st,
                      p->tbox_lo, p->tbox_hi, p->tcount, ntile, ngroup, dglo.p,
                      dghi.p);
   GTRY(hipGetLastError());
   // ---- culled lists: count, scan, fill ----
-  GTRY(dstat.alloc(5 * (size_t)nblk));
+  GTRY(dstat.alloc(p->ctx, 5 * (size_t)nblk));
   ListArgs la{};
   la.tlo = p->tbox_lo;
   la.thi = p->tbox_hi;
@@ -1177,7 +1187,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   GTRY(ds::palloc(p, &p->hrec, tot[1]));
   GTRY(ds::palloc(p, &p->gslot, tot[1]));
   GTRY(ds::palloc(p, &p->gptr, ntile + 1));
-  GTRY(dgcnt.alloc(ntile));
+  GTRY(dgcnt.alloc(p->ctx, ntile));
   HalfArgs ha{};
   ha.tptr = p->tptr;
   ha.tidx = p->tidx;
