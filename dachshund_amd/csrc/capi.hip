@@ -1206,7 +1206,9 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     p->du = p->d;
   }
   HIP_OR_FAIL(ctx, ds::palloc(p, &p->du32, npad));
-  HIP_OR_FAIL(ctx, ds::palloc(p, &p->partials, 2 * (size_t)nblk));
+  // two partial vectors, then the d . q partials of a single volume's
+  // mixed iterations (summed by the update kernel)
+  HIP_OR_FAIL(ctx, ds::palloc(p, &p->partials, 3 * (size_t)nblk));
   HIP_OR_FAIL(ctx, ds::palloc(p, &p->sc, n_sub));
   HIP_OR_FAIL(ctx, ds::pool_alloc(p, (void**)&p->sc_host,
                                   sizeof(ds_cg_scalars) * n_sub, true));

@@ -12,6 +12,7 @@
 //                (every step_r-th: x-update | residual64+precond+dots+stop)
 //   mixed:       matvec32+dq/alpha | update(+dots, refine test) |
 //                residual64 if requested (true residual) | dir
+//                (a single volume: alpha formed inside the update kernel)
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -50,6 +51,10 @@ struct UpdArgs {
   SymRows sym;
   const int* sub_sym;
   const double4* sub_cv;
+  // FA (single volume): the pair kernel's d . q partials, one per block,
+  // summed here by every workgroup instead of by a finalize kernel
+  const double* dq_part;
+  ds_cg_scalars* sc_w;
 };
 
 // q_i of row `lane` of block I, subvolume sv (see UpdArgs::sym).
@@ -69,13 +74,50 @@ __device__ __forceinline__ double update_q(const UpdArgs& a, int I, int sv,
 // its dependent loads (one wave per block, a few waves per SIMD), so every
 // load that depends only on the row is issued first, the block's inverse
 // in one batch, and the register budget keeps every block resident.
-template <int BJ, bool W, bool SYM>
+// FA (a single volume): alpha = delta / (d . q) is formed here (lib/pcg.h:
+// 35-37).  Every workgroup sums the nblk partials itself, in k_finalize<1>'s
+// order at 256 threads (the order a batch's finalize uses, so a volume
+// solved alone still equals its solve in a batch), and the first
+// workgroup records dq and alpha as k_finalize<1> does.  No other
+// workgroup reads those scalars in this kernel.
+template <int BJ, bool W, bool SYM, bool FA>
 __global__ __launch_bounds__(64 * kVecWaves)
 __attribute__((amdgpu_waves_per_eu(BJ ? 5 : 1))) void k_cg_update(
     UpdArgs a) {
   // wave-uniform: the per-block scalars come by scalar loads
   const int I = __builtin_amdgcn_readfirstlane(blockIdx.x * kVecWaves +
                                                (threadIdx.x >> 6));
+  double alpha_fa = 0.0;
+  if (FA) {
+    __shared__ double red[kVecWaves];
+    // gather_partials<1>'s sums (each thread adds its partials in index
+    // order), with fewer loads in flight to stay inside this kernel's
+    // register budget
+    double v[1] = {0.0};
+    {
+      const double* pk = a.dq_part;
+      const int n = a.nblk;
+      int b = threadIdx.x;
+      for (; b + 3 * kVecThreads < n; b += 4 * kVecThreads) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = pk[b + u * kVecThreads];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[0] += t[u];
+      }
+      for (; b < n; b += kVecThreads) v[0] += pk[b];
+    }
+    block_sum<1>(v, red);
+    const double delta = a.sc->delta;
+    // uniform: held in scalar registers across the row work
+    const double al = delta / v[0];
+    const long long bits = __double_as_longlong(al);
+    alpha_fa = __longlong_as_double(
+        ((long long)__builtin_amdgcn_readfirstlane((int)(bits >> 32)) << 32) |
+        (unsigned)__builtin_amdgcn_readfirstlane((int)bits));
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !a.sc->done)
+      finalize_alpha(a.sc_w, v[0]);
+  }
   if (I >= a.nblk) return;
   const int lane = threadIdx.x & 63;
   const int i = I * DS_BS + lane;
@@ -106,7 +148,7 @@ __attribute__((amdgpu_waves_per_eu(BJ ? 5 : 1))) void k_cg_update(
   const double var_s = SYM ? a.sub_cv[sv].x : 0.0;
   const int nreal = SYM ? a.sym.blk_nreal[I] : 0;
   const bool live = !sc->done;
-  const double alpha = sc->alpha;
+  const double alpha = FA ? alpha_fa : sc->alpha;
   const double xi = x0 + alpha * di;
   a.x[i] = live ? xi : x0;
   if (W) a.xu[i] = wi * (live ? xi : x0);
@@ -447,7 +489,7 @@ namespace {
 // sym: the FP32 MFMA operator produced q's sums only (its symmetric form
 // on the subvolumes flagged in sub_sym); the update kernel forms q there.
 hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
-                         hipStream_t st, int sym = 0) {
+                         hipStream_t st, int sym = 0, int fold_alpha = 0) {
   UpdArgs a{};
   a.d = p->d;
   a.q = p->q;
@@ -470,6 +512,10 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
     a.sub_sym = p->sub_sym;
     a.sub_cv = p->sub_cv;
   }
+  if (fold_alpha) {
+    a.dq_part = p->partials + 2 * (size_t)p->nblk;
+    a.sc_w = p->sc;
+  }
   const int nblk1 = p->nblk / p->n_sub;
   if (a.minv && p->shared_geometry && p->n_sub > 1 && p->minv_mod == nblk1) {
     if (mixed)
@@ -484,8 +530,13 @@ hipError_t launch_update(ds_problem* p, int only_x, int mixed, double drop,
     // 2 FP32 broadcast of r (mixed solver)
     const int bj = (a.minv && !only_x) ? (mixed ? 2 : 1) : 0;
     const bool w = a.wvec, sy = a.sub_sym;
-#define DS_UPD(B, W_, S) \
-  hipLaunchKernelGGL((k_cg_update<B, W_, S>), g, b, 0, st, a)
+#define DS_UPD(B, W_, S)                                             \
+  do {                                                               \
+    if (fold_alpha)                                                  \
+      hipLaunchKernelGGL((k_cg_update<B, W_, S, true>), g, b, 0, st, a); \
+    else                                                             \
+      hipLaunchKernelGGL((k_cg_update<B, W_, S, false>), g, b, 0, st, a); \
+  } while (0)
 #define DS_UPD_B(B)                                        \
   do {                                                     \
     if (w) { if (sy) DS_UPD(B, true, true); else DS_UPD(B, true, false); } \
@@ -550,11 +601,13 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
       DS_TRY(dense_gemm(p, st));
       DS_TRY(dense_epilogue(p, st));
     } else {
-      DS_TRY(launch_matvec32_ex(p, p->du32, p->d, p->q, 1, st));
+      DS_TRY(launch_matvec32_ex(p, p->du32, p->d, p->q, 1, st,
+                                p->n_sub == 1));
     }
     const int sym = !p->use_fused && !p->use_dense &&
                     p->f32_kind != DS_F32_VALU && p->n32_fast > 0;
-    DS_TRY(launch_update(p, 0, 1, drop, st, sym));
+    DS_TRY(launch_update(p, 0, 1, drop, st, sym,
+                         !p->use_fused && !p->use_dense && p->n_sub == 1));
     CgEpilogue tr;
     tr.kind = EK_CG_RES;
     tr.out2 = p->s;

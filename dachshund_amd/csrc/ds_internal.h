@@ -336,7 +336,8 @@ struct ds_problem {
   double* wvec = nullptr;              // w (W form), 0 on pad slots
   double *du = nullptr, *xu = nullptr; // w .* d, w .* x (W form)
   float* du32 = nullptr;               // FP32 copy of du
-  double* partials = nullptr;          // 2 * nblk (two partial vectors)
+  double* partials = nullptr;          // 3 * nblk: two partial vectors,
+                                       // then a single volume's d . q ones
   ds_cg_scalars* sc = nullptr;         // n_sub entries
   double* norms = nullptr;             // residual histories, n_sub x cap
   double* deltas = nullptr;            // delta histories (after norms)
@@ -436,10 +437,12 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
 // FP32 A in.  With cg, in = the CG direction and the d . q partials are
 // stored for launch_finalize; the symmetric MFMA kernel then leaves `out`
 // of its subvolumes (sub_sym) to the update kernel, which forms each row
-// from ydir and the column-sum slots (launch_update with sym).
+// from ydir and the column-sum slots (launch_update with sym).  With
+// fold_alpha (a single volume) the partials go to the third nblk of
+// `partials` and no finalize is launched: the update kernel sums them.
 hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                               const double* in, double* out, int cg,
-                              hipStream_t st);
+                              hipStream_t st, int fold_alpha = 0);
 // Sum the per-block partials of the kernel just launched and finalize the
 // CG scalars of every subvolume: EK_CG_DQ -> alpha; EK_CG_RES -> delta,
 // beta, |r|, stop rule (init / mixed / is_true / need_want as in e).

@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void k_sym_epi(Matvec32Args a, int nb) {
 
 hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
                               const double* in, double* out, int cg,
-                              hipStream_t st) {
+                              hipStream_t st, int fold_alpha) {
   Matvec32Args a{};
   a.loc = p->loc32;
   a.tgt = p->tgt32;
@@ -438,7 +438,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
   a.blk_nreal = p->blk_nreal;
   a.sub_blk0 = p->sub_blk0;
   a.sub_cv = p->sub_cv;
-  a.partials = p->partials;
+  a.partials = p->partials + (fold_alpha ? 2 * (size_t)p->nblk : 0);
   a.sc = p->sc;
   a.tlast = (float)(p->cv.table_n - 1);
   a.nblk = p->nblk;
@@ -480,7 +480,7 @@ hipError_t launch_matvec32_ex(ds_problem* p, const float* u32,
 #undef DS_L32M
 #undef DS_L32
   const hipError_t err = hipGetLastError();
-  if (err != hipSuccess || !cg) return err;
+  if (err != hipSuccess || !cg || fold_alpha) return err;
   CgEpilogue dq;
   dq.kind = EK_CG_DQ;
   return launch_finalize(p, dq, st);
