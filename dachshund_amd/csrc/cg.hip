@@ -280,24 +280,46 @@ __global__ __launch_bounds__(64 * kVecWaves) void k_cg_update_shared(
     mt[k] = src[k];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int sv = wave; sv < n_sub; sv += kVecWaves) {
-    const ds_cg_scalars* sc = a.sc + sv;
-    if (sc->done) continue;
-    const double alpha = sc->alpha;
-    const int I = sv * nblk1 + J;
-    const int i = I * DS_BS + lane;
-    const double xi = a.x[i] + alpha * a.d[i];
-    a.x[i] = xi;
-    if (a.wvec) a.xu[i] = a.wvec[i] * xi;
-    if (a.only_x) continue;
-    const double ri = a.r[i] - alpha * update_q(a, I, sv, lane, i);
-    a.r[i] = ri;
-    const double si = (R32 ? bj_apply_r32(mt, ri) : bj_apply(mt, ri));
-    a.s[i] = si;
-    const double v0 = wave_sum(ri * si), v1 = wave_sum(ri * ri);
-    if (lane == 0) {
-      a.partials[I] = v0;
-      a.partials[(size_t)a.nblk + I] = v1;
+  // Two subvolumes per pass: both rows' loads go out before either
+  // subvolume's scalars are known (a chain of dependent loads), so their
+  // latencies overlap.  As in k_cg_update, a finished subvolume's rows store
+  // x, xu and r back unchanged (its s and partials are read by nobody).
+  constexpr int kP = 2;
+  for (int sv0 = wave; sv0 < n_sub; sv0 += kP * kVecWaves) {
+    double xv[kP], dv[kP], wv[kP], qv[kP], rv[kP];
+#pragma unroll
+    for (int h = 0; h < kP; ++h) {
+      const int sv = min(sv0 + h * kVecWaves, n_sub - 1);
+      const int I = sv * nblk1 + J;
+      const int i = I * DS_BS + lane;
+      xv[h] = a.x[i];
+      dv[h] = a.d[i];
+      wv[h] = a.wvec ? a.wvec[i] : 0.0;
+      qv[h] = a.only_x ? 0.0 : update_q(a, I, sv, lane, i);
+      rv[h] = a.only_x ? 0.0 : a.r[i];
+    }
+#pragma unroll
+    for (int h = 0; h < kP; ++h) {
+      const int sv = sv0 + h * kVecWaves;
+      if (sv >= n_sub) break;   // uniform
+      const ds_cg_scalars* sc = a.sc + sv;
+      const bool live = !sc->done;
+      const double alpha = sc->alpha;
+      const int I = sv * nblk1 + J;
+      const int i = I * DS_BS + lane;
+      const double xi = live ? xv[h] + alpha * dv[h] : xv[h];
+      a.x[i] = xi;
+      if (a.wvec) a.xu[i] = wv[h] * xi;
+      if (a.only_x) continue;
+      const double ri = live ? rv[h] - alpha * qv[h] : rv[h];
+      a.r[i] = ri;
+      const double si = (R32 ? bj_apply_r32(mt, ri) : bj_apply(mt, ri));
+      a.s[i] = si;
+      const double v0 = wave_sum(ri * si), v1 = wave_sum(ri * ri);
+      if (lane == 0) {
+        a.partials[I] = v0;
+        a.partials[(size_t)a.nblk + I] = v1;
+      }
     }
   }
 }
@@ -618,7 +640,7 @@ hipError_t enqueue_iteration(ds_problem* p, int iter, int step_r, int mixed,
     if (p->use_fused || p->use_dense) {
       // shared geometry: one pass for every subvolume's true residual
       DS_TRY(launch_res64_mrhs(p, p->x, p->form == DS_FORM_W ? p->xu : p->x,
-                               p->r, 0, st));
+                               p->r, 0, st, 1));
       DS_TRY(launch_res_precond(p, p->s, 0, st));
       DS_TRY(launch_finalize(p, tr, st));
     } else {

@@ -315,10 +315,12 @@ __device__ __forceinline__ void finalize_step(ds_cg_scalars* sc, double* norms,
       return;
     }
   } else {
+    // kept for a true residual that replaces this step's residual, asked
+    // for here or by another subvolume of a shared-geometry batch
+    sc->saved_delta_old = delta_old;
     if (norm_r <= sc->tol_norm_b || norm_r <= drop * sc->ref_norm ||
         iter - sc->last_true >= sc->refresh_every) {
       sc->want_true = 1;
-      sc->saved_delta_old = delta_old;
       return;
     }
   }

@@ -380,10 +380,12 @@ hipError_t launch_precond_p(const ds_problem* p, const double* b, double* x,
                             hipStream_t st);
 // Shared-geometry batches: the true residual of every subvolume that wants
 // one (one pass over the tile lists for all right-hand sides), then its
-// preconditioned residual and dot-product partials.
+// preconditioned residual and dot-product partials.  sync (the mixed
+// solver's refinement): once any subvolume wants one, every live subvolume
+// is marked and gets its true residual in the same pass.
 hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
                              const double* u, double* out, int all,
-                             hipStream_t st);
+                             hipStream_t st, int sync = 0);
 // s = M^-1 r and the r.s, r.r partials of the subvolumes that asked for a
 // true residual (all: every live one); with q, r = rhs - q is formed first
 // and stored to r.

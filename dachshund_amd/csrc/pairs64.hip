@@ -254,6 +254,11 @@ struct Res64Args {
   double var_s, c_perp, c_para, dx_inv, tlast;
   int nblk1, ld, n_sub, table_n;
   int all;                // every live subvolume (initial residual)
+  // mixed solver's refinement: once any subvolume asks for a true
+  // residual, every live one gets it (the pass evaluates all right-hand
+  // sides anyway), so their refinements stay in step
+  int sync;
+  ds_cg_scalars* sc_w;
 };
 
 template <int FORM>
@@ -263,10 +268,20 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
   extern __shared__ double2 tab[];
   __shared__ double us[4][DS_TS][kMR];
   __shared__ double red[3][64];
-  bool want = false;
-  for (int r = 0; r < a.n_sub; ++r)
-    want = want || (!a.sc[r].done && (a.all || a.sc[r].want_true));
-  if (!want) return;   // uniform: no subvolume asked for a true residual
+  // uniform: no subvolume asked for a true residual.  Every subvolume's
+  // flags are loaded in one batch (a short-circuit test would wait for
+  // each load in turn: ~16 us for 32 subvolumes).
+  int want = 0;
+#pragma unroll
+  for (int r = 0; r < kMR; ++r) {
+    const int rr = r < a.n_sub ? r : 0;
+    const int w = (a.sc[rr].done == 0) & (a.all | (a.sc[rr].want_true != 0));
+    want |= r < a.n_sub ? w : 0;
+  }
+  if (!want) return;
+  if (a.sync && blockIdx.x == 0 && threadIdx.x == 0)
+    for (int r = 0; r < a.n_sub; ++r)
+      if (!a.sc[r].done) a.sc_w[r].want_true = 1;
   for (int k = threadIdx.x; k < a.table_n; k += blockDim.x) tab[k] = a.table[k];
   __syncthreads();
   const int I = blockIdx.x;
@@ -311,7 +326,8 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
   }
   const bool real = lane < a.blk_nreal[I];
   for (int r = 0; r < a.n_sub; ++r) {
-    if (a.sc[r].done || !(a.all || a.sc[r].want_true)) continue;  // uniform
+    if (a.sc[r].done || !(a.all || a.sync || a.sc[r].want_true))
+      continue;  // uniform
     double mine = acc[0];
 #pragma unroll
     for (int q = 1; q < kMR; ++q)
@@ -577,8 +593,10 @@ hipError_t launch_matvec64_ex(ds_problem* p, int residual, const double* in,
 
 hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
                              const double* u, double* out, int all,
-                             hipStream_t st) {
+                             hipStream_t st, int sync) {
   Res64Args a{};
+  a.sync = sync;
+  a.sc_w = p->sc;
   a.pos = p->pos64;
   a.x = in;
   a.xu = u;
