@@ -117,11 +117,15 @@ __device__ __forceinline__ void mfma_chunk(const uint4 (&a)[4],
         0, 0, 0);
 }
 
+// kSub 64-wide K chunks per barrier: the workgroup stages them in LDS
+// together and each wave keeps the next group's S loads in flight.
+constexpr int kSub = 2;
+
 __global__ __launch_bounds__(64 * kW) void k_gemm_bf16(
     const unsigned short* __restrict__ S, const unsigned short* __restrict__ U,
     float* __restrict__ Yp, int ld, int mpad, int klen,
     const int* __restrict__ done_count, int n_sub) {
-  __shared__ __align__(16) unsigned char ub[2][kR * kUPitch];
+  __shared__ __align__(16) unsigned char ub[2][kSub][kR * kUPitch];
   if (*done_count >= n_sub) return;   // every right-hand side converged
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 31, h = lane >> 5;
@@ -130,39 +134,58 @@ __global__ __launch_bounds__(64 * kW) void k_gemm_bf16(
   const int kbeg = split * klen;
   const int kend = min(ld, kbeg + klen);
   const int nch = (kend - kbeg + 63) / 64;   // klen and ld are multiples of 64
+  const int ngr = (nch + kSub - 1) / kSub;   // groups of kSub chunks
   const unsigned short* srow = S + (size_t)(row0 + r) * ld + 32 * h;
   // U chunk loader: thread t < 256 moves 16 B (column t / 8, part t % 8)
   const int ucol = threadIdx.x >> 3, upart = threadIdx.x & 7;
-  auto stage = [&](int c, int buf) {
+  auto stage = [&](int g, int buf) {
     if (threadIdx.x < kR * 8) {
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          U + (size_t)ucol * ld + kbeg + 64 * c + 8 * upart);
-      *reinterpret_cast<uint4*>(&ub[buf][ucol * kUPitch + 16 * upart]) = v;
+#pragma unroll
+      for (int u = 0; u < kSub; ++u) {
+        const int c = kSub * g + u;
+        if (c < nch) {
+          const uint4 v = *reinterpret_cast<const uint4*>(
+              U + (size_t)ucol * ld + kbeg + 64 * c + 8 * upart);
+          *reinterpret_cast<uint4*>(&ub[buf][u][ucol * kUPitch + 16 * upart]) =
+              v;
+        }
+      }
+    }
+  };
+  auto load_s = [&](int g, uint4 (&dst)[kSub][4]) {
+#pragma unroll
+    for (int u = 0; u < kSub; ++u) {
+      const int c = min(kSub * g + u, nch - 1);   // past the end: unused
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        dst[u][q] = reinterpret_cast<const uint4*>(srow + kbeg + 64 * c)[q];
     }
   };
   v16f acc = {};
-  uint4 a[4], an[4];
+  uint4 a[kSub][4], an[kSub][4];
   if (nch > 0) {
     stage(0, 0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      a[q] = reinterpret_cast<const uint4*>(srow + kbeg)[q];
+    load_s(0, a);
   }
-  for (int c = 0; c < nch; ++c) {
-    __syncthreads();   // chunk c staged; chunk c - 1's buffer is free
-    if (c + 1 < nch) {
-      stage(c + 1, (c + 1) & 1);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        an[q] = reinterpret_cast<const uint4*>(srow + kbeg + 64 * (c + 1))[q];
+  for (int g = 0; g < ngr; ++g) {
+    __syncthreads();   // group g staged; group g - 1's buffer is free
+    if (g + 1 < ngr) {
+      stage(g + 1, (g + 1) & 1);
+      load_s(g + 1, an);
     }
-    uint4 b[4];
-    const unsigned char* bp = &ub[c & 1][r * kUPitch + 64 * h];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) b[q] = reinterpret_cast<const uint4*>(bp)[q];
-    mfma_chunk(a, b, acc);
+    for (int u = 0; u < kSub; ++u) {
+      if (kSub * g + u >= nch) break;   // uniform
+      uint4 b[4];
+      const unsigned char* bp = &ub[g & 1][u][r * kUPitch + 64 * h];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a[q] = an[q];
+      for (int q = 0; q < 4; ++q) b[q] = reinterpret_cast<const uint4*>(bp)[q];
+      mfma_chunk(a[u], b, acc);
+    }
+#pragma unroll
+    for (int u = 0; u < kSub; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[u][q] = an[u][q];
   }
   float* y = Yp + ((size_t)split * kR + r) * mpad + row0;
 #pragma unroll
