@@ -234,11 +234,18 @@ __global__ __launch_bounds__(256) void k_matvec64(MatvecArgs a) {
 // same pixels, weights and covariance; subvolume r's vector at r * ld):
 // out_r = rhs_r - A x_r for every subvolume that wants a true residual,
 // from one pass over subvolume 0's tile lists.  Each pair's value is the
-// parity kernel's (same operations), and each right-hand side accumulates
-// it in the same source order with the same wave split, so every residual
-// equals the per-subvolume kernel's bit for bit.  The 16 sources x 32
-// right-hand sides of a tile are staged in LDS per wave and read as
-// broadcasts.
+// parity kernel's (same operations, FP64); the products with the 32
+// right-hand sides run on v_mfma_f64_16x16x4f64: per list entry (64 target
+// rows x 16 sources) and k-step of 4 sources, lane l = 16 g + m evaluates
+// the pair (row 16 rg + m, source 4 ks + g) of each 16-row group rg (the
+// MFMA's A element [m][g]) and passes U[source 4 ks + g][rhs 16 ct + m]
+// (B[g][m]) from the tile's 16 x 32 values staged in LDS; D element q of
+// lane l is row g + 4 q, right-hand side 16 ct + m.  The row sums are FP64
+// in another order than the per-subvolume kernel's (the matrix cores sum
+// four sources at a time), so the residuals agree with it to FP64
+// rounding, not bit for bit.  (With a lane-per-row layout every product
+// needs a broadcast LDS read of the right-hand side's value: the pass was
+// LDS-bound at ~0.69 ms for 100k pixels x 32.)
 constexpr int kMR = DS_DENSE_R;   // right-hand sides per pass
 
 struct Res64Args {
@@ -261,13 +268,15 @@ struct Res64Args {
   ds_cg_scalars* sc_w;
 };
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 template <int FORM>
 __global__ __launch_bounds__(256) void k_res64_mrhs(
     const double4* __restrict__ pos, const int* __restrict__ tidx,
     Res64Args a) {
   extern __shared__ double2 tab[];
   __shared__ double us[4][DS_TS][kMR];
-  __shared__ double red[3][64];
+  __shared__ double red[2 * 4 * 4 * 64];   // one wave's 32 sums per lane
   // uniform: no subvolume asked for a true residual.  Every subvolume's
   // flags are loaded in one batch (a short-circuit test would wait for
   // each load in turn: ~16 us for 32 subvolumes).
@@ -286,13 +295,24 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
   __syncthreads();
   const int I = blockIdx.x;
   const int lane = threadIdx.x & 63;
+  const int m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
-  const int row = I * DS_BS + lane;   // subvolume 0's slot
-  const double4 pi = pos[row];
-  double acc[kMR];
+  const int row0 = I * DS_BS;   // subvolume 0's slots
+  // targets of this lane's A elements: row 16 rg + m of each group
+  double px[4], py[4], pz[4];
 #pragma unroll
-  for (int r = 0; r < kMR; ++r) acc[r] = 0.0;
+  for (int rg = 0; rg < 4; ++rg) {
+    const double4 t = pos[row0 + 16 * rg + m];
+    px[rg] = t.x;
+    py[rg] = t.y;
+    pz[rg] = t.z;
+  }
+  v4d acc[4][2];
+#pragma unroll
+  for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) acc[rg][ct] = v4d{0.0, 0.0, 0.0, 0.0};
   const int kb = __builtin_amdgcn_readfirstlane(a.tptr[I]) + wave;
   const int ke = __builtin_amdgcn_readfirstlane(a.tptr[I + 1]);
   for (int k = kb; k < ke; k += nw) {
@@ -305,48 +325,75 @@ __global__ __launch_bounds__(256) void k_res64_mrhs(
           r < a.n_sub ? a.xu[(size_t)r * a.ld + j0 + jj] : 0.0;
     }
     __builtin_amdgcn_wave_barrier();
-#pragma unroll 2
-    for (int jj = 0; jj < DS_TS; ++jj) {
-      const double4 pj = pos[j0 + jj];
-      const double dx = pi.x - pj.x;
-      const double dy = pi.y - pj.y;
-      const double dz = pi.z - pj.z;
-      const double x_perp_2 = dx * dx + dy * dy;
-      const double x_para_2 = dz * dz;
-      double xx = (a.c_perp * x_perp_2 + a.c_para * x_para_2) * a.dx_inv;
-      xx = fmin(xx, a.tlast);
-      const int ix = (int)xx;
-      const double t = __builtin_amdgcn_fract(xx);   // == xx - ix, exactly
-      const double2 f = tab[ix];
-      const double gp = fma(f.y, t, f.x);
 #pragma unroll
-      for (int r = 0; r < kMR; ++r) acc[r] = fma(gp, us[wave][jj][r], acc[r]);
+    for (int ks = 0; ks < DS_TS / 4; ++ks) {
+      const int src = 4 * ks + g;
+      const double4 pj = pos[j0 + src];
+      const double b0 = us[wave][src][m], b1 = us[wave][src][16 + m];
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg) {
+        const double dx = px[rg] - pj.x;
+        const double dy = py[rg] - pj.y;
+        const double dz = pz[rg] - pj.z;
+        const double x_perp_2 = dx * dx + dy * dy;
+        const double x_para_2 = dz * dz;
+        double xx = (a.c_perp * x_perp_2 + a.c_para * x_para_2) * a.dx_inv;
+        xx = fmin(xx, a.tlast);
+        const int ix = (int)xx;
+        const double t = __builtin_amdgcn_fract(xx);   // == xx - ix, exactly
+        const double2 f = tab[ix];
+        const double gp = fma(f.y, t, f.x);
+        acc[rg][0] =
+            __builtin_amdgcn_mfma_f64_16x16x4f64(gp, b0, acc[rg][0], 0, 0, 0);
+        acc[rg][1] =
+            __builtin_amdgcn_mfma_f64_16x16x4f64(gp, b1, acc[rg][1], 0, 0, 0);
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
-  const bool real = lane < a.blk_nreal[I];
-  for (int r = 0; r < a.n_sub; ++r) {
-    if (a.sc[r].done || !(a.all || a.sync || a.sc[r].want_true))
-      continue;  // uniform
-    double mine = acc[0];
+  // the waves' sums, added in wave order by wave 0
+  for (int w = 1; w < nw; ++w) {
+    __syncthreads();
+    if (wave == w)
 #pragma unroll
-    for (int q = 1; q < kMR; ++q)
-      if (q == r) mine = acc[q];
-    if (wave > 0) red[wave - 1][lane] = mine;
+      for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            red[((rg * 2 + ct) * 4 + q) * 64 + lane] = acc[rg][ct][q];
     __syncthreads();
-    if (wave == 0) {
-      // rows in wave order, as the per-subvolume kernel sums them
-      double y = mine;
-      for (int w = 1; w < nw; ++w) y += red[w - 1][lane];
-      const size_t g = (size_t)r * a.ld + row;
-      const double ai = a.x[g];
-      double bi = (FORM == DS_FORM_W) ? pi.w * (a.var_s * y) + ai
-                                      : a.var_s * y + pi.w * ai;
-      bi = a.rhs[g] - bi;
-      if (!real) bi = 0.0;
-      a.out[g] = bi;
-    }
-    __syncthreads();
+    if (wave == 0)
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[rg][ct][q] += red[((rg * 2 + ct) * 4 + q) * 64 + lane];
+  }
+  if (wave != 0) return;
+  const int nreal = a.blk_nreal[I];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const int r = 16 * ct + m;   // this lane's right-hand side
+    if (r >= a.n_sub) continue;
+    if (a.sc[r].done || !(a.all || a.sync || a.sc[r].want_true)) continue;
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rw = 16 * rg + g + 4 * q;   // row within the block
+        const size_t gi = (size_t)r * a.ld + row0 + rw;
+        const double y = acc[rg][ct][q];
+        const double ai = a.x[gi];
+        const double wn = pos[row0 + rw].w;
+        double bi = (FORM == DS_FORM_W) ? wn * (a.var_s * y) + ai
+                                        : a.var_s * y + wn * ai;
+        bi = a.rhs[gi] - bi;
+        if (rw >= nreal) bi = 0.0;
+        a.out[gi] = bi;
+      }
   }
 }
 

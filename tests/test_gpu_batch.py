@@ -169,9 +169,11 @@ def test_dense_bf16_block_solve(ora, mode):
 @pytest.mark.parametrize("waves", [1, 4])
 def test_multi_rhs_residual_equals_per_subvolume(waves):
     """The one-pass FP64 residual of a shared-geometry batch (the mixed
-    solver's true residual when a multi-RHS operator is selected) equals the
-    per-subvolume parity kernel at its 64-row tile bit for bit: same pair
-    values, same source order, same wave split."""
+    solver's true residual when a multi-RHS operator is selected: the same
+    FP64 pair values, multiplied into all right-hand sides on the FP64
+    matrix cores) equals the per-subvolume parity kernel up to FP64
+    rounding of the row sums: |difference| <= 1e-13 (sum_j |A_ij x_j| +
+    |b_i|) per row (A is entrywise non-negative, so A |x| is that sum)."""
     import dachshund_amd as D
     w, sets = _realizations(7, seed=31)
     sp = D.SignalCovarParams(w.var_s, w.l_perp, w.l_para)
@@ -180,9 +182,11 @@ def test_multi_rhs_residual_equals_per_subvolume(waves):
     batch.set_tile(64)
     x = np.random.default_rng(2).standard_normal(batch.n)
     r_sub = batch.A_residual(x)
+    scale = batch.A_product(np.abs(x)) + np.abs(r_sub + batch.A_product(x))
     batch.set_fused(True)
     r_one = batch.A_residual(x)
-    assert np.array_equal(r_one, r_sub)
+    assert np.all(np.abs(r_one - r_sub) <= 1e-13 * scale)
+    assert not np.array_equal(scale, 0 * scale)
 
 
 @pytest.mark.parametrize("rows", [16, 32])
