@@ -251,6 +251,15 @@ __global__ __launch_bounds__(256) void k_matvec32m(
   const BufRsrc rslot = buf_rsrc(a.gslot, 0x7FFFFFFFu);
   const BufRsrc rloc = buf_rsrc(loc, a.npad * 16u);
   const BufRsrc ru = buf_rsrc(u32, a.npad * 4u);
+  // column-sum slots: a vector store at lane offset 4 c from the slot's
+  // wave-uniform base (scalar offset)
+  // (problem creation keeps 64 bytes x slots within 32-bit offsets)
+  const BufRsrc rpb = buf_rsrc(a.pbuf, 0xFFFFFFFFu);
+  // byte offsets in uown of this lane's 4 own values for each sub-block
+  // (the 16 zeros at 64 for a sub-block without a transposed product)
+  unsigned uoff[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) uoff[s] = 4u * (16u * s + 4u * g);
   const int vrec = (int)(lz * 16u), vslot = (int)(lz * 4u);
   const int vloc = (int)(cu * 16u), vu = (int)(cu * 4u);
   // One entry e: its record o (frame offset; .w holds the bits of live
@@ -283,7 +292,8 @@ __global__ __launch_bounds__(256) void k_matvec32m(
     for (int s = 0; s < 4; ++s) {
       if (!(live & (1u << s))) continue;
       const float4 uo = *reinterpret_cast<const float4*>(
-          &uown[(tr & (1u << s)) ? 16 * s + 4 * g : 64]);
+          reinterpret_cast<const char*>(uown) +
+          ((tr & (1u << s)) ? uoff[s] : 256u));
       const float uor[4] = {uo.x, uo.y, uo.z, uo.w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -305,7 +315,9 @@ __global__ __launch_bounds__(256) void k_matvec32m(
           __float_as_uint(at), __float_as_uint(at), false, false);
       at = __uint_as_float(h16[0]) + __uint_as_float(h16[1]);
       if (g == 0) {
-        a.pbuf[16 * (size_t)slot + c] = at;
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __float_as_int(at), rpb, (int)(cu * 4u),
+            (int)(64u * (unsigned)__builtin_amdgcn_readfirstlane(slot)), 0);
         if (CG) dqt = fmaf(at, uj, dqt);
       }
     }
