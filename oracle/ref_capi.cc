@@ -1,7 +1,7 @@
 // ref_capi.cc -- a C surface over the reference library, built from the
-// reference's own sources into oracle/_ref/libdsref.so.  TEST / BASELINE
-// INFRASTRUCTURE ONLY: the benchmark times these calls as the reference
-// OpenMP CPU path; the product never loads this library.
+// reference's own sources into oracle/_ref/libdsref.so.  TEST AND CPU
+// TIMING INFRASTRUCTURE ONLY: the benchmark times these calls as the
+// reference OpenMP CPU path; the product never loads this library.
 //
 // Pixels are reference WPixel records {x,y,z,w,wd} (lib/pixel.h:39-45),
 // points are {x,y,z} (lib/pixel.h:8-12).
