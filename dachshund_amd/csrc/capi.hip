@@ -199,6 +199,38 @@ int check_grid(ds_ctx* ctx, double lx, double ly, double lz, int nx, int ny,
   return DS_OK;
 }
 
+// Exclusive offsets of the per-map-tile list counts: ptr[T] = sum of
+// cnt[t < T], ptr[nt] = the total, or -1 when the total exceeds cap
+// entries (then no offset is written).  One workgroup of 1024 threads, each
+// summing a contiguous run of counts.
+__global__ __launch_bounds__(1024) void k_list_offsets(const int* cnt, int nt,
+                                                       long long cap,
+                                                       int* ptr) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (nt + 1023) / 1024;
+  const int b = min(nt, t * per), e = min(nt, b + per);
+  long long sum = 0;
+  for (int i = b; i < e; ++i) sum += cnt[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan
+    const long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const long long total = part[1023];
+  const bool ok = total <= cap;
+  long long run = t > 0 ? part[t - 1] : 0;
+  if (ok)
+    for (int i = b; i < e; ++i) {
+      ptr[i] = (int)run;
+      run += cnt[i];
+    }
+  if (t == 0) ptr[nt] = ok ? (int)total : -1;
+}
+
 // Map onto a grid from a blocked source set (u in sorted order on device)
 // whose ntile tile boxes are on the device (dlo / dhi, 3 per tile); hblk,
 // when given, is the host blocking of the same set.
@@ -208,11 +240,11 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g, int ntile,
                  const CovarDev& cv, double* m_host) {
   const int nt = g.tx * g.ty * g.tz;
   std::vector<int> ptr, idx;
-  DevBuf<int> dptr, didx;
-  DevBuf<double> dm;
+  ds::PoolTmp<int> dptr, didx;
+  ds::PoolTmp<double> dm;
   const size_t n_map = (size_t)g.nx * g.ny * g.nz;
   hipStream_t st = ctx->stream;
-  HIP_OR_FAIL(ctx, dptr.alloc((size_t)nt + 1));
+  HIP_OR_FAIL(ctx, dptr.alloc(ctx, (size_t)nt + 1));
   const char* host_lists = std::getenv("DS_HOST_GRID_LISTS");
   if (hblk && host_lists && host_lists[0] == '1') {
     const Blocking& blk = *hblk;
@@ -220,7 +252,7 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g, int ntile,
     std::vector<double> lo, hi;
     grid_tiles(g, lo, hi);
     ds::build_tile_lists(nt, lo.data(), hi.data(), blk, cv, ptr, idx);
-    HIP_OR_FAIL(ctx, didx.alloc(std::max<size_t>(idx.size(), 1)));
+    HIP_OR_FAIL(ctx, didx.alloc(ctx, std::max<size_t>(idx.size(), 1)));
     HIP_OR_FAIL(ctx, hipMemcpyAsync(dptr.p, ptr.data(),
                                     sizeof(int) * ptr.size(),
                                     hipMemcpyHostToDevice, st));
@@ -229,36 +261,29 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g, int ntile,
                                       sizeof(int) * idx.size(),
                                       hipMemcpyHostToDevice, st));
   } else {
-    // full scan on the device: count, scan on the host, fill
-    DevBuf<int> dcnt;
-    HIP_OR_FAIL(ctx, dcnt.alloc(nt));
+    // full scan on the device: count, offsets on the device, fill
+    ds::PoolTmp<int> dcnt;
+    HIP_OR_FAIL(ctx, dcnt.alloc(ctx, nt));
     HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, ntile, cv, dcnt.p,
                                            nullptr, nullptr, st));
-    std::vector<int> cnt(nt);
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * nt,
+    hipLaunchKernelGGL(k_list_offsets, dim3(1), dim3(1024), 0, st, dcnt.p,
+                       nt, (long long)INT_MAX, dptr.p);
+    HIP_OR_FAIL(ctx, hipGetLastError());
+    int total = 0;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&total, dptr.p + nt, sizeof(int),
                                     hipMemcpyDeviceToHost, st));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
-    ptr.assign((size_t)nt + 1, 0);
-    for (int T = 0; T < nt; ++T) {
-      if ((long long)ptr[T] + cnt[T] > INT_MAX)
-        return fail(ctx, DS_E_ARG, "map grid: too many tile-list entries");
-      ptr[T + 1] = ptr[T] + cnt[T];
-    }
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(dptr.p, ptr.data(),
-                                    sizeof(int) * ptr.size(),
-                                    hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(ctx, didx.alloc(std::max(ptr[nt], 1)));
+    if (total < 0)
+      return fail(ctx, DS_E_ARG, "map grid: too many tile-list entries");
+    HIP_OR_FAIL(ctx, didx.alloc(ctx, std::max(total, 1)));
     HIP_OR_FAIL(ctx, ds::launch_grid_lists(g, dlo, dhi, ntile, cv, nullptr,
                                            dptr.p, didx.p, st));
-    // the counts are freed on return: finish with them first
-    HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
   }
-  HIP_OR_FAIL(ctx, dm.alloc(n_map));
+  HIP_OR_FAIL(ctx, dm.alloc(ctx, n_map));
   HIP_OR_FAIL(ctx, ds::launch_map64(mode, g, pos, u, dptr.p, didx.p, table64,
                                     cv, dm.p, st));
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(m_host, dm.p, sizeof(double) * n_map,
-                                  hipMemcpyDeviceToHost, st));
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(st));
+  HIP_OR_FAIL(ctx, ds::copy_to_host(ctx, m_host, dm.p,
+                                    sizeof(double) * n_map, st));
   return DS_OK;
 }
 
@@ -539,6 +564,71 @@ void host_put(ds_ctx* c, void* p, size_t cap) {
 
 }  // namespace
 
+hipError_t copy_to_host(ds_ctx* c, void* dst, const void* src, size_t bytes,
+                        hipStream_t st) {
+  constexpr size_t kMin = size_t(4) << 20;
+  void* stage = nullptr;
+  size_t cap = 0;
+  if (bytes < kMin || bytes > kHostCacheBytes ||
+      host_get(c, &stage, bytes, &cap) != hipSuccess) {
+    (void)hipGetLastError();
+    const hipError_t e =
+        hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+  }
+  // DMA into the pinned buffer in chunks; each host thread copies its
+  // share of a chunk to dst as soon as that chunk has arrived
+  constexpr int kChunks = 8, kThreads = 4;
+  const size_t chunk = ((bytes + kChunks - 1) / kChunks + 4095) & ~size_t(4095);
+  hipEvent_t ev[kChunks];
+  int nev = 0;
+  hipError_t err = hipSuccess;
+  for (size_t off = 0; off < bytes && err == hipSuccess; off += chunk) {
+    const size_t len = std::min(chunk, bytes - off);
+    err = hipEventCreateWithFlags(&ev[nev], hipEventDisableTiming);
+    if (err != hipSuccess) break;
+    ++nev;
+    err = hipMemcpyAsync(static_cast<char*>(stage) + off,
+                         static_cast<const char*>(src) + off, len,
+                         hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipEventRecord(ev[nev - 1], st);
+  }
+  if (err == hipSuccess) {
+    std::atomic<int> bad{0};
+    auto work = [&](int t) {
+      if (t > 0 && hipSetDevice(c->device) != hipSuccess) {
+        bad = 1;
+        return;
+      }
+      for (int k = 0; k < nev; ++k) {
+        if (hipEventSynchronize(ev[k]) != hipSuccess) {
+          bad = 1;
+          return;
+        }
+        const size_t off = (size_t)k * chunk;
+        const size_t len = std::min(chunk, bytes - off);
+        const size_t part = (len + kThreads - 1) / kThreads;
+        const size_t b = std::min(len, (size_t)t * part);
+        const size_t e = std::min(len, b + part);
+        if (e > b)
+          std::memcpy(static_cast<char*>(dst) + off + b,
+                      static_cast<const char*>(stage) + off + b, e - b);
+      }
+    };
+    std::thread th[kThreads - 1];
+    for (int t = 1; t < kThreads; ++t) th[t - 1] = std::thread(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    if (bad) err = hipErrorUnknown;
+  }
+  // every chunk's copy has completed (or failed) before the buffer returns
+  const hipError_t e2 = hipStreamSynchronize(st);
+  if (err == hipSuccess) err = e2;
+  for (int k = 0; k < nev; ++k) (void)hipEventDestroy(ev[k]);
+  host_put(c, stage, cap);
+  return err;
+}
+
 hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes, bool host) {
   size_t cap = 0;
   const hipError_t e = host ? host_get(p->ctx, ptr, bytes, &cap)
@@ -595,37 +685,6 @@ hipError_t noalloc_setup(int device, unsigned char** out) {
   return hipSuccess;
 }
 
-// Exclusive offsets of the per-map-tile list counts: ptr[T] = sum of
-// cnt[t < T], ptr[nt] = the total, or -1 when the total exceeds cap
-// entries (then no offset is written).  One workgroup of 1024 threads, each
-// summing a contiguous run of counts.
-__global__ __launch_bounds__(1024) void k_list_offsets(const int* cnt, int nt,
-                                                       long long cap,
-                                                       int* ptr) {
-  __shared__ long long part[1024];
-  const int t = threadIdx.x;
-  const int per = (nt + 1023) / 1024;
-  const int b = min(nt, t * per), e = min(nt, b + per);
-  long long sum = 0;
-  for (int i = b; i < e; ++i) sum += cnt[i];
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan
-    const long long v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  const long long total = part[1023];
-  const bool ok = total <= cap;
-  long long run = t > 0 ? part[t - 1] : 0;
-  if (ok)
-    for (int i = b; i < e; ++i) {
-      ptr[i] = (int)run;
-      run += cnt[i];
-    }
-  if (t == 0) ptr[nt] = ok ? (int)total : -1;
-}
 
 }  // namespace
 

@@ -223,6 +223,29 @@ void pfree(ds_problem* p, T*& ptr) {
   if (ptr) pool_free(p, (void*)ptr);
   ptr = nullptr;
 }
+// A temporary device buffer from the context's pool, returned when it goes
+// out of scope once the context's stream has finished with it.
+template <class T>
+struct PoolTmp {
+  ds_ctx* c = nullptr;
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t alloc(ds_ctx* ctx, size_t n) {
+    c = ctx;
+    return ctx_buf_get(ctx, (void**)&p, sizeof(T) * (n ? n : 1), &cap);
+  }
+  ~PoolTmp() {
+    if (!p) return;
+    (void)hipStreamSynchronize(c->stream);
+    ctx_buf_put(c, (void*)p, cap);
+  }
+};
+// dst (pageable host memory) = bytes of device memory at src, enqueued
+// after the work on st, complete on return.  Large copies go through a
+// pinned buffer of the context's host pool in chunks, each chunk copied on
+// by host threads as soon as it arrives.
+hipError_t copy_to_host(ds_ctx* c, void* dst, const void* src, size_t bytes,
+                        hipStream_t st);
 }  // namespace ds
 
 struct ds_problem {
