@@ -280,10 +280,23 @@ int map_grid_run(ds_ctx* ctx, int mode, const ds::MapGrid& g, int ntile,
                                            dptr.p, didx.p, st));
   }
   HIP_OR_FAIL(ctx, dm.alloc(ctx, n_map));
-  HIP_OR_FAIL(ctx, ds::launch_map64(mode, g, pos, u, dptr.p, didx.p, table64,
-                                    cv, dm.p, st));
-  HIP_OR_FAIL(ctx, ds::copy_to_host(ctx, m_host, dm.p,
-                                    sizeof(double) * n_map, st));
+  // the map tiles in up to 4 slabs along x: each slab's cells are one
+  // contiguous run of the map, copied out while the next slab is computed
+  ds::HostCopy out;
+  HIP_OR_FAIL(ctx, out.begin(ctx, m_host, sizeof(double) * n_map, st));
+  const int parts = std::min(4, g.tx);
+  const size_t slab = (size_t)DS_MAP_T * g.ny * g.nz;   // cells per tile row
+  for (int q = 0; q < parts; ++q) {
+    const int x0 = (int)((long long)g.tx * q / parts);
+    const int x1 = (int)((long long)g.tx * (q + 1) / parts);
+    HIP_OR_FAIL(ctx, ds::launch_map64(mode, g, pos, u, dptr.p, didx.p,
+                                      table64, cv, dm.p, st, x0, x1));
+    const size_t c0 = std::min(n_map, (size_t)x0 * slab);
+    const size_t c1 = std::min(n_map, (size_t)x1 * slab);
+    HIP_OR_FAIL(ctx, out.enqueue(dm.p, sizeof(double) * c0,
+                                 sizeof(double) * (c1 - c0)));
+  }
+  HIP_OR_FAIL(ctx, out.finish());
   return DS_OK;
 }
 
@@ -564,52 +577,69 @@ void host_put(ds_ctx* c, void* p, size_t cap) {
 
 }  // namespace
 
-hipError_t copy_to_host(ds_ctx* c, void* dst, const void* src, size_t bytes,
-                        hipStream_t st) {
+hipError_t HostCopy::begin(ds_ctx* ctx, void* d, size_t n, hipStream_t s) {
+  c = ctx;
+  dst = d;
+  bytes = n;
+  st = s;
   constexpr size_t kMin = size_t(4) << 20;
-  void* stage = nullptr;
-  size_t cap = 0;
-  if (bytes < kMin || bytes > kHostCacheBytes ||
-      host_get(c, &stage, bytes, &cap) != hipSuccess) {
+  direct = bytes < kMin || bytes > kHostCacheBytes ||
+           host_get(c, &stage, bytes, &cap) != hipSuccess;
+  if (direct) {
+    stage = nullptr;
     (void)hipGetLastError();
-    const hipError_t e =
-        hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
-    return e != hipSuccess ? e : hipStreamSynchronize(st);
   }
-  // DMA into the pinned buffer in chunks; each host thread copies its
-  // share of a chunk to dst as soon as that chunk has arrived
-  constexpr int kChunks = 8, kThreads = 4;
-  const size_t chunk = ((bytes + kChunks - 1) / kChunks + 4095) & ~size_t(4095);
-  hipEvent_t ev[kChunks];
-  int nev = 0;
-  hipError_t err = hipSuccess;
-  for (size_t off = 0; off < bytes && err == hipSuccess; off += chunk) {
-    const size_t len = std::min(chunk, bytes - off);
-    err = hipEventCreateWithFlags(&ev[nev], hipEventDisableTiming);
-    if (err != hipSuccess) break;
-    ++nev;
+  return hipSuccess;
+}
+
+hipError_t HostCopy::enqueue(const void* src, size_t off, size_t len) {
+  if (len == 0) return hipSuccess;
+  if (direct)
+    return hipMemcpyAsync(static_cast<char*>(dst) + off,
+                          static_cast<const char*>(src) + off, len,
+                          hipMemcpyDeviceToHost, st);
+  // the part's copy runs on the context's second stream once the work
+  // enqueued so far on st has produced it, so st goes on with the next part
+  hipEvent_t ready, e;
+  hipError_t err = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+  if (err != hipSuccess) return err;
+  err = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  if (err != hipSuccess) {
+    (void)hipEventDestroy(ready);
+    return err;
+  }
+  parts.push_back({off, len});
+  ev.push_back(e);
+  ev.push_back(ready);
+  err = hipEventRecord(ready, st);
+  if (err == hipSuccess) err = hipStreamWaitEvent(c->aux, ready, 0);
+  if (err == hipSuccess)
     err = hipMemcpyAsync(static_cast<char*>(stage) + off,
                          static_cast<const char*>(src) + off, len,
-                         hipMemcpyDeviceToHost, st);
-    if (err == hipSuccess) err = hipEventRecord(ev[nev - 1], st);
-  }
-  if (err == hipSuccess) {
+                         hipMemcpyDeviceToHost, c->aux);
+  return err != hipSuccess ? err : hipEventRecord(e, c->aux);
+}
+
+hipError_t HostCopy::finish() {
+  hipError_t err = hipSuccess;
+  if (!direct && !parts.empty()) {
+    // each host thread copies its share of a part as soon as it arrives
+    constexpr int kThreads = 4;
     std::atomic<int> bad{0};
     auto work = [&](int t) {
       if (t > 0 && hipSetDevice(c->device) != hipSuccess) {
         bad = 1;
         return;
       }
-      for (int k = 0; k < nev; ++k) {
-        if (hipEventSynchronize(ev[k]) != hipSuccess) {
+      for (size_t k = 0; k < parts.size(); ++k) {
+        if (hipEventSynchronize(ev[2 * k]) != hipSuccess) {
           bad = 1;
           return;
         }
-        const size_t off = (size_t)k * chunk;
-        const size_t len = std::min(chunk, bytes - off);
-        const size_t part = (len + kThreads - 1) / kThreads;
-        const size_t b = std::min(len, (size_t)t * part);
-        const size_t e = std::min(len, b + part);
+        const size_t off = parts[k].first, len = parts[k].second;
+        const size_t per = (len + kThreads - 1) / kThreads;
+        const size_t b = std::min(len, (size_t)t * per);
+        const size_t e = std::min(len, b + per);
         if (e > b)
           std::memcpy(static_cast<char*>(dst) + off + b,
                       static_cast<const char*>(stage) + off + b, e - b);
@@ -621,12 +651,21 @@ hipError_t copy_to_host(ds_ctx* c, void* dst, const void* src, size_t bytes,
     for (auto& x : th) x.join();
     if (bad) err = hipErrorUnknown;
   }
-  // every chunk's copy has completed (or failed) before the buffer returns
-  const hipError_t e2 = hipStreamSynchronize(st);
+  // every copy has completed (or failed) before the buffer goes back
+  hipError_t e2 = hipStreamSynchronize(st);
   if (err == hipSuccess) err = e2;
-  for (int k = 0; k < nev; ++k) (void)hipEventDestroy(ev[k]);
-  host_put(c, stage, cap);
+  e2 = hipStreamSynchronize(c->aux);
+  if (err == hipSuccess) err = e2;
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+  parts.clear();
+  if (stage) host_put(c, stage, cap);
+  stage = nullptr;
   return err;
+}
+
+HostCopy::~HostCopy() {
+  if (stage || !ev.empty()) (void)finish();
 }
 
 hipError_t pool_alloc(ds_problem* p, void** ptr, size_t bytes, bool host) {

@@ -240,12 +240,26 @@ struct PoolTmp {
     ctx_buf_put(c, (void*)p, cap);
   }
 };
-// dst (pageable host memory) = bytes of device memory at src, enqueued
-// after the work on st, complete on return.  Large copies go through a
-// pinned buffer of the context's host pool in chunks, each chunk copied on
-// by host threads as soon as it arrives.
-hipError_t copy_to_host(ds_ctx* c, void* dst, const void* src, size_t bytes,
-                        hipStream_t st);
+// A copy of device memory into pageable host memory (dst, bytes) in parts
+// enqueued on st as their data is produced (enqueue: bytes [off, off + len)
+// of src), complete when finish() returns.  Large copies go through a
+// pinned buffer of the context's host pool: each part's DMA runs on the
+// context's second stream once st has produced it, and host threads copy
+// it on as soon as it arrives, while st computes the next part.
+struct HostCopy {
+  ds_ctx* c = nullptr;
+  void* dst = nullptr;
+  void* stage = nullptr;
+  size_t cap = 0, bytes = 0;
+  hipStream_t st = nullptr;
+  bool direct = true;
+  std::vector<std::pair<size_t, size_t>> parts;
+  std::vector<hipEvent_t> ev;
+  hipError_t begin(ds_ctx* ctx, void* d, size_t n, hipStream_t s);
+  hipError_t enqueue(const void* src, size_t off, size_t len);
+  hipError_t finish();
+  ~HostCopy();
+};
 }  // namespace ds
 
 struct ds_problem {
@@ -483,10 +497,11 @@ struct MapGrid {
   int tx, ty, tz;      // tiles per axis
 };
 enum MapMode { MM_SMP = 0, MM_SIGMA = 1 };
+// (the map tiles with x tile index in [x0, x1); x1 < 0: all of them)
 hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
                         const double* u, const int* tptr, const int* tidx,
                         const double2* table64, const CovarDev& cv,
-                        double* m, hipStream_t st);
+                        double* m, hipStream_t st, int x0 = 0, int x1 = -1);
 // Source-tile lists of the grid's tiles on the device: count[T] when idx is
 // null, else idx at the offsets ptr; equal to build_tile_lists' lists.
 hipError_t launch_grid_lists(const MapGrid& g, const double* slo,

@@ -410,6 +410,7 @@ struct MapArgs {
   int nx, ny, nz, tx, ty, tz;
   double var_s, c_perp, c_para, dx_inv, tlast;
   int table_n;
+  int t0;       // first map tile of the launch
 };
 
 template <bool SQUARE>
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(256) void k_map64(MapArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
-  const int T = blockIdx.x;
+  const int T = a.t0 + (int)blockIdx.x;
   const int tiz = T % a.tz, tiy = (T / a.tz) % a.ty, tix = T / (a.tz * a.ty);
   const int ix = tix * DS_MAP_T + (lane >> 4);
   const int iy = tiy * DS_MAP_T + ((lane >> 2) & 3);
@@ -677,7 +678,9 @@ hipError_t launch_res64_mrhs(ds_problem* p, const double* in,
 hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
                         const double* u, const int* tptr, const int* tidx,
                         const double2* table64, const CovarDev& cv,
-                        double* m, hipStream_t st) {
+                        double* m, hipStream_t st, int x0, int x1) {
+  if (x1 < 0) x1 = g.tx;
+  if (x1 <= x0) return hipSuccess;
   MapArgs a{};
   a.pos = pos;
   a.u = u;
@@ -694,7 +697,9 @@ hipError_t launch_map64(int mode, const MapGrid& g, const double4* pos,
   a.dx_inv = cv.dx_inv;
   a.tlast = (double)(cv.table_n - 1);
   a.table_n = cv.table_n;
-  const dim3 grid((unsigned)((size_t)g.tx * g.ty * g.tz)), block(256);
+  const size_t per_x = (size_t)g.ty * g.tz;   // tiles per x tile index
+  a.t0 = (int)(per_x * x0);
+  const dim3 grid((unsigned)(per_x * (x1 - x0))), block(256);
   const size_t shm = sizeof(double2) * cv.table_n;
   if (mode == MM_SIGMA)
     hipLaunchKernelGGL(k_map64<true>, grid, block, shm, st, a);
