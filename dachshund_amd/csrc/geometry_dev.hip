@@ -12,10 +12,13 @@
 //      extent; its left child takes the `left` points of smallest key
 //      (quantized coordinate, point index) and both children keep the
 //      node's order.  Nodes above 1024 points are split level by level by
-//      all workgroups (radix select of the split key over 8-bit digits, a
-//      per-chunk count, a scan and a stable scatter); each node of at most
-//      1024 points is then finished, down to its 16-slot tiles, by one
-//      wavefront in LDS.  The splits are those of build_blocking.
+//      all workgroups (radix select of the split key over 8-bit digits,
+//      each launch taking the previous digit from its counts, a per-chunk
+//      count, a scan and a stable scatter); each node of at most 1024
+//      points is then finished, down to its 16-slot tiles, by one
+//      wavefront in LDS.  A select stops at the first digit whose bucket
+//      starts at the split's rank: that prefix, lower digits zero, puts
+//      the same keys below it.  The splits are those of build_blocking.
 //   2. One wavefront per block: per-slot records (positions, right-hand
 //      side, diagonal, weights), the FP32 frames (per-axis medians of the
 //      block and of each tile), tile and block boxes, the far-point test.
@@ -143,12 +146,16 @@ struct BigArgs {
   const BigNode* nodes;
   const Chunk* chunks;
   unsigned* nq;               // per node: qlo[3], qhi[3]
-  unsigned long long* pre;    // per node: split key prefix
-  int* rank;                  // per node: rank left within the prefix
-  int* hist;                  // per node: 256 bins
+  // The split key's prefix and the split's rank within it after an even
+  // (first nn entries) or odd (next nn) number of digits; rank 0: found.
+  // After the last digit the first nn prefixes hold the split keys.
+  unsigned long long* pre;
+  int* rank;
+  int* hist;                  // 3 x nn x 256 digit counts, used in turn
   int* ccount;                // per chunk: keys below the split
   int* coff;                  // per chunk: left keys of earlier chunks
-  int* err;                   // bit 8: a split count differs from `left`
+  int* err;                   // bit 8: a split count differs from `left`,
+                              // bit 16: a digit histogram does not add up
   int nn, nc;
 };
 
@@ -162,6 +169,56 @@ __device__ inline int node_axis(const BigArgs& a, int i) {
   return kd_axis(lo, hi, a.kd.quant + 9 * (size_t)a.nodes[i].sub + 6);
 }
 
+// Adds the lanes with m set to their digit's bin; a wavefront whose
+// counted lanes share one digit (the common case near a node's edges and
+// in the prefix's digits) adds them with one atomic.
+__device__ inline void lds_count(int* lh, bool m, unsigned dg) {
+  const unsigned long long am = __ballot(m);
+  if (!am) return;
+  const int leader = __builtin_ctzll(am);
+  const unsigned dl = (unsigned)__shfl((int)dg, leader);
+  if (__ballot(m && dg == dl) == am) {
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&lh[dl], __popcll(am));
+  } else if (m) {
+    atomicAdd(&lh[dg], 1);
+  }
+}
+
+// The next digit of a split key: h counts that digit over the node's keys
+// that carry the prefix; the digit holding `rank` joins the prefix and the
+// rank becomes the rank within it.  At rank 0 the split is the first key
+// of the prefix's bucket, so the prefix with every lower digit zero splits
+// the keys the same way, and nothing changes.  All 256 threads call this
+// with the same prefix and rank.
+__device__ inline void big_select(const int* h, int shift,
+                                  unsigned long long& pre, int& rank,
+                                  int* sel, int* err) {
+  if (rank == 0) return;
+  const int t = threadIdx.x, wave = t >> 6;
+  const int c = h[t];
+  const int iw = wscan_incl(c);
+  if ((t & 63) == 63) sel[wave] = iw;
+  if (t == 0) sel[4] = -1;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += sel[w];
+  const int incl = before + iw, excl = incl - c;
+  if (excl <= rank && rank < incl) {
+    sel[4] = t;
+    sel[5] = excl;
+  }
+  __syncthreads();
+  const int d = sel[4], below = sel[5];
+  __syncthreads();
+  if (d < 0) {
+    if (t == 0) atomicOr(err, 16);
+    rank = 0;
+    return;
+  }
+  pre |= (unsigned long long)d << shift;
+  rank -= below;
+}
+
 __global__ __launch_bounds__(256) void k_big_init(BigArgs a) {
   const int i = blockIdx.x, t = threadIdx.x;
   a.hist[256 * (size_t)i + t] = 0;
@@ -169,118 +226,147 @@ __global__ __launch_bounds__(256) void k_big_init(BigArgs a) {
     a.nq[6 * i + t] = ~0u;
     a.nq[6 * i + 3 + t] = 0u;
   }
-  if (t == 0) {
-    a.pre[i] = 0ull;
-    a.rank[i] = a.nodes[i].left;
-  }
 }
 
 __global__ __launch_bounds__(256) void k_big_bbox(BigArgs a) {
+  __shared__ unsigned red[4][6];
   const Chunk c = a.chunks[blockIdx.x];
+  const int t = threadIdx.x;
+  uint4 v[kEpt];
+#pragma unroll
+  for (int u = 0; u < kEpt; ++u) {
+    const int j = c.b + t + 256 * u;
+    v[u] = a.kd.rec[min(j, c.e - 1)];
+  }
   unsigned lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0u, 0u, 0u};
-  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
-    const uint4 r = a.kd.rec[j];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      lo[t] = min(lo[t], qax(r, t));
-      hi[t] = max(hi[t], qax(r, t));
+  for (int u = 0; u < kEpt; ++u) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      lo[x] = min(lo[x], qax(v[u], x));
+      hi[x] = max(hi[x], qax(v[u], x));
     }
   }
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    lo[t] = wmin_u(lo[t]);
-    hi[t] = wmax_u(hi[t]);
+  for (int x = 0; x < 3; ++x) {
+    lo[x] = wmin_u(lo[x]);
+    hi[x] = wmax_u(hi[x]);
   }
-  if ((threadIdx.x & 63) == 0) {
+  if ((t & 63) == 0) {
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      atomicMin(&a.nq[6 * c.node + t], lo[t]);
-      atomicMax(&a.nq[6 * c.node + 3 + t], hi[t]);
+    for (int x = 0; x < 3; ++x) {
+      red[t >> 6][x] = lo[x];
+      red[t >> 6][3 + x] = hi[x];
     }
+  }
+  __syncthreads();
+  if (t < 6) {
+    unsigned r = red[0][t];
+    for (int w = 1; w < 4; ++w)
+      r = t < 3 ? min(r, red[w][t]) : max(r, red[w][t]);
+    if (t < 3) atomicMin(&a.nq[6 * c.node + t], r);
+    else atomicMax(&a.nq[6 * c.node + t], r);
   }
 }
 
+// Digit `pass` of the split keys (8 bits, most significant first).  Each
+// workgroup first takes the previous digit from its counts (every
+// workgroup of a node the same way; the node's first chunk records the
+// result and clears the counts the next launch adds to), then counts its
+// chunk's keys that carry the prefix, unless the split is already found.
 __global__ __launch_bounds__(256) void k_big_hist(BigArgs a, int pass) {
   __shared__ int lh[256];
-  lh[threadIdx.x] = 0;
-  __syncthreads();
+  __shared__ int sel[8];
+  const int t = threadIdx.x;
+  lh[t] = 0;
   const Chunk c = a.chunks[blockIdx.x];
-  const int axis = node_axis(a, c.node);
+  const int i = c.node;
+  const BigNode nd = a.nodes[i];
+  const size_t hs = 256 * (size_t)a.nn;
+  unsigned long long pre = 0ull;
+  int rank = nd.left;
+  if (pass > 0) {
+    const int s = (pass - 1) & 1;
+    pre = a.pre[s * a.nn + i];
+    rank = a.rank[s * a.nn + i];
+    big_select(a.hist + ((pass - 1) % 3) * hs + 256 * (size_t)i,
+               64 - 8 * pass, pre, rank, sel, a.err);
+  }
+  if ((int)blockIdx.x == nd.c0) {
+    if (t == 0) {
+      a.pre[(pass & 1) * a.nn + i] = pre;
+      a.rank[(pass & 1) * a.nn + i] = rank;
+    }
+    a.hist[((pass + 1) % 3) * hs + 256 * (size_t)i + t] = 0;
+  }
+  if (rank == 0) return;
+  const int axis = node_axis(a, i);
   const int shift = 56 - 8 * pass;
-  const unsigned long long p = a.pre[c.node];
-  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
-    const uint4 r = a.kd.rec[j];
-    const unsigned long long key = kd_key(qax(r, axis), (int)r.x);
-    if (pass == 0 || (key >> (shift + 8)) == (p >> (shift + 8)))
-      atomicAdd(&lh[(key >> shift) & 255u], 1);
+  uint4 v[kEpt];
+#pragma unroll
+  for (int u = 0; u < kEpt; ++u) {
+    const int j = c.b + t + 256 * u;
+    v[u] = a.kd.rec[min(j, c.e - 1)];
   }
   __syncthreads();
-  const int v = lh[threadIdx.x];
-  if (v) atomicAdd(&a.hist[256 * (size_t)c.node + threadIdx.x], v);
+#pragma unroll
+  for (int u = 0; u < kEpt; ++u) {
+    const int j = c.b + t + 256 * u;
+    const unsigned long long key = kd_key(qax(v[u], axis), (int)v[u].x);
+    const bool m = j < c.e && (pass == 0 ||
+                               (key >> (shift + 8)) == (pre >> (shift + 8)));
+    lds_count(lh, m, (unsigned)(key >> shift) & 255u);
+  }
+  __syncthreads();
+  const int cnt = lh[t];
+  if (cnt) atomicAdd(&a.hist[(pass % 3) * hs + 256 * (size_t)i + t], cnt);
 }
 
-// One wavefront per node: the digit holding rank `rank`, appended to the
-// prefix; the histogram is cleared for the next digit.
-__global__ __launch_bounds__(64) void k_big_select(BigArgs a, int pass) {
-  const int i = blockIdx.x, lane = threadIdx.x;
-  int* h = a.hist + 256 * (size_t)i;
-  int c[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) c[t] = h[4 * lane + t];
-  const int s = c[0] + c[1] + c[2] + c[3];
-  const int incl = wscan_incl(s), excl = incl - s;
-  const int rk = a.rank[i];
-  const bool mine = excl <= rk && rk < incl;
-  int d = 0, below = excl;
-  if (mine) {
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (rk < below + c[t]) {
-        d = 4 * lane + t;
-        break;
-      }
-      below += c[t];
-    }
-  }
-  const unsigned long long m = __ballot(mine);
-  const int src = m ? __builtin_ctzll(m) : 0;
-  d = __shfl(d, src);
-  below = __shfl(below, src);
-#pragma unroll
-  for (int t = 0; t < 4; ++t) h[4 * lane + t] = 0;
-  if (lane == 0 && m) {
-    a.pre[i] |= (unsigned long long)d << (56 - 8 * pass);
-    a.rank[i] = rk - below;
-  }
-}
-
+// The last digit (as in k_big_hist), then the chunk's keys below the
+// split key; the node's first chunk records the key for the scatter.
 __global__ __launch_bounds__(256) void k_big_count(BigArgs a) {
   __shared__ int part[4];
+  __shared__ int sel[8];
+  const int t = threadIdx.x;
   const Chunk c = a.chunks[blockIdx.x];
-  const int axis = node_axis(a, c.node);
-  const unsigned long long K = a.pre[c.node];
-  int n = 0;
-  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256) {
-    const uint4 r = a.kd.rec[j];
-    n += kd_key(qax(r, axis), (int)r.x) < K;
+  const int i = c.node;
+  unsigned long long K = a.pre[a.nn + i];
+  int rank = a.rank[a.nn + i];
+  big_select(a.hist + 256 * ((size_t)a.nn + i), 0, K, rank, sel, a.err);
+  if ((int)blockIdx.x == a.nodes[i].c0 && t == 0) {
+    a.pre[i] = K;
+    if (rank != 0) atomicOr(a.err, 16);
   }
+  const int axis = node_axis(a, i);
+  uint4 v[kEpt];
+#pragma unroll
+  for (int u = 0; u < kEpt; ++u) {
+    const int j = c.b + t + 256 * u;
+    v[u] = a.kd.rec[min(j, c.e - 1)];
+  }
+  int n = 0;
+#pragma unroll
+  for (int u = 0; u < kEpt; ++u)
+    n += c.b + t + 256 * u < c.e && kd_key(qax(v[u], axis), (int)v[u].x) < K;
   n = wsum_i(n);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+  if ((t & 63) == 0) part[t >> 6] = n;
   __syncthreads();
-  if (threadIdx.x == 0)
-    a.ccount[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+  if (t == 0) a.ccount[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
+// One wavefront per node: the chunks' left-key offsets.
 __global__ __launch_bounds__(64) void k_big_scan(BigArgs a) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= a.nn) return;
-  const BigNode nd = a.nodes[i];
+  const int lane = threadIdx.x;
+  const BigNode nd = a.nodes[blockIdx.x];
   int run = 0;
-  for (int c = nd.c0; c < nd.c0 + nd.nc; ++c) {
-    a.coff[c] = run;
-    run += a.ccount[c];
+  for (int c0 = 0; c0 < nd.nc; c0 += 64) {
+    const int c = c0 + lane;
+    const int v = c < nd.nc ? a.ccount[nd.c0 + c] : 0;
+    const int incl = wscan_incl(v);
+    if (c < nd.nc) a.coff[nd.c0 + c] = run + incl - v;
+    run += __shfl(incl, 63);
   }
-  if (run != nd.left) atomicOr(a.err, 8);
+  if (lane == 0 && run != nd.left) atomicOr(a.err, 8);
 }
 
 // Stable partition of each chunk into tmp: keys below the split to the
@@ -373,19 +459,29 @@ __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
     const int axis = kd_axis(lo, hi, W);
     const int nleaf = (m + L - 1) / L;
     const int left = L * (nleaf / 2);
-    // radix select of the key of rank `left`
-    unsigned long long pre = 0ull;
+    // radix select of the key of rank `left`.  The digits above the first
+    // one in which the node's coordinates on this axis differ are the
+    // box's; the search ends once the rank falls on the first key of a
+    // bucket, whose prefix (lower digits zero) splits the node the same way
+    const unsigned qd = lo[axis] ^ hi[axis];
+    int pass = qd ? (int)(__builtin_clz(qd) >> 3) : 4;
+    unsigned long long pre =
+        pass ? (unsigned long long)(lo[axis] >> (32 - 8 * pass))
+                   << (64 - 8 * pass)
+             : 0ull;
     int rk = left;
-    for (int pass = 0; pass < 8; ++pass) {
+    for (; pass < 8 && rk > 0; ++pass) {
       const int shift = 56 - 8 * pass;
 #pragma unroll
       for (int t = 0; t < 4; ++t) hist[4 * lane + t] = 0;
       __syncthreads();
-      for (int j = lb + lane; j < le; j += 64) {
-        const uint4 r = buf[j];
+      for (int j0 = lb; j0 < le; j0 += 64) {
+        const int j = j0 + lane;
+        const uint4 r = buf[min(j, le - 1)];
         const unsigned long long key = kd_key(qax(r, axis), (int)r.x);
-        if (pass == 0 || (key >> (shift + 8)) == (pre >> (shift + 8)))
-          atomicAdd(&hist[(key >> shift) & 255u], 1);
+        const bool m = j < le && (pass == 0 || (key >> (shift + 8)) ==
+                                                   (pre >> (shift + 8)));
+        lds_count(hist, m, (unsigned)(key >> shift) & 255u);
       }
       __syncthreads();
       int c[4];
@@ -1029,9 +1125,9 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     GTRY(dchunks.alloc(p->ctx, tot_nc));
     GTRY(up(dchunks.p, all_chunks.data(), sizeof(Chunk) * tot_nc));
     GTRY(dnq.alloc(p->ctx, 6 * max_nn));
-    GTRY(dpre.alloc(p->ctx, max_nn));
-    GTRY(drank.alloc(p->ctx, max_nn));
-    GTRY(dhist.alloc(p->ctx, 256 * max_nn));
+    GTRY(dpre.alloc(p->ctx, 2 * max_nn));
+    GTRY(drank.alloc(p->ctx, 2 * max_nn));
+    GTRY(dhist.alloc(p->ctx, 3 * 256 * max_nn));
     GTRY(dcc.alloc(p->ctx, max_nc));
     GTRY(dco.alloc(p->ctx, max_nc));
   }
@@ -1067,13 +1163,10 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     const dim3 gn(b.nn), gc(b.nc);
     hipLaunchKernelGGL(k_big_init, gn, dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_big_bbox, gc, dim3(256), 0, st, b);
-    for (int pass = 0; pass < 8; ++pass) {
+    for (int pass = 0; pass < 8; ++pass)
       hipLaunchKernelGGL(k_big_hist, gc, dim3(256), 0, st, b, pass);
-      hipLaunchKernelGGL(k_big_select, gn, dim3(64), 0, st, b, pass);
-    }
     hipLaunchKernelGGL(k_big_count, gc, dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_big_scan, dim3((b.nn + 63) / 64), dim3(64), 0, st,
-                       b);
+    hipLaunchKernelGGL(k_big_scan, gn, dim3(64), 0, st, b);
     hipLaunchKernelGGL(k_big_scatter, gc, dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_big_copy, gc, dim3(256), 0, st, b);
     GTRY(hipGetLastError());
