@@ -6,6 +6,7 @@
 // caller's order.  Errors are returned as ds_status codes with a message in
 // the context; nothing here exits or falls back to host arithmetic.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdarg>
@@ -13,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -1064,6 +1066,135 @@ int geometry_host(ds_problem* p, const double* pixels,
   return DS_OK;
 }
 
+// The device builder's copy of the caller's pixel records, sent from a
+// host thread of its own so that it overlaps the host's checks of the same
+// records.  The copy only reads them, and nothing on the device uses it
+// before join() (the stream keeps the order from there on).
+struct PixelUpload {
+  ds_ctx* c = nullptr;
+  double* d = nullptr;
+  size_t cap = 0;
+  std::thread th;
+  hipError_t err = hipSuccess;
+  hipError_t start(ds_ctx* ctx, const double* src, size_t count) {
+    c = ctx;
+    const hipError_t e =
+        ds::ctx_buf_get(ctx, (void**)&d, sizeof(double) * count, &cap);
+    if (e != hipSuccess) return e;
+    th = std::thread([this, src, count]() {
+      err = hipSetDevice(c->device);
+      if (err == hipSuccess)
+        err = hipMemcpyAsync(d, src, sizeof(double) * count,
+                             hipMemcpyHostToDevice, c->stream);
+    });
+    return hipSuccess;
+  }
+  hipError_t join() {
+    if (th.joinable()) th.join();
+    return err;
+  }
+  // after the stream has finished with the copy
+  void release() {
+    (void)join();
+    if (d) ds::ctx_buf_put(c, d, cap);
+    d = nullptr;
+  }
+  ~PixelUpload() {
+    (void)join();
+    if (d) {
+      (void)hipStreamSynchronize(c->stream);
+      release();
+    }
+  }
+};
+
+// One pass over the pixel records on host threads: the finite checks of
+// the coordinates (check_points) and of the weights and data, reporting the
+// first failing record in index order; each subvolume's coordinate bounds
+// (kd_quant: min and max, exact in any order) and |b|, each subvolume's sum
+// in index order as lib/wf_vector.cc:105-112 forms it, on one thread.
+int scan_pixels(ds_ctx* ctx, const double* pixels, int n_sub,
+                const int* sub_n, const std::vector<int>& sub_off,
+                const std::vector<ds::CovarDev>& cvs,
+                std::vector<ds::KdQuant>& quant, std::vector<double>& norm_b) {
+  constexpr int kRun = 32768;
+  struct Run {
+    int k, b, e;
+    double lo[3], hi[3];
+    long long bad_x = -1;   // 3 * record + coordinate
+    int bad_wd = -1;
+  };
+  std::vector<Run> runs;
+  for (int k = 0; k < n_sub; ++k)
+    for (int b = sub_off[k]; b < sub_off[k + 1]; b += kRun)
+      runs.push_back({k, b, std::min(sub_off[k + 1], b + kRun), {}, {}});
+  norm_b.assign(n_sub, 0.0);
+  const int ntask = n_sub + (int)runs.size();
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    for (int t = next++; t < ntask; t = next++) {
+      if (t < n_sub) {
+        const double* pk = pixels + 5 * (size_t)sub_off[t];
+        double sum_b2 = 0.0;
+        for (int i = 0; i < sub_n[t]; ++i) {
+          const double bi = pk[5 * (size_t)i + 4];
+          sum_b2 += bi * bi;
+        }
+        norm_b[t] = std::sqrt(sum_b2);
+        continue;
+      }
+      Run& r = runs[t - n_sub];
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = std::numeric_limits<double>::infinity();
+        r.hi[a] = -r.lo[a];
+      }
+      for (int i = r.b; i < r.e; ++i) {
+        const double* q = pixels + 5 * (size_t)i;
+        for (int a = 0; a < 3; ++a) {
+          if (r.bad_x < 0 && !std::isfinite(q[a])) r.bad_x = 3LL * i + a;
+          r.lo[a] = std::min(r.lo[a], q[a]);
+          r.hi[a] = std::max(r.hi[a], q[a]);
+        }
+        if (r.bad_wd < 0 && !(std::isfinite(q[3]) && std::isfinite(q[4])))
+          r.bad_wd = i;
+      }
+    }
+  };
+  const long long nt = std::min<long long>(
+      16, std::max<long long>(1, std::thread::hardware_concurrency()));
+  const int nth = (int)std::min<long long>(nt, ntask);
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; ++t) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
+  for (const Run& r : runs)
+    if (r.bad_x >= 0)
+      return fail(ctx, DS_E_ARG, "pixels: coordinate %d of record %lld is "
+                  "not finite", (int)(r.bad_x % 3), r.bad_x / 3);
+  for (const Run& r : runs)
+    if (r.bad_wd >= 0)
+      return fail(ctx, DS_E_ARG, "pixel %d: weight or data not finite",
+                  r.bad_wd);
+  quant.assign(n_sub, ds::KdQuant());
+  size_t ri = 0;
+  for (int k = 0; k < n_sub; ++k) {
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::numeric_limits<double>::infinity();
+      hi[a] = -lo[a];
+    }
+    for (; ri < runs.size() && runs[ri].k == k; ++ri)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], runs[ri].lo[a]);
+        hi[a] = std::max(hi[a], runs[ri].hi[a]);
+      }
+    double sc[3];
+    axis_scale(cvs[k], sc);
+    for (int a = 0; a < 3; ++a) ds::kd_quant_axis(lo[a], hi[a], sc[a], quant[k], a);
+  }
+  return DS_OK;
+}
+
 int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
                    const double* pixels, const ds_covar* covars,
                    ds_problem** out) {
@@ -1084,13 +1215,12 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   }
   if (ntot > 500000000LL) return fail(ctx, DS_E_ARG, "too many pixels");
   const int n = (int)ntot;
-  if (int rc = check_points(ctx, pixels, 5, n, "pixels")) return rc;
-  for (int i = 0; i < n; ++i)
-    if (!std::isfinite(pixels[5 * (size_t)i + 3]) ||
-        !std::isfinite(pixels[5 * (size_t)i + 4]))
-      return fail(ctx, DS_E_ARG, "pixel %d: weight or data not finite", i);
+  if (!pixels) return fail(ctx, DS_E_ARG, "pixels: NULL array");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device));
   const double t0 = now_s();
+  PixelUpload upload;
+  if (!ctx->host_geometry)
+    HIP_OR_FAIL(ctx, upload.start(ctx, pixels, 5 * (size_t)n));
   std::unique_ptr<ds_problem, void (*)(ds_problem*)> P(
       new ds_problem, [](ds_problem* q) {
         // a failed creation: its enqueued work ends before the buffers go
@@ -1110,35 +1240,14 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
   p->sub_off.assign(n_sub + 1, 0);
   for (int k = 0; k < n_sub; ++k) p->sub_off[k + 1] = p->sub_off[k] + sub_n[k];
   p->sub_cv_h.resize(n_sub);
-  p->sub_norm_b.assign(n_sub, 0.0);
-  // per subvolume: covariance constants, the ordering's quantization and
-  // |b| in the reference's order (lib/wf_vector.cc:105-112, :253-260)
-  std::vector<ds::KdQuant> quant(n_sub);
-  auto prep = [&](int k) {
-    const ds::CovarDev cv = ds::make_covar(&covars[k]);
-    p->sub_cv_h[k] = cv;
-    double sc[3];
-    axis_scale(cv, sc);
-    const double* pk = pixels + 5 * (size_t)p->sub_off[k];
-    quant[k] = ds::kd_quant(pk, 5, sub_n[k], sc);
-    double sum_b2 = 0.0;
-    for (int i = 0; i < sub_n[k]; ++i) {
-      const double bi = pk[5 * (size_t)i + 4];
-      sum_b2 += bi * bi;
-    }
-    p->sub_norm_b[k] = std::sqrt(sum_b2);
-  };
-  if (n_sub == 1) {
-    prep(0);
-  } else {
-    std::vector<std::thread> th;
-    const int nt = std::max(1, std::min(n_sub, 16));
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&, t]() {
-        for (int k = t; k < n_sub; k += nt) prep(k);
-      });
-    for (auto& x : th) x.join();
-  }
+  for (int k = 0; k < n_sub; ++k) p->sub_cv_h[k] = ds::make_covar(&covars[k]);
+  // per subvolume: the ordering's quantization and |b| in the reference's
+  // order (lib/wf_vector.cc:105-112, :253-260)
+  std::vector<ds::KdQuant> quant;
+  if (int rc = scan_pixels(ctx, pixels, n_sub, sub_n, p->sub_off,
+                           p->sub_cv_h, quant, p->sub_norm_b))
+    return rc;
+  HIP_OR_FAIL(ctx, upload.join());
   p->norm_b = p->sub_norm_b[0];
   // The neighbour search's first ring radius (square root of table-argument
   // units): the ball that holds about 64 points at subvolume 0's mean
@@ -1185,8 +1294,9 @@ int create_problem(ds_ctx* ctx, int form, int n_sub, const int* sub_n,
     if (int rc = geometry_host(p, pixels, max_src, bstat)) return rc;
   } else {
     int gerr = 0;
-    HIP_OR_FAIL(ctx, ds::build_geometry_device(p, pixels, quant, max_src,
+    HIP_OR_FAIL(ctx, ds::build_geometry_device(p, upload.d, quant, max_src,
                                                bstat, &gerr));
+    upload.release();
     if (gerr == 1)
       return fail(ctx, DS_E_ARG, "more than INT_MAX tile-list entries");
     if (gerr)

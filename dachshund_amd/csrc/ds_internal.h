@@ -387,14 +387,15 @@ struct ds_problem {
 
 namespace ds {
 // A problem's ordering, culled lists, per-slot arrays and column-sum slots
-// built on the device from the caller's pixel records (geometry_dev.hip);
+// built on the device from the pixel records (geometry_dev.hip), which
+// dpix holds on the device (n x 5 doubles, copied there on p's stream);
 // p's sizes, subvolume arrays and covariance constants are set.  Fills
 // p->perm, p->perm_dev, p->nslot and the device arrays, bstat (per block:
 // list length, half-list length, live sub-block entries, live 32-row
 // halves, evaluated sub-block entries) and max_src (per subvolume: largest
 // |source coordinate| in its tile frame).  *error: 1 = more than INT_MAX
 // entries, 2 = inconsistent symmetric lists.
-hipError_t build_geometry_device(ds_problem* p, const double* pixels,
+hipError_t build_geometry_device(ds_problem* p, const double* dpix,
                                  const std::vector<KdQuant>& quant,
                                  std::vector<float>& max_src,
                                  std::vector<int>& bstat, int* error);

@@ -1004,7 +1004,7 @@ struct SyncGuard {
 
 }  // namespace
 
-hipError_t build_geometry_device(ds_problem* p, const double* pixels,
+hipError_t build_geometry_device(ds_problem* p, const double* dpix,
                                  const std::vector<KdQuant>& quant,
                                  std::vector<float>& max_src,
                                  std::vector<int>& bstat, int* error) {
@@ -1083,7 +1083,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     all_chunks.insert(all_chunks.end(), lv_chunks[l].begin(),
                       lv_chunks[l].end());
   }
-  Tmp<double> dpix, dq, ds3, dcen, dtcen, dglo, dghi;
+  Tmp<double> dq, ds3, dcen, dtcen, dglo, dghi;
   Tmp<int> dsub_n, dsub_off, dcc, dco, drank, dhist, dgcnt, derr, dstat;
   Tmp<double4> dcv;
   Tmp<uint4> drec, dtmp;
@@ -1097,8 +1097,6 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
                  : hipSuccess;
   };
-  GTRY(dpix.alloc(p->ctx, 5 * (size_t)n));
-  GTRY(up(dpix.p, pixels, sizeof(double) * 5 * (size_t)n));
   GTRY(dq.alloc(p->ctx, hq.size()));
   GTRY(up(dq.p, hq.data(), sizeof(double) * hq.size()));
   GTRY(ds3.alloc(p->ctx, hs3.size()));
@@ -1132,7 +1130,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
     GTRY(dco.alloc(p->ctx, max_nc));
   }
   KdArgs kd{};
-  kd.pix = dpix.p;
+  kd.pix = dpix;
   kd.rec = drec.p;
   kd.tmp = dtmp.p;
   kd.blk_sub = p->blk_sub;
@@ -1198,7 +1196,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* pixels,
   GTRY(hipMemsetAsync(dmsrc.p, 0, sizeof(unsigned) * n_sub, st));
   BlkArgs ba{};
   ba.rec = drec.p;
-  ba.pix = dpix.p;
+  ba.pix = dpix;
   ba.blk_sub = p->blk_sub;
   ba.sub_blk0 = p->sub_blk0;
   ba.sub_n = dsub_n.p;
