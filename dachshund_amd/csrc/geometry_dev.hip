@@ -419,8 +419,10 @@ __global__ __launch_bounds__(256) void k_big_copy(BigArgs a) {
 
 // ---- nodes of at most kWaveMax points: one wavefront each, in LDS -------
 
-// Node (b, e) of subvolume `sub` in block phase (leaf 64, then the tiles of
-// each block with leaf 16): every descendant split made in place.
+// Node (b, e) of subvolume `sub` (more than 64 points) in block phase:
+// every descendant split down to leaves of 64 made in place (the splits of
+// each block into its tiles are k_kd_blocks').  The stack keeps the leaf
+// size with each node for the split rule, always 64 here.
 __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
   __shared__ uint4 buf[kWaveMax], buf2[kWaveMax];
   __shared__ int hist[256];
@@ -434,9 +436,9 @@ __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
   if (lane == 0) {
     stk[0] = 0;
     stk[1] = cnt;
-    stk[2] = cnt > DS_BS ? DS_BS : DS_TS;
+    stk[2] = DS_BS;
   }
-  sp = cnt > DS_TS ? 1 : 0;
+  sp = cnt > DS_BS ? 1 : 0;
   __syncthreads();
   while (sp > 0) {
     --sp;
@@ -530,15 +532,13 @@ __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
     __syncthreads();
     for (int j = lb + lane; j < le; j += 64) buf[j] = buf2[j];
     __syncthreads();
-    // children: leaf 64 until a child fits a block, then leaf 16
+    // children that hold more than one block
     const int cb[2] = {lb, lb + left}, ce[2] = {lb + left, le};
     int push = 0;
 #pragma unroll
     for (int h = 1; h >= 0; --h) {
       const int cn = ce[h] - cb[h];
-      int Lc = 0;
-      if (L == DS_BS && cn > DS_BS) Lc = DS_BS;
-      else if (cn > DS_TS) Lc = DS_TS;
+      const int Lc = cn > DS_BS ? DS_BS : 0;
       if (Lc) {
         if (lane == 0) {
           stk[3 * (sp + push)] = cb[h];
@@ -559,6 +559,7 @@ __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
 struct BlkArgs {
   const uint4* rec;
   const double* pix;
+  const double* quant;      // per subvolume: lo[3], inv[3], w[3]
   const int* blk_sub;
   const int* sub_blk0;
   const int* sub_n;
@@ -600,6 +601,73 @@ __device__ inline double group_median(double v, bool real, int cnt) {
   return __shfl(v, src);
 }
 
+// The splits of one 64-slot block into its 16-slot tiles (build_blocking
+// with leaf 16: the node's box picks the axis, its left child takes the
+// `left` smallest keys, both children keep the node's order).  Lane i
+// holds the record of slot i (lanes from nreal on hold none).  Every node
+// is a range of lanes made of whole 16-lane groups (16 divides each split
+// point), and the nodes of one level split together: the box from the
+// groups' boxes, each key's rank within its node by a pass over the
+// node's lanes, then every record pushed to its new lane.
+__device__ inline uint4 tile_order(uint4 r, int nreal, const double w[3]) {
+  const int lane = threadIdx.x & 63;
+  const bool real = lane < nreal;
+  int nb = real ? 0 : lane, ne = real ? nreal : lane;   // this lane's node
+  for (int level = 0; level < 3; ++level) {
+    const bool act = ne - nb > DS_TS;
+    if (!__ballot(act)) break;
+    unsigned lo[3], hi[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      unsigned gl = real ? qax(r, ax) : ~0u, gh = real ? qax(r, ax) : 0u;
+#pragma unroll
+      for (int o = 8; o >= 1; o >>= 1) {
+        gl = min(gl, (unsigned)__shfl_xor((int)gl, o));
+        gh = max(gh, (unsigned)__shfl_xor((int)gh, o));
+      }
+      lo[ax] = ~0u;
+      hi[ax] = 0u;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const unsigned sl = (unsigned)__builtin_amdgcn_readlane((int)gl, 16 * g);
+        const unsigned sh = (unsigned)__builtin_amdgcn_readlane((int)gh, 16 * g);
+        if (16 * g >= nb && 16 * g < ne) {
+          lo[ax] = min(lo[ax], sl);
+          hi[ax] = max(hi[ax], sh);
+        }
+      }
+    }
+    const int axis = kd_axis(lo, hi, w);
+    const unsigned q = qax(r, axis), id = r.x;
+    int rank = 0;
+    for (int j = 0; j < nreal; ++j) {
+      const unsigned qj = (unsigned)__builtin_amdgcn_readlane((int)q, j);
+      const unsigned ij = (unsigned)__builtin_amdgcn_readlane((int)id, j);
+      const bool less = qj < q || (qj == q && ij < id);
+      rank += (j >= nb && j < ne && less) ? 1 : 0;
+    }
+    const int m = ne - nb;
+    const int left = DS_TS * (((m + DS_TS - 1) / DS_TS) / 2);
+    const bool isl = act && rank < left;
+    const unsigned long long bl = __ballot(isl);
+    int dst = lane;
+    if (act) {
+      const unsigned long long below_nb = (1ull << nb) - 1ull;
+      const int lb4 = mbcnt64(bl) - __popcll(bl & below_nb);
+      dst = isl ? nb + lb4 : nb + left + (lane - nb - lb4);
+    }
+    r.x = (unsigned)__builtin_amdgcn_ds_permute(4 * dst, (int)r.x);
+    r.y = (unsigned)__builtin_amdgcn_ds_permute(4 * dst, (int)r.y);
+    r.z = (unsigned)__builtin_amdgcn_ds_permute(4 * dst, (int)r.z);
+    r.w = (unsigned)__builtin_amdgcn_ds_permute(4 * dst, (int)r.w);
+    if (act) {
+      if (lane < nb + left) ne = nb + left;
+      else nb += left;
+    }
+  }
+  return r;
+}
+
 __global__ __launch_bounds__(256) void k_kd_blocks(BlkArgs a, int nblk) {
   const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (B >= nblk) return;
@@ -608,7 +676,18 @@ __global__ __launch_bounds__(256) void k_kd_blocks(BlkArgs a, int nblk) {
   const int nreal = min(DS_BS, a.sub_n[k] - (B - a.sub_blk0[k]) * DS_BS);
   const bool real = lane < nreal;
   const size_t g = (size_t)B * DS_BS + lane;
-  const uint4 r = a.rec[(size_t)B * DS_BS + (real ? lane : nreal - 1)];
+  uint4 r = a.rec[(size_t)B * DS_BS + (real ? lane : nreal - 1)];
+  {
+    const double* Q = a.quant + 9 * (size_t)k + 6;
+    const double w[3] = {Q[0], Q[1], Q[2]};
+    r = tile_order(r, nreal, w);
+    // the slots after the last real one repeat its (ordered) record
+    const int src = real ? lane : nreal - 1;
+    r.x = (unsigned)__shfl((int)r.x, src);
+    r.y = (unsigned)__shfl((int)r.y, src);
+    r.z = (unsigned)__shfl((int)r.z, src);
+    r.w = (unsigned)__shfl((int)r.w, src);
+  }
   int idx = (int)r.x;
   if (idx < 0 || idx >= a.n) {
     atomicOr(a.err, 4);
@@ -1026,7 +1105,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     const Pend d = todo.back();
     todo.pop_back();
     const int cnt = d.e - d.b;
-    if (cnt <= DS_TS) continue;
+    if (cnt <= DS_BS) continue;   // one block: k_kd_blocks makes its tiles
     if (cnt <= kWaveMax) {
       wave_nodes.push_back(make_int4(d.b, d.e, d.sub, 0));
       continue;
@@ -1197,6 +1276,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
   BlkArgs ba{};
   ba.rec = drec.p;
   ba.pix = dpix;
+  ba.quant = dq.p;
   ba.blk_sub = p->blk_sub;
   ba.sub_blk0 = p->sub_blk0;
   ba.sub_n = dsub_n.p;
