@@ -628,3 +628,31 @@ def test_context_released_before_its_problem(D):
     ctx.close()
     assert np.array_equal(wf.A_product(a), before)
     wf.close()
+
+
+def test_creation_rejects_non_finite_pixels(D):
+    """Problem creation checks the records on several host threads while the
+    upload runs beside them: a non-finite coordinate is reported before a
+    non-finite weight or datum, each at its smallest record index, the
+    upload's buffer goes back to the context, and the next problem builds."""
+    from dachshund_amd import _lib
+    from dachshund_amd import synth
+    pix = synth.make_pixels(synth.workload("c1"))
+    big = np.concatenate([pix] * 10)            # 81,920 records: three runs
+    sp = D.SignalCovarParams(0.05, 2.5, 2.1)
+    bad = big.copy()
+    bad[40000, 3] = np.nan
+    bad[70000, 0] = np.nan
+    bad[50000, 2] = np.inf
+    with pytest.raises(_lib.DsError, match="coordinate 2 of record 50000"):
+        D.WFVectorW(D.pixel_to_wpixel(bad), sp)
+    bad[50000, 2] = big[50000, 2]
+    bad[70000, 0] = big[70000, 0]
+    with pytest.raises(_lib.DsError, match="pixel 40000: weight or data"):
+        D.WFVectorW(D.pixel_to_wpixel(bad), sp)
+    wf = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    a = np.random.default_rng(3).standard_normal(len(pix))
+    ref = D.WFVectorW(D.pixel_to_wpixel(pix), sp)
+    assert np.array_equal(wf.A_product(a), ref.A_product(a))
+    wf.close()
+    ref.close()
