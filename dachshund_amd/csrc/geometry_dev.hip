@@ -1378,20 +1378,37 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
   hipLaunchKernelGGL(k_kd_gather<true>, gb, dim3(256), 0, st, ha);
   GTRY(hipGetLastError());
   // ---- what the host needs: per-block counts, far points, slot order ----
+  // (one pinned buffer from the context's pool, copied back in one go)
   bstat.resize(5 * (size_t)nblk);
   std::vector<unsigned> ms(n_sub);
   int nslot = 0, gerr = 0;
   p->perm.resize(npad);
-  GTRY(hipMemcpyAsync(bstat.data(), dstat.p, sizeof(int) * bstat.size(),
-                      hipMemcpyDeviceToHost, st));
-  GTRY(hipMemcpyAsync(ms.data(), dmsrc.p, sizeof(unsigned) * n_sub,
-                      hipMemcpyDeviceToHost, st));
-  GTRY(hipMemcpyAsync(&nslot, p->gptr + ntile, sizeof(int),
-                      hipMemcpyDeviceToHost, st));
-  GTRY(hipMemcpyAsync(&gerr, derr.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  GTRY(hipMemcpyAsync(p->perm.data(), p->perm_dev, sizeof(int) * npad,
-                      hipMemcpyDeviceToHost, st));
-  GTRY(hipStreamSynchronize(st));
+  const size_t o_stat = 0, o_ms = o_stat + sizeof(int) * bstat.size(),
+               o_slot = o_ms + sizeof(unsigned) * n_sub,
+               o_err = o_slot + sizeof(int), o_perm = o_err + sizeof(int),
+               o_end = o_perm + sizeof(int) * (size_t)npad;
+  char* hb = nullptr;
+  GTRY(ds::pool_alloc(p, (void**)&hb, o_end, true));
+  auto down = [&](size_t off, const void* src, size_t bytes) {
+    return hipMemcpyAsync(hb + off, src, bytes, hipMemcpyDeviceToHost, st);
+  };
+  hipError_t ce = down(o_stat, dstat.p, o_ms - o_stat);
+  if (ce == hipSuccess) ce = down(o_ms, dmsrc.p, o_slot - o_ms);
+  if (ce == hipSuccess) ce = down(o_slot, p->gptr + ntile, sizeof(int));
+  if (ce == hipSuccess) ce = down(o_err, derr.p, sizeof(int));
+  if (ce == hipSuccess) ce = down(o_perm, p->perm_dev, o_end - o_perm);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  if (ce == hipSuccess) {
+    std::memcpy(bstat.data(), hb + o_stat, o_ms - o_stat);
+    std::memcpy(ms.data(), hb + o_ms, o_slot - o_ms);
+    std::memcpy(&nslot, hb + o_slot, sizeof(int));
+    std::memcpy(&gerr, hb + o_err, sizeof(int));
+    std::memcpy(p->perm.data(), hb + o_perm, o_end - o_perm);
+  } else {
+    (void)hipStreamSynchronize(st);
+  }
+  ds::pool_free(p, hb);
+  GTRY(ce);
   if (nslot < 0 || gerr) {
     *error = nslot < 0 ? 1 : 2 + gerr;
     return hipSuccess;
