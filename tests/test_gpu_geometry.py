@@ -145,3 +145,27 @@ def test_device_geometry_solves_like_host(D, host_ctx):
     rd = D.WFVectorW(recs, sp).solve_pcg(prm, xd)
     rh = D.WFVectorW(recs, sp, ctx=host_ctx).solve_pcg(prm, xh)
     assert rd.success and rd.num_iters == rh.num_iters and _same(xd, xh)
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_device_tile_order_every_block_size(D, host_ctx, ties):
+    """One subvolume per block size from 17 to 64 points (each a single
+    block whose tiles the block kernel orders in registers), and two of 65
+    and 200 points (blocks cut from a larger node), on spread-out points and
+    on points with only three distinct values per axis (ties in every
+    split key, settled by the point index)."""
+    rng = np.random.default_rng(11 if ties else 12)
+    parts, sps = [], []
+    for n in list(range(17, 65)) + [65, 200]:
+        p = np.empty((n, 5))
+        if ties:
+            p[:, :3] = rng.integers(0, 3, (n, 3)) * 1.5
+        else:
+            p[:, :3] = rng.uniform(0, 5.0, (n, 3))
+        p[:, 3] = rng.uniform(0.1, 0.3, n)
+        p[:, 4] = rng.standard_normal(n)
+        parts.append(D.pixel_to_wpixel(p))
+        sps.append(D.SignalCovarParams(0.05, 2.0, 2.5))
+    dev = D.BatchWF(parts, sps)
+    host = D.BatchWF(parts, sps, ctx=host_ctx)
+    _compare(dev, host)
