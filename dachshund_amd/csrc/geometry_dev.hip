@@ -411,18 +411,14 @@ __global__ __launch_bounds__(256) void k_big_scatter(BigArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_big_copy(BigArgs a) {
-  const Chunk c = a.chunks[blockIdx.x];
-  for (int j = c.b + (int)threadIdx.x; j < c.e; j += 256)
-    a.kd.rec[j] = a.kd.tmp[j];
-}
-
 // ---- nodes of at most kWaveMax points: one wavefront each, in LDS -------
 
 // Node (b, e) of subvolume `sub` (more than 64 points) in block phase:
 // every descendant split down to leaves of 64 made in place (the splits of
-// each block into its tiles are k_kd_blocks').  The stack keeps the leaf
-// size with each node for the split rule, always 64 here.
+// each block into its tiles are k_kd_blocks').  The node's records come
+// from the record buffer its last big-node level wrote (wn.w: 1 = tmp), and
+// the finished order goes to rec.  The stack keeps the leaf size with each
+// node for the split rule, always 64 here.
 __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
   __shared__ uint4 buf[kWaveMax], buf2[kWaveMax];
   __shared__ int hist[256];
@@ -431,7 +427,8 @@ __global__ __launch_bounds__(64) void k_kd_wave(KdArgs a, const int4* wn) {
   const int4 node = wn[blockIdx.x];
   const int b = node.x, cnt = node.y - node.x;
   const double* W = a.quant + 9 * (size_t)node.z + 6;
-  for (int j = lane; j < cnt; j += 64) buf[j] = a.rec[b + j];
+  const uint4* src = node.w ? a.tmp : a.rec;
+  for (int j = lane; j < cnt; j += 64) buf[j] = src[b + j];
   int sp = 0;
   if (lane == 0) {
     stk[0] = 0;
@@ -1107,7 +1104,9 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     const int cnt = d.e - d.b;
     if (cnt <= DS_BS) continue;   // one block: k_kd_blocks makes its tiles
     if (cnt <= kWaveMax) {
-      wave_nodes.push_back(make_int4(d.b, d.e, d.sub, 0));
+      // depth d >= 1: written by level d - 1's scatter, which writes tmp
+      // at even levels and rec at odd ones (depth 0: rec, from k_kd_init)
+      wave_nodes.push_back(make_int4(d.b, d.e, d.sub, d.depth & 1));
       continue;
     }
     if ((int)lv_nodes.size() <= d.depth) {
@@ -1221,11 +1220,14 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
   hipLaunchKernelGGL(k_kd_init, dim3((npad + 255) / 256), dim3(256), 0, st,
                      kd);
   GTRY(hipGetLastError());
-  // ---- big nodes, level by level ----
+  // ---- big nodes, level by level; each level reads one record buffer
+  // and scatters into the other ----
   size_t no = 0, co = 0;
   for (size_t l = 0; l < lv_nodes.size(); ++l) {
     BigArgs b{};
     b.kd = kd;
+    b.kd.rec = (l & 1) ? dtmp.p : drec.p;
+    b.kd.tmp = (l & 1) ? drec.p : dtmp.p;
     b.nodes = dnodes.p + no;
     b.chunks = dchunks.p + co;
     b.nq = dnq.p;
@@ -1245,7 +1247,6 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     hipLaunchKernelGGL(k_big_count, gc, dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_big_scan, gn, dim3(64), 0, st, b);
     hipLaunchKernelGGL(k_big_scatter, gc, dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_big_copy, gc, dim3(256), 0, st, b);
     GTRY(hipGetLastError());
     no += lv_nodes[l].size();
     co += lv_chunks[l].size();
