@@ -151,12 +151,16 @@ struct BigArgs {
   // After the last digit the first nn prefixes hold the split keys.
   unsigned long long* pre;
   int* rank;
-  int* hist;                  // 3 x nn x 256 digit counts, used in turn
+  int* hist;                  // 3 x max_nn x 256 digit counts, in turn
+  unsigned* nq_next;          // the next level's boxes (the other half of
+                              // a double buffer), set up by k_big_count
   int* ccount;                // per chunk: keys below the split
   int* coff;                  // per chunk: left keys of earlier chunks
   int* err;                   // bit 8: a split count differs from `left`,
                               // bit 16: a digit histogram does not add up
   int nn, nc;
+  int max_nn;                 // node capacity of hist and of each nq half
+  int nn_next;                // the next level's node count
 };
 
 __device__ inline int node_axis(const BigArgs& a, int i) {
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(256) void k_big_hist(BigArgs a, int pass) {
   const Chunk c = a.chunks[blockIdx.x];
   const int i = c.node;
   const BigNode nd = a.nodes[i];
-  const size_t hs = 256 * (size_t)a.nn;
+  const size_t hs = 256 * (size_t)a.max_nn;
   unsigned long long pre = 0ull;
   int rank = nd.left;
   if (pass > 0) {
@@ -332,10 +336,16 @@ __global__ __launch_bounds__(256) void k_big_count(BigArgs a) {
   const int i = c.node;
   unsigned long long K = a.pre[a.nn + i];
   int rank = a.rank[a.nn + i];
-  big_select(a.hist + 256 * ((size_t)a.nn + i), 0, K, rank, sel, a.err);
+  big_select(a.hist + 256 * ((size_t)a.max_nn + i), 0, K, rank, sel, a.err);
   if ((int)blockIdx.x == a.nodes[i].c0 && t == 0) {
     a.pre[i] = K;
     if (rank != 0) atomicOr(a.err, 16);
+  }
+  // the next level's first counts (slot 0, last read by this level's
+  // seventh digit launch) and boxes
+  for (int m = blockIdx.x; m < a.nn_next; m += gridDim.x) {
+    a.hist[256 * (size_t)m + t] = 0;
+    if (t < 6) a.nq_next[6 * m + t] = t < 3 ? ~0u : 0u;
   }
   const int axis = node_axis(a, i);
   uint4 v[kEpt];
@@ -1200,7 +1210,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     GTRY(up(dnodes.p, all_nodes.data(), sizeof(BigNode) * tot_nn));
     GTRY(dchunks.alloc(p->ctx, tot_nc));
     GTRY(up(dchunks.p, all_chunks.data(), sizeof(Chunk) * tot_nc));
-    GTRY(dnq.alloc(p->ctx, 6 * max_nn));
+    GTRY(dnq.alloc(p->ctx, 2 * 6 * max_nn));
     GTRY(dpre.alloc(p->ctx, 2 * max_nn));
     GTRY(drank.alloc(p->ctx, 2 * max_nn));
     GTRY(dhist.alloc(p->ctx, 3 * 256 * max_nn));
@@ -1230,7 +1240,10 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     b.kd.tmp = (l & 1) ? drec.p : dtmp.p;
     b.nodes = dnodes.p + no;
     b.chunks = dchunks.p + co;
-    b.nq = dnq.p;
+    b.nq = dnq.p + (l & 1) * 6 * max_nn;
+    b.nq_next = dnq.p + ((l + 1) & 1) * 6 * max_nn;
+    b.max_nn = (int)max_nn;
+    b.nn_next = l + 1 < lv_nodes.size() ? (int)lv_nodes[l + 1].size() : 0;
     b.pre = dpre.p;
     b.rank = drank.p;
     b.hist = dhist.p;
@@ -1240,7 +1253,7 @@ hipError_t build_geometry_device(ds_problem* p, const double* dpix,
     b.nn = (int)lv_nodes[l].size();
     b.nc = (int)lv_chunks[l].size();
     const dim3 gn(b.nn), gc(b.nc);
-    hipLaunchKernelGGL(k_big_init, gn, dim3(256), 0, st, b);
+    if (l == 0) hipLaunchKernelGGL(k_big_init, gn, dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_big_bbox, gc, dim3(256), 0, st, b);
     for (int pass = 0; pass < 8; ++pass)
       hipLaunchKernelGGL(k_big_hist, gc, dim3(256), 0, st, b, pass);
